@@ -1,0 +1,989 @@
+// Host orchestration of one discharge case on one GPU, and the C ABI of
+// include/picmcc.h.  A case owns every device buffer (no globals, unlike the
+// reference's src/Variables.cu) and runs on its own HIP stream, so several
+// cases can share a device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/picmcc.h"
+#include "pic_kernels.h"
+#include "pic_setup.h"
+
+namespace pic {
+
+static thread_local std::string g_err;
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess) return fail(-50, "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t n) {
+    return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+}
+template <typename T>
+static hipError_t upload(T **p, const std::vector<T> &v) {
+    hipError_t e = dalloc(p, v.size());
+    if (e != hipSuccess) return e;
+    if (!v.empty()) e = hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+struct CaseBase {
+    virtual ~CaseBase() {}
+    virtual int advance_time() = 0;
+    virtual int field_solve() = 0;
+    virtual int efield() = 0;
+    virtual int move() = 0;
+    virtual int sort_boundary() = 0;
+    virtual int mcc() = 0;
+    virtual int deposit() = 0;
+    virtual int diagnostic() = 0;
+    virtual int step_fused(int n) = 0;
+    virtual int sync() = 0;
+    virtual int info(pic_info *) = 0;
+    virtual int get_particles(int, void *, void *, void *, void *, void *, uint32_t *, int32_t *) = 0;
+    virtual int set_particles(int, long long, const void *, const void *, const void *, const void *, const void *,
+                              const uint32_t *, const int32_t *, uint32_t) = 0;
+    virtual int array_info(const char *, size_t *, int *, int *) = 0;
+    virtual int get_array(const char *, void *, size_t) = 0;
+    virtual int set_array(const char *, const void *, size_t) = 0;
+    virtual int set_step(long long) = 0;
+    virtual int last_solve(int *, int *, double *) = 0;
+    int rf_cycles = 0, dt_pic = 0;
+};
+
+template <typename R>
+struct Case : CaseBase {
+    Setup<R> S;
+    int precision = 0;
+    int device = 0;
+    double inner_tol = 1e-4;
+    int pcg_maxit = 100000;
+    hipStream_t st = nullptr;
+    int G = 0, nsp = 2;
+    DevConst<R> kc{};
+    NodeConst<R> nc{};
+    VoltConst vc{};
+    // node / setup arrays
+    R *Ex = nullptr, *Ey = nullptr, *P1 = nullptr, *P2 = nullptr, *Pion = nullptr, *BackVel1 = nullptr;
+    R *vsave = nullptr, *xe = nullptr, *cx = nullptr, *lap = nullptr, *phi = nullptr, *tot = nullptr;
+    R *Sigma = nullptr, *Source = nullptr, *den = nullptr, *sigma_sp = nullptr, *V = nullptr, *part = nullptr;
+    R *brhs = nullptr, *Area = nullptr, *eps = nullptr, *stage = nullptr;
+    int *StructIdx = nullptr, *Boundary = nullptr, *Face = nullptr, *CondID = nullptr, *A_idx = nullptr;
+    Route *route = nullptr;
+    int *inv_start = nullptr, *inv_ent = nullptr;
+    // field solve
+    int *pcol = nullptr;
+    float *pvalf = nullptr, *pmdf = nullptr;
+    double *pvald = nullptr, *pmdd = nullptr, *b64 = nullptr, *x64 = nullptr;
+    PcgWork *pw = nullptr;
+    PcgResult *d_res = nullptr;
+    PcgOperator op{};
+    // particles: two buffers per species
+    PView<R> cur[MAXSP]{}, nxt[MAXSP]{};
+    uint8_t *tag[MAXSP] = {nullptr};
+    int *key[MAXSP] = {nullptr}, *prank = nullptr;
+    int *cnt[MAXSP] = {nullptr}, *base[MAXSP] = {nullptr}, *pbase[MAXSP] = {nullptr};
+    int *pcnt = nullptr, *pid_off = nullptr, *total = nullptr, *alive = nullptr, *ptotal = nullptr;
+    int *pstart = nullptr, *scan_tmp = nullptr, *cellp = nullptr;
+    uint32_t *next_id = nullptr;
+    unsigned long long *mccr = nullptr;
+    int *condcharge = nullptr, *d_err = nullptr;
+    double *sum_den = nullptr, *sum_phi = nullptr, *sum_ex = nullptr, *sum_ey = nullptr;
+    long long n_accum = 0;
+    long long tstep = 0;
+    double t = 0.0;
+    int tag_valid = 0;
+
+    ~Case() override {
+        if (st) hipStreamSynchronize(st);
+        void *ptrs[] = {Ex, Ey, P1, P2, Pion, BackVel1, vsave, xe, cx, lap, phi, tot, Sigma, Source, den, sigma_sp,
+                        V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
+                        inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
+                        ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
+                        sum_ey};
+        for (void *p : ptrs)
+            if (p) hipFree(p);
+        for (int s = 0; s < MAXSP; ++s) {
+            PView<R> *vs[2] = {&cur[s], &nxt[s]};
+            for (PView<R> *v : vs) {
+                void *q[] = {v->x, v->y, v->vx, v->vy, v->vz, v->id, v->cell_start};
+                for (void *p : q)
+                    if (p) hipFree(p);
+            }
+            if (tag[s]) hipFree(tag[s]);
+            if (key[s]) hipFree(key[s]);
+            if (cnt[s]) hipFree(cnt[s]);
+            if (base[s]) hipFree(base[s]);
+            if (pbase[s]) hipFree(pbase[s]);
+        }
+        pcg_work_destroy(pw);
+        if (st) hipStreamDestroy(st);
+    }
+
+    // ------------------------------------------------------------ routing tables
+    // Destination of a particle leaving cell c in each direction, restating
+    // the structure-map lookup of src/cuda_Sortboundary.cu:68-257.
+    void build_routes(std::vector<Route> &rt, std::vector<int> &istart, std::vector<int> &ient) {
+        const int ngy = S.ngy, ncx = S.ncx, ncy = S.ncy;
+        rt.assign((size_t)G * ROUTE_SLOTS, Route{-1, 0});
+        std::vector<std::vector<std::pair<int, int>>> inv(G);   // (source cell, entry)
+        for (int i = 0; i < ncx; ++i)
+            for (int j = 0; j < ncy; ++j) {
+                const int c = i * ngy + j;
+                int dests[ROUTE_SLOTS];
+                int nslot = 0;
+                for (int code = 0; code < ROUTE_SLOTS; ++code) {
+                    Route r{-1, 0};
+                    if (code == 4) {
+                        r.dest = c;
+                        r.info = 0;
+                        rt[(size_t)c * ROUTE_SLOTS + code] = r;
+                        continue;
+                    }
+                    const int mcid = code_mcid(code, ngy);
+                    const int SI = S.StructIdx[padded_index(c, mcid, ngy)];
+                    const int cid = c + mcid;
+                    int flags = 0;
+                    if (SI == 0) r.dest = cid;
+                    else if (SI >= 1 && SI < 100) r.dest = -3;
+                    else if (SI >= 100) { r.dest = -2; r.info = SI - 101; }
+                    else {
+                        int sum = 0, ok = 1;
+                        switch (SI) {
+                        case -2: sum = ngy; flags = 16 | 64; break;
+                        case -6: sum = -ngy; flags = 16 | 64; break;
+                        case -4: sum = 1; flags = 32 | 128; break;
+                        case -8: sum = -1; flags = 32 | 128; break;
+                        case -12: sum = ngy + 1; flags = 16 | 32 | 64 | 128; break;
+                        case -16: sum = 1 - ngy; flags = 16 | 32 | 64 | 128; break;
+                        case -20: sum = ngy - 1; flags = 16 | 32 | 64 | 128; break;
+                        case -24: sum = -ngy - 1; flags = 16 | 32 | 64 | 128; break;
+                        default: ok = 0;
+                        }
+                        r.dest = ok ? cid + sum : -1;
+                    }
+                    if (r.dest >= 0) {
+                        int slot = -1;
+                        for (int q = 0; q < nslot; ++q)
+                            if (dests[q] == r.dest) slot = q + 1;
+                        if (slot < 0) {
+                            dests[nslot++] = r.dest;
+                            slot = nslot;
+                            inv[r.dest].push_back({c, c * ROUTE_SLOTS + slot});
+                        }
+                        r.info = slot | flags;
+                    }
+                    rt[(size_t)c * ROUTE_SLOTS + code] = r;
+                }
+            }
+        istart.assign(G + 1, 0);
+        ient.clear();
+        for (int d = 0; d < G; ++d) {
+            const int i = d / ngy, j = d % ngy;
+            if (i < ncx && j < ncy) ient.push_back(d * ROUTE_SLOTS + 0);
+            auto &v = inv[d];
+            std::stable_sort(v.begin(), v.end(), [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+                return a.first < b.first;
+            });
+            for (auto &e : v) ient.push_back(e.second);
+            istart[d + 1] = (int)ient.size();
+        }
+    }
+
+    int alloc_species(int s, int cap) {
+        for (PView<R> *v : {&cur[s], &nxt[s]}) {
+            HIPCHK(dalloc(&v->x, cap)); HIPCHK(dalloc(&v->y, cap)); HIPCHK(dalloc(&v->vx, cap));
+            HIPCHK(dalloc(&v->vy, cap)); HIPCHK(dalloc(&v->vz, cap)); HIPCHK(dalloc(&v->id, cap));
+            HIPCHK(dalloc(&v->cell_start, G + 1));
+            HIPCHK(hipMemset(v->cell_start, 0, (G + 1) * sizeof(int)));
+            v->cap = cap;
+        }
+        HIPCHK(dalloc(&tag[s], cap));
+        HIPCHK(dalloc(&key[s], cap));
+        HIPCHK(dalloc(&cnt[s], (size_t)G * ROUTE_SLOTS));
+        HIPCHK(dalloc(&base[s], (size_t)G * ROUTE_SLOTS));
+        HIPCHK(dalloc(&pbase[s], (size_t)G * ROUTE_SLOTS));
+        return 0;
+    }
+
+    int init(const Deck &d, const pic_options &o) {
+        precision = o.precision;
+        device = o.device;
+        if (o.inner_tol > 0) inner_tol = o.inner_tol;
+        if (o.pcg_maxit > 0) pcg_maxit = o.pcg_maxit;
+        HIPCHK(hipSetDevice(device));
+        std::string err;
+        int e = build_setup<R>(d, S, err, o.load != 0);
+        if (e) return fail(e, "%s", err.c_str());
+        rf_cycles = d.rf_cycles;
+        dt_pic = d.dt_pic;
+        G = S.Gsize;
+        nsp = S.nsp;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        // node arrays
+        HIPCHK(dalloc(&Ex, G)); HIPCHK(dalloc(&Ey, G)); HIPCHK(dalloc(&phi, G)); HIPCHK(dalloc(&tot, G));
+        HIPCHK(dalloc(&Sigma, G)); HIPCHK(dalloc(&Source, G));
+        HIPCHK(dalloc(&den, (size_t)nsp * G)); HIPCHK(dalloc(&sigma_sp, (size_t)nsp * G));
+        HIPCHK(dalloc(&part, (size_t)nsp * G * 4)); HIPCHK(dalloc(&V, 64));
+        HIPCHK(dalloc(&brhs, S.A_size));
+        const void *zero_targets[] = {};
+        (void)zero_targets;
+        HIPCHK(hipMemset(Ex, 0, G * sizeof(R))); HIPCHK(hipMemset(Ey, 0, G * sizeof(R)));
+        HIPCHK(hipMemset(phi, 0, G * sizeof(R))); HIPCHK(hipMemset(tot, 0, G * sizeof(R)));
+        HIPCHK(hipMemset(Sigma, 0, G * sizeof(R))); HIPCHK(hipMemset(Source, 0, G * sizeof(R)));
+        HIPCHK(hipMemset(den, 0, (size_t)nsp * G * sizeof(R)));
+        HIPCHK(hipMemset(sigma_sp, 0, (size_t)nsp * G * sizeof(R)));
+        HIPCHK(hipMemset(part, 0, (size_t)nsp * G * 4 * sizeof(R)));
+        HIPCHK(hipMemset(brhs, 0, std::max(1, S.A_size) * sizeof(R)));
+        HIPCHK(upload(&P1, S.P1)); HIPCHK(upload(&P2, S.P2)); HIPCHK(upload(&Pion, S.Pion));
+        HIPCHK(upload(&BackVel1, S.BackVel1)); HIPCHK(upload(&vsave, S.vsave));
+        HIPCHK(upload(&xe, S.xe)); HIPCHK(upload(&cx, S.cx)); HIPCHK(upload(&lap, S.lap_phi));
+        HIPCHK(upload(&StructIdx, S.StructIdx)); HIPCHK(upload(&Boundary, S.Boundary));
+        HIPCHK(upload(&Face, S.Face)); HIPCHK(upload(&CondID, S.CondID)); HIPCHK(upload(&A_idx, S.A_idx));
+        HIPCHK(upload(&Area, S.Area)); HIPCHK(upload(&eps, S.eps));
+        std::vector<Route> rt;
+        std::vector<int> istart, ient;
+        build_routes(rt, istart, ient);
+        HIPCHK(upload(&route, rt)); HIPCHK(upload(&inv_start, istart)); HIPCHK(upload(&inv_ent, ient));
+        // field operator in padded 5-entry rows
+        {
+            const int N = S.A_size;
+            std::vector<int> col(5 * (size_t)N);
+            std::vector<float> vf(5 * (size_t)N, 0.0f), mf(N);
+            std::vector<double> vd(5 * (size_t)N, 0.0), md(N);
+            for (int r = 0; r < N; ++r) {
+                int k = 0;
+                for (int q = S.Ai[r]; q < S.Ai[r + 1]; ++q, ++k) {
+                    col[5 * r + k] = S.Aj[q];
+                    vf[5 * r + k] = (float)S.Aval[q];
+                    vd[5 * r + k] = (double)S.Aval[q];
+                }
+                for (; k < 5; ++k) col[5 * r + k] = r;
+                mf[r] = (float)S.Mdiag[r];
+                md[r] = (double)S.Mdiag[r];
+            }
+            HIPCHK(upload(&pcol, col)); HIPCHK(upload(&pvalf, vf)); HIPCHK(upload(&pvald, vd));
+            HIPCHK(upload(&pmdf, mf)); HIPCHK(upload(&pmdd, md));
+            HIPCHK(dalloc(&b64, N)); HIPCHK(dalloc(&x64, N));
+            HIPCHK(hipMemset(x64, 0, std::max(1, N) * sizeof(double)));
+            HIPCHK(hipMemset(b64, 0, std::max(1, N) * sizeof(double)));
+            op.N = N; op.col = pcol; op.valf = pvalf; op.vald = pvald; op.mdiagf = pmdf; op.mdiagd = pmdd;
+            int werr = 0;
+            pw = pcg_work_create(N, &werr);
+            if (!pw) return fail(-40, "field solver workspace allocation failed");
+            HIPCHK(dalloc(&d_res, 1));
+            HIPCHK(hipMemset(d_res, 0, sizeof(PcgResult)));
+        }
+        // particles
+        int capmax = 0;
+        for (int s = 0; s < nsp; ++s) {
+            int cap = std::max(S.sp[s].cap, 1);
+            capmax = std::max(capmax, cap);
+            if (int e2 = alloc_species(s, cap)) return e2;
+        }
+        HIPCHK(dalloc(&stage, (size_t)6 * capmax));
+        HIPCHK(dalloc(&prank, capmax));
+        HIPCHK(dalloc(&pcnt, (size_t)G * ROUTE_SLOTS));
+        HIPCHK(dalloc(&pid_off, (size_t)G * ROUTE_SLOTS));
+        HIPCHK(dalloc(&total, G)); HIPCHK(dalloc(&alive, G)); HIPCHK(dalloc(&ptotal, G));
+        HIPCHK(dalloc(&pstart, G + 1)); HIPCHK(dalloc(&cellp, G));
+        HIPCHK(dalloc(&scan_tmp, (G + 1023) / 1024 + 2));
+        HIPCHK(dalloc(&next_id, MAXSP));
+        HIPCHK(dalloc(&mccr, (size_t)G * S.TnRct));
+        HIPCHK(hipMemset(mccr, 0, (size_t)G * S.TnRct * sizeof(unsigned long long)));
+        HIPCHK(dalloc(&condcharge, nsp * S.CondNUMR));
+        HIPCHK(hipMemset(condcharge, 0, nsp * S.CondNUMR * sizeof(int)));
+        HIPCHK(dalloc(&d_err, 1));
+        HIPCHK(hipMemset(d_err, 0, sizeof(int)));
+        HIPCHK(dalloc(&sum_den, (size_t)nsp * G)); HIPCHK(dalloc(&sum_phi, G));
+        HIPCHK(dalloc(&sum_ex, G)); HIPCHK(dalloc(&sum_ey, G));
+        HIPCHK(hipMemset(sum_den, 0, (size_t)nsp * G * sizeof(double)));
+        HIPCHK(hipMemset(sum_phi, 0, G * sizeof(double)));
+        HIPCHK(hipMemset(sum_ex, 0, G * sizeof(double)));
+        HIPCHK(hipMemset(sum_ey, 0, G * sizeof(double)));
+        {
+            uint32_t nid[MAXSP] = {0};
+            for (int s = 0; s < nsp; ++s) nid[s] = (uint32_t)S.sp[s].n;
+            HIPCHK(hipMemcpy(next_id, nid, sizeof nid, hipMemcpyHostToDevice));
+        }
+        for (int s = 0; s < nsp; ++s) {
+            SpeciesInit<R> &si = S.sp[s];
+            if (si.n > 0) {
+                HIPCHK(hipMemcpy(cur[s].x, si.x.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(cur[s].y, si.y.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(cur[s].vx, si.vx.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(cur[s].vy, si.vy.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(cur[s].vz, si.vz.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(cur[s].id, si.id.data(), si.n * sizeof(uint32_t), hipMemcpyHostToDevice));
+            }
+            if (!si.cell_start.empty())
+                HIPCHK(hipMemcpy(cur[s].cell_start, si.cell_start.data(), (G + 1) * sizeof(int), hipMemcpyHostToDevice));
+            // free the host copy of the load; the device holds the state now
+            std::vector<R>().swap(si.x); std::vector<R>().swap(si.y); std::vector<R>().swap(si.vx);
+            std::vector<R>().swap(si.vy); std::vector<R>().swap(si.vz); std::vector<uint32_t>().swap(si.id);
+        }
+        // constants
+        kc.ngx = S.ngx; kc.ngy = S.ngy; kc.ncx = S.ncx; kc.ncy = S.ncy; kc.Gsize = G; kc.Csize = S.Csize;
+        kc.nsp = nsp; kc.CondNUMR = S.CondNUMR; kc.TnRct = S.TnRct; kc.nvel = S.nvel; kc.MCCn = S.MCCn;
+        kc.nlogx = N_LOGX; kc.dt_dx = S.dt_dx; kc.dt_dy = S.dt_dy; kc.idLOGX = S.idLOGX;
+        for (int s = 0; s < MAXSP; ++s) { kc.Ascale[s] = S.Ascale[s]; kc.Escale[s] = S.Escale[s]; }
+        for (int r = 0; r < 3; ++r) kc.sigv[r] = S.sigv[r];
+        for (int r = 0; r < 16; ++r) kc.Th_e[r] = S.Th_e[r];
+        kc.seed = (uint64_t)d.rng_seed;
+        kc.Ex = Ex; kc.Ey = Ey; kc.P1 = P1; kc.P2 = P2; kc.Pion = Pion; kc.BackVel1 = BackVel1;
+        kc.vsave = vsave; kc.xe = xe; kc.cx = cx; kc.StructIdx = StructIdx; kc.route = route;
+        nc.Gsize = G; nc.ngx = S.ngx; nc.ngy = S.ngy; nc.nsp = nsp; nc.CondNUMR = S.CondNUMR;
+        nc.hdx = S.hdx; nc.hdy = S.hdy; nc.idx = S.idx; nc.idy = S.idy; nc.idxy = S.idxy;
+        nc.hxy = (R)(0.5 * (double)S.dx / (double)S.dy);
+        nc.hyx = (R)(0.5 * (double)S.dy / (double)S.dx);
+        for (int s = 0; s < MAXSP; ++s) nc.q_density[s] = S.q_density[s];
+        nc.Boundary = Boundary; nc.Face = Face; nc.CondID = CondID; nc.A_idx = A_idx; nc.Area = Area; nc.eps = eps;
+        if (S.CondNUMR > MAX_SRC || (int)d.src.size() > MAX_SRC) return fail(-41, "too many conductors or sources");
+        vc.ncond = S.CondNUMR;
+        vc.nsrc = (int)d.src.size();
+        for (int i = 0; i < S.CondNUMR; ++i) vc.flag[i] = S.Efield_Flag[i];
+        const double PI = 3.14159265358979323846;
+        for (int j = 0; j < vc.nsrc; ++j) {
+            vc.mid[j] = d.src[j].mid;
+            vc.dc[j] = (double)(R)d.src[j].dc;
+            vc.ac[j] = (double)(R)d.src[j].ac;
+            vc.w[j] = (double)(R)(2 * PI * (double)(R)d.src[j].freq);
+            vc.ph[j] = (double)(R)(PI / 180 * (double)(R)d.src[j].phase);
+        }
+        // the initial deposit (src/cuda_main.cu:28)
+        if (int e3 = deposit()) return e3;
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+
+    void swap_species(int s) { std::swap(cur[s], nxt[s]); }
+
+    // ------------------------------------------------------------ stages
+    int advance_time() override {
+        ++tstep;
+        t = (double)tstep * (double)S.dt;
+        return 0;
+    }
+    int field_solve() override {
+        const int N = S.A_size;
+        launch_rhs_to_fp64<R>(N, brhs, b64, st);
+        const double tol = S.deck.pcg_tol > 0 ? S.deck.pcg_tol : 1e-10;
+        if (precision == PIC_FP64_PLAIN) pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st);
+        else pcg_solve_refined(op, b64, x64, tol, inner_tol, pcg_maxit, pw, d_res, st);
+        HIPCHK(hipGetLastError());
+        launch_x_to_phi<R>(G, A_idx, x64, phi, st);
+        return 0;
+    }
+    int efield() override {
+        launch_efield<R>(nc, vc, t, lap, V, phi, tot, den, Sigma, Ex, Ey, st);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int move() override {
+        for (int s = 0; s < nsp; ++s) launch_push<R>(kc, cur[s], s, tag[s], st);
+        tag_valid = 1;
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int sort_boundary() override {
+        if (!tag_valid) return fail(-42, "SortBounndary_cuda called without a preceding Move_cuda");
+        for (int s = 0; s < nsp; ++s) {
+            launch_sort_count<R>(kc, cur[s], s, tag[s], key[s], cnt[s], condcharge, sigma_sp, st);
+            launch_totals(G, inv_start, inv_ent, cnt[s], nullptr, total, alive, nullptr, st);
+            launch_scan(total, nxt[s].cell_start, G, scan_tmp, st);
+            launch_assign_bases(G, inv_start, inv_ent, cnt[s], nullptr, nxt[s].cell_start, alive, nullptr, base[s],
+                                nullptr, nullptr, st);
+            launch_sort_scatter<R>(kc, cur[s], nxt[s], tag[s], key[s], base[s], nxt[s].cap, d_err, st);
+            swap_species(s);
+        }
+        tag_valid = 0;
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int mcc() override {
+        const uint32_t step = (uint32_t)tstep;
+        // electrons first in the event order is irrelevant: each particle owns its stream
+        launch_mcc_collide<R>(kc, cur[1], 1, step, mccr, cellp, prank, stage, st);   // ions (no products)
+        launch_mcc_collide<R>(kc, cur[0], 0, step, mccr, pcnt, prank, stage, st);    // electrons
+        launch_scan(pcnt, pstart, G, scan_tmp, st);
+        for (int s = 0; s < nsp; ++s) {
+            launch_cell_counts(G, cur[s].cell_start, pcnt, total, st);
+            launch_scan(total, nxt[s].cell_start, G, scan_tmp, st);
+            launch_mcc_insert<R>(kc, cur[s], nxt[s], nxt[s].cell_start, pstart, cur[0], prank, stage, s == 0 ? 0 : 3,
+                                 next_id + s, nxt[s].cap, d_err, st);
+        }
+        for (int s = 0; s < nsp; ++s) swap_species(s);
+        launch_bump_ids(next_id, pstart + G, nsp, st);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int deposit() override {
+        launch_deposit<R>(kc, nc, cur, part, den, sigma_sp, Sigma, Source, brhs, st);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int diagnostic() override {
+        launch_accumulate<R>(G, nsp, den, tot, Ex, Ey, sum_den, sum_phi, sum_ex, sum_ey, st);
+        ++n_accum;
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int particle_stage_fused() {
+        const uint32_t step = (uint32_t)tstep;
+        launch_fused_count<R>(kc, cur[0], 0, step, cnt[0], pcnt, mccr, condcharge, sigma_sp, st);
+        launch_fused_count<R>(kc, cur[1], 1, step, cnt[1], nullptr, mccr, condcharge, sigma_sp, st);
+        // electrons: survivors + ionisation electrons; global product order
+        launch_totals(G, inv_start, inv_ent, cnt[0], pcnt, total, alive, ptotal, st);
+        launch_scan(total, nxt[0].cell_start, G, scan_tmp, st);
+        launch_scan(ptotal, pstart, G, scan_tmp, st);
+        launch_assign_bases(G, inv_start, inv_ent, cnt[0], pcnt, nxt[0].cell_start, alive, pstart, base[0], pbase[0],
+                            pid_off, st);
+        // ions: survivors + ionisation ions (same per-entry product counts)
+        launch_totals(G, inv_start, inv_ent, cnt[1], pcnt, total, alive, nullptr, st);
+        launch_scan(total, nxt[1].cell_start, G, scan_tmp, st);
+        launch_assign_bases(G, inv_start, inv_ent, cnt[1], pcnt, nxt[1].cell_start, alive, pstart, base[1], pbase[1],
+                            cellp == nullptr ? pid_off : pid_off, st);
+        // pid_off recomputed identically (same product counts and order)
+        launch_fused_scatter<R>(kc, cur[0], nxt[0], nxt[1], 0, step, base[0], pbase[0], pbase[1], pid_off, next_id,
+                                nxt[0].cap, nxt[1].cap, d_err, st);
+        launch_fused_scatter<R>(kc, cur[1], nxt[1], nxt[1], 1, step, base[1], pbase[1], pbase[1], pid_off, next_id,
+                                nxt[1].cap, nxt[1].cap, d_err, st);
+        launch_bump_ids(next_id, pstart + G, nsp, st);
+        swap_species(0);
+        swap_species(1);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int step_fused(int n) override {
+        for (int i = 0; i < n; ++i) {
+            advance_time();
+            if (int e = field_solve()) return e;
+            if (int e = efield()) return e;
+            if (int e = particle_stage_fused()) return e;
+            if (int e = deposit()) return e;
+            if (int e = diagnostic()) return e;
+        }
+        return 0;
+    }
+    int sync() override {
+        HIPCHK(hipStreamSynchronize(st));
+        int e = 0;
+        HIPCHK(hipMemcpy(&e, d_err, sizeof(int), hipMemcpyDeviceToHost));
+        if (e) return fail(-30, "particle capacity exceeded (Max_np too small for this case)");
+        PcgResult r{};
+        HIPCHK(hipMemcpy(&r, d_res, sizeof r, hipMemcpyDeviceToHost));
+        if (r.status == 1) return fail(-31, "field solve hit the iteration limit (rel. residual %g)", r.rel_res);
+        if (std::isnan(r.rel_res)) return fail(-32, "Field solver Error! (NaN residual)");
+        return 0;
+    }
+    long long np_of(int s) {
+        int v = 0;
+        hipMemcpy(&v, cur[s].cell_start + G, sizeof(int), hipMemcpyDeviceToHost);
+        return v;
+    }
+    int info(pic_info *o) override {
+        HIPCHK(hipStreamSynchronize(st));
+        o->ngx = S.ngx; o->ngy = S.ngy; o->nsp = nsp; o->A_size = S.A_size; o->CondNUMR = S.CondNUMR;
+        o->TnRct = S.TnRct; o->MCCn = S.MCCn; o->nvel = S.nvel;
+        o->dt = (double)S.dt; o->dx = (double)S.dx; o->dy = (double)S.dy; o->t = t; o->tstep = tstep;
+        o->precision = precision;
+        for (int s = 0; s < 5; ++s) {
+            o->np[s] = s < nsp ? np_of(s) : 0;
+            o->cap[s] = s < nsp ? cur[s].cap : 0;
+        }
+        return 0;
+    }
+    int get_particles(int s, void *x, void *y, void *vx, void *vy, void *vz, uint32_t *id, int32_t *cs) override {
+        if (s < 0 || s >= nsp) return fail(-3, "bad species %d", s);
+        HIPCHK(hipStreamSynchronize(st));
+        long long n = np_of(s);
+        const size_t b = (size_t)n * sizeof(R);
+        if (x) HIPCHK(hipMemcpy(x, cur[s].x, b, hipMemcpyDeviceToHost));
+        if (y) HIPCHK(hipMemcpy(y, cur[s].y, b, hipMemcpyDeviceToHost));
+        if (vx) HIPCHK(hipMemcpy(vx, cur[s].vx, b, hipMemcpyDeviceToHost));
+        if (vy) HIPCHK(hipMemcpy(vy, cur[s].vy, b, hipMemcpyDeviceToHost));
+        if (vz) HIPCHK(hipMemcpy(vz, cur[s].vz, b, hipMemcpyDeviceToHost));
+        if (id) HIPCHK(hipMemcpy(id, cur[s].id, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (cs) HIPCHK(hipMemcpy(cs, cur[s].cell_start, (G + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    int set_particles(int s, long long n, const void *x, const void *y, const void *vx, const void *vy, const void *vz,
+                      const uint32_t *id, const int32_t *cs, uint32_t nid) override {
+        if (s < 0 || s >= nsp) return fail(-3, "bad species %d", s);
+        if (n < 0 || n > cur[s].cap) return fail(-30, "particle count %lld exceeds capacity %d", n, cur[s].cap);
+        if (cs[0] != 0 || cs[G] != n) return fail(-3, "cell_start must run from 0 to n");
+        for (int c = 0; c < G; ++c)
+            if (cs[c + 1] < cs[c]) return fail(-3, "cell_start must be non-decreasing");
+        HIPCHK(hipStreamSynchronize(st));
+        const size_t b = (size_t)n * sizeof(R);
+        HIPCHK(hipMemcpy(cur[s].x, x, b, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cur[s].y, y, b, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cur[s].vx, vx, b, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cur[s].vy, vy, b, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cur[s].vz, vz, b, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cur[s].id, id, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cur[s].cell_start, cs, (G + 1) * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(next_id + s, &nid, sizeof nid, hipMemcpyHostToDevice));
+        tag_valid = 0;
+        return 0;
+    }
+
+    // ------------------------------------------------------------ named arrays
+    struct Arr {
+        void *dev;
+        const void *host;
+        size_t count;
+        int elsize;
+        int is_float;
+        bool writable;
+    };
+    bool lookup(const std::string &n, Arr &a) {
+        const int rs = (int)sizeof(R);
+        auto D = [&](void *p, size_t c, int es, int fl, bool w = true) { a = Arr{p, nullptr, c, es, fl, w}; return true; };
+        auto H = [&](const void *p, size_t c, int es, int fl) { a = Arr{nullptr, p, c, es, fl, false}; return true; };
+        if (n == "Ex") return D(Ex, G, rs, 1);
+        if (n == "Ey") return D(Ey, G, rs, 1);
+        if (n == "phi") return D(phi, G, rs, 1);
+        if (n == "TotPot") return D(tot, G, rs, 1);
+        if (n == "Sigma") return D(Sigma, G, rs, 1);
+        if (n == "Source") return D(Source, G, rs, 1);
+        if (n == "den") return D(den, (size_t)nsp * G, rs, 1);
+        if (n == "sigma_sp") return D(sigma_sp, (size_t)nsp * G, rs, 1);
+        if (n == "b") return D(brhs, S.A_size, rs, 1);
+        if (n == "xsol") return D(x64, S.A_size, 8, 1);
+        if (n == "V_cond") return D(V, S.CondNUMR, rs, 1);
+        if (n == "mccr") return D(mccr, (size_t)G * S.TnRct, 8, 0);
+        if (n == "CondCharge") return D(condcharge, (size_t)nsp * S.CondNUMR, 4, 0);
+        if (n == "sum_den") return D(sum_den, (size_t)nsp * G, 8, 1);
+        if (n == "sum_phi") return D(sum_phi, G, 8, 1);
+        if (n == "sum_ex") return D(sum_ex, G, 8, 1);
+        if (n == "sum_ey") return D(sum_ey, G, 8, 1);
+        if (n == "next_id") return D(next_id, nsp, 4, 0);
+        if (n == "Boundary") return H(S.Boundary.data(), G, 4, 0);
+        if (n == "CondID") return H(S.CondID.data(), G, 4, 0);
+        if (n == "Face") return H(S.Face.data(), G, 4, 0);
+        if (n == "DensRegion") return H(S.DensRegion.data(), G, 4, 0);
+        if (n == "Area") return H(S.Area.data(), G, rs, 1);
+        if (n == "Temp") return H(S.Temp.data(), G, rs, 1);
+        if (n == "PlasmaRegion") return H(S.PlasmaRegion.data(), S.Csize, 4, 0);
+        if (n == "eps_r") return H(S.eps_r.data(), S.Csize, rs, 1);
+        if (n == "eps") return H(S.eps.data(), S.Csize, rs, 1);
+        if (n == "StructIdx") return H(S.StructIdx.data(), S.StructIdx.size(), 4, 0);
+        if (n == "A_idx") return H(S.A_idx.data(), G, 4, 0);
+        if (n == "Ai") return H(S.Ai.data(), S.Ai.size(), 4, 0);
+        if (n == "Aj") return H(S.Aj.data(), S.Aj.size(), 4, 0);
+        if (n == "Aval") return H(S.Aval.data(), S.Aval.size(), rs, 1);
+        if (n == "Atemp") return H(S.Atemp.data(), S.Atemp.size(), rs, 1);
+        if (n == "Mdiag") return H(S.Mdiag.data(), S.Mdiag.size(), rs, 1);
+        if (n == "cond_b") return H(S.cond_b.data(), S.cond_b.size(), rs, 1);
+        if (n == "temp_b") return H(S.temp_b.data(), S.temp_b.size(), rs, 1);
+        if (n == "lap_phi") return H(S.lap_phi.data(), S.lap_phi.size(), rs, 1);
+        if (n == "BackDen1") return H(S.BackDen1.data(), G, rs, 1);
+        if (n == "BackVel1") return H(S.BackVel1.data(), G, rs, 1);
+        if (n == "FluidN") return H(S.FluidN.data(), S.FluidN.size(), rs, 1);
+        if (n == "xe") return H(S.xe.data(), S.xe.size(), rs, 1);
+        if (n == "xee") return H(S.xee.data(), S.xee.size(), rs, 1);
+        if (n == "cx") return H(S.cx.data(), S.cx.size(), rs, 1);
+        if (n == "sigv") return H(S.sigv, 3, rs, 1);
+        if (n == "P1") return H(S.P1.data(), G, rs, 1);
+        if (n == "P2") return H(S.P2.data(), G, rs, 1);
+        if (n == "Pion") return H(S.Pion.data(), G, rs, 1);
+        if (n == "vsave") return H(S.vsave.data(), S.vsave.size(), rs, 1);
+        return false;
+    }
+    int array_info(const char *name, size_t *count, int *elsize, int *is_float) override {
+        Arr a;
+        if (!lookup(name, a)) return fail(-4, "unknown array '%s'", name);
+        *count = a.count; *elsize = a.elsize; *is_float = a.is_float;
+        return 0;
+    }
+    int get_array(const char *name, void *host, size_t bytes) override {
+        Arr a;
+        if (!lookup(name, a)) return fail(-4, "unknown array '%s'", name);
+        const size_t need = a.count * a.elsize;
+        if (bytes < need) return fail(-5, "buffer too small for '%s' (%zu < %zu)", name, bytes, need);
+        if (a.dev) {
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipMemcpy(host, a.dev, need, hipMemcpyDeviceToHost));
+        } else if (need) {
+            std::memcpy(host, a.host, need);
+        }
+        return 0;
+    }
+    int set_array(const char *name, const void *host, size_t bytes) override {
+        Arr a;
+        if (!lookup(name, a)) return fail(-4, "unknown array '%s'", name);
+        if (!a.writable) return fail(-6, "array '%s' is read-only", name);
+        const size_t need = a.count * a.elsize;
+        if (bytes != need) return fail(-5, "size mismatch for '%s' (%zu != %zu)", name, bytes, need);
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpy(a.dev, host, need, hipMemcpyHostToDevice));
+        return 0;
+    }
+    int set_step(long long s) override {
+        tstep = s;
+        t = (double)s * (double)S.dt;
+        return 0;
+    }
+    int last_solve(int *iters, int *outer, double *rel) override {
+        PcgResult r{};
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpy(&r, d_res, sizeof r, hipMemcpyDeviceToHost));
+        if (iters) *iters = r.iters;
+        if (outer) *outer = r.outer;
+        if (rel) *rel = r.rel_res;
+        return 0;
+    }
+};
+
+}  // namespace pic
+
+namespace pic {
+// Host-only view of a case setup (no device needed): lets the CPU test suite
+// check deck parsing, geometry, operator, tables and the initial load.
+template <typename R>
+static bool setup_lookup(Setup<R> &S, const std::string &n, const void *&p, size_t &count, int &es, int &fl) {
+    const int rs = (int)sizeof(R);
+    auto H = [&](const void *q, size_t c, int e, int f) { p = q; count = c; es = e; fl = f; return true; };
+    const size_t G = (size_t)S.Gsize;
+    if (n == "Boundary") return H(S.Boundary.data(), G, 4, 0);
+    if (n == "CondID") return H(S.CondID.data(), G, 4, 0);
+    if (n == "Face") return H(S.Face.data(), G, 4, 0);
+    if (n == "DensRegion") return H(S.DensRegion.data(), G, 4, 0);
+    if (n == "Area") return H(S.Area.data(), G, rs, 1);
+    if (n == "Temp") return H(S.Temp.data(), G, rs, 1);
+    if (n == "PlasmaRegion") return H(S.PlasmaRegion.data(), S.PlasmaRegion.size(), 4, 0);
+    if (n == "eps_r") return H(S.eps_r.data(), S.eps_r.size(), rs, 1);
+    if (n == "eps") return H(S.eps.data(), S.eps.size(), rs, 1);
+    if (n == "StructIdx") return H(S.StructIdx.data(), S.StructIdx.size(), 4, 0);
+    if (n == "A_idx") return H(S.A_idx.data(), G, 4, 0);
+    if (n == "Ai") return H(S.Ai.data(), S.Ai.size(), 4, 0);
+    if (n == "Aj") return H(S.Aj.data(), S.Aj.size(), 4, 0);
+    if (n == "Aval") return H(S.Aval.data(), S.Aval.size(), rs, 1);
+    if (n == "Atemp") return H(S.Atemp.data(), S.Atemp.size(), rs, 1);
+    if (n == "Mdiag") return H(S.Mdiag.data(), S.Mdiag.size(), rs, 1);
+    if (n == "cond_b") return H(S.cond_b.data(), S.cond_b.size(), rs, 1);
+    if (n == "temp_b") return H(S.temp_b.data(), S.temp_b.size(), rs, 1);
+    if (n == "lap_phi") return H(S.lap_phi.data(), S.lap_phi.size(), rs, 1);
+    if (n == "BackDen1") return H(S.BackDen1.data(), S.BackDen1.size(), rs, 1);
+    if (n == "BackVel1") return H(S.BackVel1.data(), S.BackVel1.size(), rs, 1);
+    if (n == "FluidN") return H(S.FluidN.data(), S.FluidN.size(), rs, 1);
+    if (n == "xe") return H(S.xe.data(), S.xe.size(), rs, 1);
+    if (n == "xee") return H(S.xee.data(), S.xee.size(), rs, 1);
+    if (n == "cx") return H(S.cx.data(), S.cx.size(), rs, 1);
+    if (n == "sigv") return H(S.sigv, 3, rs, 1);
+    if (n == "P1") return H(S.P1.data(), S.P1.size(), rs, 1);
+    if (n == "P2") return H(S.P2.data(), S.P2.size(), rs, 1);
+    if (n == "Pion") return H(S.Pion.data(), S.Pion.size(), rs, 1);
+    if (n == "vsave") return H(S.vsave.data(), S.vsave.size(), rs, 1);
+    if (n.size() > 4 && n.compare(0, 2, "sp") == 0 && n[3] == '.') {
+        int s = n[2] - '0';
+        if (s < 0 || s >= S.nsp) return false;
+        SpeciesInit<R> &sp = S.sp[s];
+        std::string f = n.substr(4);
+        if (f == "x") return H(sp.x.data(), sp.n, rs, 1);
+        if (f == "y") return H(sp.y.data(), sp.n, rs, 1);
+        if (f == "vx") return H(sp.vx.data(), sp.n, rs, 1);
+        if (f == "vy") return H(sp.vy.data(), sp.n, rs, 1);
+        if (f == "vz") return H(sp.vz.data(), sp.n, rs, 1);
+        if (f == "id") return H(sp.id.data(), sp.n, 4, 0);
+        if (f == "cell_start") return H(sp.cell_start.data(), sp.cell_start.size(), 4, 0);
+    }
+    return false;
+}
+
+template <typename R>
+static bool setup_scalar(Setup<R> &S, const std::string &n, double &v) {
+    if (n == "dt") { v = (double)S.dt; return true; }
+    if (n == "dx") { v = (double)S.dx; return true; }
+    if (n == "dy") { v = (double)S.dy; return true; }
+    if (n == "A_size") { v = S.A_size; return true; }
+    if (n == "CondNUMR") { v = S.CondNUMR; return true; }
+    if (n == "MCCn") { v = S.MCCn; return true; }
+    if (n == "dt_mcc") { v = (double)S.dt_mcc; return true; }
+    if (n == "nvel") { v = S.nvel; return true; }
+    if (n == "nsp") { v = S.nsp; return true; }
+    if (n.compare(0, 2, "np") == 0 && n.size() == 3) { v = S.sp[n[2] - '0'].n; return true; }
+    if (n.compare(0, 6, "Ascale") == 0 && n.size() == 7) { v = (double)S.Ascale[n[6] - '0']; return true; }
+    if (n.compare(0, 6, "Escale") == 0 && n.size() == 7) { v = (double)S.Escale[n[6] - '0']; return true; }
+    if (n.compare(0, 3, "vti") == 0 && n.size() == 4) { v = (double)S.vti[n[3] - '0']; return true; }
+    if (n.compare(0, 9, "q_density") == 0 && n.size() == 10) { v = (double)S.q_density[n[9] - '0']; return true; }
+    return false;
+}
+
+struct HostSetupBase {
+    virtual ~HostSetupBase() {}
+    virtual bool lookup(const std::string &, const void *&, size_t &, int &, int &) = 0;
+    virtual bool scalar(const std::string &, double &) = 0;
+};
+template <typename R>
+struct HostSetup : HostSetupBase {
+    Setup<R> S;
+    bool lookup(const std::string &n, const void *&p, size_t &c, int &e, int &f) override {
+        return setup_lookup<R>(S, n, p, c, e, f);
+    }
+    bool scalar(const std::string &n, double &v) override { return setup_scalar<R>(S, n, v); }
+};
+}  // namespace pic
+
+// ================================================================== C ABI
+using namespace pic;
+
+struct pic_ctx {
+    std::unique_ptr<CaseBase> c;
+};
+
+extern "C" {
+
+const char *pic_last_error(void) { return g_err.c_str(); }
+
+int pic_create(const char *deck_json, const pic_options *opt, pic_ctx **out) {
+    if (!deck_json || !out) return fail(-1, "null argument");
+    *out = nullptr;
+    pic_options o{};
+    o.load = 1;
+    if (opt) o = *opt;
+    Deck d;
+    std::string err;
+    if (int e = parse_deck(deck_json, d, err)) return fail(e, "%s", err.c_str());
+    std::unique_ptr<pic_ctx> h(new pic_ctx());
+    int e = 0;
+    try {
+        if (o.precision == PIC_FP64_PLAIN) {
+            auto *c = new Case<double>();
+            h->c.reset(c);
+            e = c->init(d, o);
+        } else if (o.precision == PIC_FP32_REFINED) {
+            auto *c = new Case<float>();
+            h->c.reset(c);
+            e = c->init(d, o);
+        } else {
+            return fail(-1, "unknown precision %d", o.precision);
+        }
+    } catch (const std::bad_alloc &) {
+        return fail(-7, "host out of memory during setup");
+    }
+    if (e) return e;
+    *out = h.release();
+    return 0;
+}
+
+int pic_create_from_file(const char *path, const pic_options *opt, pic_ctx **out) {
+    if (!path) return fail(-1, "null path");
+    std::ifstream f(path);
+    if (!f) return fail(-1, "cannot open deck '%s'", path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::string s = ss.str();
+    return pic_create(s.c_str(), opt, out);
+}
+
+int pic_destroy(pic_ctx *ctx) {
+    delete ctx;
+    return 0;
+}
+
+#define CTX_OR_FAIL(ctx) \
+    if (!(ctx) || !(ctx)->c) return fail(-1, "null context")
+
+int pic_info_get(pic_ctx *ctx, pic_info *info) {
+    CTX_OR_FAIL(ctx);
+    if (!info) return fail(-1, "null info");
+    return ctx->c->info(info);
+}
+int pic_sync(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->sync(); }
+int pic_advance_time(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->advance_time(); }
+int pic_field_solve(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->field_solve(); }
+int pic_efield(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->efield(); }
+int pic_move(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->move(); }
+int pic_sort_boundary(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->sort_boundary(); }
+int pic_mcc(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->mcc(); }
+int pic_deposit(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->deposit(); }
+int pic_diagnostic(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->diagnostic(); }
+
+int pic_run_staged(pic_ctx *ctx, int nsteps) {
+    CTX_OR_FAIL(ctx);
+    CaseBase *c = ctx->c.get();
+    for (int i = 0; i < nsteps; ++i) {
+        int e;
+        if ((e = c->advance_time()) || (e = c->field_solve()) || (e = c->efield()) || (e = c->move()) ||
+            (e = c->sort_boundary()) || (e = c->mcc()) || (e = c->deposit()) || (e = c->diagnostic()))
+            return e;
+    }
+    return 0;
+}
+
+int pic_step(pic_ctx *ctx, int nsteps) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->step_fused(nsteps);
+}
+
+int pic_get_particles(pic_ctx *ctx, int isp, void *x, void *y, void *vx, void *vy, void *vz, uint32_t *id,
+                      int32_t *cell_start) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->get_particles(isp, x, y, vx, vy, vz, id, cell_start);
+}
+int pic_set_particles(pic_ctx *ctx, int isp, long long n, const void *x, const void *y, const void *vx,
+                      const void *vy, const void *vz, const uint32_t *id, const int32_t *cell_start,
+                      uint32_t next_id) {
+    CTX_OR_FAIL(ctx);
+    if (!x || !y || !vx || !vy || !vz || !id || !cell_start) return fail(-1, "null particle buffer");
+    return ctx->c->set_particles(isp, n, x, y, vx, vy, vz, id, cell_start, next_id);
+}
+int pic_array_info(pic_ctx *ctx, const char *name, size_t *count, int *elsize, int *is_float) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->array_info(name, count, elsize, is_float);
+}
+int pic_get_array(pic_ctx *ctx, const char *name, void *host, size_t bytes) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->get_array(name, host, bytes);
+}
+int pic_set_array(pic_ctx *ctx, const char *name, const void *host, size_t bytes) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->set_array(name, host, bytes);
+}
+const char *pic_array_names(void) {
+    return "Ex Ey phi TotPot Sigma Source den sigma_sp b xsol V_cond mccr CondCharge sum_den sum_phi sum_ex sum_ey "
+           "next_id Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai Aj Aval "
+           "Atemp Mdiag cond_b temp_b lap_phi BackDen1 BackVel1 FluidN xe xee cx sigv P1 P2 Pion vsave";
+}
+int pic_set_step(pic_ctx *ctx, long long tstep) { CTX_OR_FAIL(ctx); return ctx->c->set_step(tstep); }
+int pic_last_field_solve(pic_ctx *ctx, int *iters, int *outer, double *rel_res) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->last_solve(iters, outer, rel_res);
+}
+
+// ------------------------------------------------------------------ host-only setup
+struct pic_setup_h {
+    std::unique_ptr<HostSetupBase> h;
+};
+
+int pic_setup_open(const char *deck_json, int precision, int load, pic_setup_h **out) {
+    if (!deck_json || !out) return fail(-1, "null argument");
+    *out = nullptr;
+    Deck d;
+    std::string err;
+    if (int e = parse_deck(deck_json, d, err)) return fail(e, "%s", err.c_str());
+    std::unique_ptr<pic_setup_h> h(new pic_setup_h());
+    int e;
+    if (precision == PIC_FP64_PLAIN) {
+        auto *s = new HostSetup<double>();
+        h->h.reset(s);
+        e = build_setup<double>(d, s->S, err, load != 0);
+    } else {
+        auto *s = new HostSetup<float>();
+        h->h.reset(s);
+        e = build_setup<float>(d, s->S, err, load != 0);
+    }
+    if (e) return fail(e, "%s", err.c_str());
+    *out = h.release();
+    return 0;
+}
+int pic_setup_array_info(pic_setup_h *h, const char *name, size_t *count, int *elsize, int *is_float) {
+    if (!h) return fail(-1, "null handle");
+    const void *p;
+    if (!h->h->lookup(name, p, *count, *elsize, *is_float)) return fail(-4, "unknown array '%s'", name);
+    return 0;
+}
+int pic_setup_get_array(pic_setup_h *h, const char *name, void *host, size_t bytes) {
+    if (!h) return fail(-1, "null handle");
+    const void *p;
+    size_t c;
+    int e, f;
+    if (!h->h->lookup(name, p, c, e, f)) return fail(-4, "unknown array '%s'", name);
+    if (bytes < c * e) return fail(-5, "buffer too small for '%s'", name);
+    if (c) std::memcpy(host, p, c * e);
+    return 0;
+}
+int pic_setup_scalar(pic_setup_h *h, const char *name, double *v) {
+    if (!h) return fail(-1, "null handle");
+    if (!h->h->scalar(name, *v)) return fail(-4, "unknown scalar '%s'", name);
+    return 0;
+}
+void pic_setup_close(pic_setup_h *h) { delete h; }
+
+// ------------------------------------------------------------------ shims
+static pic_ctx *g_default = nullptr;
+
+static void shim_check(int e, const char *where) {
+    if (e) {
+        fprintf(stderr, "\n%s: %s (status %d)\n", where, pic_last_error(), e);
+        exit(1);
+    }
+}
+static CaseBase *shim_case(const char *where) {
+    if (!g_default) {
+        fprintf(stderr, "\n%s: no case open (call pic_shim_open first)\n", where);
+        exit(1);
+    }
+    return g_default->c.get();
+}
+
+int pic_shim_open(const char *deck_path, int precision) {
+    pic_shim_close();
+    pic_options o{};
+    o.precision = precision;
+    o.load = 1;
+    return pic_create_from_file(deck_path, &o, &g_default);
+}
+void pic_shim_close(void) {
+    if (g_default) pic_destroy(g_default);
+    g_default = nullptr;
+}
+void info_Device(void) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    hipGetDevice(&dev);
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess)
+        printf(" Device %d: %s, %d CUs, %.1f GB\n", dev, p.name, p.multiProcessorCount, p.totalGlobalMem / 1e9);
+}
+// the setup stages run inside pic_shim_open; these keep the reference's call sequence valid
+void start_cuda(void) { shim_case("start_cuda"); }
+void Set_Device_Parameter(void) { shim_case("Set_Device_Parameter"); }
+void Set_Particle_cuda(void) { shim_case("Set_Particle_cuda"); }
+void Set_NullCollisionTime_cuda(void) { shim_case("Set_NullCollisionTime_cuda"); }
+void Set_SortBoundary_cuda(void) { shim_case("Set_SortBoundary_cuda"); }
+void Set_MatrixPCG_cuda(void) { shim_case("Set_MatrixPCG_cuda"); }
+void PCG_SOLVER_Laplace(void) { shim_case("PCG_SOLVER_Laplace"); }
+void PCG_SOLVER(void) { shim_check(shim_case("PCG_SOLVER")->field_solve(), "PCG_SOLVER"); }
+void Efield_cuda_Basic(void) { shim_check(shim_case("Efield_cuda_Basic")->efield(), "Efield_cuda_Basic"); }
+void Move_cuda(void) { shim_check(shim_case("Move_cuda")->move(), "Move_cuda"); }
+void SortBounndary_cuda(void) { shim_check(shim_case("SortBounndary_cuda")->sort_boundary(), "SortBounndary_cuda"); }
+void MCC_Ar_cuda(void) { shim_check(shim_case("MCC_Ar_cuda")->mcc(), "MCC_Ar_cuda"); }
+void Deposit_cuda(void) { shim_check(shim_case("Deposit_cuda")->deposit(), "Deposit_cuda"); }
+void Deposit_Basic(void) { shim_check(shim_case("Deposit_Basic")->deposit(), "Deposit_Basic"); }
+void main_cuda(void) {
+    CaseBase *c = shim_case("main_cuda");
+    long long nsteps = (long long)c->rf_cycles * c->dt_pic;
+    printf("-------------GPU START-------------\n");
+    for (long long i = 0; i < nsteps; ++i) {
+        shim_check(c->step_fused(1), "main_cuda");
+        if ((i + 1) % c->dt_pic == 0) shim_check(c->sync(), "main_cuda");
+    }
+    shim_check(c->sync(), "main_cuda");
+}
+
+}  // extern "C"

@@ -1,0 +1,848 @@
+// Particle and node kernels of the PIC-MCC step for gfx950.
+//
+// Particles of one species live in structure-of-arrays form, sorted by cell
+// (a cell is named by its lower-left node, as in the reference), with a CSR
+// offset array cell_start[Gsize+1].  Every particle kernel runs one 64-lane
+// wavefront per cell: the cell's four E nodes and its 9-entry route table are
+// wave-uniform, and ranks inside the cell come from ballots, so the new order
+// is fixed without atomics:
+//   new bucket(d) = stayers of d (old order)
+//                 ++ arrivals grouped by source cell ascending (old order)
+//                 ++ collision products in parent order.
+#include <hip/hip_runtime.h>
+
+#include "pic_device.h"
+#include "pic_kernels.h"
+#include "pic_setup.h"
+
+namespace pic {
+
+namespace {
+
+__device__ inline int wave_cell(int ncy, int ngy, int Csize, int &lane) {
+    lane = threadIdx.x & 63;
+    int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (w >= Csize) return -1;
+    int c = (w / ncy) * ngy + (w % ncy);
+    return __builtin_amdgcn_readfirstlane(c);
+}
+
+__device__ inline unsigned long long lanes_below(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+template <typename R>
+__device__ inline R wave_sum(R v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename R>
+struct CellE {
+    R ews, ewn, ees, een, yws, ywn, yes, yen;
+    __device__ void load(const R *Ex, const R *Ey, int c, int ngy) {
+        ews = Ex[c]; ewn = Ex[c + 1]; ees = Ex[c + ngy]; een = Ex[c + ngy + 1];
+        yws = Ey[c]; ywn = Ey[c + 1]; yes = Ey[c + ngy]; yen = Ey[c + ngy + 1];
+    }
+};
+
+}  // namespace
+
+// ============================================================ staged: Move
+template <typename R>
+__global__ __launch_bounds__(256) void k_push(DevConst<R> k, PView<R> p, int isp, uint8_t *tag) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    if (k0 == k1) return;
+    CellE<R> e;
+    e.load(k.Ex, k.Ey, c, k.ngy);
+    const R As = k.Ascale[isp];
+    for (int kk = k0 + lane; kk < k1; kk += 64) {
+        R x = p.x[kk], y = p.y[kk], vx = p.vx[kk], vy = p.vy[kk];
+        int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx, k.dt_dy,
+                                    x, y, vx, vy);
+        p.x[kk] = x; p.y[kk] = y; p.vx[kk] = vx; p.vy[kk] = vy;
+        tag[kk] = (uint8_t)code;
+    }
+}
+
+// side effects of a particle that leaves through a wall
+template <typename R>
+__device__ inline void wall_effects(const DevConst<R> &k, const Route &rt, int isp, int c, int code, R x, R y, R vx,
+                                    R vy, int *condcharge, R *sigma_sp) {
+    if (rt.dest == -2) {
+        atomicAdd(&condcharge[isp * k.CondNUMR + rt.info], 1);
+    } else if (rt.dest == -3) {
+        int mcid = code_mcid(code, k.ngy);
+        int sm = padded_index(c, mcid, k.ngy);
+        int pa, pb;
+        R a, b;
+        dielectric_split<R>(k.StructIdx, k.ngy, c, mcid, sm, x, y, vx, vy, pa, a, pb, b);
+        atomicAdd(&sigma_sp[(size_t)isp * k.Gsize + pa], a);
+        atomicAdd(&sigma_sp[(size_t)isp * k.Gsize + pb], b);
+    }
+}
+
+// ============================================================ staged: Sort count
+// key[k] = slot << 24 | rank within (cell, slot); -1 when the particle leaves
+template <typename R>
+__global__ __launch_bounds__(256) void k_sort_count(DevConst<R> k, PView<R> p, int isp, const uint8_t *tag, int *key,
+                                                    int *cnt, int *condcharge, R *sigma_sp) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    const unsigned long long lt = lanes_below(lane);
+    int run[ROUTE_SLOTS];
+#pragma unroll
+    for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] = 0;
+    for (int base = k0; base < k1; base += 64) {
+        int kk = base + lane;
+        int slot = -1;
+        if (kk < k1) {
+            int code = tag[kk];
+            Route rt = k.route[c * ROUTE_SLOTS + code];
+            if (rt.dest >= 0) slot = rt.info & 15;
+            else wall_effects<R>(k, rt, isp, c, code, p.x[kk], p.y[kk], p.vx[kk], p.vy[kk], condcharge, sigma_sp);
+        }
+        int rank = -1;
+#pragma unroll
+        for (int s = 0; s < ROUTE_SLOTS; ++s) {
+            unsigned long long m = __ballot(slot == s);
+            if (slot == s) rank = run[s] + __popcll(m & lt);
+            run[s] += __popcll(m);
+        }
+        if (kk < k1) key[kk] = slot < 0 ? -1 : ((slot << 24) | rank);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < ROUTE_SLOTS; ++s) cnt[c * ROUTE_SLOTS + s] = run[s];
+    }
+}
+
+// ============================================================ bases
+// Per destination cell d: new total, then the base offset of every
+// (source cell, slot) entry feeding d in canonical order.  prod_cnt may be
+// null (no products).  Products of d follow all survivors of d.
+__global__ void k_totals(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt, const int *prod_cnt,
+                         int *total, int *alive_total, int *prod_total) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= Gsize) return;
+    int a = 0, pr = 0;
+    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) {
+        int ent = inv_ent[e];
+        a += cnt[ent];
+        if (prod_cnt) pr += prod_cnt[ent];
+    }
+    total[d] = a + pr;
+    alive_total[d] = a;
+    if (prod_total) prod_total[d] = pr;
+}
+
+__global__ void k_assign_bases(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt, const int *prod_cnt,
+                               const int *start, const int *alive_total, const int *prod_start, int *base,
+                               int *pbase, int *pid_off) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= Gsize) return;
+    int run = start[d];
+    int prun = start[d] + alive_total[d];
+    int irun = prod_start ? prod_start[d] : 0;
+    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) {
+        int ent = inv_ent[e];
+        base[ent] = run;
+        run += cnt[ent];
+        if (prod_cnt) {
+            pbase[ent] = prun;
+            pid_off[ent] = irun;
+            prun += prod_cnt[ent];
+            irun += prod_cnt[ent];
+        }
+    }
+}
+
+// exclusive scan, three phases, 1024 elements per block
+__global__ __launch_bounds__(1024) void k_scan_blocks(const int *in, int *out, int n, int *block_sums) {
+    __shared__ int sh[1024];
+    int i = blockIdx.x * 1024 + threadIdx.x;
+    int v = i < n ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) out[i] = sh[threadIdx.x] - v;
+    if (threadIdx.x == 1023) block_sums[blockIdx.x] = sh[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(int *block_sums, int nblocks, int *total_out) {
+    __shared__ int sh[1024];
+    // serial over chunks of 1024 block sums (nblocks is small)
+    int carry = 0;
+    for (int base = 0; base < nblocks; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < nblocks ? block_sums[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblocks) block_sums[i] = carry + sh[threadIdx.x] - v;
+        int add = sh[1023];
+        __syncthreads();
+        carry += add;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(int *out, int n, const int *block_sums, const int *total, int *end_slot) {
+    int i = blockIdx.x * 1024 + threadIdx.x;
+    if (i < n) out[i] += block_sums[blockIdx.x];
+    if (i == 0 && end_slot) *end_slot = *total;
+}
+
+// ============================================================ staged: Sort scatter
+template <typename R>
+__global__ __launch_bounds__(256) void k_sort_scatter(DevConst<R> k, PView<R> p, PView<R> q, const uint8_t *tag,
+                                                      const int *key, const int *base, int cap, int *err) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    for (int kk = k0 + lane; kk < k1; kk += 64) {
+        int ky = key[kk];
+        if (ky < 0) continue;
+        int slot = ky >> 24, rank = ky & 0xffffff;
+        int code = tag[kk];
+        Route rt = k.route[c * ROUTE_SLOTS + code];
+        R x = p.x[kk], y = p.y[kk], vx = p.vx[kk], vy = p.vy[kk];
+        apply_reflect<R>(rt.info, x, y, vx, vy);
+        int pos = base[c * ROUTE_SLOTS + slot] + rank;
+        if (pos >= cap) { atomicOr(err, 1); continue; }
+        q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = p.vz[kk]; q.id[pos] = p.id[kk];
+    }
+}
+
+// ============================================================ staged: MCC
+// Collisions in place; ionisation products are staged per parent slot.
+// prank[k] = rank of the parent's product within its cell (-1: none).
+template <typename R>
+__global__ __launch_bounds__(256) void k_mcc_collide(DevConst<R> k, PView<R> p, int isp, uint32_t step,
+                                                     unsigned long long *mccr, int *pcnt, int *prank, R *stage) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    const unsigned long long lt = lanes_below(lane);
+    int run = 0;
+    const size_t cap = (size_t)p.cap;
+    for (int base = k0; base < k1; base += 64) {
+        int kk = base + lane;
+        bool prod = false;
+        if (kk < k1) {
+            R vx = p.vx[kk], vy = p.vy[kk], vz = p.vz[kk];
+            int ev;
+            if (isp == 0) {
+                R evx = 0, evy = 0, evz = 0, ivx = 0, ivy = 0, ivz = 0;
+                ev = mcc_ar_electron<R>(k, c, p.id[kk], step, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
+                if (ev == 3 || ev == 4) {
+                    prod = true;
+                    stage[kk] = evx; stage[cap + kk] = evy; stage[2 * cap + kk] = evz;
+                    stage[3 * cap + kk] = ivx; stage[4 * cap + kk] = ivy; stage[5 * cap + kk] = ivz;
+                }
+            } else {
+                ev = mcc_ar_ion<R>(k, c, p.id[kk], step, vx, vy, vz);
+            }
+            if (ev >= 0) {
+                atomicAdd(&mccr[(size_t)c * k.TnRct + ev], 1ull);
+                p.vx[kk] = vx; p.vy[kk] = vy; p.vz[kk] = vz;
+            }
+        }
+        unsigned long long m = __ballot(prod);
+        if (kk < k1) prank[kk] = prod ? run + __popcll(m & lt) : -1;
+        run += __popcll(m);
+    }
+    if (lane == 0) pcnt[c] = run;
+}
+
+// new bucket d = old bucket d ++ products of d; ids of products are
+// next_id + global product rank
+template <typename R>
+__global__ __launch_bounds__(256) void k_mcc_insert(DevConst<R> k, PView<R> src, PView<R> dst, const int *new_start,
+                                                    const int *prod_start, const int *prod_parent_start,
+                                                    const PView<R> parent, const int *prank, const R *stage,
+                                                    int stage_off, const uint32_t *next_id, int cap, int *err) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = src.cell_start[c], k1 = src.cell_start[c + 1];
+    int ns = new_start[c];
+    for (int kk = k0 + lane; kk < k1; kk += 64) {
+        int pos = ns + (kk - k0);
+        if (pos >= cap) { atomicOr(err, 1); continue; }
+        dst.x[pos] = src.x[kk]; dst.y[pos] = src.y[kk];
+        dst.vx[pos] = src.vx[kk]; dst.vy[pos] = src.vy[kk]; dst.vz[pos] = src.vz[kk]; dst.id[pos] = src.id[kk];
+    }
+    if (!prank) return;
+    int m = k1 - k0;
+    int q0 = parent.cell_start[c], q1 = parent.cell_start[c + 1];
+    const size_t pcap = (size_t)parent.cap;
+    for (int kk = q0 + lane; kk < q1; kk += 64) {
+        int r = prank[kk];
+        if (r < 0) continue;
+        int pos = ns + m + r;
+        if (pos >= cap) { atomicOr(err, 1); continue; }
+        dst.x[pos] = parent.x[kk]; dst.y[pos] = parent.y[kk];
+        dst.vx[pos] = stage[(size_t)stage_off * pcap + kk];
+        dst.vy[pos] = stage[(size_t)(stage_off + 1) * pcap + kk];
+        dst.vz[pos] = stage[(size_t)(stage_off + 2) * pcap + kk];
+        dst.id[pos] = *next_id + (uint32_t)(prod_start[c] + r);
+    }
+    (void)prod_parent_start;
+}
+
+// ============================================================ fused step
+// Pass 1: push + walls + collision, counting survivors and products per
+// (cell, slot).  Side effects (wall counters, surface charge, event counts)
+// happen here only.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, uint32_t step, int *cnt,
+                                                     int *pcnt, unsigned long long *mccr, int *condcharge,
+                                                     R *sigma_sp) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    CellE<R> e;
+    e.load(k.Ex, k.Ey, c, k.ngy);
+    const R As = k.Ascale[isp];
+    int run[ROUTE_SLOTS], prun[ROUTE_SLOTS];
+#pragma unroll
+    for (int s = 0; s < ROUTE_SLOTS; ++s) { run[s] = 0; prun[s] = 0; }
+    for (int base = k0; base < k1; base += 64) {
+        int kk = base + lane;
+        int slot = -1;
+        bool prod = false;
+        if (kk < k1) {
+            R x = p.x[kk], y = p.y[kk], vx = p.vx[kk], vy = p.vy[kk], vz = p.vz[kk];
+            int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx,
+                                        k.dt_dy, x, y, vx, vy);
+            Route rt = k.route[c * ROUTE_SLOTS + code];
+            if (rt.dest >= 0) {
+                apply_reflect<R>(rt.info, x, y, vx, vy);
+                slot = rt.info & 15;
+                int ev;
+                if (isp == 0) {
+                    R evx, evy, evz, ivx, ivy, ivz;
+                    ev = mcc_ar_electron<R>(k, rt.dest, p.id[kk], step, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
+                    prod = (ev == 3 || ev == 4);
+                } else {
+                    ev = mcc_ar_ion<R>(k, rt.dest, p.id[kk], step, vx, vy, vz);
+                }
+                if (ev >= 0) atomicAdd(&mccr[(size_t)rt.dest * k.TnRct + ev], 1ull);
+            } else {
+                wall_effects<R>(k, rt, isp, c, code, x, y, vx, vy, condcharge, sigma_sp);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < ROUTE_SLOTS; ++s) {
+            run[s] += __popcll(__ballot(slot == s));
+            prun[s] += __popcll(__ballot(slot == s && prod));
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < ROUTE_SLOTS; ++s) {
+            cnt[c * ROUTE_SLOTS + s] = run[s];
+            if (pcnt) pcnt[c * ROUTE_SLOTS + s] = prun[s];
+        }
+    }
+}
+
+// Pass 2: recompute (same bits) and write each particle and product to its
+// final slot.  For electrons, ionisation products go to both the electron
+// and the ion arrays.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p, PView<R> q, PView<R> qion, int isp,
+                                                       uint32_t step, const int *base, const int *pbase_e,
+                                                       const int *pbase_i, const int *pid_off, const uint32_t *next_id,
+                                                       int cap, int cap_ion, int *err) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    if (k0 == k1) return;
+    CellE<R> e;
+    e.load(k.Ex, k.Ey, c, k.ngy);
+    const R As = k.Ascale[isp];
+    const unsigned long long lt = lanes_below(lane);
+    int run[ROUTE_SLOTS], prun[ROUTE_SLOTS];
+#pragma unroll
+    for (int s = 0; s < ROUTE_SLOTS; ++s) { run[s] = 0; prun[s] = 0; }
+    for (int b0 = k0; b0 < k1; b0 += 64) {
+        int kk = b0 + lane;
+        int slot = -1;
+        bool prod = false;
+        R x = 0, y = 0, vx = 0, vy = 0, vz = 0, evx = 0, evy = 0, evz = 0, ivx = 0, ivy = 0, ivz = 0;
+        uint32_t id = 0;
+        if (kk < k1) {
+            x = p.x[kk]; y = p.y[kk]; vx = p.vx[kk]; vy = p.vy[kk]; vz = p.vz[kk]; id = p.id[kk];
+            int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx,
+                                        k.dt_dy, x, y, vx, vy);
+            Route rt = k.route[c * ROUTE_SLOTS + code];
+            if (rt.dest >= 0) {
+                apply_reflect<R>(rt.info, x, y, vx, vy);
+                slot = rt.info & 15;
+                if (isp == 0) {
+                    int ev = mcc_ar_electron<R>(k, rt.dest, id, step, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
+                    prod = (ev == 3 || ev == 4);
+                } else {
+                    mcc_ar_ion<R>(k, rt.dest, id, step, vx, vy, vz);
+                }
+            }
+        }
+        int pos = -1, ppos = -1, prank = 0;
+#pragma unroll
+        for (int s = 0; s < ROUTE_SLOTS; ++s) {
+            unsigned long long m = __ballot(slot == s);
+            unsigned long long pm = __ballot(slot == s && prod);
+            if (slot == s) {
+                pos = run[s] + __popcll(m & lt);
+                prank = prun[s] + __popcll(pm & lt);
+            }
+            run[s] += __popcll(m);
+            prun[s] += __popcll(pm);
+        }
+        if (slot >= 0) {
+            int ent = c * ROUTE_SLOTS + slot;
+            pos += base[ent];
+            if (pos < cap) {
+                q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = vz; q.id[pos] = id;
+            } else {
+                atomicOr(err, 1);
+            }
+            if (prod) {
+                int off = pid_off[ent] + prank;
+                ppos = pbase_e[ent] + prank;
+                if (ppos < cap) {
+                    q.x[ppos] = x; q.y[ppos] = y; q.vx[ppos] = evx; q.vy[ppos] = evy; q.vz[ppos] = evz;
+                    q.id[ppos] = next_id[0] + (uint32_t)off;
+                } else {
+                    atomicOr(err, 1);
+                }
+                int ipos = pbase_i[ent] + prank;
+                if (ipos < cap_ion) {
+                    qion.x[ipos] = x; qion.y[ipos] = y; qion.vx[ipos] = ivx; qion.vy[ipos] = ivy; qion.vz[ipos] = ivz;
+                    qion.id[ipos] = next_id[1] + (uint32_t)off;
+                } else {
+                    atomicOr(err, 1);
+                }
+            }
+        }
+    }
+}
+
+// ============================================================ deposit
+// CIC weights summed per cell with a wave reduction (src/cuda_Deposit.cu:289-317)
+template <typename R>
+__global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PView<R> p, R *part) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    R WS = 0, WN = 0, ES = 0, EN = 0;
+    for (int kk = k0 + lane; kk < k1; kk += 64) {
+        R lx = p.x[kk], ly = p.y[kk];
+        WS += (1 - lx) * (1 - ly);
+        WN += (1 - lx) * ly;
+        ES += lx * (1 - ly);
+        EN += lx * ly;
+    }
+    WS = wave_sum(WS); WN = wave_sum(WN); ES = wave_sum(ES); EN = wave_sum(EN);
+    if (lane == 0) {
+        part[(size_t)c * 4 + 0] = WS; part[(size_t)c * 4 + 1] = WN;
+        part[(size_t)c * 4 + 2] = ES; part[(size_t)c * 4 + 3] = EN;
+    }
+}
+
+// Node density from the four adjacent cells, boundary factors
+// (src/cuda_Deposit.cu:277-288), then charge density, source and PCG rhs
+// (src/cuda_Deposit.cu:30-104).  part holds nsp x Gsize x 4.
+template <typename R>
+__global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_sp, R *Sigma, R *Source, R *b) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n.Gsize) return;
+    const int ngy = n.ngy, ngx = n.ngx;
+    const int x = g / ngy, y = g % ngy;
+    const bool c00 = x < ngx - 1 && y < ngy - 1;   // cell g
+    const bool c01 = x < ngx - 1 && y > 0;         // cell g-1 (node is its north-west corner)
+    const bool c10 = x > 0 && y < ngy - 1;         // cell g-ngy (south-east corner)
+    const bool c11 = x > 0 && y > 0;               // cell g-ngy-1 (north-east corner)
+    const int f = n.Face[g], bnd = n.Boundary[g];
+    R ss = 0;
+    for (int isp = 0; isp < n.nsp; ++isp) {
+        const R *pp = part + (size_t)isp * n.Gsize * 4;
+        R d = 0;   // cells in ascending index order
+        if (c11) d += pp[(size_t)(g - ngy - 1) * 4 + 3];
+        if (c10) d += pp[(size_t)(g - ngy) * 4 + 2];
+        if (c01) d += pp[(size_t)(g - 1) * 4 + 1];
+        if (c00) d += pp[(size_t)g * 4 + 0];
+        if (f == F_LEFT || f == F_RIGHT || f == F_UP || f == F_DOWN) d *= 2;
+        else if (f == F_UL || f == F_LL || f == F_UR || f == F_LR) d *= 4;
+        if (bnd == NEUMANN) d *= 2;
+        else if (bnd == DIRICHLET && f != F_NOFACE && n.CondID[g] != 0) d *= 2;
+        den[(size_t)isp * n.Gsize + g] = d;
+    }
+    int nb = -1;
+    if (x == 0) nb = g + ngy;
+    else if (x == ngx - 1) nb = g - ngy;
+    else if (y == 0) nb = g + 1;
+    else if (y == ngy - 1) nb = g - 1;
+    int nbnd = nb >= 0 ? n.Boundary[nb] : bnd;
+    R fac = nb >= 0 ? (R)2 : (R)1;
+    R SD = 0;
+    if (nbnd == DIELECTRIC) {
+        for (int i = 0; i < n.nsp; ++i) ss += sigma_sp[(size_t)i * n.Gsize + g] * n.q_density[i];
+        Sigma[g] = fac * ss / n.Area[g];
+        SD = ss;
+    } else if (nbnd == CONDUCTOR) {
+        for (int i = 0; i < n.nsp; ++i) sigma_sp[(size_t)i * n.Gsize + g] = 0;
+        SD = 0;
+    }
+    ss = 0;
+    for (int i = 0; i < n.nsp; ++i) ss += den[(size_t)i * n.Gsize + g] * n.q_density[i];
+    R src = (R)((double)(ss + SD) / 8.8542e-12);
+    Source[g] = src;
+    int ai = n.A_idx[g];
+    if (ai) b[ai - 1] = src;
+}
+
+// ============================================================ E field
+// Total potential = sum_c V_c phi_c + phi_Poisson (src/cuda_Field.cu:22-33)
+template <typename R>
+__global__ void k_potential(NodeConst<R> n, const R *lap, const R *V, const R *phi, R *tot) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n.Gsize) return;
+    R tp = 0;
+    for (int i = 0; i < n.CondNUMR; ++i) tp += V[i] * lap[(size_t)i * n.Gsize + g];
+    tp += (R)1 * phi[g];
+    tot[g] = tp;
+}
+
+template <typename R>
+__device__ inline void eps_idx(int TID, int ngx, int ngy, int &ET, int &EL, int &ED, int &EC) {
+    int x = TID / ngy, y = TID % ngy;
+    ET = x * (ngy - 1) + y;
+    EL = x ? ET - ngy + 1 : ET;
+    ED = y ? ET - 1 : ET;
+    EC = EL + ED - ET;
+    int xo = (x == ngx - 1) ? ngy - 1 : 0;
+    int yo = (y == ngy - 1) ? 1 : 0;
+    ET -= (xo + yo); EL -= yo; ED -= xo;
+}
+
+// Surface charge of conductor/Dirichlet faces (src/cuda_Field.cu:500-558,
+// from the Poisson potential and the last deposit) and node E
+// (src/cuda_Field.cu:698-846, from the total potential).
+template <typename R>
+__global__ void k_efield(NodeConst<R> n, const R *phi, const R *tot, const R *den, R *Sigma, R *Ex, R *Ey) {
+    int TID = blockIdx.x * blockDim.x + threadIdx.x;
+    if (TID >= n.Gsize) return;
+    const int ngx = n.ngx, ngy = n.ngy;
+    const int x = TID / ngy, y = TID % ngy;
+    int ET, EL, ED, EC;
+    eps_idx<R>(TID, ngx, ngy, ET, EL, ED, EC);
+    const int pl = x ? TID - ngy : TID, pr = (x == ngx - 1) ? TID : TID + ngy;
+    const int pd = y ? TID - 1 : TID, pu = (y == ngy - 1) ? TID : TID + 1;
+    const int b = n.Boundary[TID], f = n.Face[TID];
+    const R *e = n.eps;
+    const R hdx = n.hdx, hdy = n.hdy, idx = n.idx, idy = n.idy, idxy = n.idxy;
+    // --- Cond_Sigma (Poisson part only)
+    R sig = Sigma[TID];
+    if ((b == CONDUCTOR || b == DIRICHLET) && f != F_NOFACE) {
+        R ss = 0;
+        for (int i = 0; i < n.nsp; ++i) ss += den[TID + i * n.Gsize] * n.q_density[i];
+        const R P = phi[TID];
+        switch (f) {
+        case F_UP: sig = hdy * (e[ET] + e[EL]) * (P - phi[pu]) - hdx * ss; break;
+        case F_DOWN: sig = hdy * (e[ED] + e[EC]) * (P - phi[pd]) - hdx * ss; break;
+        case F_LEFT: sig = hdx * (e[EL] + e[EC]) * (P - phi[pl]) - hdy * ss; break;
+        case F_RIGHT: sig = hdx * (e[ET] + e[ED]) * (P - phi[pr]) - hdy * ss; break;
+        case F_UL: sig = hdy * (e[ET] + e[EL]) * (P - phi[pu]) + hdx * (e[EL] + e[EC]) * (P - phi[pl]) - idxy * ss; break;
+        case F_UR: sig = hdy * (e[ET] + e[EL]) * (P - phi[pu]) + hdx * (e[ET] + e[ED]) * (P - phi[pr]) - idxy * ss; break;
+        case F_LL: sig = hdy * (e[ED] + e[EC]) * (P - phi[pd]) + hdx * (e[EL] + e[EC]) * (P - phi[pl]) - idxy * ss; break;
+        case F_LR: sig = hdy * (e[ED] + e[EC]) * (P - phi[pd]) + hdx * (e[ET] + e[ED]) * (P - phi[pr]) - idxy * ss; break;
+        default: break;
+        }
+        Sigma[TID] = sig;
+    }
+    // --- VtoEfield
+    const R *Phi = tot;
+    const R P = Phi[TID];
+    R ex = Ex[TID], ey = Ey[TID];
+    const R hxy = n.hxy, hyx = n.hyx;
+    const bool wall = (b == CONDUCTOR || b == NEUMANN || b == DIRICHLET);
+    const bool die = (b == DIELECTRIC || b >= 100);
+    const bool hard = (b == CONDUCTOR || b == DIRICHLET || b == DIELECTRIC);
+    if (f == F_LEFT) {
+        if (wall) { ex = -2 * sig / (e[EL] + e[EC]); ey = 0; }
+        else if (die) { ex = idx * (Phi[pl] - P); ey = hdy * (Phi[pd] - Phi[pu]); }
+        else {
+            ex = (R)0.5 * (e[ET] + e[ED]) * (Phi[pr] - P) * idx;
+            ex -= hxy * e[ET] * (P - Phi[pu]) * idy;
+            ex -= hxy * e[ED] * (P - Phi[pd]) * idy;
+            ex /= (R)-0.5 * (e[EL] + e[EC]);
+            ey = hdy * (Phi[pd] - Phi[pu]);
+        }
+    } else if (f == F_RIGHT) {
+        if (wall) { ex = 2 * sig / (e[ET] + e[ED]); ey = 0; }
+        else if (die) { ex = idx * (P - Phi[pr]); ey = hdy * (Phi[pd] - Phi[pu]); }
+        else {
+            ex = (R)0.5 * (e[EL] + e[EC]) * (Phi[pl] - P) * idx;
+            ex -= hxy * e[EL] * (P - Phi[pu]) * idy;
+            ex -= hxy * e[EC] * (P - Phi[pd]) * idy;
+            ex /= (R)0.5 * (e[ET] + e[ED]);
+            ey = hdy * (Phi[pd] - Phi[pu]);
+        }
+    } else if (f == F_UP) {
+        if (wall) { ex = 0; ey = 2 * sig / (e[ET] + e[EL]); }
+        else if (die) { ex = hdx * (Phi[pl] - Phi[pr]); ey = idy * (P - Phi[pu]); }
+        else {
+            ex = hdx * (Phi[pl] - Phi[pr]);
+            ey = (R)0.5 * (e[ED] + e[EC]) * (Phi[pd] - P) * idy;
+            ey -= hyx * e[ED] * (P - Phi[pr]) * idx;
+            ey -= hyx * e[EC] * (P - Phi[pl]) * idx;
+            ey /= (R)0.5 * (e[ET] + e[EL]);
+        }
+    } else if (f == F_DOWN) {
+        if (wall) { ex = 0; ey = -2 * sig / (e[ED] + e[EC]); }
+        else if (die) { ex = hdx * (Phi[pl] - Phi[pr]); ey = idy * (Phi[pd] - P); }
+        else {
+            ex = hdx * (Phi[pl] - Phi[pr]);
+            ey = (R)0.5 * (e[ET] + e[EL]) * (Phi[pu] - P) * idy;
+            ey -= hyx * e[ET] * (P - Phi[pr]) * idx;
+            ey -= hyx * e[EL] * (P - Phi[pl]) * idx;
+            ey /= (R)-0.5 * (e[ED] + e[EC]);
+        }
+    } else if (f == F_UL) {
+        if (hard) { ex = idx * (Phi[pl] - P); ey = idy * (P - Phi[pu]); }
+        else { ex = hdx * (Phi[pl] - Phi[pr]); ey = hdy * (Phi[pd] - Phi[pu]); }
+    } else if (f == F_UR) {
+        if (hard) { ex = idx * (P - Phi[pr]); ey = idy * (P - Phi[pu]); }
+        else { ex = hdx * (Phi[pl] - Phi[pr]); ey = hdy * (Phi[pd] - Phi[pu]); }
+    } else if (f == F_LL) {
+        if (hard) { ex = idx * (Phi[pl] - P); ey = idy * (Phi[pd] - P); }
+        else { ex = hdx * (Phi[pl] - Phi[pr]); ey = hdy * (Phi[pd] - Phi[pu]); }
+    } else if (f == F_LR) {
+        if (hard) { ex = idx * (P - Phi[pr]); ey = idy * (Phi[pd] - P); }
+        else { ex = hdx * (Phi[pl] - Phi[pr]); ey = hdy * (Phi[pd] - Phi[pu]); }
+    } else if (b == NEUMANN) {
+        if (x == 0 || x == ngx - 1) { ex = 0; ey = hdy * (Phi[pd] - Phi[pu]); }
+        else if (y == 0 || y == ngy - 1) { ex = hdx * (Phi[pl] - Phi[pr]); ey = 0; }
+    } else {
+        ex = hdx * (Phi[pl] - Phi[pr]);
+        ey = hdy * (Phi[pd] - Phi[pu]);
+    }
+    Ex[TID] = ex;
+    Ey[TID] = ey;
+}
+
+// Conductor voltages at t = step * dt: V = sum DC + AC sin(w t + phase)
+template <typename R>
+__global__ void k_voltages(VoltConst v, double t, R *V) {
+    int i = threadIdx.x;
+    if (i >= v.ncond) return;
+    R acc = 0;
+    if (v.flag[i]) {
+        for (int j = 0; j < v.nsrc; ++j)
+            if (v.mid[j] == i + 1)
+                acc = (R)((double)acc + ((double)(R)v.dc[j] + (double)(R)v.ac[j] * dm_sin((double)(R)v.w[j] * t + (double)(R)v.ph[j])));
+    }
+    V[i] = acc;
+}
+
+// profile accumulators in FP64 (src/cuda_Diagnostic.cu:505-547)
+template <typename R>
+__global__ void k_accumulate(int Gsize, int nsp, const R *den, const R *tot, const R *Ex, const R *Ey, double *sden,
+                             double *sphi, double *sex, double *sey) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= Gsize) return;
+    for (int i = 0; i < nsp; ++i) sden[(size_t)i * Gsize + g] += (double)den[(size_t)i * Gsize + g];
+    sphi[g] += (double)tot[g];
+    sex[g] += (double)Ex[g];
+    sey[g] += (double)Ey[g];
+}
+
+// ============================================================ small helpers
+// new_total[d] = particles already in d + products appended to d
+__global__ void k_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= Gsize) return;
+    total[d] = cell_start[d + 1] - cell_start[d] + (pcnt ? pcnt[d] : 0);
+}
+
+// next_id[s] += number of products created this step (same count for the
+// electron and the ion array: one of each per ionisation)
+__global__ void k_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp) {
+    if (threadIdx.x == 0) {
+        uint32_t add = (uint32_t)*prod_total_end;
+        for (int s = 0; s < nsp; ++s) next_id[s] += add;
+    }
+}
+
+template <typename R>
+__global__ void k_rhs_to_fp64(int N, const R *b, double *b64) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) b64[i] = (double)b[i];
+}
+
+template <typename R>
+__global__ void k_x_to_phi(int Gsize, const int *A_idx, const double *x, R *phi) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= Gsize) return;
+    int a = A_idx[g];
+    phi[g] = a ? (R)x[a - 1] : (R)0;
+}
+
+// ============================================================ launchers
+static inline int cell_blocks(int Csize) { return (Csize + 3) / 4; }
+static inline int node_blocks(int n) { return (n + 255) / 256; }
+
+template <typename R>
+void launch_push(const DevConst<R> &k, const PView<R> &p, int isp, uint8_t *tag, hipStream_t s) {
+    hipLaunchKernelGGL(k_push<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag);
+}
+template <typename R>
+void launch_sort_count(const DevConst<R> &k, const PView<R> &p, int isp, const uint8_t *tag, int *key, int *cnt,
+                       int *cc, R *sg, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag, key, cnt, cc, sg);
+}
+template <typename R>
+void launch_sort_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const uint8_t *tag,
+                         const int *key, const int *base, int cap, int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_scatter<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, q, tag, key, base, cap, err);
+}
+template <typename R>
+void launch_mcc_collide(const DevConst<R> &k, const PView<R> &p, int isp, uint32_t step, unsigned long long *mccr,
+                        int *pcnt, int *prank, R *stage, hipStream_t s) {
+    hipLaunchKernelGGL(k_mcc_collide<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, step, mccr, pcnt,
+                       prank, stage);
+}
+template <typename R>
+void launch_mcc_insert(const DevConst<R> &k, const PView<R> &src, const PView<R> &dst, const int *new_start,
+                       const int *prod_start, const PView<R> &parent, const int *prank, const R *stage, int stage_off,
+                       const uint32_t *next_id, int cap, int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_mcc_insert<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, src, dst, new_start,
+                       prod_start, (const int *)nullptr, parent, prank, stage, stage_off, next_id, cap, err);
+}
+template <typename R>
+void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, uint32_t step, int *cnt, int *pcnt,
+                        unsigned long long *mccr, int *cc, R *sg, hipStream_t s) {
+    hipLaunchKernelGGL(k_fused_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, step, cnt, pcnt, mccr,
+                       cc, sg);
+}
+template <typename R>
+void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
+                          uint32_t step, const int *base, const int *pbase_e, const int *pbase_i, const int *pid_off,
+                          const uint32_t *next_id, int cap, int cap_ion, int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_fused_scatter<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, q, qion, isp, step, base,
+                       pbase_e, pbase_i, pid_off, next_id, cap, cap_ion, err);
+}
+template <typename R>
+void launch_deposit(const DevConst<R> &k, const NodeConst<R> &n, const PView<R> *ps, R *part, R *den, R *sigma_sp,
+                    R *Sigma, R *Source, R *b, hipStream_t s) {
+    for (int isp = 0; isp < n.nsp; ++isp)
+        hipLaunchKernelGGL(k_deposit_cells<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, ps[isp],
+                           part + (size_t)isp * n.Gsize * 4);
+    hipLaunchKernelGGL(k_deposit_nodes<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, (const R *)part, den,
+                       sigma_sp, Sigma, Source, b);
+}
+template <typename R>
+void launch_efield(const NodeConst<R> &n, const VoltConst &v, double t, const R *lap, R *V, const R *phi, R *tot,
+                   const R *den, R *Sigma, R *Ex, R *Ey, hipStream_t s) {
+    hipLaunchKernelGGL(k_voltages<R>, dim3(1), dim3(64), 0, s, v, t, V);
+    hipLaunchKernelGGL(k_potential<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, lap, (const R *)V, phi, tot);
+    hipLaunchKernelGGL(k_efield<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, phi, (const R *)tot, den, Sigma, Ex,
+                       Ey);
+}
+template <typename R>
+void launch_accumulate(int Gsize, int nsp, const R *den, const R *tot, const R *Ex, const R *Ey, double *sden,
+                       double *sphi, double *sex, double *sey, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate<R>, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, nsp, den, tot, Ex, Ey, sden,
+                       sphi, sex, sey);
+}
+
+void launch_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_counts, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, cell_start, pcnt, total);
+}
+void launch_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp, hipStream_t s) {
+    hipLaunchKernelGGL(k_bump_ids, dim3(1), dim3(64), 0, s, next_id, prod_total_end, nsp);
+}
+template <typename R>
+void launch_rhs_to_fp64(int N, const R *b, double *b64, hipStream_t s) {
+    hipLaunchKernelGGL(k_rhs_to_fp64<R>, dim3(node_blocks(N)), dim3(256), 0, s, N, b, b64);
+}
+template <typename R>
+void launch_x_to_phi(int Gsize, const int *A_idx, const double *x, R *phi, hipStream_t s) {
+    hipLaunchKernelGGL(k_x_to_phi<R>, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, A_idx, x, phi);
+}
+template void launch_rhs_to_fp64<float>(int, const float *, double *, hipStream_t);
+template void launch_rhs_to_fp64<double>(int, const double *, double *, hipStream_t);
+template void launch_x_to_phi<float>(int, const int *, const double *, float *, hipStream_t);
+template void launch_x_to_phi<double>(int, const int *, const double *, double *, hipStream_t);
+
+void launch_totals(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt, const int *prod_cnt, int *total,
+                   int *alive_total, int *prod_total, hipStream_t s) {
+    hipLaunchKernelGGL(k_totals, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent, cnt, prod_cnt,
+                       total, alive_total, prod_total);
+}
+void launch_assign_bases(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt, const int *prod_cnt,
+                         const int *start, const int *alive_total, const int *prod_start, int *base, int *pbase,
+                         int *pid_off, hipStream_t s) {
+    hipLaunchKernelGGL(k_assign_bases, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent, cnt,
+                       prod_cnt, start, alive_total, prod_start, base, pbase, pid_off);
+}
+// exclusive scan of n ints into out[0..n]; out[n] = total.  tmp needs
+// ceil(n/1024)+1 ints.
+void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
+    int nb = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, s, in, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tmp, nb, tmp + nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, out, n, (const int *)tmp, (const int *)(tmp + nb), out + n);
+}
+
+#define INST(R)                                                                                                         \
+    template void launch_push<R>(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);                 \
+    template void launch_sort_count<R>(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *,      \
+                                       int *, R *, hipStream_t);                                                       \
+    template void launch_sort_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const uint8_t *,     \
+                                         const int *, const int *, int, int *, hipStream_t);                           \
+    template void launch_mcc_collide<R>(const DevConst<R> &, const PView<R> &, int, uint32_t, unsigned long long *,    \
+                                        int *, int *, R *, hipStream_t);                                               \
+    template void launch_mcc_insert<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const int *,           \
+                                       const int *, const PView<R> &, const int *, const R *, int, const uint32_t *,   \
+                                       int, int *, hipStream_t);                                                       \
+    template void launch_fused_count<R>(const DevConst<R> &, const PView<R> &, int, uint32_t, int *, int *,            \
+                                        unsigned long long *, int *, R *, hipStream_t);                                \
+    template void launch_fused_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const PView<R> &,   \
+                                          int, uint32_t, const int *, const int *, const int *, const int *,           \
+                                          const uint32_t *, int, int, int *, hipStream_t);                             \
+    template void launch_deposit<R>(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *, R *, R *, R *,   \
+                                    R *, R *, hipStream_t);                                                            \
+    template void launch_efield<R>(const NodeConst<R> &, const VoltConst &, double, const R *, R *, const R *, R *,    \
+                                   const R *, R *, R *, R *, hipStream_t);                                             \
+    template void launch_accumulate<R>(int, int, const R *, const R *, const R *, const R *, double *, double *,       \
+                                       double *, double *, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace pic

@@ -1,0 +1,341 @@
+// Field solve: Jacobi-preconditioned conjugate gradients as one persistent
+// cooperative kernel per solve (the reference's PCG_float,
+// src/cuda_Field.cu:942-1072, uses the same single-launch structure).
+//
+//  * plain FP64 path: vectors, operator and dot products in FP64;
+//  * refined path: FP64 outer refinement (residual b - A x recomputed in FP64,
+//    x accumulated in FP64) around an FP32 inner PCG whose vectors and
+//    stencil sweeps are FP32 and whose global dot products are FP64.
+//
+// Both stop on the FP64 relative residual ||b - A x||_2 <= tol ||b||_2.
+// Global reductions are deterministic: each block writes its partial sum,
+// and after the grid barrier every block adds all partials in the same
+// order.  The recurrence q = A p is carried as q <- A z + beta q so each
+// iteration needs two grid barriers.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "pic_kernels.h"
+
+namespace pic {
+
+constexpr int PCG_BLOCK = 256;
+constexpr int PCG_NRED = 4;   // values reduced per barrier
+
+struct GridSync {
+    unsigned int count;
+    unsigned int gen;
+};
+
+struct PcgWork {
+    int N = 0, nblk = 0;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;   // FP64 vectors (plain path / outer residual)
+    float *rf = nullptr, *zf = nullptr, *pf = nullptr, *qf = nullptr, *df = nullptr;
+    double *partials = nullptr;   // 2 sets x nblk x PCG_NRED
+    GridSync *gs = nullptr;
+};
+
+namespace {
+
+__device__ inline void grid_sync(GridSync *gs, unsigned nblk) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned g = __hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned arrived = __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblk - 1) {
+            __hip_atomic_store(&gs->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+                __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    __threadfence();
+}
+
+// block-level sum of NV doubles with a fixed tree; result valid in all threads
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *sh) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) sh[j * PCG_BLOCK + t] = v[j];
+    __syncthreads();
+    for (int off = PCG_BLOCK / 2; off > 0; off >>= 1) {
+        if (t < off) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) sh[j * PCG_BLOCK + t] += sh[j * PCG_BLOCK + t + off];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = sh[j * PCG_BLOCK];
+    __syncthreads();
+}
+
+// publish block partials, barrier, and sum all partials in block order
+template <int NV>
+__device__ inline void grid_sum(double (&v)[NV], double *sh, double *partials, int &phase, GridSync *gs, unsigned nblk) {
+    block_sum<NV>(v, sh);
+    double *set = partials + (size_t)(phase & 1) * nblk * PCG_NRED;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) set[blockIdx.x * PCG_NRED + j] = v[j];
+    }
+    grid_sync(gs, nblk);
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+    for (int b = threadIdx.x; b < (int)nblk; b += PCG_BLOCK) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) acc[j] = set[b * PCG_NRED + j];
+    }
+    // nblk <= PCG_BLOCK: thread b holds partial b; fixed-tree block sum
+    block_sum<NV>(acc, sh);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = acc[j];
+    ++phase;
+}
+
+template <typename V>
+__device__ inline V spmv_row(const int *col, const V *val, const V *x, int i) {
+    const int o = 5 * i;
+    V s = val[o] * x[col[o]];
+    s += val[o + 1] * x[col[o + 1]];
+    s += val[o + 2] * x[col[o + 2]];
+    s += val[o + 3] * x[col[o + 3]];
+    s += val[o + 4] * x[col[o + 4]];
+    return s;
+}
+
+// ------------------------------------------------------------ plain FP64
+__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_plain(PcgOperator A, const double *b, double *x, double *r,
+                                                        double *z, double *p, double *q, double *partials,
+                                                        GridSync *gs, double tol, int maxit, PcgResult *res) {
+    __shared__ double sh[PCG_NRED * PCG_BLOCK];
+    const unsigned nblk = gridDim.x;
+    const int N = A.N;
+    const int stride = nblk * PCG_BLOCK;
+    const int t0 = blockIdx.x * PCG_BLOCK + threadIdx.x;
+    int phase = 0;
+    double v3[3] = {0, 0, 0};
+    for (int i = t0; i < N; i += stride) {
+        double ri = b[i] - spmv_row<double>(A.col, A.vald, x, i);
+        double zi = A.mdiagd[i] * ri;
+        r[i] = ri; z[i] = zi; p[i] = zi;
+        v3[0] += ri * zi; v3[1] += ri * ri; v3[2] += b[i] * b[i];
+    }
+    grid_sum<3>(v3, sh, partials, phase, gs, nblk);
+    double rz = v3[0], rr = v3[1], bb = v3[2];
+    const double tol2 = tol * tol * bb;
+    int it = 0;
+    int status = 0;
+    if (bb == 0.0) {
+        for (int i = t0; i < N; i += stride) x[i] = 0.0;
+        rr = 0.0;
+    } else if (rr > tol2) {
+        double v1[1] = {0};
+        for (int i = t0; i < N; i += stride) {
+            double qi = spmv_row<double>(A.col, A.vald, p, i);
+            q[i] = qi;
+            v1[0] += p[i] * qi;
+        }
+        grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+        double pq = v1[0];
+        for (;;) {
+            double alpha = pq != 0.0 ? rz / pq : 0.0;
+            double v2[2] = {0, 0};
+            for (int i = t0; i < N; i += stride) {
+                x[i] += alpha * p[i];
+                double ri = r[i] - alpha * q[i];
+                double zi = A.mdiagd[i] * ri;
+                r[i] = ri; z[i] = zi;
+                v2[0] += ri * zi; v2[1] += ri * ri;
+            }
+            grid_sum<2>(v2, sh, partials, phase, gs, nblk);
+            ++it;
+            double rz_new = v2[0];
+            rr = v2[1];
+            if (rr <= tol2) break;
+            if (it >= maxit) { status = 1; break; }
+            double beta = rz != 0.0 ? rz_new / rz : 0.0;
+            rz = rz_new;
+            v1[0] = 0.0;
+            for (int i = t0; i < N; i += stride) {
+                double pi = z[i] + beta * p[i];
+                double qi = spmv_row<double>(A.col, A.vald, z, i) + beta * q[i];
+                p[i] = pi; q[i] = qi;
+                v1[0] += pi * qi;
+            }
+            grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+            pq = v1[0];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        res->iters = it;
+        res->outer = 1;
+        res->rel_res = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+        res->status = status;
+    }
+}
+
+// ------------------------------------------------------------ refined
+__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const double *b, double *x, double *r64,
+                                                          float *r, float *z, float *p, float *q, float *d,
+                                                          double *partials, GridSync *gs, double tol, double inner_tol,
+                                                          int maxit, PcgResult *res) {
+    __shared__ double sh[PCG_NRED * PCG_BLOCK];
+    const unsigned nblk = gridDim.x;
+    const int N = A.N;
+    const int stride = nblk * PCG_BLOCK;
+    const int t0 = blockIdx.x * PCG_BLOCK + threadIdx.x;
+    int phase = 0;
+    int it = 0, outer = 0, status = 0;
+    double bb = -1.0, rr64 = 0.0;
+    for (;;) {
+        // FP64 residual of the current iterate
+        double v2[2] = {0, 0};
+        for (int i = t0; i < N; i += stride) {
+            double ri = b[i] - spmv_row<double>(A.col, A.vald, x, i);
+            r64[i] = ri;
+            v2[0] += ri * ri;
+            if (bb < 0.0) v2[1] += b[i] * b[i];
+        }
+        grid_sum<2>(v2, sh, partials, phase, gs, nblk);
+        rr64 = v2[0];
+        if (bb < 0.0) bb = v2[1];
+        if (bb == 0.0) {
+            for (int i = t0; i < N; i += stride) x[i] = 0.0;
+            rr64 = 0.0;
+            break;
+        }
+        if (rr64 <= tol * tol * bb) break;
+        if (it >= maxit) { status = 1; break; }
+        ++outer;
+        // inner FP32 PCG on A d = r, d0 = 0
+        double v3[2] = {0, 0};
+        for (int i = t0; i < N; i += stride) {
+            float ri = (float)r64[i];
+            float zi = A.mdiagf[i] * ri;
+            r[i] = ri; z[i] = zi; p[i] = zi; d[i] = 0.0f;
+            v3[0] += (double)ri * (double)zi;
+            v3[1] += (double)ri * (double)ri;
+        }
+        grid_sum<2>(v3, sh, partials, phase, gs, nblk);
+        double rz = v3[0];
+        const double stop2 = inner_tol * inner_tol * v3[1];
+        double v1[1] = {0};
+        for (int i = t0; i < N; i += stride) {
+            float qi = spmv_row<float>(A.col, A.valf, p, i);
+            q[i] = qi;
+            v1[0] += (double)p[i] * (double)qi;
+        }
+        grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+        double pq = v1[0];
+        for (;;) {
+            float alpha = (float)(pq != 0.0 ? rz / pq : 0.0);
+            double w2[2] = {0, 0};
+            for (int i = t0; i < N; i += stride) {
+                d[i] += alpha * p[i];
+                float ri = r[i] - alpha * q[i];
+                float zi = A.mdiagf[i] * ri;
+                r[i] = ri; z[i] = zi;
+                w2[0] += (double)ri * (double)zi;
+                w2[1] += (double)ri * (double)ri;
+            }
+            grid_sum<2>(w2, sh, partials, phase, gs, nblk);
+            ++it;
+            double rz_new = w2[0];
+            if (w2[1] <= stop2 || it >= maxit) break;
+            float beta = (float)(rz != 0.0 ? rz_new / rz : 0.0);
+            rz = rz_new;
+            v1[0] = 0.0;
+            for (int i = t0; i < N; i += stride) {
+                float pi = z[i] + beta * p[i];
+                float qi = spmv_row<float>(A.col, A.valf, z, i) + beta * q[i];
+                p[i] = pi; q[i] = qi;
+                v1[0] += (double)pi * (double)qi;
+            }
+            grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+            pq = v1[0];
+        }
+        for (int i = t0; i < N; i += stride) x[i] += (double)d[i];
+        // x is read by other blocks in the residual sweep
+        grid_sync(gs, nblk);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        res->iters = it;
+        res->outer = outer;
+        res->rel_res = bb > 0.0 ? sqrt(rr64 / bb) : 0.0;
+        res->status = status;
+    }
+}
+
+}  // namespace
+
+PcgWork *pcg_work_create(int N, int *err) {
+    PcgWork *w = new PcgWork();
+    w->N = N;
+    int dev = 0, ncu = 0, per_cu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_pcg_refined), PCG_BLOCK, 0);
+    int per_cu2 = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, reinterpret_cast<const void *>(k_pcg_plain), PCG_BLOCK, 0);
+    if (per_cu2 < per_cu) per_cu = per_cu2;
+    int resident = ncu * (per_cu > 0 ? per_cu : 1);
+    int need = (N + PCG_BLOCK - 1) / PCG_BLOCK;
+    int nblk = need < ncu ? need : ncu;          // one block per CU at most
+    if (nblk > resident) nblk = resident;
+    if (nblk > PCG_BLOCK) nblk = PCG_BLOCK;      // grid_sum keeps one partial per thread
+    if (nblk < 1) nblk = 1;
+    w->nblk = nblk;
+    size_t n = (size_t)N;
+    hipError_t e = hipSuccess;
+    e = (hipError_t)(e | hipMalloc(&w->r, n * sizeof(double)));
+    e = (hipError_t)(e | hipMalloc(&w->z, n * sizeof(double)));
+    e = (hipError_t)(e | hipMalloc(&w->p, n * sizeof(double)));
+    e = (hipError_t)(e | hipMalloc(&w->q, n * sizeof(double)));
+    e = (hipError_t)(e | hipMalloc(&w->rf, n * sizeof(float)));
+    e = (hipError_t)(e | hipMalloc(&w->zf, n * sizeof(float)));
+    e = (hipError_t)(e | hipMalloc(&w->pf, n * sizeof(float)));
+    e = (hipError_t)(e | hipMalloc(&w->qf, n * sizeof(float)));
+    e = (hipError_t)(e | hipMalloc(&w->df, n * sizeof(float)));
+    e = (hipError_t)(e | hipMalloc(&w->partials, 2 * (size_t)nblk * PCG_NRED * sizeof(double)));
+    e = (hipError_t)(e | hipMalloc(&w->gs, sizeof(GridSync)));
+    if (e != hipSuccess) { *err = -40; pcg_work_destroy(w); return nullptr; }
+    hipMemset(w->gs, 0, sizeof(GridSync));
+    hipMemset(w->r, 0, n * sizeof(double));
+    *err = 0;
+    return w;
+}
+
+void pcg_work_destroy(PcgWork *w) {
+    if (!w) return;
+    hipFree(w->r); hipFree(w->z); hipFree(w->p); hipFree(w->q);
+    hipFree(w->rf); hipFree(w->zf); hipFree(w->pf); hipFree(w->qf); hipFree(w->df);
+    hipFree(w->partials); hipFree(w->gs);
+    delete w;
+}
+
+int pcg_grid_blocks(PcgWork *w) { return w ? w->nblk : 0; }
+
+void pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, double tol, int maxit, PcgWork *w,
+                          PcgResult *res, hipStream_t s) {
+    void *args[] = {(void *)&A, (void *)&b, (void *)&x, (void *)&w->r, (void *)&w->z, (void *)&w->p, (void *)&w->q,
+                    (void *)&w->partials, (void *)&w->gs, (void *)&tol, (void *)&maxit, (void *)&res};
+    hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_plain), dim3(w->nblk), dim3(PCG_BLOCK), args, 0, s);
+}
+
+void pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol, int maxit,
+                       PcgWork *w, PcgResult *res, hipStream_t s) {
+    void *args[] = {(void *)&A, (void *)&b, (void *)&x, (void *)&w->r, (void *)&w->rf, (void *)&w->zf,
+                    (void *)&w->pf, (void *)&w->qf, (void *)&w->df, (void *)&w->partials, (void *)&w->gs,
+                    (void *)&tol, (void *)&inner_tol, (void *)&maxit, (void *)&res};
+    hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_refined), dim3(w->nblk), dim3(PCG_BLOCK), args, 0,
+                               s);
+}
+
+}  // namespace pic

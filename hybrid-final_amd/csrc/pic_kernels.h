@@ -1,0 +1,102 @@
+// Kernel launch interface shared by the host orchestration (ctx.hip) and the
+// kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pic_device.h"
+
+namespace pic {
+
+template <typename R>
+struct PView {
+    R *x, *y, *vx, *vy, *vz;
+    uint32_t *id;
+    int *cell_start;
+    int cap;
+};
+
+template <typename R>
+struct NodeConst {
+    int Gsize, ngx, ngy, nsp, CondNUMR;
+    R hdx, hdy, idx, idy, idxy, hxy, hyx;
+    R q_density[5];
+    const int *Boundary, *Face, *CondID, *A_idx;
+    const R *Area, *eps;
+};
+
+constexpr int MAX_SRC = 16;
+struct VoltConst {
+    int ncond, nsrc;
+    int flag[MAX_SRC], mid[MAX_SRC];
+    double dc[MAX_SRC], ac[MAX_SRC], w[MAX_SRC], ph[MAX_SRC];   // values already rounded to the path precision
+};
+
+template <typename R> void launch_push(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);
+template <typename R>
+void launch_sort_count(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *, int *, R *, hipStream_t);
+template <typename R>
+void launch_sort_scatter(const DevConst<R> &, const PView<R> &, const PView<R> &, const uint8_t *, const int *,
+                         const int *, int, int *, hipStream_t);
+template <typename R>
+void launch_mcc_collide(const DevConst<R> &, const PView<R> &, int, uint32_t, unsigned long long *, int *, int *, R *,
+                        hipStream_t);
+template <typename R>
+void launch_mcc_insert(const DevConst<R> &, const PView<R> &, const PView<R> &, const int *, const int *,
+                       const PView<R> &, const int *, const R *, int, const uint32_t *, int, int *, hipStream_t);
+template <typename R>
+void launch_fused_count(const DevConst<R> &, const PView<R> &, int, uint32_t, int *, int *, unsigned long long *, int *,
+                        R *, hipStream_t);
+template <typename R>
+void launch_fused_scatter(const DevConst<R> &, const PView<R> &, const PView<R> &, const PView<R> &, int, uint32_t,
+                          const int *, const int *, const int *, const int *, const uint32_t *, int, int, int *,
+                          hipStream_t);
+template <typename R>
+void launch_deposit(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *, R *, R *, R *, R *, R *,
+                    hipStream_t);
+template <typename R>
+void launch_efield(const NodeConst<R> &, const VoltConst &, double, const R *, R *, const R *, R *, const R *, R *, R *,
+                   R *, hipStream_t);
+template <typename R>
+void launch_accumulate(int, int, const R *, const R *, const R *, const R *, double *, double *, double *, double *,
+                       hipStream_t);
+void launch_totals(int, const int *, const int *, const int *, const int *, int *, int *, int *, hipStream_t);
+void launch_assign_bases(int, const int *, const int *, const int *, const int *, const int *, const int *, const int *,
+                         int *, int *, int *, hipStream_t);
+void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s);
+void launch_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total, hipStream_t s);
+void launch_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp, hipStream_t s);
+template <typename R> void launch_rhs_to_fp64(int N, const R *b, double *b64, hipStream_t s);
+template <typename R> void launch_x_to_phi(int Gsize, const int *A_idx, const double *x, R *phi, hipStream_t s);
+
+// ---- field solver (pcg.hip)
+struct PcgResult {
+    int iters;          // total inner iterations
+    int outer;          // refinement cycles (1 for the plain path)
+    double rel_res;     // final ||b - A x|| / ||b|| in FP64
+    int status;         // 0 converged, 1 hit the iteration limit
+};
+
+// Operator in padded 5-entry row form (missing entries have col = row, val = 0).
+struct PcgOperator {
+    int N;
+    const int *col;          // 5N
+    const float *valf;       // 5N
+    const double *vald;      // 5N
+    const float *mdiagf;     // N   Jacobi preconditioner 1/diag
+    const double *mdiagd;
+};
+
+struct PcgWork;   // device scratch owned by the context
+PcgWork *pcg_work_create(int N, int *err);
+void pcg_work_destroy(PcgWork *);
+int pcg_grid_blocks(PcgWork *);
+
+// x (FP64, warm start, in/out), b (FP64).  Writes the result to *res (device
+// memory) so the solve can run inside a captured graph.
+void pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, double tol, int maxit, PcgWork *w,
+                          PcgResult *res_dev, hipStream_t s);
+void pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol, int maxit,
+                       PcgWork *w, PcgResult *res_dev, hipStream_t s);
+
+}  // namespace pic

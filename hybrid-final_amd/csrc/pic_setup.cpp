@@ -1,0 +1,724 @@
+// Host-side case setup; see pic_setup.h for the reference lines restated.
+#include "pic_setup.h"
+
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "detmath.h"
+#include "json.h"
+#include "xsec_data.inc"
+
+namespace pic {
+
+namespace {
+constexpr double kEps0 = 8.8542e-12;
+constexpr double kCQ = 1.602e-19;
+constexpr double kAMU = 1.66053e-27;
+constexpr double kNperTorr = 8.3221e20;
+constexpr double kDblMin = 1e-200;
+constexpr double kPi = 3.14159265358979323846;
+
+const char *kArKeys[AR_NRCT] = {"0.e+Ar>e+Ar", "1.e+Ar>e+Ar*", "2.e+Ar>e+Ar*m", "3.e+Ar>2e+Ar^",
+                                "4.e+Ar*m>2e+Ar^", "5.Ar+Ar^>Ar^+Ar", "6.Ar+Ar^>Ar+Ar^",
+                                "7.e+Ar*m>e+Ar", "8.Ar*m+Ar>e+Ar^+Ar", "9.Ar+Ar*m>Ar+Ar"};
+}  // namespace
+
+int parse_deck(const std::string &text, Deck &d, std::string &err) {
+    JValue root;
+    try {
+        root = JParser(text).parse();
+    } catch (const std::exception &e) {
+        err = e.what();
+        return -2;
+    }
+    const JValue &geo = root.object("Geometry");
+    const JValue &ss = geo.object("SystemSpec");
+    d.xlength = ss.number("X_length(m)");
+    d.ylength = ss.number("Y_length(m)");
+    d.zlength = ss.number("Z_length(m)");
+    d.ngx = (int)ss.number("NumGridx");
+    d.ngy = (int)ss.number("NumGridy");
+    for (const JValue &b : geo.array("BoundaryCondition")) {
+        Line l;
+        l.x0 = (int)b.number("X0"); l.x1 = (int)b.number("X1");
+        l.y0 = (int)b.number("Y0"); l.y1 = (int)b.number("Y1");
+        l.bc = (int)b.number("B.C"); l.temp = b.number("Temp");
+        d.bnd.push_back(l);
+    }
+    for (const JValue &c : geo.array("ConductorSpec")) {
+        Line l;
+        l.id = (int)c.number("M_ID");
+        l.x0 = (int)c.number("X0"); l.x1 = (int)c.number("X1");
+        l.y0 = (int)c.number("Y0"); l.y1 = (int)c.number("Y1");
+        l.temp = c.number("Temp"); l.a = c.number("R(Ohm)"); l.b = c.number("L(H)"); l.c = c.number("C(F)");
+        d.cond.push_back(l);
+    }
+    for (const JValue &s : geo.array("Source")) {
+        Src x;
+        x.mid = (int)s.number("M_ID"); x.dc = s.number("DC(V)"); x.pow = s.number("Power(W)");
+        x.ac = s.number("AC(V)"); x.freq = s.number("Freq(1/s)"); x.phase = s.number("Phase(deg)");
+        d.src.push_back(x);
+    }
+    for (const JValue &e : geo.array("DielectricSpec")) {
+        Line l;
+        l.id = (int)e.number("M_ID");
+        l.x0 = (int)e.number("X0"); l.x1 = (int)e.number("X1");
+        l.y0 = (int)e.number("Y0"); l.y1 = (int)e.number("Y1");
+        l.a = e.number("Epsilon");
+        d.diel.push_back(l);
+    }
+    const JValue &gas = root.object("GasSpecies");
+    d.gas = (int)gas.number("Type(0:Ar,1:O2,2:Ar/O2)");
+    d.total_pres = gas.number("TotalPres(Torr)");
+    const JValue &bg = gas.object("Background");
+    d.bg_ratio[0] = bg.object("Argon").number("Ratio");
+    d.bg_temp[0] = bg.object("Argon").number("Temp(eV)");
+    d.bg_ratio[1] = bg.object("Oxygen").number("Ratio");
+    d.bg_temp[1] = bg.object("Oxygen").number("Temp(eV)");
+    const JValue &ns = gas.object("NeutralSpecies");
+    d.fl_loadtype = (int)ns.number("Loadtype");
+    std::vector<const char *> fln, spn;
+    if (d.gas == 0) { fln = {"Ar*"}; spn = {"Electron", "Ar+"}; }
+    else if (d.gas == 1) { fln = {"O2A", "O2B", "OP", "OD"}; spn = {"Electron", "O2+", "O+", "O-"}; }
+    else if (d.gas == 2) { fln = {"Ar*", "O2A", "O2B", "OP", "OD"}; spn = {"Electron", "Ar+", "O2+", "O+", "O-"}; }
+    for (size_t k = 0; k < fln.size(); ++k) {
+        const JValue &o = ns.object(fln[k]);
+        d.fl_dens[k] = o.number("Density"); d.fl_temp[k] = o.number("Temp(eV)"); d.fl_quench[k] = o.number("Quenching");
+    }
+    const JValue &cs = gas.object("ChargeSpecies");
+    d.sp_loadtype = (int)cs.number("Loadtype");
+    const JValue &lp = cs.object("LoadPosition(m)");
+    d.sp_x0 = lp.number("X0"); d.sp_x1 = lp.number("X1"); d.sp_y0 = lp.number("Y0"); d.sp_y1 = lp.number("Y1");
+    for (size_t k = 0; k < spn.size(); ++k) {
+        const JValue &o = cs.object(spn[k]);
+        d.sp_dens[k] = o.number("Density"); d.sp_temp[k] = o.number("Temp(eV)"); d.sp_np2c[k] = o.number("np2c");
+        d.sp_ratio[k] = (int)o.number("Ratio"); d.sp_maxnp[k] = (int)o.number("Max_np");
+    }
+    const JValue &sm = root.object("SimulationMethod");
+    d.dt_pic = (int)sm.number("TimeStep_PIC");
+    d.pcg_tol = sm.number("PCGMarginOfError");
+    d.n_ave = (int)sm.number("AverageIter");
+    d.n_smt = (int)sm.number("DensitySmoothing");
+    d.np_limit = (int)sm.number("ParticleLimit");
+    d.seed = (int64_t)sm.number("LoaderSeed");
+    if (d.seed == 0) d.seed = 1;
+    d.rng_seed = (int64_t)sm.number("PhiloxSeed");
+    d.rf_cycles = (int)sm.number("RFCycles");
+    if (d.gas == 0) {
+        const JValue &ac = root.object("Collision_On_Off").object("ArgonCase");
+        for (int r = 0; r < AR_NRCT; ++r) d.coll_flag[r] = ac.number(kArKeys[r]);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- BandLU
+template <typename R>
+int BandLU::factor(int n, const std::vector<int> &Ai, const std::vector<int> &Aj, const std::vector<R> &val) {
+    N = n;
+    bw = 0;
+    for (int r = 0; r < n; ++r)
+        for (int k = Ai[r]; k < Ai[r + 1]; ++k) {
+            int dc = std::abs(Aj[k] - r);
+            if (dc > bw) bw = dc;
+        }
+    const int W = 2 * bw + 1;
+    a.assign((size_t)n * W, 0.0);
+    auto at = [&](int rr, int cc) -> double & { return a[(size_t)rr * W + (cc - rr + bw)]; };
+    for (int r = 0; r < n; ++r)
+        for (int k = Ai[r]; k < Ai[r + 1]; ++k) at(r, Aj[k]) += (double)val[k];
+    for (int k = 0; k < n; ++k) {
+        double piv = at(k, k);
+        int rmax = k + bw < n - 1 ? k + bw : n - 1;
+        for (int i2 = k + 1; i2 <= rmax; ++i2) {
+            double f = at(i2, k);
+            if (f == 0.0) continue;
+            f /= piv;
+            at(i2, k) = f;
+            for (int j2 = k + 1; j2 <= rmax; ++j2) at(i2, j2) -= f * at(k, j2);
+        }
+    }
+    return 0;
+}
+
+void BandLU::solve(const double *b, double *x) const {
+    const int W = 2 * bw + 1;
+    auto at = [&](int rr, int cc) { return a[(size_t)rr * W + (cc - rr + bw)]; };
+    for (int r = 0; r < N; ++r) {
+        double s = b[r];
+        int k0 = r - bw > 0 ? r - bw : 0;
+        for (int k = k0; k < r; ++k) s -= at(r, k) * x[k];
+        x[r] = s;
+    }
+    for (int r = N - 1; r >= 0; --r) {
+        double s = x[r];
+        int k1 = r + bw < N - 1 ? r + bw : N - 1;
+        for (int k = r + 1; k <= k1; ++k) s -= at(r, k) * x[k];
+        x[r] = s / at(r, r);
+    }
+}
+
+namespace {
+
+template <typename R>
+constexpr bool is_f32() { return std::is_same<R, float>::value; }
+
+// ------------------------------------------------ constants (start.c, Efield.c)
+template <typename R>
+int constants(const Deck &d, Setup<R> &s, std::string &err) {
+    s.ngx = d.ngx; s.ngy = d.ngy;
+    if (d.bnd.size() < 4) { err = "\"BoundaryCondition\" is too small!(BoundaryNum>=4)"; return -10; }
+    if (d.cond.size() < 2) { err = "\"Conductor\" is too small!(CondNum>=2)"; return -11; }
+    if (d.cond.size() > 100) { err = "\"Conductor\" is too large!(CondNum<100)"; return -11; }
+    if (d.ngx < 2 || d.ngy < 2) { err = "NumGridx/NumGridy must be >= 2"; return -12; }
+    for (auto &b : d.bnd)
+        if (b.x0 > b.x1 || b.x1 >= d.ngx || b.y0 > b.y1 || b.y1 >= d.ngy || b.bc > 4) {
+            err = "bad line in \"BoundaryCondition\""; return -12;
+        }
+    {
+        std::vector<int> ids;
+        for (auto &c : d.cond) {
+            if (c.id == 0) { err = "Error : \"M_ID\" > 0"; return -13; }
+            if (ids.empty()) {
+                if (c.id != 1) { err = "Error : First \"M_ID\" must be 1"; return -13; }
+                ids.push_back(c.id);
+                continue;
+            }
+            if (ids.back() > c.id) { err = "Error : \"M_ID\" must be written in order."; return -13; }
+            bool dup = false;
+            for (int m : ids) dup |= (m == c.id);
+            if (!dup) ids.push_back(c.id);
+        }
+        s.CondNUMR = (int)ids.size();
+    }
+    for (auto &c : d.cond)
+        if (c.x0 > c.x1 || c.x1 >= d.ngx || c.y0 > c.y1 || c.y1 >= d.ngy) { err = "bad line in \"ConductorSpec\""; return -14; }
+    float max_freq = 0.0f;
+    double max_freq_d = 0.0;
+    for (auto &x : d.src) {
+        bool ok = false;
+        for (auto &c : d.cond) ok |= (c.id == x.mid);
+        if (!ok) { err = "Source is no matching Conductor"; return -15; }
+        if ((float)x.freq != 0.0f && (float)x.freq > max_freq) max_freq = (float)x.freq;
+        if (x.freq > max_freq_d) max_freq_d = x.freq;
+        if ((float)x.dc == 0.0f && (float)x.pow == 0.0f && (float)x.ac == 0.0f) { err = "Source is error(delete Line)"; return -15; }
+    }
+    if (max_freq <= 0.0f) { err = "no source frequency"; return -16; }
+    if (d.dt_pic == 0) { err = "ERROR! - \"TimeStep_PIC\" = 0"; return -17; }
+    if (d.gas < 0 || d.gas > 2) { err = "\"Type\" is error in GasSpecies."; return -18; }
+    if (d.gas != 0) { err = "O2 and Ar/O2 chemistry is not available in this build"; return -19; }
+
+    s.Gsize = s.ngx * s.ngy;
+    s.ncx = s.ngx - 1; s.ncy = s.ngy - 1;
+    s.Csize = s.ncx * s.ncy;
+    R xl = (R)d.xlength, yl = (R)d.ylength;
+    s.zl = (R)d.zlength;
+    s.dx = xl / s.ncx;
+    s.dy = yl / s.ncy;
+    s.idx = (R)1 / s.dx;
+    s.idy = (R)1 / s.dy;
+    s.idxy = (R)(1.5 / (double)(s.dx + s.dy));
+    s.hdx = (R)(0.5 / (double)s.dx);
+    s.hdy = (R)(0.5 / (double)s.dy);
+    if (is_f32<R>()) s.dt = (R)(1.0 / (double)max_freq / (double)(float)d.dt_pic);
+    else s.dt = (R)(1.0 / max_freq_d / (double)d.dt_pic);
+    s.dt_dx = s.dt / s.dx;
+    s.dt_dy = s.dt / s.dy;
+
+    s.nsp = 2; s.nfsp = 1;
+    s.bg_mass[0] = (R)(39.950 * kAMU);
+    s.bg_pres[0] = (R)d.total_pres;
+    s.bg_dens[0] = (R)(kNperTorr * (double)s.bg_pres[0] / ((double)(R)d.bg_temp[0] + kDblMin));
+    s.fl_mass[0] = (R)(39.950 * kAMU);
+    s.fl_dens[0] = (R)d.fl_dens[0];
+    s.mass[0] = (R)(5.485799486e-4 * kAMU); s.q[0] = (R)(-1.0 * kCQ);
+    s.mass[1] = (R)(39.9500 * kAMU);        s.q[1] = (R)(1.0 * kCQ);
+    for (int i = 0; i < s.nsp; ++i) {
+        s.np2c[i] = (R)d.sp_np2c[i];
+        R temp = (R)d.sp_temp[i];
+        s.q_density[i] = s.q[i] * s.np2c[i];
+        s.vti[i] = (R)std::sqrt(kCQ * (double)temp / (double)s.mass[i]);
+        s.qm[i] = s.q[i] / s.mass[i];
+        s.Escale[i] = (R)(0.5 * (double)s.mass[i] / kCQ);
+        s.Ascale[i] = s.qm[i] * s.dt;
+        s.Denscale[i] = s.np2c[i] / s.dx / s.dy;
+    }
+    s.Efield_Flag.assign(s.CondNUMR, 0);
+    for (int i = 0; i < s.CondNUMR; ++i) {
+        int cnt = 0;
+        for (auto &x : d.src) cnt += (x.mid == i + 1);
+        s.Efield_Flag[i] = (cnt > 0 && cnt < 4) ? 1 : 0;
+    }
+    return 0;
+}
+
+// ------------------------------------------------ geometry (start.c:1464-1851)
+template <typename R>
+void geometry(const Deck &d, Setup<R> &s) {
+    const int ngx = s.ngx, ngy = s.ngy, ncx = s.ncx, ncy = s.ncy;
+    const R dx = s.dx, dy = s.dy, zl = s.zl;
+    s.Boundary.assign(s.Gsize, 0); s.CondID.assign(s.Gsize, 0);
+    s.Face.assign(s.Gsize, F_NOFACE); s.DensRegion.assign(s.Gsize, 0);
+    s.Area.assign(s.Gsize, dy * zl); s.Temp.assign(s.Gsize, 0);
+    s.PlasmaRegion.assign(s.Csize, 1); s.eps_r.assign(s.Csize, 1); s.eps.assign(s.Csize, 0);
+    const R corner_area = (R)(0.5 * (double)(dx + dy) * (double)zl);
+    for (int j = 0; j < ngy; ++j)
+        for (int i = 0; i < ngx; ++i) {
+            int ID = i * ngy + j;
+            for (auto &b : d.bnd) {
+                if (i >= b.x0 && i <= b.x1 && j >= b.y0 && j <= b.y1) {
+                    if (b.bc == DIRICHLET) { s.Boundary[ID] = DIRICHLET; s.Temp[ID] = (R)b.temp; }
+                    else if (s.Boundary[ID] != DIRICHLET && b.bc == NEUMANN) { s.Boundary[ID] = NEUMANN; s.Temp[ID] = (R)b.temp; }
+                }
+            }
+        }
+    for (auto &e : d.diel)
+        for (int i = e.x0; i <= e.x1; ++i)
+            for (int j = e.y0; j <= e.y1; ++j) {
+                int ID = i * ngy + j;
+                if (s.Boundary[ID] == 0) s.Boundary[ID] = DIELECTRIC;
+                if (i < e.x1 && j < e.y1) { s.PlasmaRegion[i * ncy + j] = 0; s.eps_r[i * ncy + j] = (R)e.a; }
+            }
+    for (auto &c : d.cond)
+        for (int i = c.x0; i <= c.x1; ++i)
+            for (int j = c.y0; j <= c.y1; ++j) {
+                int ID = i * ngy + j;
+                s.CondID[ID] = c.id;
+                if (s.Boundary[ID] == 0 || s.Boundary[ID] == DIELECTRIC) { s.Boundary[ID] = CONDUCTOR; s.Temp[ID] = (R)c.temp; }
+                if (i < c.x1 && j < c.y1) { s.PlasmaRegion[i * ncy + j] = 0; s.eps_r[i * ncy + j] = 0; }
+            }
+    for (int i = 0; i < ncx; ++i)
+        for (int j = 0; j < ncy; ++j)
+            if (s.PlasmaRegion[i * ncy + j] == 1) {
+                int ID = i * ngy + j;
+                s.DensRegion[ID] = s.DensRegion[ID + 1] = s.DensRegion[ID + ngy] = s.DensRegion[ID + ngy + 1] = 1;
+            }
+    auto face_area = [&](int f) -> R {
+        if (f == F_LEFT || f == F_RIGHT) return dy * zl;
+        if (f == F_UP || f == F_DOWN) return dx * zl;
+        return corner_area;
+    };
+    for (int i = 1; i < ngx - 1; ++i)
+        for (int j = 1; j < ngy - 1; ++j) {
+            int ID = i * ngy + j, CID = i * ncy + j;
+            int P1 = (int)s.eps_r[CID], P2 = (int)s.eps_r[CID - ncy];
+            int P3 = (int)s.eps_r[CID - 1], P4 = (int)s.eps_r[CID - ncy - 1];
+            int b = s.Boundary[ID];
+            if (b != DIELECTRIC && b != CONDUCTOR) continue;
+            int f = -1;
+            if ((P2 == 1 && P1 == 1 && P4 == 1) || (P2 != 0 && P1 != 0 && P4 != 0 && P3 == 0)) f = F_UL;
+            else if ((P2 == 1 && P1 == 1 && P3 == 1) || (P2 != 0 && P1 != 0 && P3 != 0 && P4 == 0)) f = F_UR;
+            else if ((P2 == 1 && P1 == 1 && P4 != 1 && P3 != 1) || (P2 != 0 && P1 != 0 && P4 == 0 && P3 == 0)) f = F_UP;
+            else if ((P4 == 1 && P3 == 1 && P2 == 1) || (P2 != 0 && P4 != 0 && P3 != 0 && P1 == 0)) f = F_LL;
+            else if ((P4 == 1 && P3 == 1 && P1 == 1) || (P1 != 0 && P4 != 0 && P3 != 0 && P2 == 0)) f = F_LR;
+            else if ((P4 == 1 && P3 == 1 && P1 != 1 && P2 != 1) || (P4 != 0 && P3 != 0 && P1 == 0 && P2 == 0)) f = F_DOWN;
+            else if ((P4 == 1 && P2 == 1 && P1 != 1 && P3 != 1) || (P4 != 0 && P2 != 0 && P1 == 0 && P3 == 0)) f = F_LEFT;
+            else if ((P4 != 1 && P2 != 1 && P1 == 1 && P3 == 1) || (P4 == 0 && P2 == 0 && P1 != 0 && P3 != 0)) f = F_RIGHT;
+            if (f < 0) continue;
+            s.Face[ID] = f;
+            s.Area[ID] = face_area(f);
+        }
+    // wall nodes; a CondID lookup past either end of the node array reads 0
+    auto cid = [&](int n) { return (n >= 0 && n < s.Gsize) ? s.CondID[n] : 0; };
+    for (int j = 1; j < ncy; ++j)
+        for (int side = 0; side < 2; ++side) {
+            int ID = side ? ncx * ngy + j : j;
+            int in = side ? ID - ngy : ngy + ID;
+            int up = cid(ID + 1), dn = cid(ID - 1), inw = cid(in);
+            int wallface = side ? F_LEFT : F_RIGHT;
+            if (s.Boundary[ID] == DIRICHLET) {
+                s.Face[ID] = wallface; s.Area[ID] = dy * zl;
+                if (up == 0 && dn != 0 && inw != 0) { s.Face[ID] = F_UP; s.Area[ID] = dx * zl; }
+                else if (up != 0 && dn == 0 && inw != 0) { s.Face[ID] = F_DOWN; s.Area[ID] = dx * zl; }
+                else if (up != 0 && dn != 0 && inw != 0) { s.Face[ID] = F_NOFACE; s.Area[ID] = dy * zl; }
+                else if (up == 0 && dn == 0 && inw != 0) { s.Face[ID] = F_NOFACE; s.Area[ID] = dy * zl; }
+            } else if (s.Boundary[ID] == NEUMANN) {
+                s.Face[ID] = F_NOFACE; s.Area[ID] = dy * zl;
+                if (up == 0 && dn != 0 && inw != 0) { s.Face[ID] = F_UP; s.Area[ID] = dx * zl; }
+                else if (up != 0 && dn == 0 && inw != 0) { s.Face[ID] = F_DOWN; s.Area[ID] = dx * zl; }
+                else if (up != 0 && dn != 0 && inw == 0) { s.Face[ID] = wallface; s.Area[ID] = dy * zl; }
+            }
+        }
+    for (int i = 1; i < ncx; ++i)
+        for (int side = 0; side < 2; ++side) {
+            int ID = side ? i * ngy + ncy : i * ngy;
+            int in = side ? ID - 1 : ID + 1;
+            int rt = cid(ID + ngy), lf = cid(ID - ngy), inw = cid(in);
+            int wallface = side ? F_DOWN : F_UP;
+            if (s.Boundary[ID] == DIRICHLET) {
+                s.Face[ID] = wallface; s.Area[ID] = dx * zl;
+                if (rt == 0 && lf != 0 && inw != 0) { s.Face[ID] = F_RIGHT; s.Area[ID] = dy * zl; }
+                else if (rt != 0 && lf == 0 && inw != 0) { s.Face[ID] = F_LEFT; s.Area[ID] = dy * zl; }
+                else if (rt != 0 && lf != 0 && inw != 0) { s.Face[ID] = F_NOFACE; s.Area[ID] = dy * zl; }
+                else if (rt == 0 && lf == 0 && inw != 0) { s.Face[ID] = F_NOFACE; s.Area[ID] = dy * zl; }
+            } else if (s.Boundary[ID] == NEUMANN) {
+                s.Face[ID] = F_NOFACE; s.Area[ID] = dy * zl;
+                if (rt == 0 && lf != 0 && inw != 0) { s.Face[ID] = F_RIGHT; s.Area[ID] = dy * zl; }
+                else if (rt != 0 && lf == 0 && inw != 0) { s.Face[ID] = F_LEFT; s.Area[ID] = dy * zl; }
+                else if (rt != 0 && lf != 0 && inw == 0) { s.Face[ID] = wallface; s.Area[ID] = dx * zl; }
+            }
+        }
+    {
+        const int corners[4] = {0, ncx * ngy, ncy, ncx * ngy + ncy};
+        const int faces[4] = {F_UR, F_UL, F_LR, F_LL};
+        for (int k = 0; k < 4; ++k) {
+            int ID = corners[k];
+            if (s.Boundary[ID] == DIRICHLET && cid(ID + 1) == 0) { s.Face[ID] = faces[k]; s.Area[ID] = corner_area; }
+        }
+    }
+    // padded structure map: 0 plasma, 1..99 dielectric, >=100 conductor, <0 walls
+    const int W = ncy + 2;
+    s.StructIdx.assign((size_t)(ncx + 2) * W, 0);
+    auto SI = [&](int a, int b) -> int & { return s.StructIdx[(size_t)a * W + b]; };
+    for (int i = 0; i < ncx; ++i)
+        for (int j = 0; j < ncy; ++j) {
+            SI(i + 1, j + 1) = 0;
+            for (auto &e : d.diel)
+                if (i >= e.x0 && i < e.x1 && j >= e.y0 && j < e.y1) SI(i + 1, j + 1) = e.id;
+            for (auto &c : d.cond)
+                if (i >= c.x0 && i < c.x1 && j >= c.y0 && j < c.y1) SI(i + 1, j + 1) = 100 + c.id;
+        }
+    const auto &B = s.Boundary;
+    for (int i = 1; i < ngx; ++i) {
+        if ((B[(i - 1) * ngy] == NEUMANN || B[i * ngy] == NEUMANN) && SI(i, 1) == 0) SI(i, 0) = -B[(i - 1) * ngy] - 2;
+        else SI(i, 0) = SI(i, 1) ? SI(i, 1) : -1;
+        if ((B[(i - 1) * ngy + ncy] == NEUMANN || B[i * ngy + ncy] == NEUMANN) && SI(i, ngy - 1) == 0)
+            SI(i, ngy) = -B[(i - 1) * ngy + ncy] - 6;
+        else SI(i, ngy) = SI(i, ngy - 1) ? SI(i, ngy - 1) : -1;
+    }
+    for (int j = 1; j < ngy; ++j) {
+        if ((B[j - 1] == NEUMANN || B[j] == NEUMANN) && SI(1, j) == 0) SI(0, j) = -B[j - 1];
+        else SI(0, j) = SI(1, j) ? SI(1, j) : -1;
+        if ((B[ncx * ngy + j - 1] == NEUMANN || B[ncx * ngy + j] == NEUMANN) && SI(ngx - 1, j) == 0)
+            SI(ngx, j) = -B[ncx * ngy + j] - 4;
+        else SI(ngx, j) = SI(ngx - 1, j) ? SI(ngx - 1, j) : -1;
+    }
+    SI(0, 0) = -B[0];
+    SI(ncx + 1, 0) = -B[(ncx - 1) * ngy];
+    SI(0, ncy + 1) = -B[ngy + ncy];
+    SI(ncx + 1, ncy + 1) = -B[(ncy - 1) * ngy + ncy];
+    for (int i = 0; i < s.Csize; ++i) s.eps[i] = (R)(kEps0 * (double)s.eps_r[i]);
+}
+
+// ------------------------------------------------ field operator (load.c:121-245)
+template <typename R>
+void field_matrix(Setup<R> &s) {
+    const int ngx = s.ngx, ngy = s.ngy, ncx = s.ncx, ncy = s.ncy;
+    s.A_idx.assign(s.Gsize, 0);
+    int n = 1;
+    for (int i = 0; i < ngx; ++i)
+        for (int j = 0; j < ngy; ++j) {
+            int g = i * ngy + j;
+            if (!s.CondID[g] && s.Boundary[g] != DIRICHLET) s.A_idx[g] = n++;
+        }
+    const int N = n - 1;
+    s.A_size = N;
+    s.Ai.assign(N + 1, 0);
+    s.Aj.clear(); s.Aval.clear(); s.Atemp.clear();
+    s.Mdiag.assign(N, 0);
+    s.cond_b.assign((size_t)s.CondNUMR * N, 0);
+    s.temp_b.assign(N, 0);
+    const R dydx = (R)(0.5 * (double)s.dy / (double)s.dx);
+    const R dxdy = (R)(0.5 * (double)s.dx / (double)s.dy);
+    const auto &er = s.eps_r;
+    int l = 0;
+    for (int i = 0; i < ngx; ++i)
+        for (int j = 0; j < ngy; ++j) {
+            int g = i * ngy + j;
+            if (!s.A_idx[g]) continue;
+            int x1 = (i == 0) ? i : i - 1, y1 = (j == ncy) ? j - 1 : j, y2 = (j == 0) ? j : j - 1;
+            R L = dydx * (er[x1 * ncy + y1] + er[x1 * ncy + y2]);
+            x1 = (i == ncx) ? i - 1 : i;
+            R Rc = dydx * (er[x1 * ncy + y1] + er[x1 * ncy + y2]);
+            int x2 = (i == 0) ? i : i - 1;
+            y1 = (j == ncy) ? j - 1 : j;
+            R U = dxdy * (er[x1 * ncy + y1] + er[x2 * ncy + y1]);
+            y1 = (j == 0) ? j : j - 1;
+            R D = dxdy * (er[x1 * ncy + y1] + er[x2 * ncy + y1]);
+            const int LN = (i == 0) ? 2 : 1, RN = (i == ncx) ? 2 : 1, UN = (j == ncy) ? 2 : 1, DN = (j == 0) ? 2 : 1;
+            auto neighbour = [&](int nb, R coef, R aval, R atemp) {
+                if (s.CondID[nb]) {
+                    s.cond_b[(size_t)(s.CondID[nb] - 1) * N + l] += coef;
+                    s.temp_b[l] += coef * s.Temp[nb];
+                } else if (s.Boundary[nb] == DIRICHLET) {
+                    s.temp_b[l] += coef * s.Temp[nb];
+                } else {
+                    s.Aval.push_back(aval); s.Atemp.push_back(atemp); s.Aj.push_back(s.A_idx[nb] - 1);
+                }
+            };
+            if (i) neighbour((i - 1) * ngy + j, L, (-L) * RN, (-L) * RN);
+            if (j) neighbour(i * ngy + j - 1, D, (-D) * UN, (-D));
+            s.Aval.push_back(L + D + U + Rc);
+            s.Mdiag[l] = (R)1 / s.Aval.back();
+            s.Atemp.push_back(L + D + U + Rc);
+            s.Aj.push_back(s.A_idx[g] - 1);
+            if (j != ncy) neighbour(i * ngy + j + 1, U, (-U) * DN, (-U));
+            if (i != ncx) neighbour((i + 1) * ngy + j, Rc, (-Rc) * LN, (-Rc));
+            ++l;
+            s.Ai[l] = (int)s.Aval.size();
+        }
+}
+
+// ------------------------------------------------ Laplace + gas fields
+template <typename R>
+int presolves(Setup<R> &s, std::string &err) {
+    const int N = s.A_size;
+    if (N <= 0) { err = "field operator has no unknowns"; return -20; }
+    BandLU lu;
+    lu.factor(N, s.Ai, s.Aj, s.Aval);
+    std::vector<double> b(N), x(N);
+    s.lap_phi.assign((size_t)s.CondNUMR * s.Gsize, 0);
+    for (int k = 0; k < s.CondNUMR; ++k) {
+        for (int r = 0; r < N; ++r) b[r] = (double)s.cond_b[(size_t)k * N + r];
+        lu.solve(b.data(), x.data());
+        R *phi = s.lap_phi.data() + (size_t)k * s.Gsize;
+        for (int g = 0; g < s.Gsize; ++g) {
+            if (s.A_idx[g]) phi[g] = (R)x[s.A_idx[g] - 1];
+            else if (s.CondID[g] == k + 1) phi[g] = 1;
+        }
+    }
+    BandLU lut;
+    lut.factor(N, s.Ai, s.Aj, s.Atemp);
+    for (int r = 0; r < N; ++r) b[r] = (double)s.temp_b[r];
+    lut.solve(b.data(), x.data());
+    for (int g = 0; g < s.Gsize; ++g)
+        if (s.A_idx[g]) s.Temp[g] = (R)x[s.A_idx[g] - 1];
+    s.BackDen1.assign(s.Gsize, 0);
+    s.BackVel1.assign(s.Gsize, 0);
+    for (int g = 0; g < s.Gsize; ++g) {
+        if (!s.DensRegion[g]) continue;
+        double T = (double)s.Temp[g];
+        s.BackDen1[g] = (R)(kNperTorr * (double)s.bg_pres[0] / (T * 8.6142e-5 + kDblMin));
+        s.BackVel1[g] = (R)std::sqrt(T * 1.38e-23 / (double)s.bg_mass[0]);
+    }
+    // metastable density at nodes (src/Fluid.c:3-68, src/cuda_Fluid.cu:321-339)
+    s.FluidN.assign((size_t)s.nfsp * s.Gsize, 0);
+    for (int f = 0; f < s.nfsp; ++f)
+        for (int i = 0; i < s.ncx; ++i)
+            for (int j = 0; j < s.ncy; ++j) {
+                R den = 0;
+                if (s.PlasmaRegion[i * s.ncy + j] != 0 && s.deck.fl_loadtype == 0) den = s.fl_dens[f];
+                s.FluidN[(size_t)f * s.Gsize + i * s.ngy + j] = den;
+                if (i == s.ncx - 1 || j == s.ncy - 1) s.FluidN[(size_t)f * s.Gsize + (i + 1) * s.ngy + j + 1] = den;
+            }
+    return 0;
+}
+
+// ------------------------------------------------ cross sections (PhysicsData.c)
+template <typename R>
+R func_decay(const float *mx, const float *my, int n, R in_x) {
+    if (n > 1) {
+        if (in_x < mx[0]) return my[0];
+        if (in_x > mx[n - 1]) {
+            if (is_f32<R>()) {
+                float inc = (float)(std::log10((double)(my[n - 2] / my[n - 1])) / std::log10((double)(mx[n - 1] / mx[n - 2])));
+                return (R)((double)my[n - 1] * std::pow((double)mx[n - 1], (double)inc) / std::pow((double)in_x, (double)inc));
+            }
+            double inc = std::log10((double)my[n - 2] / (double)my[n - 1]) / std::log10((double)mx[n - 1] / (double)mx[n - 2]);
+            return (R)((double)my[n - 1] * std::pow((double)mx[n - 1], inc) / std::pow((double)in_x, inc));
+        }
+        int lo = 0, hi = n - 1;
+        while (hi - lo > 1) {
+            int k = (hi + lo) >> 1;
+            if (mx[k] > in_x) hi = k; else lo = k;
+        }
+        R h = (R)mx[hi] - (R)mx[lo];
+        R a = ((R)mx[hi] - in_x) / h;
+        R b = (in_x - (R)mx[lo]) / h;
+        return a * (R)my[lo] + b * (R)my[hi];
+    }
+    return my[0];
+}
+
+template <typename R>
+void argon_tables(Setup<R> &s) {
+    s.TnRct = AR_NRCT;
+    s.idLOGX = is_f32<R>() ? (R)((R)1 / (R)0.1f) : (R)10.0;
+    s.xe.assign(N_LOGX, 0); s.xee.assign(N_LOGX, 0); s.cx.assign(N_LOGX * AR_NRCT, 0);
+    for (int i = 0; i < N_LOGX; ++i) {
+        if (is_f32<R>()) s.xe[i] = (R)(0.1f * (float)(i - 40));
+        else s.xe[i] = (R)(0.1 * (double)(i - 40));
+        s.xee[i] = (R)std::pow(10.0, (double)s.xe[i]);
+    }
+    const int set[7] = {1, 2, 3, 4, 5, 7, 9};
+    const int sub[7] = {-1, 3, -1, -1, -1, -1, -1};
+    const float th[7] = {0.0f, 11.55f, 11.548f, 15.76f, 4.22f, 0.0f, 0.0f};
+    for (int r = 0; r < AR_NRCT; ++r) { s.Flag[r] = (R)s.deck.coll_flag[r]; if (s.Flag[r] < 0) s.Flag[r] = 0; }
+    for (int r = 0; r < 7; ++r) {
+        s.Th_e[r] = (R)th[r];
+        for (int i = 0; i < N_LOGX; ++i) {
+            R v = 0;
+            if (!(s.xee[i] < s.Th_e[r])) {
+                v = s.Flag[r] * func_decay<R>(pic_xsec_x[set[r]], pic_xsec_y[set[r]], pic_xsec_n[set[r]], s.xee[i]);
+                if (sub[r] >= 0)
+                    v = v - s.Flag[r] * func_decay<R>(pic_xsec_x[sub[r]], pic_xsec_y[sub[r]], pic_xsec_n[sub[r]], s.xee[i]);
+            }
+            s.cx[i * AR_NRCT + r] = v;
+        }
+    }
+    R me = s.mass[0], mi = s.mass[1], mAr = s.bg_mass[0], mAs = s.fl_mass[0];
+    s.mofM[0] = s.mofM[1] = s.mofM[2] = s.mofM[3] = me / mAr;
+    s.mofM[4] = me / mAs; s.mofM[5] = s.mofM[6] = mAr / mi; s.mofM[7] = me / mAs;
+    s.mofM[8] = mAs / mAr; s.mofM[9] = mAr / mAs;
+    s.RR[7] = s.Flag[7] * (R)2.0e-13; s.RR[8] = s.Flag[8] * (R)6.2e-16; s.RR[9] = s.Flag[9] * (R)3.0e-21;
+}
+
+template <typename R>
+void null_collision(Setup<R> &s) {
+    R sv[3] = {0, 0, 0};
+    for (int i = 0; i < N_LOGX; ++i) {
+        const R *row = s.cx.data() + i * AR_NRCT;
+        double e = (double)s.xee[i];
+        R s0 = row[0] + row[1] + row[2] + row[3];
+        R s1 = row[4];
+        R s2 = row[5] + row[6];
+        R v0 = (R)(std::sqrt(2 * 1.602e-19 * e / (double)s.mass[0]) * (double)s0);
+        R v1 = (R)(std::sqrt(2 * 1.602e-19 * e / (double)s.mass[0]) * (double)s1);
+        R v2 = (R)(std::sqrt(2 * 1.602e-19 * e / (double)s.mass[1]) * (double)s2);
+        if (v0 > sv[0]) sv[0] = v0;
+        if (v1 > sv[1]) sv[1] = v1;
+        if (v2 > sv[2]) sv[2] = v2;
+    }
+    s.sigv[0] = sv[0]; s.sigv[1] = sv[1]; s.sigv[2] = sv[2];
+    double t0 = 1.0 - std::exp(-1 * (double)s.dt * (double)sv[0] * (double)s.bg_dens[0]);
+    double t1 = 1.0 - std::exp(-1 * (double)s.dt * (double)sv[1] * (double)s.fl_dens[0]);
+    R colprob = (R)t0 + (R)t1;
+    if (colprob > (R)0.08) {
+        s.MCCn = (int)std::ceil((double)(colprob / (R)0.08));
+        R ratio = (R)1 / (R)s.MCCn;
+        s.dt_mcc = s.dt * ratio;
+    } else {
+        s.MCCn = 1;
+        s.dt_mcc = s.dt;
+    }
+    s.P1.assign(s.Gsize, 0); s.P2.assign(s.Gsize, 0); s.Pion.assign(s.Gsize, 0);
+    for (int g = 0; g < s.Gsize; ++g) {
+        double dtm = (double)s.dt_mcc, dtd = (double)s.dt;
+        R p1 = (R)(1.0 - dm_exp(-dtm * (double)s.sigv[0] * (double)s.BackDen1[g]));
+        R p2b = (R)(1.0 - dm_exp(-dtm * (double)s.sigv[1] * (double)s.FluidN[g]));
+        s.P1[g] = p1;
+        s.P2[g] = p1 + p2b;
+        s.Pion[g] = (R)(1.0 - dm_exp(-dtd * (double)s.sigv[2] * (double)s.BackDen1[g]));
+    }
+}
+
+// ------------------------------------------------ Maxwellian table + loader
+struct Lcg {
+    long seed;
+    float next() {   // Park-Miller minimal standard, Schrage's method
+        long a = 16807, m = 2147483647, q = 127773, r = 2836;
+        long hi = seed / q, lo = seed - q * hi;
+        seed = a * lo - r * hi;
+        if (seed <= 0) seed += m;
+        return (float)(seed / 2147483646.0);
+    }
+};
+
+inline float maxwell_cdf(float v) {
+    return (float)(-2 * (double)v * std::exp(-(double)v * (double)v) / std::sqrt(kPi) + std::erf((double)v));
+}
+
+template <typename R>
+void maxwell_table(Setup<R> &s) {
+    int nvel = (int)(1. / (1 - (double)maxwell_cdf(3.3f)));
+    float dv = (float)(std::sqrt(kPi) / (4.0 * nvel));
+    s.nvel = nvel;
+    s.vsave.assign(nvel, 0);
+    int i = 0;
+    for (int n = 0; n < nvel; ++n) {
+        double rr = (1.0 * n) / nvel;
+        while ((double)maxwell_cdf((float)i * dv) < rr) ++i;
+        s.vsave[n] = (R)(float)(std::sqrt(2.0) * i * (double)dv);
+    }
+}
+
+template <typename R>
+int load(Setup<R> &s, std::string &err) {
+    const Deck &d = s.deck;
+    if (d.sp_loadtype != 0) { err = "only the UNIFORM particle load is available"; return -21; }
+    Lcg g{(long)d.seed};
+    for (int isp = 0; isp < s.nsp; ++isp) {
+        SpeciesInit<R> &sp = s.sp[isp];
+        float dxf = (float)s.dx, dyf = (float)s.dy, zl = (float)s.zl;
+        float ninit = (float)d.sp_dens[isp];
+        int npc = (int)(ninit * dxf * dyf * zl / (float)d.sp_np2c[isp]);
+        float y0 = (float)d.sp_y0 / dyf, y1 = (float)d.sp_y1 / dyf - 1;
+        float x0 = (float)d.sp_x0 / dxf, x1 = (float)d.sp_x1 / dxf - 1;
+        long total = 0;
+        for (int i = 0; i < s.ncx; ++i)
+            for (int j = 0; j < s.ncy; ++j)
+                if (s.PlasmaRegion[i * s.ncy + j] == 1 && i >= x0 && i <= x1 && j >= y0 && j <= y1) total += npc;
+        if (total > d.np_limit) { err = "LOAD: too many particles"; return -22; }
+        sp.cap = d.sp_maxnp[isp] > total ? d.sp_maxnp[isp] : (int)total;
+        std::vector<int> cellof(total), cnt(s.Gsize, 0);
+        std::vector<float> tx(total), ty(total), tvx(total), tvy(total), tvz(total);
+        const float vti = (float)s.vti[isp];
+        long idx = 0;
+        for (int i = 0; i < s.ncx; ++i)
+            for (int j = 0; j < s.ncy; ++j) {
+                if (s.PlasmaRegion[i * s.ncy + j] != 1) continue;
+                int num = (i >= x0 && i <= x1 && j >= y0 && j <= y1) ? npc : 0;
+                for (int n = 0; n < num; ++n) {
+                    int vi = (int)((s.nvel - 1) * g.next());
+                    float aphi = (float)(2 * kPi * g.next());
+                    float costheta = 1 - 2 * g.next();
+                    float sintheta = (float)std::sqrt((double)(1 - costheta * costheta));
+                    float vs = (float)s.vsave[vi];
+                    tvx[idx] = (float)((double)(vti * vs * sintheta) * std::cos((double)aphi));
+                    tvy[idx] = (float)((double)(vti * vs * sintheta) * std::sin((double)aphi));
+                    tvz[idx] = vti * vs * costheta;
+                    float px = (float)i + g.next();
+                    float py = (float)j + g.next();
+                    int xid = (int)px, yid = (int)py;
+                    if (xid > s.ncx - 1) xid = s.ncx - 1;
+                    if (yid > s.ncy - 1) yid = s.ncy - 1;
+                    tx[idx] = px - (float)xid;
+                    ty[idx] = py - (float)yid;
+                    cellof[idx] = xid * s.ngy + yid;
+                    cnt[cellof[idx]]++;
+                    ++idx;
+                }
+            }
+        sp.cell_start.assign(s.Gsize + 1, 0);
+        for (int c = 0; c < s.Gsize; ++c) sp.cell_start[c + 1] = sp.cell_start[c] + cnt[c];
+        sp.x.assign(idx, 0); sp.y.assign(idx, 0); sp.vx.assign(idx, 0); sp.vy.assign(idx, 0); sp.vz.assign(idx, 0);
+        sp.id.assign(idx, 0);
+        for (int c = 0; c < s.Gsize; ++c) cnt[c] = sp.cell_start[c];
+        for (long n = 0; n < idx; ++n) {
+            int dst = cnt[cellof[n]]++;
+            sp.x[dst] = (R)tx[n]; sp.y[dst] = (R)ty[n];
+            sp.vx[dst] = (R)tvx[n]; sp.vy[dst] = (R)tvy[n]; sp.vz[dst] = (R)tvz[n];
+            sp.id[dst] = (uint32_t)n;
+        }
+        sp.n = (int)idx;
+    }
+    return 0;
+}
+
+}  // namespace
+
+template <typename R>
+int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool do_load) {
+    s.deck = d;
+    int e = constants(d, s, err);
+    if (e) return e;
+    geometry(d, s);
+    field_matrix(s);
+    e = presolves(s, err);
+    if (e) return e;
+    argon_tables(s);
+    null_collision(s);
+    maxwell_table(s);
+    if (do_load) {
+        e = load(s, err);
+        if (e) return e;
+    }
+    return 0;
+}
+
+template int build_setup<float>(const Deck &, Setup<float> &, std::string &, bool);
+template int build_setup<double>(const Deck &, Setup<double> &, std::string &, bool);
+template int BandLU::factor<float>(int, const std::vector<int> &, const std::vector<int> &, const std::vector<float> &);
+template int BandLU::factor<double>(int, const std::vector<int> &, const std::vector<int> &, const std::vector<double> &);
+
+}  // namespace pic

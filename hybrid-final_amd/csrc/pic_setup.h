@@ -1,0 +1,99 @@
+// Host-side case setup: the JSON deck and everything the time loop needs
+// that is computed once (geometry maps, field operator, Laplace solutions,
+// gas fields, cross-section tables, null-collision constants, Maxwellian
+// table, initial particles).  Restates src/start.c:19-2052, src/load.c:121-245,
+// src/Efield.c:3-14, src/PhysicsData.c:638-705,1579-1718,
+// src/cuda_mcc.cu:202-291, src/Particle.c:3-230 and
+// src/cuda_Field.cu:182-236,646-678.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pic {
+
+enum { DIRICHLET = 1, NEUMANN, CONDUCTOR, DIELECTRIC };
+enum { F_LEFT, F_RIGHT, F_UP, F_DOWN, F_UL, F_UR, F_LL, F_LR, F_NOFACE };
+constexpr int MAXSP = 5;
+constexpr int AR_NRCT = 10;
+constexpr int N_LOGX = 81;
+
+struct Line { int x0 = 0, x1 = 0, y0 = 0, y1 = 0, id = 0, bc = 0; double temp = 0, a = 0, b = 0, c = 0; };
+struct Src { int mid = 0; double dc = 0, pow = 0, ac = 0, freq = 0, phase = 0; };
+
+// One case deck, with the reference's key names and "missing reads as 0".
+struct Deck {
+    double xlength = 0, ylength = 0, zlength = 0;
+    int ngx = 0, ngy = 0;
+    std::vector<Line> bnd, cond, diel;   // diel: a = epsilon; cond: a,b,c = R,L,C
+    std::vector<Src> src;
+    int gas = 0;
+    double total_pres = 0, bg_ratio[2] = {0, 0}, bg_temp[2] = {0, 0};
+    int fl_loadtype = 0;
+    double fl_dens[5] = {0}, fl_temp[5] = {0}, fl_quench[5] = {0};
+    int sp_loadtype = 0;
+    double sp_x0 = 0, sp_x1 = 0, sp_y0 = 0, sp_y1 = 0;
+    double sp_dens[MAXSP] = {0}, sp_temp[MAXSP] = {0}, sp_np2c[MAXSP] = {0};
+    int sp_ratio[MAXSP] = {0}, sp_maxnp[MAXSP] = {0};
+    int dt_pic = 0;
+    double pcg_tol = 0;
+    int n_ave = 0, n_smt = 0, np_limit = 0;
+    double coll_flag[100] = {0};
+    int64_t seed = 1, rng_seed = 0;
+    int rf_cycles = 0;
+};
+
+// Parses deck text; returns 0 or a negative code with a message.
+int parse_deck(const std::string &text, Deck &d, std::string &err);
+
+template <typename R>
+struct SpeciesInit {
+    std::vector<R> x, y, vx, vy, vz;
+    std::vector<uint32_t> id;
+    std::vector<int> cell_start;   // Gsize + 1
+    int n = 0, cap = 0;
+};
+
+template <typename R>
+struct Setup {
+    Deck deck;
+    int ngx = 0, ngy = 0, ncx = 0, ncy = 0, Gsize = 0, Csize = 0, nsp = 0, nfsp = 0, CondNUMR = 0;
+    R dx = 0, dy = 0, zl = 0, dt = 0, dt_dx = 0, dt_dy = 0, hdx = 0, hdy = 0, idx = 0, idy = 0, idxy = 0;
+    R mass[MAXSP] = {0}, q[MAXSP] = {0}, np2c[MAXSP] = {0}, q_density[MAXSP] = {0}, vti[MAXSP] = {0};
+    R qm[MAXSP] = {0}, Escale[MAXSP] = {0}, Ascale[MAXSP] = {0}, Denscale[MAXSP] = {0};
+    R bg_mass[2] = {0, 0}, bg_pres[2] = {0, 0}, bg_dens[2] = {0, 0}, fl_mass[5] = {0}, fl_dens[5] = {0};
+    // geometry
+    std::vector<int> Boundary, CondID, Face, DensRegion, PlasmaRegion, StructIdx, A_idx;
+    std::vector<R> Area, Temp, eps_r, eps;
+    // field operator, CSR 0-based (row = unknown)
+    int A_size = 0;
+    std::vector<int> Ai, Aj;
+    std::vector<R> Aval, Atemp, Mdiag, cond_b, temp_b;
+    // Laplace solutions (CondNUMR x Gsize) and gas fields
+    std::vector<R> lap_phi, BackDen1, BackVel1, FluidN;
+    std::vector<int> Efield_Flag;
+    // collision data
+    int TnRct = 0, MCCn = 1;
+    R idLOGX = 0, dt_mcc = 0;
+    std::vector<R> xe, xee, cx;      // cx: N_LOGX x TnRct
+    R Th_e[100] = {0}, Flag[100] = {0}, mofM[100] = {0}, RR[100] = {0};
+    R sigv[32] = {0};
+    std::vector<R> P1, P2, Pion;
+    int nvel = 0;
+    std::vector<R> vsave;
+    SpeciesInit<R> sp[MAXSP];
+};
+
+template <typename R>
+int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool load = true);
+
+// FP64 banded LU without pivoting (the field operator is an M-matrix).
+struct BandLU {
+    int N = 0, bw = 0;
+    std::vector<double> a;
+    template <typename R>
+    int factor(int n, const std::vector<int> &Ai, const std::vector<int> &Aj, const std::vector<R> &val);
+    void solve(const double *b, double *x) const;
+};
+
+}  // namespace pic

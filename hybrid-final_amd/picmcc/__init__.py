@@ -1,0 +1,255 @@
+"""Python host mirror of the MI355X PIC-MCC library (libpicmcc.so).
+
+A thin ctypes layer over include/picmcc.h.  ``Case`` is one discharge case
+resident on the GPU (the reference's stage table, src/cuda_main.cu:112-235,
+as methods); ``HostSetup`` is the host-side setup alone (no device needed).
+There is no CPU fallback: if the HIP library is missing the import of the
+compute path raises.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libpicmcc.so")
+
+FP32_REFINED = 0
+FP64_PLAIN = 1
+
+_lib = None
+
+
+class PicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"picmcc error {code}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("inner_tol", C.c_double),
+                ("pcg_maxit", C.c_int), ("load", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("ngx", C.c_int), ("ngy", C.c_int), ("nsp", C.c_int), ("A_size", C.c_int),
+                ("CondNUMR", C.c_int), ("TnRct", C.c_int), ("MCCn", C.c_int), ("nvel", C.c_int),
+                ("dt", C.c_double), ("dx", C.c_double), ("dy", C.c_double), ("t", C.c_double),
+                ("tstep", C.c_longlong), ("precision", C.c_int),
+                ("np", C.c_longlong * 5), ("cap", C.c_longlong * 5)]
+
+
+EXPORTS = {
+    "pic_create": (C.c_int, [C.c_char_p, C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "pic_create_from_file": (C.c_int, [C.c_char_p, C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "pic_destroy": (C.c_int, [C.c_void_p]),
+    "pic_last_error": (C.c_char_p, []),
+    "pic_info_get": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
+    "pic_sync": (C.c_int, [C.c_void_p]),
+    "pic_advance_time": (C.c_int, [C.c_void_p]),
+    "pic_field_solve": (C.c_int, [C.c_void_p]),
+    "pic_efield": (C.c_int, [C.c_void_p]),
+    "pic_move": (C.c_int, [C.c_void_p]),
+    "pic_sort_boundary": (C.c_int, [C.c_void_p]),
+    "pic_mcc": (C.c_int, [C.c_void_p]),
+    "pic_deposit": (C.c_int, [C.c_void_p]),
+    "pic_diagnostic": (C.c_int, [C.c_void_p]),
+    "pic_run_staged": (C.c_int, [C.c_void_p, C.c_int]),
+    "pic_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "pic_get_particles": (C.c_int, [C.c_void_p, C.c_int] + [C.c_void_p] * 7),
+    "pic_set_particles": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong] + [C.c_void_p] * 7 + [C.c_uint32]),
+    "pic_array_info": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)]),
+    "pic_get_array": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
+    "pic_set_array": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
+    "pic_array_names": (C.c_char_p, []),
+    "pic_set_step": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "pic_last_field_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_double)]),
+    "pic_setup_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "pic_setup_array_info": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
+    "pic_setup_get_array": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
+    "pic_setup_scalar": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    "pic_setup_close": (None, [C.c_void_p]),
+    "pic_shim_open": (C.c_int, [C.c_char_p, C.c_int]),
+    "pic_shim_close": (None, []),
+}
+SHIMS = ["main_cuda", "start_cuda", "info_Device", "Set_Device_Parameter", "Set_Particle_cuda",
+         "Set_NullCollisionTime_cuda", "Set_SortBoundary_cuda", "Set_MatrixPCG_cuda", "PCG_SOLVER_Laplace",
+         "PCG_SOLVER", "Efield_cuda_Basic", "Move_cuda", "SortBounndary_cuda", "MCC_Ar_cuda", "Deposit_cuda",
+         "Deposit_Basic"]
+
+
+def lib():
+    """Load libpicmcc.so (raises if it is not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is not built; run `make -C hybrid-final_amd` or __graft_entry__.build()")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        for name in SHIMS:
+            getattr(L, name).restype = None
+            getattr(L, name).argtypes = []
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        raise PicError(code, lib().pic_last_error().decode())
+    return code
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+_INT_ARRAYS = {"Boundary", "CondID", "Face", "DensRegion", "PlasmaRegion", "StructIdx", "A_idx", "Ai", "Aj",
+               "CondCharge", "next_id"}
+
+
+def _dtype(name, elsize, is_float):
+    if name == "mccr":
+        return np.uint64
+    if name == "next_id" or name.endswith(".id"):
+        return np.uint32
+    if not is_float:
+        return np.int32 if elsize == 4 else np.int64
+    return np.float32 if elsize == 4 else np.float64
+
+
+def deck_text(deck):
+    return deck if isinstance(deck, str) else json.dumps(deck)
+
+
+class HostSetup:
+    """Host-side case setup only (geometry, operator, tables, initial load)."""
+
+    def __init__(self, deck, precision=FP32_REFINED, load=True):
+        L = lib()
+        h = C.c_void_p()
+        _check(L.pic_setup_open(deck_text(deck).encode(), precision, int(load), C.byref(h)))
+        self.h = h
+        self.precision = precision
+
+    def array(self, name):
+        L = lib()
+        n, es, fl = C.c_size_t(), C.c_int(), C.c_int()
+        _check(L.pic_setup_array_info(self.h, name.encode(), C.byref(n), C.byref(es), C.byref(fl)))
+        out = np.empty(n.value, dtype=_dtype(name, es.value, fl.value))
+        _check(L.pic_setup_get_array(self.h, name.encode(), _ptr(out), out.nbytes))
+        return out
+
+    def scalar(self, name):
+        v = C.c_double()
+        _check(lib().pic_setup_scalar(self.h, name.encode(), C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().pic_setup_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Case:
+    """One discharge case resident on the GPU."""
+
+    def __init__(self, deck, precision=FP32_REFINED, device=0, inner_tol=0.0, pcg_maxit=0, load=True):
+        L = lib()
+        o = Options(precision, device, inner_tol, pcg_maxit, int(load))
+        h = C.c_void_p()
+        _check(L.pic_create(deck_text(deck).encode(), C.byref(o), C.byref(h)))
+        self.h = h
+        self.precision = precision
+        self.real = np.float64 if precision == FP64_PLAIN else np.float32
+
+    # ---- lifecycle
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = Info()
+        _check(lib().pic_info_get(self.h, C.byref(i)))
+        return i
+
+    def sync(self):
+        _check(lib().pic_sync(self.h))
+
+    # ---- stage table
+    def advance_time(self): _check(lib().pic_advance_time(self.h))
+    def field_solve(self): _check(lib().pic_field_solve(self.h))
+    def efield(self): _check(lib().pic_efield(self.h))
+    def move(self): _check(lib().pic_move(self.h))
+    def sort_boundary(self): _check(lib().pic_sort_boundary(self.h))
+    def mcc(self): _check(lib().pic_mcc(self.h))
+    def deposit(self): _check(lib().pic_deposit(self.h))
+    def diagnostic(self): _check(lib().pic_diagnostic(self.h))
+
+    def run_staged(self, n=1):
+        _check(lib().pic_run_staged(self.h, n))
+
+    def step(self, n=1):
+        _check(lib().pic_step(self.h, n))
+
+    def set_step(self, tstep):
+        _check(lib().pic_set_step(self.h, tstep))
+
+    def last_field_solve(self):
+        it, out, rr = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().pic_last_field_solve(self.h, C.byref(it), C.byref(out), C.byref(rr)))
+        return it.value, out.value, rr.value
+
+    # ---- state
+    def particles(self, isp):
+        inf = self.info()
+        n = inf.np[isp]
+        G = inf.ngx * inf.ngy
+        out = {k: np.empty(n, dtype=self.real) for k in ("x", "y", "vx", "vy", "vz")}
+        out["id"] = np.empty(n, dtype=np.uint32)
+        out["cell_start"] = np.empty(G + 1, dtype=np.int32)
+        _check(lib().pic_get_particles(self.h, isp, _ptr(out["x"]), _ptr(out["y"]), _ptr(out["vx"]),
+                                       _ptr(out["vy"]), _ptr(out["vz"]), _ptr(out["id"]),
+                                       _ptr(out["cell_start"])))
+        return out
+
+    def set_particles(self, isp, st, next_id):
+        arrs = [np.ascontiguousarray(st[k], dtype=self.real) for k in ("x", "y", "vx", "vy", "vz")]
+        ids = np.ascontiguousarray(st["id"], dtype=np.uint32)
+        cs = np.ascontiguousarray(st["cell_start"], dtype=np.int32)
+        _check(lib().pic_set_particles(self.h, isp, len(arrs[0]), *[_ptr(a) for a in arrs], _ptr(ids), _ptr(cs),
+                                       int(next_id)))
+
+    def array(self, name):
+        L = lib()
+        n, es, fl = C.c_size_t(), C.c_int(), C.c_int()
+        _check(L.pic_array_info(self.h, name.encode(), C.byref(n), C.byref(es), C.byref(fl)))
+        out = np.empty(n.value, dtype=_dtype(name, es.value, fl.value))
+        _check(L.pic_get_array(self.h, name.encode(), _ptr(out), out.nbytes))
+        return out
+
+    def set_array(self, name, values):
+        L = lib()
+        n, es, fl = C.c_size_t(), C.c_int(), C.c_int()
+        _check(L.pic_array_info(self.h, name.encode(), C.byref(n), C.byref(es), C.byref(fl)))
+        a = np.ascontiguousarray(values, dtype=_dtype(name, es.value, fl.value))
+        _check(L.pic_set_array(self.h, name.encode(), _ptr(a), a.nbytes))

@@ -1,0 +1,123 @@
+/*
+ * picmcc.h - C ABI of the MI355X PIC-MCC step library (libpicmcc.so).
+ *
+ * Two layers:
+ *  1. pic_* functions on an explicit context (one discharge case: deck,
+ *     device-resident particles, fields and random-stream state).  Each
+ *     returns 0 or a negative status; pic_last_error() gives the message.
+ *  2. The reference's zero-argument stage functions (inc/cuda_main.cuh:36-61,
+ *     inc/main.h:20,34-41) as shims acting on a default context opened with
+ *     pic_shim_open(); they print the error and exit(1) like the reference's
+ *     gpuErrchk (src/cuda_main.cu:2-10).
+ * No torch types, plain pointers and sizes only.
+ */
+#ifndef PICMCC_H
+#define PICMCC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pic_ctx pic_ctx;
+
+enum pic_precision {
+    PIC_FP32_REFINED = 0, /* FP32 particles/fields, FP32 PCG inside FP64 refinement */
+    PIC_FP64_PLAIN = 1    /* the reference algorithm promoted to FP64 */
+};
+
+typedef struct {
+    int precision;        /* enum pic_precision */
+    int device;           /* HIP device ordinal */
+    double inner_tol;     /* refined path: relative residual drop per FP32 inner solve (0 = 1e-4) */
+    int pcg_maxit;        /* iteration cap per field solve (0 = 100000, the reference's cap) */
+    int load;             /* 1 = load the initial particles from the deck (default when opt is NULL) */
+} pic_options;
+
+typedef struct {
+    int ngx, ngy, nsp, A_size, CondNUMR, TnRct, MCCn, nvel;
+    double dt, dx, dy, t;
+    long long tstep;
+    int precision;
+    long long np[5];
+    long long cap[5];
+} pic_info;
+
+/* ---------------- context ---------------- */
+/* deck = JSON text in the reference format (src/start.c:19-1431) */
+int pic_create(const char *deck_json, const pic_options *opt, pic_ctx **out);
+/* same, reading the deck from a file (the reference's -i <case.json>, src/main.c:27-39) */
+int pic_create_from_file(const char *path, const pic_options *opt, pic_ctx **out);
+int pic_destroy(pic_ctx *ctx);
+const char *pic_last_error(void);
+int pic_info_get(pic_ctx *ctx, pic_info *info);
+int pic_sync(pic_ctx *ctx);   /* waits for the stream; reports device-side overflow */
+
+/* ---------------- stage table (src/cuda_main.cu:112-235) ---------------- */
+int pic_advance_time(pic_ctx *ctx);  /* t += dt, tstep++ (src/cuda_main.cu:114-116) */
+int pic_field_solve(pic_ctx *ctx);   /* PCG_SOLVER, src/cuda_Field.cu:133-181 */
+int pic_efield(pic_ctx *ctx);        /* Efield_cuda_Basic, src/cuda_Field.cu:2-42 */
+int pic_move(pic_ctx *ctx);          /* Move_cuda, src/cuda_Move.cu:3-7 */
+int pic_sort_boundary(pic_ctx *ctx); /* SortBounndary_cuda, src/cuda_Sortboundary.cu:4-13 */
+int pic_mcc(pic_ctx *ctx);           /* MCC_Ar_cuda, src/cuda_mcc.cu:97-106 */
+int pic_deposit(pic_ctx *ctx);       /* Deposit_cuda / Deposit_Basic, src/cuda_Deposit.cu:3-29 */
+int pic_diagnostic(pic_ctx *ctx);    /* profile accumulators, src/cuda_Diagnostic.cu:505-547 */
+/* one full timestep through the stage functions above, nsteps times */
+int pic_run_staged(pic_ctx *ctx, int nsteps);
+/* one full timestep on the fused path (push+walls+MCC counted, scanned and
+ * scattered in two passes), nsteps times; no host synchronisation inside */
+int pic_step(pic_ctx *ctx, int nsteps);
+
+/* ---------------- state exchange ---------------- */
+/* particle arrays of species isp: element type double for PIC_FP64_PLAIN,
+ * float otherwise; buffers must hold np (from pic_info_get) elements;
+ * cell_start holds ngx*ngy+1 ints */
+int pic_get_particles(pic_ctx *ctx, int isp, void *x, void *y, void *vx, void *vy, void *vz, uint32_t *id,
+                      int32_t *cell_start);
+int pic_set_particles(pic_ctx *ctx, int isp, long long n, const void *x, const void *y, const void *vx,
+                      const void *vy, const void *vz, const uint32_t *id, const int32_t *cell_start,
+                      uint32_t next_id);
+/* named arrays (node fields, setup maps, tables, counters); see the
+ * library's array list in pic_array_names() */
+int pic_array_info(pic_ctx *ctx, const char *name, size_t *count, int *elsize, int *is_float);
+int pic_get_array(pic_ctx *ctx, const char *name, void *host, size_t bytes);
+int pic_set_array(pic_ctx *ctx, const char *name, const void *host, size_t bytes);
+const char *pic_array_names(void);
+int pic_set_step(pic_ctx *ctx, long long tstep);
+int pic_last_field_solve(pic_ctx *ctx, int *iters, int *outer, double *rel_res);
+
+/* ---------------- host-only setup (no device needed) ---------------- */
+/* deck -> geometry maps, field operator, Laplace solutions, tables and the
+ * initial particle load, computed on the host exactly as pic_create does */
+typedef struct pic_setup_h pic_setup_h;
+int pic_setup_open(const char *deck_json, int precision, int load, pic_setup_h **out);
+int pic_setup_array_info(pic_setup_h *h, const char *name, size_t *count, int *elsize, int *is_float);
+int pic_setup_get_array(pic_setup_h *h, const char *name, void *host, size_t bytes);
+int pic_setup_scalar(pic_setup_h *h, const char *name, double *value);
+void pic_setup_close(pic_setup_h *h);
+
+/* ---------------- reference-named zero-argument shims ---------------- */
+int pic_shim_open(const char *deck_path, int precision);
+void pic_shim_close(void);
+void main_cuda(void);           /* src/cuda_main.cu:11 (runs RFCycles cycles) */
+void start_cuda(void);          /* src/cuda_start.cu:2 */
+void info_Device(void);         /* src/cuda_start.cu:36 */
+void Set_Device_Parameter(void);
+void Set_Particle_cuda(void);
+void Set_NullCollisionTime_cuda(void);
+void Set_SortBoundary_cuda(void);
+void Set_MatrixPCG_cuda(void);
+void PCG_SOLVER_Laplace(void);
+void PCG_SOLVER(void);          /* src/cuda_Field.cu:133 */
+void Efield_cuda_Basic(void);   /* src/cuda_Field.cu:2 */
+void Move_cuda(void);           /* src/cuda_Move.cu:3 */
+void SortBounndary_cuda(void);  /* src/cuda_Sortboundary.cu:4 (sic) */
+void MCC_Ar_cuda(void);         /* src/cuda_mcc.cu:97 dispatch */
+void Deposit_cuda(void);        /* src/cuda_Deposit.cu:3 */
+void Deposit_Basic(void);       /* src/cuda_Deposit.cu:21 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

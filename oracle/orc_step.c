@@ -1,0 +1,797 @@
+/*
+ * ORACLE - test infrastructure only (see pic_oracle.h).
+ * Per-timestep restatement of src/cuda_main.cu:112-235 for argon:
+ * field solve, E-field, push, sort/boundary, null-collision MCC, deposit,
+ * profile accumulation.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "pic_oracle.h"
+#include "detmath.h"
+#include "philox_ref.h"
+
+#define EPS0 8.8542e-12
+#define ORC_PI 3.14159265358979323846
+
+int orc_setup(orc_case *c);
+void orc_band_solve(int N, const double *lu, int bw, const double *b, double *x);
+
+static void *zalloc(size_t n, size_t s) {
+    void *p = calloc(n ? n : 1, s);
+    if (!p) { fprintf(stderr, "oracle: out of memory\n"); exit(2); }
+    return p;
+}
+
+/* ================= lifecycle ================= */
+orc_case *orc_create(const orc_deck *d, int *err) {
+    orc_case *c = zalloc(1, sizeof(orc_case));
+    c->deck = *d;
+    int e = orc_setup(c);
+    if (err) *err = e;
+    if (e) { return c; }
+    int G = c->Gsize, s;
+    c->phi = zalloc(G, sizeof(real)); c->TotPot = zalloc(G, sizeof(real));
+    c->Sigma = zalloc(G, sizeof(real)); c->Source = zalloc(G, sizeof(real));
+    c->Ex = zalloc(G, sizeof(real)); c->Ey = zalloc(G, sizeof(real));
+    c->den = zalloc((size_t)c->nsp * G, sizeof(real));
+    c->sigma_sp = zalloc((size_t)c->nsp * G, sizeof(real));
+    c->b = zalloc(c->A_size, sizeof(real));
+    c->xsol = zalloc(c->A_size, sizeof(double));
+    c->V_cond = zalloc(c->CondNUMR, sizeof(real));
+    c->CondCharge = zalloc((size_t)c->nsp * c->CondNUMR, sizeof(real));
+    c->mccr = zalloc((size_t)G * c->TnRct, sizeof(int64_t));
+    c->sum_den = zalloc((size_t)c->nsp * G, sizeof(double));
+    c->sum_phi = zalloc(G, sizeof(double));
+    c->sum_ex = zalloc(G, sizeof(double)); c->sum_ey = zalloc(G, sizeof(double));
+    int maxcap = 0;
+    for (s = 0; s < c->nsp; ++s) if (c->sp[s].cap > maxcap) maxcap = c->sp[s].cap;
+    c->tag = zalloc((size_t)c->nsp * maxcap, sizeof(int));
+    c->Efield_Flag = zalloc(c->CondNUMR, sizeof(int));
+    {
+        int i, j;
+        for (i = 0; i < c->CondNUMR; ++i) {
+            int cnt = 0;
+            for (j = 0; j < c->deck.nsrc; ++j) if (c->deck.smid[j] == i + 1) cnt++;
+            c->Efield_Flag[i] = (cnt > 0 && cnt < 4) ? 1 : 0;
+        }
+    }
+    return c;
+}
+
+/* ================= field solve: src/cuda_Field.cu:133-181, 639-645 =========
+ * The oracle solves A x = b exactly (FP64 banded LU); b comes from the last
+ * deposit.  x is scattered to node phi with 0 at eliminated nodes. */
+void orc_field(orc_case *c) {
+    int N = c->A_size, g;
+    double *bd = zalloc(N, sizeof(double));
+    for (g = 0; g < N; ++g) bd[g] = (double)c->b[g];
+    orc_band_solve(N, c->lu, c->lu_bw, bd, c->xsol);
+    for (g = 0; g < c->Gsize; ++g) c->phi[g] = c->A_idx[g] ? (real)c->xsol[c->A_idx[g] - 1] : 0;
+    free(bd);
+}
+
+/* ================= E-field: src/cuda_Field.cu:2-42, 500-558, 698-846 ======= */
+static void eps_index(const orc_case *c, int TID, int *e_tid, int *e_left, int *e_down, int *e_cross) {
+    int ngx = c->ngx, ngy = c->ngy;
+    int x = TID / ngy, y = TID % ngy;
+    int ET = x * (ngy - 1) + y;
+    int EL = x ? ET - ngy + 1 : ET;
+    int ED = y ? ET - 1 : ET;
+    int EC = EL + ED - ET;
+    int xo = (x == ngx - 1) ? ngy - 1 : 0;
+    int yo = (y == ngy - 1) ? 1 : 0;
+    ET -= (xo + yo); EL -= yo; ED -= xo;
+    *e_tid = ET; *e_left = EL; *e_down = ED; *e_cross = EC;
+}
+
+static void cond_sigma(orc_case *c, const real *Phi, real *Sigma) {
+    int ngx = c->ngx, ngy = c->ngy, TID, i;
+    for (TID = 0; TID < c->Gsize; ++TID) {
+        int x = TID / ngy, y = TID % ngy;
+        int ET, EL, ED, EC;
+        eps_index(c, TID, &ET, &EL, &ED, &EC);
+        int pl = x ? TID - ngy : TID, pr = (x == ngx - 1) ? TID : TID + ngy;
+        int pd = y ? TID - 1 : TID, pu = (y == ngy - 1) ? TID : TID + 1;
+        real ss = 0;
+        for (i = 0; i < c->nsp; ++i) ss += c->den[TID + i * c->Gsize] * c->q_density[i];
+        int b = c->Boundary[TID], f = c->Face[TID];
+        if (!((b == ORC_CONDUCTOR || b == ORC_DIRICHLET) && f != ORC_NOFACE)) continue;
+        const real *e = c->eps;
+        real hdx = c->hdx, hdy = c->hdy, idxy = c->idxy, P = Phi[TID];
+        switch (f) {
+        case ORC_UP: Sigma[TID] = hdy * (e[ET] + e[EL]) * (P - Phi[pu]) - hdx * ss; break;
+        case ORC_DOWN: Sigma[TID] = hdy * (e[ED] + e[EC]) * (P - Phi[pd]) - hdx * ss; break;
+        case ORC_LEFT: Sigma[TID] = hdx * (e[EL] + e[EC]) * (P - Phi[pl]) - hdy * ss; break;
+        case ORC_RIGHT: Sigma[TID] = hdx * (e[ET] + e[ED]) * (P - Phi[pr]) - hdy * ss; break;
+        case ORC_UL: Sigma[TID] = hdy * (e[ET] + e[EL]) * (P - Phi[pu]) + hdx * (e[EL] + e[EC]) * (P - Phi[pl]) - idxy * ss; break;
+        case ORC_UR: Sigma[TID] = hdy * (e[ET] + e[EL]) * (P - Phi[pu]) + hdx * (e[ET] + e[ED]) * (P - Phi[pr]) - idxy * ss; break;
+        case ORC_LL: Sigma[TID] = hdy * (e[ED] + e[EC]) * (P - Phi[pd]) + hdx * (e[EL] + e[EC]) * (P - Phi[pl]) - idxy * ss; break;
+        case ORC_LR: Sigma[TID] = hdy * (e[ED] + e[EC]) * (P - Phi[pd]) + hdx * (e[ET] + e[ED]) * (P - Phi[pr]) - idxy * ss; break;
+        default: break;
+        }
+    }
+}
+
+static void v_to_efield(orc_case *c, const real *Sigma, const real *Phi) {
+    int ngx = c->ngx, ngy = c->ngy, TID;
+    const real *e = c->eps;
+    real dx = c->dx, dy = c->dy, hdx = c->hdx, hdy = c->hdy, idx = c->idx, idy = c->idy;
+    for (TID = 0; TID < c->Gsize; ++TID) {
+        int x = TID / ngy, y = TID % ngy;
+        int pl = x ? TID - ngy : TID, pr = (x == ngx - 1) ? TID : TID + ngy;
+        int pd = y ? TID - 1 : TID, pu = (y == ngy - 1) ? TID : TID + 1;
+        int ET, EL, ED, EC;
+        eps_index(c, TID, &ET, &EL, &ED, &EC);
+        int b = c->Boundary[TID], f = c->Face[TID];
+        real P = Phi[TID], Ex, Ey;
+        real hxy = (real)(0.5 * (double)dx / (double)dy), hyx = (real)(0.5 * (double)dy / (double)dx);
+        int wall = (b == ORC_CONDUCTOR || b == ORC_NEUMANN || b == ORC_DIRICHLET);
+        int die = (b == ORC_DIELECTRIC || b >= 100);
+        int hard = (b == ORC_CONDUCTOR || b == ORC_DIRICHLET || b == ORC_DIELECTRIC);
+        if (f == ORC_LEFT) {
+            if (wall) { Ex = -2 * Sigma[TID] / (e[EL] + e[EC]); Ey = 0; }
+            else if (die) { Ex = idx * (Phi[pl] - P); Ey = hdy * (Phi[pd] - Phi[pu]); }
+            else {
+                Ex = (real)0.5 * (e[ET] + e[ED]) * (Phi[pr] - P) * idx;
+                Ex -= hxy * e[ET] * (P - Phi[pu]) * idy;
+                Ex -= hxy * e[ED] * (P - Phi[pd]) * idy;
+                Ex /= (real)-0.5 * (e[EL] + e[EC]);
+                Ey = hdy * (Phi[pd] - Phi[pu]);
+            }
+        } else if (f == ORC_RIGHT) {
+            if (wall) { Ex = 2 * Sigma[TID] / (e[ET] + e[ED]); Ey = 0; }
+            else if (die) { Ex = idx * (P - Phi[pr]); Ey = hdy * (Phi[pd] - Phi[pu]); }
+            else {
+                Ex = (real)0.5 * (e[EL] + e[EC]) * (Phi[pl] - P) * idx;
+                Ex -= hxy * e[EL] * (P - Phi[pu]) * idy;
+                Ex -= hxy * e[EC] * (P - Phi[pd]) * idy;
+                Ex /= (real)0.5 * (e[ET] + e[ED]);
+                Ey = hdy * (Phi[pd] - Phi[pu]);
+            }
+        } else if (f == ORC_UP) {
+            if (wall) { Ex = 0; Ey = 2 * Sigma[TID] / (e[ET] + e[EL]); }
+            else if (die) { Ex = hdx * (Phi[pl] - Phi[pr]); Ey = idy * (P - Phi[pu]); }
+            else {
+                Ex = hdx * (Phi[pl] - Phi[pr]);
+                Ey = (real)0.5 * (e[ED] + e[EC]) * (Phi[pd] - P) * idy;
+                Ey -= hyx * e[ED] * (P - Phi[pr]) * idx;
+                Ey -= hyx * e[EC] * (P - Phi[pl]) * idx;
+                Ey /= (real)0.5 * (e[ET] + e[EL]);
+            }
+        } else if (f == ORC_DOWN) {
+            if (wall) { Ex = 0; Ey = -2 * Sigma[TID] / (e[ED] + e[EC]); }
+            else if (die) { Ex = hdx * (Phi[pl] - Phi[pr]); Ey = idy * (Phi[pd] - P); }
+            else {
+                Ex = hdx * (Phi[pl] - Phi[pr]);
+                Ey = (real)0.5 * (e[ET] + e[EL]) * (Phi[pu] - P) * idy;
+                Ey -= hyx * e[ET] * (P - Phi[pr]) * idx;
+                Ey -= hyx * e[EL] * (P - Phi[pl]) * idx;
+                Ey /= (real)-0.5 * (e[ED] + e[EC]);
+            }
+        } else if (f == ORC_UL) {
+            if (hard) { Ex = idx * (Phi[pl] - P); Ey = idy * (P - Phi[pu]); }
+            else { Ex = hdx * (Phi[pl] - Phi[pr]); Ey = hdy * (Phi[pd] - Phi[pu]); }
+        } else if (f == ORC_UR) {
+            if (hard) { Ex = idx * (P - Phi[pr]); Ey = idy * (P - Phi[pu]); }
+            else { Ex = hdx * (Phi[pl] - Phi[pr]); Ey = hdy * (Phi[pd] - Phi[pu]); }
+        } else if (f == ORC_LL) {
+            if (hard) { Ex = idx * (Phi[pl] - P); Ey = idy * (Phi[pd] - P); }
+            else { Ex = hdx * (Phi[pl] - Phi[pr]); Ey = hdy * (Phi[pd] - Phi[pu]); }
+        } else if (f == ORC_LR) {
+            if (hard) { Ex = idx * (P - Phi[pr]); Ey = idy * (Phi[pd] - P); }
+            else { Ex = hdx * (Phi[pl] - Phi[pr]); Ey = hdy * (Phi[pd] - Phi[pu]); }
+        } else if (b == ORC_NEUMANN) {
+            if (x == 0 || x == ngx - 1) { Ex = 0; Ey = hdy * (Phi[pd] - Phi[pu]); }
+            else if (y == 0 || y == ngy - 1) { Ex = hdx * (Phi[pl] - Phi[pr]); Ey = 0; }
+            else { Ex = c->Ex[TID]; Ey = c->Ey[TID]; }   /* reference leaves E unchanged */
+        } else {
+            Ex = hdx * (Phi[pl] - Phi[pr]); Ey = hdy * (Phi[pd] - Phi[pu]);
+        }
+        c->Ex[TID] = Ex; c->Ey[TID] = Ey;
+    }
+}
+
+/* conductor voltages at time t (src/cuda_Field.cu:9-20): V = DC + AC sin(w t + phase) */
+void orc_voltages(orc_case *c, double t) {
+    int i, j;
+    for (i = 0; i < c->CondNUMR; ++i) {
+        real v = 0;
+        if (c->Efield_Flag[i]) {
+            for (j = 0; j < c->deck.nsrc; ++j)
+                if (c->deck.smid[j] == i + 1) {
+                    real w = (real)(2 * ORC_PI * (double)(real)c->deck.sfreq[j]);
+                    real ph = (real)(ORC_PI / 180 * (double)(real)c->deck.sphase[j]);
+                    v = (real)((double)v + ((double)(real)c->deck.sdc[j] +
+                                            (double)(real)c->deck.sac[j] * dm_sin((double)w * t + (double)ph)));
+                }
+        }
+        c->V_cond[i] = v;
+    }
+}
+
+void orc_efield(orc_case *c) {
+    int g, i;
+    orc_voltages(c, c->t);
+    for (g = 0; g < c->Gsize; ++g) {
+        real tp = 0;
+        for (i = 0; i < c->CondNUMR; ++i) tp += c->V_cond[i] * c->lap_phi[(size_t)i * c->Gsize + g];
+        tp += (real)1 * c->phi[g];
+        c->TotPot[g] = tp;
+    }
+    cond_sigma(c, c->phi, c->Sigma);     /* Poisson part only, as the reference */
+    v_to_efield(c, c->Sigma, c->TotPot);
+}
+
+/* ================= push: src/cuda_Move.cu:181-270 =================
+ * tag = relative cell step (0 = stays).  Positions are offsets in the cell. */
+void orc_move(orc_case *c) {
+    int isp, cell, ngy = c->ngy;
+    for (isp = 0; isp < c->nsp; ++isp) {
+        orc_species *s = &c->sp[isp];
+        int *tag = c->tag + (size_t)isp * s->cap;
+        real As = c->Ascale[isp];
+        for (cell = 0; cell < c->Gsize; ++cell) {
+            int k0 = s->cell_start[cell], k1 = s->cell_start[cell + 1], k;
+            if (k0 == k1) continue;
+            real ews = c->Ex[cell], ewn = c->Ex[cell + 1], ees = c->Ex[cell + ngy], een = c->Ex[cell + ngy + 1];
+            real yws = c->Ey[cell], ywn = c->Ey[cell + 1], yes = c->Ey[cell + ngy], yen = c->Ey[cell + ngy + 1];
+            for (k = k0; k < k1; ++k) {
+                real lx = s->x[k], ly = s->y[k];
+                real dvx = ews * (1 - lx) * (1 - ly) + ewn * (1 - lx) * ly + ees * lx * (1 - ly) + een * lx * ly;
+                real dvy = yws * (1 - lx) * (1 - ly) + ywn * (1 - lx) * ly + yes * lx * (1 - ly) + yen * lx * ly;
+                real mx = s->vx[k] + dvx * As;
+                real my = s->vy[k] + dvy * As;
+                lx += mx * c->dt_dx;
+                ly += my * c->dt_dy;
+                int t = 0;
+                if (ly >= 1 || ly < 0 || lx >= 1 || lx < 0) {
+                    if (ly >= 1) { t += 1; ly -= 1; }
+                    else if (ly < 0) { t -= 1; ly += 1; }
+                    if (lx >= 1) { t += ngy; lx -= 1; }
+                    else if (lx < 0) { t -= ngy; lx += 1; }
+                }
+                s->x[k] = lx; s->y[k] = ly; s->vx[k] = mx; s->vy[k] = my;
+                tag[k] = t;
+            }
+        }
+    }
+}
+
+/* ================= sort + boundary: src/cuda_Sortboundary.cu:41-258 =========
+ * Returns 1 and sets *dest (species-local node-indexed cell) when the
+ * particle survives; 0 when it is absorbed (conductor), deleted (Dirichlet)
+ * or deposited as surface charge (dielectric).  Position/velocity are
+ * transformed in place for Neumann reflections.  Structure codes the
+ * reference leaves undefined (e.g. -3, -5 from a Dirichlet node next to a
+ * Neumann wall) delete the particle. */
+#define RABS(v) ((v) < 0 ? -(v) : (v))
+static int resolve_crosser(orc_case *c, int isp, int ID, int MCID, real *px, real *py, real *pvx, real *pvy, int *dest) {
+    int ngy = c->ngy;
+    int x = ID / ngy;
+    int SM;
+    if (MCID > 1) SM = (ID + ngy + 2 + x) + (MCID + 1);
+    else if (MCID < -1) SM = (ID + ngy + 2 + x) + (MCID - 1);
+    else SM = (ID + ngy + 2 + x) + MCID;
+    const int *S = c->StructIdx;
+    int CID = ID + MCID;
+    int SI = S[SM];
+    real X = *px, Y = *py, VX = *pvx, VY = *pvy;
+    if (SI == 0) { *dest = CID; return 1; }
+    if (SI >= 1 && SI < 100) {
+        int pa = 0, pb = 0; real a = 0, b = 0, delta;
+        if (MCID == 1) { delta = X - VX / VY * Y; pa = CID; a = 1 - delta; pb = CID + ngy; b = delta; }
+        else if (MCID == -1) { delta = X + VX / VY * (1 - Y); pa = ID; a = 1 - delta; pb = ID + ngy; b = delta; }
+        else if (MCID == ngy) { delta = Y - VY / VX * X; pa = CID; a = 1 - delta; pb = CID + 1; b = delta; }
+        else if (MCID == -ngy) { delta = Y + VY / VX * (1 - X); pa = ID; a = 1 - delta; pb = ID + 1; b = delta; }
+        else if (MCID == ngy + 1) {
+            delta = X - VX / VY * Y;
+            if (delta >= 0) {
+                if (S[SM - 1]) { delta = Y - VY / VX * X; pa = CID - 1; a = -delta; pb = CID; b = 1 + delta; }
+                else { pa = CID; a = 1 - delta; pb = CID + ngy; b = delta; }
+            } else {
+                if (S[SM - ngy - 1]) { pa = ID + 1; a = -delta; pb = CID; b = 1 + delta; }
+                else { delta = Y - VY / VX * X; pa = CID; a = 1 - delta; pb = CID + 1; b = delta; }
+            }
+        } else if (MCID == ngy - 1) {
+            delta = X + VX / VY * (1 - Y);
+            if (delta >= 0) {
+                if (S[SM + 1]) { delta = Y - VY / VX * X; pa = CID + 1; a = 2 - delta; pb = CID + 2; b = delta - 1; }
+                else { pa = CID + 1; a = 1 - delta; pb = CID + 1 + ngy; b = delta; }
+            } else {
+                if (S[SM - ngy - 1]) { pa = ID; a = -delta; pb = CID + 1; b = 1 + delta; }
+                else { delta = Y - VY / VX * X; pa = CID; a = 1 - delta; pb = CID + 1; b = delta; }
+            }
+        } else if (MCID == -ngy + 1) {
+            delta = X - VX / VY * Y;
+            if (delta < 1) {
+                if (S[SM - 1]) { delta = Y + VY / VX * (1 - X); pa = CID + ngy - 1; a = -delta; pb = CID + ngy; b = 1 + delta; }
+                else { pa = CID; a = 1 - delta; pb = CID + ngy; b = delta; }
+            } else {
+                if (S[SM + ngy + 1]) { pa = ID + 1; a = 2 - delta; pb = ID + 1 + ngy; b = delta - 1; }
+                else { delta = Y + VY / VX * (1 - X); pa = ID + 1; a = 1 - delta; pb = ID + 2; b = delta; }
+            }
+        } else if (MCID == -ngy - 1) {
+            delta = X + VX / VY * (1 - Y);
+            if (delta < 1) {
+                if (S[SM + 1]) { delta = Y + VY / VX * (1 - X); pa = ID; a = 2 - delta; pb = ID + 1; b = delta - 1; }
+                else { pa = CID; a = 1 - delta; pb = ID; b = delta; }
+            } else {
+                if (S[SM + ngy + 1]) { pa = ID; a = 2 - delta; pb = ID + ngy; b = delta - 1; }
+                else { delta = Y + VY / VX * (1 - X); pa = ID - 1; a = 1 - delta; pb = ID; b = delta; }
+            }
+        }
+        real *sg = c->sigma_sp + (size_t)isp * c->Gsize;
+        sg[pa] += a;
+        sg[pb] += b;
+        return 0;
+    }
+    if (SI >= 100) { c->CondCharge[isp * c->CondNUMR + (SI - 101)] += 1; return 0; }
+    int sum, fx, fy, fvx, fvy;
+    switch (SI) {
+    case -2: sum = ngy; fx = 0; fy = 1; fvx = -1; fvy = 1; break;
+    case -6: sum = -ngy; fx = 0; fy = 1; fvx = -1; fvy = 1; break;
+    case -4: sum = 1; fx = 1; fy = 0; fvx = 1; fvy = -1; break;
+    case -8: sum = -1; fx = 1; fy = 0; fvx = 1; fvy = -1; break;
+    case -12: sum = ngy + 1; fx = 0; fy = 0; fvx = -1; fvy = -1; break;
+    case -16: sum = 1 - ngy; fx = 0; fy = 0; fvx = -1; fvy = -1; break;
+    case -20: sum = ngy - 1; fx = 0; fy = 0; fvx = -1; fvy = -1; break;
+    case -24: sum = -ngy - 1; fx = 0; fy = 0; fvx = -1; fvy = -1; break;
+    default: return 0;   /* -1 Dirichlet, and undefined codes */
+    }
+    *dest = CID + sum;
+    if (!fx) *px = 1 - X;
+    if (!fy) *py = 1 - Y;
+    *pvx = fvx * VX;
+    *pvy = fvy * VY;
+    return 1;
+}
+
+/* Rebucket after the push.  Order in the new bucket of cell d: particles of
+ * d that did not cross (old order), then survivors arriving in d, grouped by
+ * source cell in ascending order, each group in old order. */
+int orc_sort(orc_case *c) {
+    int isp;
+    for (isp = 0; isp < c->nsp; ++isp) {
+        orc_species *s = &c->sp[isp];
+        int *tag = c->tag + (size_t)isp * s->cap;
+        int n = s->n, G = c->Gsize, cell, k;
+        int *destof = zalloc(n, sizeof(int));
+        char *cls = zalloc(n, 1);
+        int *scnt = zalloc(G, sizeof(int)), *acnt = zalloc(G, sizeof(int));
+        for (cell = 0; cell < G; ++cell)
+            for (k = s->cell_start[cell]; k < s->cell_start[cell + 1]; ++k) {
+                if (tag[k] == 0) { destof[k] = cell; cls[k] = 0; scnt[cell]++; continue; }
+                int d;
+                if (resolve_crosser(c, isp, cell, tag[k], &s->x[k], &s->y[k], &s->vx[k], &s->vy[k], &d)) {
+                    destof[k] = d; cls[k] = 1; acnt[d]++;
+                } else { destof[k] = -1; cls[k] = 2; }
+            }
+        int *start = zalloc(G + 1, sizeof(int));
+        for (cell = 0; cell < G; ++cell) start[cell + 1] = start[cell] + scnt[cell] + acnt[cell];
+        int total = start[G];
+        real *nx = zalloc(s->cap, sizeof(real)), *ny = zalloc(s->cap, sizeof(real));
+        real *nvx = zalloc(s->cap, sizeof(real)), *nvy = zalloc(s->cap, sizeof(real)), *nvz = zalloc(s->cap, sizeof(real));
+        uint32_t *nid = zalloc(s->cap, sizeof(uint32_t));
+        int *sfill = zalloc(G, sizeof(int)), *afill = zalloc(G, sizeof(int));
+        for (k = 0; k < n; ++k) {
+            if (cls[k] == 2) continue;
+            int d = destof[k], pos;
+            if (cls[k] == 0) pos = start[d] + sfill[d]++;
+            else pos = start[d] + scnt[d] + afill[d]++;
+            nx[pos] = s->x[k]; ny[pos] = s->y[k];
+            nvx[pos] = s->vx[k]; nvy[pos] = s->vy[k]; nvz[pos] = s->vz[k]; nid[pos] = s->id[k];
+        }
+        free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->vz); free(s->id);
+        s->x = nx; s->y = ny; s->vx = nvx; s->vy = nvy; s->vz = nvz; s->id = nid;
+        memcpy(s->cell_start, start, (G + 1) * sizeof(int));
+        s->n = total;
+        free(destof); free(cls); free(scnt); free(acnt); free(start); free(sfill); free(afill);
+        memset(tag, 0, sizeof(int) * s->cap);
+    }
+    return 0;
+}
+
+/* ================= MCC (argon): src/cuda_mcc.cu:97-106, 139-201;
+ *                   src/cuda_mccAr.cu:41-232, 707-777 ================= */
+#define TWO_PI_R ((real)(2.0 * ORC_PI))
+#if defined(ORC_FLOAT)
+#define RSQRT(v) sqrtf(v)
+#else
+#define RSQRT(v) sqrt(v)
+#endif
+
+static real ar_sigma(const orc_case *c, int r, real engy) {
+    if (engy == 0) return 0;
+    int N = c->N_LOGX, T = c->TnRct;
+    real lengy = (real)dm_log10((double)engy);
+    real ee1 = c->idLOGX * (lengy - c->xe[0]);
+    int ee2 = (int)ee1;
+    real a1 = ee1 - (real)ee2;
+    real a2 = 1 - a1;
+    if (lengy < c->xe[0]) return c->cx[r];
+    if (lengy > c->xe[N - 1])
+        return (real)((double)c->cx[(N - 1) * T + r] * 0.1 * dm_exp(-(double)(lengy - c->xe[N - 1])));
+    if (ee2 >= N - 1) return c->cx[(N - 1) * T + r];   /* reference reads one past the table */
+    return a2 * c->cx[ee2 * T + r] + a1 * c->cx[(ee2 + 1) * T + r];
+}
+
+static void anewvel(real energy, real vel, real *nvx, real *nvy, real *nvz, real r1, real r2) {
+    real coschi;
+    if (energy < (real)1e-30) coschi = 1;
+    else coschi = (energy + 2 - 2 * (real)dm_pow((double)(energy + 1), (double)r1)) / energy;
+    real sinchi = RSQRT(RABS(1 - coschi * coschi));
+    real phi1 = TWO_PI_R * r2;
+    real cosphi = (real)dm_cos((double)phi1), sinphi = (real)dm_sin((double)phi1);
+    real r13 = *nvx, r23 = *nvy, r33 = *nvz, up1, up2, up3;
+    if (r33 == 1) { up1 = 0; up2 = 1; up3 = 0; } else { up1 = 0; up2 = 0; up3 = 1; }
+    real r12 = r23 * up3 - r33 * up2;
+    real r22 = r33 * up1 - r13 * up3;
+    real r32 = r13 * up2 - r23 * up1;
+    real mag = RSQRT(r12 * r12 + r22 * r22 + r32 * r32);
+    r12 /= mag; r22 /= mag; r32 /= mag;
+    real r11 = r22 * r33 - r32 * r23;
+    real r21 = r32 * r13 - r12 * r33;
+    real r31 = r12 * r23 - r22 * r13;
+    *nvx = vel * (r11 * sinchi * cosphi + r12 * sinchi * sinphi + r13 * coschi);
+    *nvy = vel * (r21 * sinchi * cosphi + r22 * sinchi * sinphi + r23 * coschi);
+    *nvz = vel * (r31 * sinchi * cosphi + r32 * sinchi * sinphi + r33 * coschi);
+}
+
+static void newvel_ionsc(real *vx, real *vy, real *vz, real vel, real r1, real r2) {
+    real coschi = RSQRT(r1);
+    real sinchi = RSQRT(RABS(1 - coschi * coschi));
+    real phi1 = TWO_PI_R * r2;
+    real cosphi = (real)dm_cos((double)phi1), sinphi = (real)dm_sin((double)phi1);
+    real r13 = *vx / vel, r23 = *vy / vel, r33 = *vz / vel, up1, up2, up3;
+    if (r33 == 1) { up1 = 0; up2 = 1; up3 = 0; } else { up1 = 0; up2 = 0; up3 = 1; }
+    real r12 = r23 * up3 - r33 * up2;
+    real r22 = r33 * up1 - r13 * up3;
+    real r32 = r13 * up2 - r23 * up1;
+    real mag = RSQRT(r12 * r12 + r22 * r22 + r32 * r32);
+    r12 /= mag; r22 /= mag; r32 /= mag;
+    real r11 = r22 * r33 - r32 * r23;
+    real r21 = r32 * r13 - r12 * r33;
+    real r31 = r12 * r23 - r22 * r13;
+    *vx = vel * coschi * (r11 * sinchi * cosphi + r12 * sinchi * sinphi + r13 * coschi);
+    *vy = vel * coschi * (r21 * sinchi * cosphi + r22 * sinchi * sinphi + r23 * coschi);
+    *vz = vel * coschi * (r31 * sinchi * cosphi + r32 * sinchi * sinphi + r33 * coschi);
+}
+
+static void dmaxwellv(real *vx, real *vy, real *vz, real vsaven, real vti, real Rphi, real Rthe) {
+    real aphi = TWO_PI_R * Rphi;
+    real costheta = 1 - 2 * Rthe;
+    real sintheta = RSQRT(1 - costheta * costheta);
+    *vx = vti * vsaven * sintheta * (real)dm_cos((double)aphi);
+    *vy = vti * vsaven * sintheta * (real)dm_sin((double)aphi);
+    *vz = vti * vsaven * costheta;
+}
+
+#define U(st) ((real)orc_uniform(st))
+#define MCC_TAG(isp) ((uint32_t)(((isp) << 8) | 1u))
+
+typedef struct { real x, y, vx, vy, vz; int cell; } orc_prod;
+
+/* Appends products to each bucket in parent order; returns 0 or -30 on
+ * capacity overflow. */
+int orc_mcc(orc_case *c) {
+    orc_species *se = &c->sp[0], *si = &c->sp[1];
+    int G = c->Gsize, ID, k;
+    uint64_t seed = (uint64_t)c->deck.rng_seed;
+    uint32_t step = (uint32_t)c->tstep;
+    int T = c->TnRct;
+    int maxp = se->n + 16;
+    orc_prod *pe = zalloc(maxp, sizeof(orc_prod)), *pi = zalloc(maxp, sizeof(orc_prod));
+    int npe = 0, npi = 0;
+    for (ID = 0; ID < G; ++ID) {
+        /* ions (Direct_Argon_ArIon) */
+        real Prob = c->Pion[ID];
+        for (k = si->cell_start[ID]; k < si->cell_start[ID + 1]; ++k) {
+            orc_stream st; orc_stream_init(&st, seed, si->id[k], step, MCC_TAG(1));
+            real R1 = U(&st);
+            if (R1 > Prob) continue;
+            int n = (int)((real)(c->nvel - 1) * U(&st));
+            real vnx, vny, vnz;
+            real rphi = U(&st), rthe = U(&st);
+            dmaxwellv(&vnx, &vny, &vnz, c->vsave[n], c->BackVel1[ID], rphi, rthe);
+            real VX = si->vx[k] - vnx, VY = si->vy[k] - vnz, VZ = si->vz[k] - vny;  /* y/z swap as reference */
+            real dum = VX * VX + VY * VY + VZ * VZ;
+            real engy = c->Escale[1] * dum;
+            real vel = RSQRT(dum);
+            R1 = U(&st) * c->sigv[2] / vel;
+            real sum = ar_sigma(c, 5, engy);
+            if (R1 <= sum) {
+                si->vx[k] = vnx; si->vy[k] = vny; si->vz[k] = vnz;
+                c->mccr[(size_t)ID * T + 5]++;
+            } else if (R1 <= (sum += ar_sigma(c, 6, engy))) {
+                real a = U(&st), b = U(&st);
+                newvel_ionsc(&VX, &VY, &VZ, vel, a, b);
+                si->vx[k] = VX + vnx; si->vy[k] = VY + vny; si->vz[k] = VZ + vnz;
+                c->mccr[(size_t)ID * T + 6]++;
+            }
+        }
+        /* electrons (Direct_Argon_Electron) */
+        real P1 = c->P1[ID], P2 = c->P2[ID];
+        for (k = se->cell_start[ID]; k < se->cell_start[ID + 1]; ++k) {
+            orc_stream st; orc_stream_init(&st, seed, se->id[k], step, MCC_TAG(0));
+            int coll = 0, j;
+            real R1 = 0;
+            for (j = 0; j < c->MCCn; ++j) { R1 = U(&st); if (R1 < P2) { coll = 1; break; } }
+            if (!coll) continue;
+            int target = (R1 <= P1) ? 0 : 1;
+            real VX = se->vx[k], VY = se->vy[k], VZ = se->vz[k];
+            real dum = VX * VX + VY * VY + VZ * VZ;
+            real vel = RSQRT(dum), vel2 = 0, rengy = 0;
+            VX /= vel; VY /= vel; VZ /= vel;
+            real engy = c->Escale[0] * dum;
+            real R2, sum;
+            int type = 1;
+            if (target == 0) {
+                R2 = U(&st) * c->sigv[0] / vel;
+                sum = ar_sigma(c, 0, engy);
+                if (R2 <= sum) { type = 0; c->mccr[(size_t)ID * T + 0]++; }
+                else if (engy > c->Th_e[1] && R2 <= (sum += ar_sigma(c, 1, engy))) {
+                    type = 0; engy -= c->Th_e[1]; vel = RSQRT(RABS(engy) / c->Escale[0]); c->mccr[(size_t)ID * T + 1]++;
+                } else if (engy > c->Th_e[2] && R2 <= (sum += ar_sigma(c, 2, engy))) {
+                    type = 0; engy -= c->Th_e[2]; vel = RSQRT(RABS(engy) / c->Escale[0]); c->mccr[(size_t)ID * T + 2]++;
+                } else if (engy > c->Th_e[3] && R2 <= (sum += ar_sigma(c, 3, engy))) {
+                    type = 2; engy -= c->Th_e[3];
+                    real u = U(&st);
+                    rengy = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+                    engy -= rengy;
+                    vel = RSQRT(RABS(rengy) / c->Escale[0]);
+                    vel2 = RSQRT(RABS(engy) / c->Escale[0]);
+                    c->mccr[(size_t)ID * T + 3]++;
+                }
+            } else {
+                R2 = U(&st) * c->sigv[1] / vel;
+                sum = ar_sigma(c, 4, engy);
+                if (engy > c->Th_e[4] && R2 <= sum) {
+                    type = 2; engy -= c->Th_e[4];
+                    real u = U(&st);
+                    rengy = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+                    engy -= rengy;
+                    vel = RSQRT(RABS(rengy) / c->Escale[0]);
+                    vel2 = RSQRT(RABS(engy) / c->Escale[0]);
+                    c->mccr[(size_t)ID * T + 4]++;
+                }
+            }
+            if (type == 0) {
+                real a = U(&st), b = U(&st);
+                anewvel(engy, vel, &VX, &VY, &VZ, a, b);
+                se->vx[k] = VX; se->vy[k] = VY; se->vz[k] = VZ;
+            } else if (type == 2) {
+                orc_prod *e = &pe[npe++], *ion = &pi[npi++];
+                e->cell = ID; e->x = se->x[k]; e->y = se->y[k];
+                e->vx = VX; e->vy = VY; e->vz = VZ;
+                real a = U(&st), b = U(&st);
+                anewvel(rengy, vel, &e->vx, &e->vy, &e->vz, a, b);
+                ion->cell = ID; ion->x = se->x[k]; ion->y = se->y[k];
+                int n = (int)((real)(c->nvel - 1) * U(&st));
+                real rphi = U(&st), rthe = U(&st);
+                dmaxwellv(&ion->vx, &ion->vy, &ion->vz, c->vsave[n], c->BackVel1[ID], rphi, rthe);
+                a = U(&st); b = U(&st);
+                anewvel(engy, vel2, &VX, &VY, &VZ, a, b);
+                se->vx[k] = VX; se->vy[k] = VY; se->vz[k] = VZ;
+            }
+            if (npe >= maxp - 1) {
+                maxp *= 2;
+                pe = realloc(pe, maxp * sizeof(orc_prod)); pi = realloc(pi, maxp * sizeof(orc_prod));
+            }
+        }
+    }
+    /* append products: bucket d = old bucket ++ products of d (parent order);
+     * new ids in global (cell, parent) order */
+    orc_species *sps[2] = {se, si};
+    orc_prod *lists[2] = {pe, pi};
+    int nps[2] = {npe, npi}, s2, rc = 0;
+    for (s2 = 0; s2 < 2; ++s2) {
+        orc_species *s = sps[s2];
+        int np = nps[s2];
+        if (s->n + np > s->cap) { rc = -30; continue; }
+        int *pc = zalloc(G, sizeof(int)), d, q;
+        for (q = 0; q < np; ++q) pc[lists[s2][q].cell]++;
+        int *start = zalloc(G + 1, sizeof(int));
+        for (d = 0; d < G; ++d) start[d + 1] = start[d] + (s->cell_start[d + 1] - s->cell_start[d]) + pc[d];
+        int tot = start[G];
+        real *nx = zalloc(s->cap, sizeof(real)), *ny = zalloc(s->cap, sizeof(real));
+        real *nvx = zalloc(s->cap, sizeof(real)), *nvy = zalloc(s->cap, sizeof(real)), *nvz = zalloc(s->cap, sizeof(real));
+        uint32_t *nid = zalloc(s->cap, sizeof(uint32_t));
+        int *fill = zalloc(G, sizeof(int));
+        for (d = 0; d < G; ++d) {
+            int o = s->cell_start[d], m = s->cell_start[d + 1] - o, r;
+            for (r = 0; r < m; ++r) {
+                int pos = start[d] + r;
+                nx[pos] = s->x[o + r]; ny[pos] = s->y[o + r];
+                nvx[pos] = s->vx[o + r]; nvy[pos] = s->vy[o + r]; nvz[pos] = s->vz[o + r]; nid[pos] = s->id[o + r];
+            }
+            fill[d] = start[d] + m;
+        }
+        for (q = 0; q < np; ++q) {     /* lists are already in (cell, parent) order */
+            orc_prod *p = &lists[s2][q];
+            int pos = fill[p->cell]++;
+            nx[pos] = p->x; ny[pos] = p->y; nvx[pos] = p->vx; nvy[pos] = p->vy; nvz[pos] = p->vz;
+            nid[pos] = s->next_id++;
+        }
+        free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->vz); free(s->id);
+        s->x = nx; s->y = ny; s->vx = nvx; s->vy = nvy; s->vz = nvz; s->id = nid;
+        memcpy(s->cell_start, start, (G + 1) * sizeof(int));
+        s->n = tot;
+        free(pc); free(start); free(fill);
+    }
+    free(pe); free(pi);
+    return rc;
+}
+
+/* ================= deposit: src/cuda_Deposit.cu:3-104, 277-326 ============= */
+void orc_deposit(orc_case *c) {
+    int G = c->Gsize, ngy = c->ngy, ngx = c->ngx, isp, cell, k, g, i;
+    for (g = 0; g < c->nsp * G; ++g) c->den[g] = 0;
+    for (isp = 0; isp < c->nsp; ++isp) {
+        orc_species *s = &c->sp[isp];
+        real *den = c->den + (size_t)isp * G;
+        for (cell = 0; cell < G; ++cell) {
+            int k0 = s->cell_start[cell], k1 = s->cell_start[cell + 1];
+            if (k0 == k1) continue;
+            real WS = 0, WN = 0, ES = 0, EN = 0;
+            for (k = k0; k < k1; ++k) {
+                real lx = s->x[k], ly = s->y[k];
+                WS += (1 - lx) * (1 - ly);
+                WN += (1 - lx) * ly;
+                ES += lx * (1 - ly);
+                EN += lx * ly;
+            }
+            den[cell] += WS; den[cell + 1] += WN; den[cell + ngy] += ES; den[cell + ngy + 1] += EN;
+        }
+        for (g = 0; g < G; ++g) {     /* DepositBoundary */
+            int f = c->Face[g];
+            if (f == ORC_LEFT || f == ORC_RIGHT || f == ORC_UP || f == ORC_DOWN) den[g] *= 2;
+            else if (f == ORC_UL || f == ORC_LL || f == ORC_UR || f == ORC_LR) den[g] *= 4;
+            if (c->Boundary[g] == ORC_NEUMANN) den[g] *= 2;
+            else if (c->Boundary[g] == ORC_DIRICHLET && f != ORC_NOFACE && c->CondID[g] != 0) den[g] *= 2;
+        }
+    }
+    /* SumSource; the reference leaves SD uninitialised on wall nodes whose
+     * inward neighbour is neither dielectric nor conductor - read as 0 here */
+    for (g = 0; g < G; ++g) {
+        int x = g / ngy, y = g % ngy, nb = -1;
+        real SD = 0, ss = 0;
+        if (x == 0) nb = g + ngy;
+        else if (x == ngx - 1) nb = g - ngy;
+        else if (y == 0) nb = g + 1;
+        else if (y == ngy - 1) nb = g - 1;
+        int bnd = nb >= 0 ? c->Boundary[nb] : c->Boundary[g];
+        real fac = nb >= 0 ? 2 : 1;
+        if (bnd == ORC_DIELECTRIC) {
+            for (i = 0; i < c->nsp; ++i) ss += c->sigma_sp[(size_t)i * G + g] * c->q_density[i];
+            c->Sigma[g] = fac * ss / c->Area[g];
+            SD = ss;
+        } else if (bnd == ORC_CONDUCTOR) {
+            for (i = 0; i < c->nsp; ++i) c->sigma_sp[(size_t)i * G + g] = 0;
+            SD = 0;
+        }
+        ss = 0;
+        for (i = 0; i < c->nsp; ++i) ss += c->den[(size_t)i * G + g] * c->q_density[i];
+        c->Source[g] = (real)((double)(ss + SD) / EPS0);
+    }
+    for (g = 0; g < G; ++g) if (c->A_idx[g]) c->b[c->A_idx[g] - 1] = c->Source[g];
+}
+
+/* ================= accumulators: src/cuda_Diagnostic.cu:505-547 (FP64) ===== */
+void orc_accumulate(orc_case *c) {
+    int g, G = c->Gsize;
+    for (g = 0; g < c->nsp * G; ++g) c->sum_den[g] += (double)c->den[g];
+    for (g = 0; g < G; ++g) {
+        c->sum_phi[g] += (double)c->TotPot[g];
+        c->sum_ex[g] += (double)c->Ex[g];
+        c->sum_ey[g] += (double)c->Ey[g];
+    }
+    c->n_accum++;
+}
+
+/* ================= one timestep: src/cuda_main.cu:112-235 ================= */
+int orc_step(orc_case *c) {
+    c->tstep++;
+    c->t = (double)c->tstep * (double)c->dt;
+    orc_field(c);
+    orc_efield(c);
+    orc_move(c);
+    int e = orc_sort(c);
+    if (e) return e;
+    e = orc_mcc(c);
+    if (e) return e;
+    orc_deposit(c);
+    orc_accumulate(c);
+    return 0;
+}
+
+int orc_run(orc_case *c, int nsteps) {
+    int i, e;
+    for (i = 0; i < nsteps; ++i) { e = orc_step(c); if (e) return e; }
+    return 0;
+}
+
+/* ================= accessors for the Python test harness ================= */
+typedef struct { const char *name; void *ptr; long count; int elsize; } orc_arr;
+
+static int lookup(orc_case *c, const char *name, orc_arr *out) {
+    long G = c->Gsize, C = c->Csize;
+    int rs = (int)sizeof(real);
+#define ENT(nm, p, cnt, es) if (!strcmp(name, nm)) { out->ptr = (void *)(p); out->count = (cnt); out->elsize = (es); return 0; }
+    ENT("Boundary", c->Boundary, G, 4) ENT("CondID", c->CondID, G, 4) ENT("Face", c->Face, G, 4)
+    ENT("DensRegion", c->DensRegion, G, 4) ENT("Area", c->Area, G, rs) ENT("Temp", c->Temp, G, rs)
+    ENT("PlasmaRegion", c->PlasmaRegion, C, 4) ENT("eps_r", c->eps_r, C, rs) ENT("eps", c->eps, C, rs)
+    ENT("StructIdx", c->StructIdx, (long)(c->ncx + 2) * (c->ncy + 2), 4)
+    ENT("A_idx", c->A_idx, G, 4) ENT("Ai", c->Ai, c->A_size + 1, 4)
+    ENT("Aj", c->Aj, c->A_size > 0 ? c->Ai[c->A_size] : 0, 4) ENT("Aval", c->Aval, c->A_size > 0 ? c->Ai[c->A_size] : 0, rs)
+    ENT("Atemp", c->Atemp_val, c->A_size > 0 ? c->Ai[c->A_size] : 0, rs)
+    ENT("Mdiag", c->Mdiag, c->A_size, rs) ENT("cond_b", c->cond_b, (long)c->CondNUMR * c->A_size, rs)
+    ENT("temp_b", c->temp_b, c->A_size, rs) ENT("lap_phi", c->lap_phi, (long)c->CondNUMR * G, rs)
+    ENT("BackDen1", c->BackDen1, G, rs) ENT("BackVel1", c->BackVel1, G, rs) ENT("FluidN", c->FluidN, (long)c->nfsp * G, rs)
+    ENT("xe", c->xe, c->N_LOGX, rs) ENT("xee", c->xee, c->N_LOGX, rs) ENT("cx", c->cx, (long)c->N_LOGX * c->TnRct, rs)
+    ENT("sigv", c->sigv, 3, rs) ENT("P1", c->P1, G, rs) ENT("P2", c->P2, G, rs) ENT("Pion", c->Pion, G, rs)
+    ENT("vsave", c->vsave, c->nvel, rs) ENT("phi", c->phi, G, rs) ENT("TotPot", c->TotPot, G, rs)
+    ENT("Sigma", c->Sigma, G, rs) ENT("Source", c->Source, G, rs) ENT("Ex", c->Ex, G, rs) ENT("Ey", c->Ey, G, rs)
+    ENT("den", c->den, (long)c->nsp * G, rs) ENT("sigma_sp", c->sigma_sp, (long)c->nsp * G, rs)
+    ENT("b", c->b, c->A_size, rs) ENT("xsol", c->xsol, c->A_size, 8) ENT("V_cond", c->V_cond, c->CondNUMR, rs)
+    ENT("CondCharge", c->CondCharge, (long)c->nsp * c->CondNUMR, rs) ENT("mccr", c->mccr, G * c->TnRct, 8)
+    ENT("sum_den", c->sum_den, (long)c->nsp * G, 8) ENT("sum_phi", c->sum_phi, G, 8)
+    ENT("sum_ex", c->sum_ex, G, 8) ENT("sum_ey", c->sum_ey, G, 8) ENT("lu", c->lu, (long)c->A_size * (2 * c->lu_bw + 1), 8)
+    if (!strncmp(name, "sp", 2) && name[2] >= '0' && name[2] <= '9' && name[3] == '.') {
+        int s = name[2] - '0';
+        if (s >= c->nsp) return -1;
+        orc_species *p = &c->sp[s];
+        const char *f = name + 4;
+        if (!strcmp(f, "x")) { out->ptr = p->x; out->count = p->cap; out->elsize = rs; return 0; }
+        if (!strcmp(f, "y")) { out->ptr = p->y; out->count = p->cap; out->elsize = rs; return 0; }
+        if (!strcmp(f, "vx")) { out->ptr = p->vx; out->count = p->cap; out->elsize = rs; return 0; }
+        if (!strcmp(f, "vy")) { out->ptr = p->vy; out->count = p->cap; out->elsize = rs; return 0; }
+        if (!strcmp(f, "vz")) { out->ptr = p->vz; out->count = p->cap; out->elsize = rs; return 0; }
+        if (!strcmp(f, "id")) { out->ptr = p->id; out->count = p->cap; out->elsize = 4; return 0; }
+        if (!strcmp(f, "cell_start")) { out->ptr = p->cell_start; out->count = G + 1; out->elsize = 4; return 0; }
+        if (!strcmp(f, "tag")) { out->ptr = c->tag + (size_t)s * p->cap; out->count = p->cap; out->elsize = 4; return 0; }
+    }
+#undef ENT
+    return -1;
+}
+
+void *orc_array(orc_case *c, const char *name, long *count, int *elsize) {
+    orc_arr a;
+    if (lookup(c, name, &a)) { *count = 0; *elsize = 0; return 0; }
+    *count = a.count; *elsize = a.elsize;
+    return a.ptr;
+}
+
+/* scalar getter/setter by name */
+double orc_scalar(orc_case *c, const char *name) {
+    if (!strcmp(name, "dt")) return (double)c->dt;
+    if (!strcmp(name, "dx")) return (double)c->dx;
+    if (!strcmp(name, "dy")) return (double)c->dy;
+    if (!strcmp(name, "t")) return c->t;
+    if (!strcmp(name, "tstep")) return (double)c->tstep;
+    if (!strcmp(name, "ngx")) return c->ngx;
+    if (!strcmp(name, "ngy")) return c->ngy;
+    if (!strcmp(name, "nsp")) return c->nsp;
+    if (!strcmp(name, "A_size")) return c->A_size;
+    if (!strcmp(name, "CondNUMR")) return c->CondNUMR;
+    if (!strcmp(name, "MCCn")) return c->MCCn;
+    if (!strcmp(name, "dt_mcc")) return (double)c->dt_mcc;
+    if (!strcmp(name, "nvel")) return c->nvel;
+    if (!strcmp(name, "TnRct")) return c->TnRct;
+    if (!strcmp(name, "n_accum")) return c->n_accum;
+    if (!strncmp(name, "np", 2)) return c->sp[name[2] - '0'].n;
+    if (!strncmp(name, "cap", 3)) return c->sp[name[3] - '0'].cap;
+    if (!strncmp(name, "next_id", 7)) return c->sp[name[7] - '0'].next_id;
+    if (!strncmp(name, "Ascale", 6)) return (double)c->Ascale[name[6] - '0'];
+    if (!strncmp(name, "Escale", 6)) return (double)c->Escale[name[6] - '0'];
+    if (!strncmp(name, "vti", 3)) return (double)c->vti[name[3] - '0'];
+    if (!strncmp(name, "q_density", 9)) return (double)c->q_density[name[9] - '0'];
+    if (!strncmp(name, "Denscale", 8)) return (double)c->Denscale[name[8] - '0'];
+    return 0.0 / 0.0;
+}
+
+void orc_set_np(orc_case *c, int isp, int n) { c->sp[isp].n = n; }
+void orc_set_tstep(orc_case *c, long tstep) { c->tstep = tstep; c->t = (double)tstep * (double)c->dt; }
+int orc_sizeof_deck(void) { return (int)sizeof(orc_deck); }
+int orc_sizeof_real(void) { return (int)sizeof(real); }
