@@ -1,0 +1,152 @@
+"""Benchmark: argon CCP PIC-MCC timesteps on one MI355X.
+
+Workload (default): 256x256 cells, dx = 0.2 mm, 200 V at 13.56 MHz between
+the top and bottom node rows, Neumann sides, Ar 50 mTorr, 200 macro-particles
+per cell per species (26.2 M), dt = T_RF/800, seeded Maxwellian loading.
+A step is one pass of the full loop (field solve, E field, push, walls, MCC,
+rebucketing, deposit, accumulators) over all particles, inputs resident in
+HBM.  The timed path is FP32 particles/fields with FP64 refinement of the
+field solve; the same workload is also timed on the FP64 plain path, and the
+CPU restatement (the oracle) is timed on a bounded sample on the host.
+
+Prints one JSON line (rank 0; this build runs on one GPU only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hybrid-final_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s, FP32-refined vs FP64 plain; p50/p99 ensemble case latency"
+
+
+def run_gpu(deck, precision, steps, warmup, timing=True):
+    from picmcc import Case
+    c = Case(deck, precision)
+    c.step(warmup)
+    c.sync()
+    inf0 = c.info()
+    if timing:
+        c.stage_timing(True)
+    t0 = time.perf_counter()
+    c.step(steps)
+    c.sync()
+    dt = time.perf_counter() - t0
+    inf1 = c.info()
+    stages, nst = c.stage_times() if timing else ({}, 0)
+    it, outer, rel = c.last_field_solve()
+    out = {
+        "seconds": dt,
+        "np0": (inf0.np[0], inf0.np[1]),
+        "np1": (inf1.np[0], inf1.np[1]),
+        "stages_ms": {k: v / max(nst, 1) for k, v in stages.items()},
+        "field_iters": it, "field_cycles": outer, "field_rel_res": rel,
+        "solver_kind": c.field_solver_kind(),
+    }
+    c.close()
+    return out
+
+
+def particles_avg(r):
+    return 0.5 * (sum(r["np0"]) + sum(r["np1"]))
+
+
+def cpu_baseline(deck, steps):
+    """Oracle (FP64 CPU restatement, one thread) on `steps` steps of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    o = Oracle(deck, "f64")
+    o.orc_deposit()
+    n0 = o.scalar("np0") + o.scalar("np1")
+    t0 = time.perf_counter()
+    rc = o.orc_run(steps)
+    dt = time.perf_counter() - t0
+    n1 = o.scalar("np0") + o.scalar("np1")
+    if rc != 0:
+        raise RuntimeError(f"oracle run failed with {rc}")
+    return 0.5 * (n0 + n1) * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=20, help="steps timed on the FP64 plain path")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs one discharge case on one GPU; --gpus must be 1 "
+                                   "(ensembles shard one case per GPU with no communication)"}))
+        sys.exit(2)
+
+    from picmcc import FP32_REFINED, FP64_PLAIN, decks
+    deck = decks.config2() if args.config == "c2" else decks.config1()
+
+    ref = run_gpu(deck, FP32_REFINED, args.steps, args.warmup)
+    value = particles_avg(ref) * args.steps / ref["seconds"]
+    plain = run_gpu(deck, FP64_PLAIN, args.plain_steps, 2, timing=False)
+    plain_value = particles_avg(plain) * args.plain_steps / plain["seconds"]
+
+    # roofline of the dominant particle kernel (HIP events on the case stream)
+    st = ref["stages_ms"]
+    ne, ni = [0.5 * (a + b) for a, b in zip(ref["np0"], ref["np1"])]
+    kernels = {
+        # name: (algorithmic bytes per launch, ms per launch); FP32 record = x,y,vx,vy,vz,id = 24 B
+        "scatter_e": (48.0 * ne, st.get("scatter_e", 0.0)),
+        "scatter_i": (48.0 * ni, st.get("scatter_i", 0.0)),
+        "count": (24.0 * (ne + ni), st.get("count", 0.0)),
+        "deposit": (8.0 * (ne + ni), st.get("deposit", 0.0)),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    bytes_, ms = kernels[dom]
+    achieved = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ref["seconds"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Maxwellian loading of the deck; no datasets)",
+        "config": {
+            "workload": ("Ar CCP 256x256 cells dx=0.2mm, 200 V 13.56 MHz, 50 mTorr, 200 particles/cell/species, "
+                         "dt=T_RF/800" if args.config == "c2" else
+                         "Ar CCP 64x64 cells dx=0.5mm, 100 V 13.56 MHz, 100 mTorr, 50 particles/cell/species, "
+                         "dt=T_RF/400"),
+            "particles": int(sum(ref["np1"])),
+            "field_solve": "FP32 separable inverse in FP64 refinement, tol 1e-10 relative"
+            if ref["solver_kind"] == 1 else "FP32 Jacobi PCG in FP64 refinement, tol 1e-10 relative",
+        },
+        "fp64_plain": {"value": plain_value, "unit": "particle-steps/s",
+                       "ms_per_step": plain["seconds"] / args.plain_steps * 1e3, "steps": args.plain_steps,
+                       "field_solve": "FP64 Jacobi PCG (reference algorithm), tol 1e-10 relative",
+                       "field_iters_last": plain["field_iters"]},
+        "refined_over_plain": value / plain_value if plain_value > 0 else None,
+        "field_solve_last": {"cycles": ref["field_cycles"], "rel_residual": ref["field_rel_res"]},
+        "stage_ms_per_step": {k: round(v, 5) for k, v in st.items()},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_, "ms_per_launch": ms},
+    }
+    if args.cpu_steps > 0:
+        cv, cdt = cpu_baseline(deck, args.cpu_steps)
+        line["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                                "sample": f"{args.cpu_steps} steps of the same workload on the FP64 oracle "
+                                          f"({cdt:.1f} s of CPU time, setup excluded)"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

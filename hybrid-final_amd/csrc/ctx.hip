@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -71,7 +72,10 @@ struct CaseBase {
     virtual int set_array(const char *, const void *, size_t) = 0;
     virtual int set_step(long long) = 0;
     virtual int last_solve(int *, int *, double *) = 0;
+    virtual int set_timing(int) = 0;
+    virtual int get_timing(double *, int, long long *) = 0;
     int rf_cycles = 0, dt_pic = 0;
+    int solver_kind = 0;   // 0 Jacobi PCG, 1 separable fast solve
 };
 
 template <typename R>
@@ -101,6 +105,14 @@ struct Case : CaseBase {
     PcgWork *pw = nullptr;
     PcgResult *d_res = nullptr;
     PcgOperator op{};
+    // separable fast solve (refined path)
+    bool spectral = false;
+    int field_solver = 0;
+    SpecDev spec{};
+    SpecState *spec_state = nullptr;
+    double *spec_part = nullptr;
+    float *spec_rf = nullptr;
+    std::vector<float *> spec_bufs;
     // particles: two buffers per species
     PView<R> cur[MAXSP]{}, nxt[MAXSP]{};
     uint8_t *tag[MAXSP] = {nullptr};
@@ -116,6 +128,15 @@ struct Case : CaseBase {
     long long tstep = 0;
     double t = 0.0;
     int tag_valid = 0;
+    // optional stage timers (HIP events on the case stream)
+    static constexpr int NT = 8;   // field, efield, count, bases, scatter, deposit, diag, (spare)
+    bool timing = false;
+    hipEvent_t ev[NT + 1] = {};
+    double t_ms[NT] = {0};
+    long long t_n = 0;
+    void mark(int k) {
+        if (timing) hipEventRecord(ev[k], st);
+    }
 
     ~Case() override {
         if (st) hipStreamSynchronize(st);
@@ -140,6 +161,11 @@ struct Case : CaseBase {
             if (pbase[s]) hipFree(pbase[s]);
         }
         pcg_work_destroy(pw);
+        for (int k = 0; k <= NT; ++k)
+            if (ev[k]) hipEventDestroy(ev[k]);
+        for (float *p : spec_bufs) hipFree(p);
+        if (spec_state) hipFree(spec_state);
+        if (spec_part) hipFree(spec_part);
         if (st) hipStreamDestroy(st);
     }
 
@@ -234,6 +260,7 @@ struct Case : CaseBase {
         device = o.device;
         if (o.inner_tol > 0) inner_tol = o.inner_tol;
         if (o.pcg_maxit > 0) pcg_maxit = o.pcg_maxit;
+        field_solver = o.field_solver;
         HIPCHK(hipSetDevice(device));
         std::string err;
         int e = build_setup<R>(d, S, err, o.load != 0);
@@ -292,10 +319,48 @@ struct Case : CaseBase {
             HIPCHK(hipMemset(b64, 0, std::max(1, N) * sizeof(double)));
             op.N = N; op.col = pcol; op.valf = pvalf; op.vald = pvald; op.mdiagf = pmdf; op.mdiagd = pmdd;
             int werr = 0;
-            pw = pcg_work_create(N, &werr);
+            const char *nb_env = getenv("PIC_PCG_BLOCKS");
+            pw = pcg_work_create(N, &werr, nb_env ? atoi(nb_env) : 0);
             if (!pw) return fail(-40, "field solver workspace allocation failed");
             HIPCHK(dalloc(&d_res, 1));
             HIPCHK(hipMemset(d_res, 0, sizeof(PcgResult)));
+            if (precision == PIC_FP32_REFINED && field_solver == 0) {
+                Spectral sp;
+                std::string why;
+                if (build_spectral<R>(S, sp, why)) {
+                    const int nx = sp.nx, ny = sp.ny;
+                    auto f32 = [&](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
+                    std::vector<double> VyiT((size_t)ny * ny), VyT((size_t)ny * ny), dinv((size_t)nx * ny);
+                    for (int a = 0; a < ny; ++a)
+                        for (int b2 = 0; b2 < ny; ++b2) {
+                            VyiT[(size_t)a * ny + b2] = sp.Vyi[(size_t)b2 * ny + a];
+                            VyT[(size_t)a * ny + b2] = sp.Vy[(size_t)b2 * ny + a];
+                        }
+                    for (int a = 0; a < nx; ++a)
+                        for (int b2 = 0; b2 < ny; ++b2) {
+                            double den = sp.cx * sp.lx[a] + sp.cy * sp.ly[b2];
+                            dinv[(size_t)a * ny + b2] = den > 1e-300 ? 1.0 / den : 0.0;
+                        }
+                    float *p;
+                    const std::vector<double> *srcs[5] = {&sp.Vxi, &VyiT, &sp.Vx, &VyT, &dinv};
+                    const float *dst[5];
+                    for (int q = 0; q < 5; ++q) {
+                        std::vector<float> f = f32(*srcs[q]);
+                        HIPCHK(upload(&p, f));
+                        spec_bufs.push_back(p);
+                        dst[q] = p;
+                    }
+                    spec.nx = nx; spec.ny = ny;
+                    spec.Vxi = dst[0]; spec.VyiT = dst[1]; spec.Vx = dst[2]; spec.VyT = dst[3]; spec.dinv = dst[4];
+                    HIPCHK(dalloc(&p, (size_t)nx * ny)); spec_bufs.push_back(p); spec.T1 = p;
+                    HIPCHK(dalloc(&p, (size_t)nx * ny)); spec_bufs.push_back(p); spec.T2 = p;
+                    HIPCHK(dalloc(&spec_rf, N)); spec_bufs.push_back(spec_rf);
+                    HIPCHK(dalloc(&spec_state, 1));
+                    HIPCHK(dalloc(&spec_part, 2 * ((N + 255) / 256) + 2));
+                    spectral = true;
+                    solver_kind = 1;
+                }
+            }
         }
         // particles
         int capmax = 0;
@@ -392,6 +457,7 @@ struct Case : CaseBase {
         launch_rhs_to_fp64<R>(N, brhs, b64, st);
         const double tol = S.deck.pcg_tol > 0 ? S.deck.pcg_tol : 1e-10;
         if (precision == PIC_FP64_PLAIN) pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st);
+        else if (spectral) spectral_solve_refined(op, spec, b64, x64, tol, 4, spec_state, spec_part, spec_rf, d_res, st);
         else pcg_solve_refined(op, b64, x64, tol, inner_tol, pcg_maxit, pw, d_res, st);
         HIPCHK(hipGetLastError());
         launch_x_to_phi<R>(G, A_idx, x64, phi, st);
@@ -455,6 +521,7 @@ struct Case : CaseBase {
         const uint32_t step = (uint32_t)tstep;
         launch_fused_count<R>(kc, cur[0], 0, step, cnt[0], pcnt, mccr, condcharge, sigma_sp, st);
         launch_fused_count<R>(kc, cur[1], 1, step, cnt[1], nullptr, mccr, condcharge, sigma_sp, st);
+        mark(3);
         // electrons: survivors + ionisation electrons; global product order
         launch_totals(G, inv_start, inv_ent, cnt[0], pcnt, total, alive, ptotal, st);
         launch_scan(total, nxt[0].cell_start, G, scan_tmp, st);
@@ -465,10 +532,12 @@ struct Case : CaseBase {
         launch_totals(G, inv_start, inv_ent, cnt[1], pcnt, total, alive, nullptr, st);
         launch_scan(total, nxt[1].cell_start, G, scan_tmp, st);
         launch_assign_bases(G, inv_start, inv_ent, cnt[1], pcnt, nxt[1].cell_start, alive, pstart, base[1], pbase[1],
-                            cellp == nullptr ? pid_off : pid_off, st);
+                            pid_off, st);
         // pid_off recomputed identically (same product counts and order)
+        mark(4);
         launch_fused_scatter<R>(kc, cur[0], nxt[0], nxt[1], 0, step, base[0], pbase[0], pbase[1], pid_off, next_id,
                                 nxt[0].cap, nxt[1].cap, d_err, st);
+        mark(5);
         launch_fused_scatter<R>(kc, cur[1], nxt[1], nxt[1], 1, step, base[1], pbase[1], pbase[1], pid_off, next_id,
                                 nxt[1].cap, nxt[1].cap, d_err, st);
         launch_bump_ids(next_id, pstart + G, nsp, st);
@@ -480,12 +549,48 @@ struct Case : CaseBase {
     int step_fused(int n) override {
         for (int i = 0; i < n; ++i) {
             advance_time();
+            mark(0);
             if (int e = field_solve()) return e;
+            mark(1);
             if (int e = efield()) return e;
+            mark(2);
             if (int e = particle_stage_fused()) return e;
+            mark(6);
             if (int e = deposit()) return e;
+            mark(7);
             if (int e = diagnostic()) return e;
+            mark(8);
+            if (timing) {
+                HIPCHK(hipEventSynchronize(ev[8]));
+                for (int k = 0; k < 8; ++k) {
+                    float ms = 0.f;
+                    if (k == 5) continue;
+                    int a = k, b = k + 1;
+                    if (k >= 2 && k <= 4) continue;
+                    hipEventElapsedTime(&ms, ev[a], ev[b]);
+                    t_ms[k] += ms;
+                }
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, ev[2], ev[3]); t_ms[2] += ms;
+                hipEventElapsedTime(&ms, ev[3], ev[4]); t_ms[3] += ms;
+                hipEventElapsedTime(&ms, ev[4], ev[5]); t_ms[4] += ms;
+                hipEventElapsedTime(&ms, ev[5], ev[6]); t_ms[5] += ms;
+                ++t_n;
+            }
         }
+        return 0;
+    }
+    int set_timing(int on) override {
+        if (on && !ev[0])
+            for (int k = 0; k <= NT; ++k) HIPCHK(hipEventCreate(&ev[k]));
+        timing = on != 0;
+        for (int k = 0; k < NT; ++k) t_ms[k] = 0;
+        t_n = 0;
+        return 0;
+    }
+    int get_timing(double *ms, int n, long long *steps) override {
+        for (int k = 0; k < n && k < NT; ++k) ms[k] = t_ms[k];
+        if (steps) *steps = t_n;
         return 0;
     }
     int sync() override {
@@ -874,6 +979,21 @@ int pic_set_step(pic_ctx *ctx, long long tstep) { CTX_OR_FAIL(ctx); return ctx->
 int pic_last_field_solve(pic_ctx *ctx, int *iters, int *outer, double *rel_res) {
     CTX_OR_FAIL(ctx);
     return ctx->c->last_solve(iters, outer, rel_res);
+}
+
+int pic_stage_timing(pic_ctx *ctx, int enable) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->set_timing(enable);
+}
+int pic_stage_times(pic_ctx *ctx, double *ms, int n, long long *steps) {
+    CTX_OR_FAIL(ctx);
+    if (!ms) return fail(-1, "null buffer");
+    return ctx->c->get_timing(ms, n, steps);
+}
+
+int pic_field_solver_kind(pic_ctx *ctx) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->solver_kind;
 }
 
 // ------------------------------------------------------------------ host-only setup

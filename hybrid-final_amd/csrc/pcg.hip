@@ -39,9 +39,12 @@ struct PcgWork {
 namespace {
 
 __device__ inline void grid_sync(GridSync *gs, unsigned nblk) {
-    __threadfence();
+    // the block's stores happen-before thread 0's agent-scope release via the
+    // workgroup barrier; thread 0's acquire makes every block's stores
+    // visible to the whole block after the second barrier
     __syncthreads();
     if (threadIdx.x == 0) {
+        __threadfence();
         unsigned g = __hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         unsigned arrived = __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblk - 1) {
@@ -51,9 +54,9 @@ __device__ inline void grid_sync(GridSync *gs, unsigned nblk) {
             while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
                 __builtin_amdgcn_s_sleep(1);
         }
+        __threadfence();
     }
     __syncthreads();
-    __threadfence();
 }
 
 // block-level sum of NV doubles with a fixed tree; result valid in all threads
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
 
 }  // namespace
 
-PcgWork *pcg_work_create(int N, int *err) {
+PcgWork *pcg_work_create(int N, int *err, int want_blocks) {
     PcgWork *w = new PcgWork();
     w->N = N;
     int dev = 0, ncu = 0, per_cu = 0;
@@ -288,6 +291,7 @@ PcgWork *pcg_work_create(int N, int *err) {
     int resident = ncu * (per_cu > 0 ? per_cu : 1);
     int need = (N + PCG_BLOCK - 1) / PCG_BLOCK;
     int nblk = need < ncu ? need : ncu;          // one block per CU at most
+    if (want_blocks > 0 && want_blocks < nblk) nblk = want_blocks;
     if (nblk > resident) nblk = resident;
     if (nblk > PCG_BLOCK) nblk = PCG_BLOCK;      // grid_sum keeps one partial per thread
     if (nblk < 1) nblk = 1;

@@ -88,7 +88,7 @@ struct PcgOperator {
 };
 
 struct PcgWork;   // device scratch owned by the context
-PcgWork *pcg_work_create(int N, int *err);
+PcgWork *pcg_work_create(int N, int *err, int want_blocks = 0);
 void pcg_work_destroy(PcgWork *);
 int pcg_grid_blocks(PcgWork *);
 
@@ -98,5 +98,19 @@ void pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, doub
                           PcgResult *res_dev, hipStream_t s);
 void pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol, int maxit,
                        PcgWork *w, PcgResult *res_dev, hipStream_t s);
+
+// ---- separable fast solve (spectral.hip)
+struct SpecDev {
+    int nx, ny;
+    const float *Vxi, *VyiT, *Vx, *VyT, *dinv;   // dinv = 1/(cx lx_a + cy ly_b), 0 for a null mode
+    float *T1, *T2;
+};
+struct SpecState {
+    int done, outer, status, pad;
+    double rel;
+};
+void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const double *b, double *x, double tol,
+                            int max_cycles, SpecState *state, double *partials, float *rf, PcgResult *res,
+                            hipStream_t st);
 
 }  // namespace pic

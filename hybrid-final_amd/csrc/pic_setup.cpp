@@ -2,6 +2,7 @@
 #include "pic_setup.h"
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <type_traits>
 
@@ -614,20 +615,23 @@ struct Lcg {
     }
 };
 
+// F(v) = erf(v) - 2 v exp(-v^2)/sqrt(pi), with the float products of
+// src/Particle.c:227-230
 inline float maxwell_cdf(float v) {
-    return (float)(-2 * (double)v * std::exp(-(double)v * (double)v) / std::sqrt(kPi) + std::erf((double)v));
+    float m2v = -2 * v, mvv = -v * v;
+    return (float)((double)m2v * std::exp((double)mvv) / std::sqrt(kPi) + std::erf((double)v));
 }
 
 template <typename R>
 void maxwell_table(Setup<R> &s) {
-    int nvel = (int)(1. / (1 - (double)maxwell_cdf(3.3f)));
+    int nvel = (int)(1. / (double)(1 - maxwell_cdf(3.3f)));
     float dv = (float)(std::sqrt(kPi) / (4.0 * nvel));
     s.nvel = nvel;
     s.vsave.assign(nvel, 0);
     int i = 0;
     for (int n = 0; n < nvel; ++n) {
-        double rr = (1.0 * n) / nvel;
-        while ((double)maxwell_cdf((float)i * dv) < rr) ++i;
+        float rr = (float)((1.0 * n) / nvel);
+        while (maxwell_cdf((float)i * dv) < rr) ++i;
         s.vsave[n] = (R)(float)(std::sqrt(2.0) * i * (double)dv);
     }
 }
@@ -715,6 +719,143 @@ int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool do_load) {
     }
     return 0;
 }
+
+// ---------------------------------------------------------------- spectral
+namespace {
+enum EndType { END_N, END_D };
+
+// eigenvectors (columns) and eigenvalues of the n x n second-difference
+// matrix with the given ends: interior rows [-1 2 -1], a Neumann end row
+// [2 -2] (mirror), a Dirichlet end row [2 -1]
+void tridiag_eigen(int n, EndType left, EndType right, std::vector<double> &V, std::vector<double> &lam) {
+    const double PI = 3.14159265358979323846;
+    V.assign((size_t)n * n, 0.0);
+    lam.assign(n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        double th;
+        for (int a = 0; a < n; ++a) {
+            double v;
+            if (left == END_N && right == END_N) { th = n > 1 ? PI * k / (n - 1) : 0.0; v = std::cos(th * a); }
+            else if (left == END_D && right == END_D) { th = PI * (k + 1) / (n + 1); v = std::sin(th * (a + 1)); }
+            else if (left == END_N) { th = PI * (2 * k + 1) / (2.0 * n); v = std::cos(th * a); }
+            else { th = PI * (2 * k + 1) / (2.0 * n); v = std::cos(th * (n - 1 - a)); }
+            V[(size_t)a * n + k] = v;
+        }
+        if (left == END_N && right == END_N) th = n > 1 ? PI * k / (n - 1) : 0.0;
+        else if (left == END_D && right == END_D) th = PI * (k + 1) / (n + 1);
+        else th = PI * (2 * k + 1) / (2.0 * n);
+        lam[k] = 2.0 - 2.0 * std::cos(th);
+    }
+}
+
+void tridiag_matrix(int n, EndType left, EndType right, std::vector<double> &T) {
+    T.assign((size_t)n * n, 0.0);
+    for (int a = 0; a < n; ++a) {
+        T[(size_t)a * n + a] = 2.0;
+        if (a > 0) T[(size_t)a * n + a - 1] = -1.0;
+        if (a < n - 1) T[(size_t)a * n + a + 1] = -1.0;
+    }
+    if (n > 1 && left == END_N) T[1] = -2.0;
+    if (n > 1 && right == END_N) T[(size_t)(n - 1) * n + n - 2] = -2.0;
+}
+
+bool invert(int n, const std::vector<double> &A, std::vector<double> &Ai) {
+    std::vector<double> M(A);
+    Ai.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Ai[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(M[(size_t)r * n + c]) > std::fabs(M[(size_t)piv * n + c])) piv = r;
+        if (M[(size_t)piv * n + c] == 0.0) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(M[(size_t)c * n + k], M[(size_t)piv * n + k]);
+                std::swap(Ai[(size_t)c * n + k], Ai[(size_t)piv * n + k]);
+            }
+        double d = 1.0 / M[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) { M[(size_t)c * n + k] *= d; Ai[(size_t)c * n + k] *= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double f = M[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
+                Ai[(size_t)r * n + k] -= f * Ai[(size_t)c * n + k];
+            }
+        }
+    }
+    return true;
+}
+}  // namespace
+
+template <typename R>
+bool build_spectral(const Setup<R> &S, Spectral &sp, std::string &why) {
+    sp = Spectral();
+    const int ngy = S.ngy;
+    int i0 = S.ngx, i1 = -1, j0 = S.ngy, j1 = -1;
+    for (int i = 0; i < S.ngx; ++i)
+        for (int j = 0; j < S.ngy; ++j)
+            if (S.A_idx[i * ngy + j]) { i0 = std::min(i0, i); i1 = std::max(i1, i); j0 = std::min(j0, j); j1 = std::max(j1, j); }
+    if (i1 < 0) { why = "no unknowns"; return false; }
+    const int nx = i1 - i0 + 1, ny = j1 - j0 + 1;
+    if ((long)nx * ny != S.A_size) { why = "unknowns do not form a rectangle"; return false; }
+    for (int i = i0; i <= i1; ++i)
+        for (int j = j0; j <= j1; ++j)
+            if (S.A_idx[i * ngy + j] != (i - i0) * ny + (j - j0) + 1) { why = "unknown numbering is not row-major"; return false; }
+    for (size_t c = 0; c < S.eps_r.size(); ++c)
+        if (S.eps_r[c] != S.eps_r[0]) { why = "non-uniform permittivity"; return false; }
+    const EndType xl = i0 == 0 ? END_N : END_D, xr = i1 == S.ncx ? END_N : END_D;
+    const EndType yl = j0 == 0 ? END_N : END_D, yr = j1 == S.ncy ? END_N : END_D;
+    if (xl == END_N && xr == END_N && yl == END_N && yr == END_N) { why = "pure Neumann operator"; return false; }
+    const double cx = (double)(R)((R)(0.5 * (double)S.dy / (double)S.dx) * (S.eps_r[0] + S.eps_r[0]));
+    const double cy = (double)(R)((R)(0.5 * (double)S.dx / (double)S.dy) * (S.eps_r[0] + S.eps_r[0]));
+    std::vector<double> Tx, Ty;
+    tridiag_matrix(nx, xl, xr, Tx);
+    tridiag_matrix(ny, yl, yr, Ty);
+    // every matrix entry must match cx Tx (x) I + I (x) cy Ty
+    const double tol = std::is_same<R, float>::value ? 1e-6 : 1e-13;
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b) {
+            const int row = a * ny + b;
+            int seen = 0;
+            for (int q = S.Ai[row]; q < S.Ai[row + 1]; ++q) {
+                const int col = S.Aj[q], a2 = col / ny, b2 = col % ny;
+                double model;
+                if (col == row) model = cx * Tx[(size_t)a * nx + a] + cy * Ty[(size_t)b * ny + b];
+                else if (b2 == b) model = cx * Tx[(size_t)a * nx + a2];
+                else if (a2 == a) model = cy * Ty[(size_t)b * ny + b2];
+                else { why = "coupling outside the 5-point stencil"; return false; }
+                const double v = (double)S.Aval[q];
+                if (std::fabs(v - model) > tol * (std::fabs(model) + cx + cy)) { why = "coefficients differ from the separable model"; return false; }
+                ++seen;
+            }
+            int expect = 1 + (a > 0) + (a < nx - 1) + (b > 0) + (b < ny - 1);
+            if (seen != expect) { why = "stencil entry count differs from the separable model"; return false; }
+        }
+    sp.nx = nx; sp.ny = ny; sp.cx = cx; sp.cy = cy;
+    tridiag_eigen(nx, xl, xr, sp.Vx, sp.lx);
+    tridiag_eigen(ny, yl, yr, sp.Vy, sp.ly);
+    if (!invert(nx, sp.Vx, sp.Vxi) || !invert(ny, sp.Vy, sp.Vyi)) { why = "singular eigenbasis"; return false; }
+    // check T V = V diag(lambda)
+    auto check = [](int n, const std::vector<double> &T, const std::vector<double> &V, const std::vector<double> &l) {
+        double err = 0, scale = 0;
+        for (int a = 0; a < n; ++a)
+            for (int k = 0; k < n; ++k) {
+                double s = 0;
+                for (int m = 0; m < n; ++m) s += T[(size_t)a * n + m] * V[(size_t)m * n + k];
+                err = std::max(err, std::fabs(s - l[k] * V[(size_t)a * n + k]));
+                scale = std::max(scale, std::fabs(V[(size_t)a * n + k]));
+            }
+        return err <= 1e-9 * 4.0 * scale;
+    };
+    if (!check(nx, Tx, sp.Vx, sp.lx) || !check(ny, Ty, sp.Vy, sp.ly)) { why = "eigenbasis check failed"; return false; }
+    sp.ok = true;
+    return true;
+}
+
+template bool build_spectral<float>(const Setup<float> &, Spectral &, std::string &);
+template bool build_spectral<double>(const Setup<double> &, Spectral &, std::string &);
 
 template int build_setup<float>(const Deck &, Setup<float> &, std::string &, bool);
 template int build_setup<double>(const Deck &, Setup<double> &, std::string &, bool);

@@ -87,6 +87,20 @@ struct Setup {
 template <typename R>
 int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool load = true);
 
+// Separable field operator: when the unknowns form a rectangle with uniform
+// coefficients, A = cx Tx (x) I + I (x) cy Ty with Tx, Ty the 1-D second
+// difference operators (Neumann mirror or Dirichlet ends), so
+// A^-1 = (Vx (x) Vy) diag(1/(cx lx_a + cy ly_b)) (Vx^-1 (x) Vy^-1).
+// Row-major nx x nx / ny x ny FP64 matrices.
+struct Spectral {
+    bool ok = false;
+    int nx = 0, ny = 0;
+    double cx = 0, cy = 0;
+    std::vector<double> Vx, Vxi, Vy, Vyi, lx, ly;
+};
+template <typename R>
+bool build_spectral(const Setup<R> &S, Spectral &sp, std::string &why);
+
 // FP64 banded LU without pivoting (the field operator is an M-matrix).
 struct BandLU {
     int N = 0, bw = 0;

@@ -29,7 +29,7 @@ class PicError(RuntimeError):
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int), ("inner_tol", C.c_double),
-                ("pcg_maxit", C.c_int), ("load", C.c_int)]
+                ("pcg_maxit", C.c_int), ("load", C.c_int), ("field_solver", C.c_int)]
 
 
 class Info(C.Structure):
@@ -67,6 +67,9 @@ EXPORTS = {
     "pic_set_step": (C.c_int, [C.c_void_p, C.c_longlong]),
     "pic_last_field_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_double)]),
+    "pic_field_solver_kind": (C.c_int, [C.c_void_p]),
+    "pic_stage_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "pic_stage_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_longlong)]),
     "pic_setup_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "pic_setup_array_info": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
@@ -166,9 +169,10 @@ class HostSetup:
 class Case:
     """One discharge case resident on the GPU."""
 
-    def __init__(self, deck, precision=FP32_REFINED, device=0, inner_tol=0.0, pcg_maxit=0, load=True):
+    def __init__(self, deck, precision=FP32_REFINED, device=0, inner_tol=0.0, pcg_maxit=0, load=True,
+                 field_solver=0):
         L = lib()
-        o = Options(precision, device, inner_tol, pcg_maxit, int(load))
+        o = Options(precision, device, inner_tol, pcg_maxit, int(load), field_solver)
         h = C.c_void_p()
         _check(L.pic_create(deck_text(deck).encode(), C.byref(o), C.byref(h)))
         self.h = h
@@ -213,6 +217,20 @@ class Case:
 
     def set_step(self, tstep):
         _check(lib().pic_set_step(self.h, tstep))
+
+    STAGES = ("field", "efield", "count", "bases", "scatter_e", "scatter_i", "deposit", "accumulate")
+
+    def stage_timing(self, enable=True):
+        _check(lib().pic_stage_timing(self.h, int(enable)))
+
+    def stage_times(self):
+        ms = (C.c_double * 8)()
+        n = C.c_longlong()
+        _check(lib().pic_stage_times(self.h, ms, 8, C.byref(n)))
+        return dict(zip(self.STAGES, list(ms))), n.value
+
+    def field_solver_kind(self):
+        return lib().pic_field_solver_kind(self.h)
 
     def last_field_solve(self):
         it, out, rr = C.c_int(), C.c_int(), C.c_double()

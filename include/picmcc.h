@@ -33,6 +33,7 @@ typedef struct {
     double inner_tol;     /* refined path: relative residual drop per FP32 inner solve (0 = 1e-4) */
     int pcg_maxit;        /* iteration cap per field solve (0 = 100000, the reference's cap) */
     int load;             /* 1 = load the initial particles from the deck (default when opt is NULL) */
+    int field_solver;     /* 0 = fast separable solve when the operator allows it (refined path), 1 = PCG */
 } pic_options;
 
 typedef struct {
@@ -86,6 +87,14 @@ int pic_set_array(pic_ctx *ctx, const char *name, const void *host, size_t bytes
 const char *pic_array_names(void);
 int pic_set_step(pic_ctx *ctx, long long tstep);
 int pic_last_field_solve(pic_ctx *ctx, int *iters, int *outer, double *rel_res);
+/* per-stage GPU time of pic_step, from HIP events on the case's stream:
+ * ms[0] field solve, [1] E field, [2] count pass (push+walls+MCC),
+ * [3] scan/bases, [4] electron scatter, [5] ion scatter, [6] deposit,
+ * [7] accumulators; summed over *steps steps since enabling */
+int pic_stage_timing(pic_ctx *ctx, int enable);
+int pic_stage_times(pic_ctx *ctx, double *ms, int n, long long *steps);
+/* 0 = Jacobi PCG, 1 = separable fast solve (FP32 spectral inverse in FP64 refinement) */
+int pic_field_solver_kind(pic_ctx *ctx);
 
 /* ---------------- host-only setup (no device needed) ---------------- */
 /* deck -> geometry maps, field operator, Laplace solutions, tables and the

@@ -589,17 +589,19 @@ static float orc_frand(orc_lcg *g) {
     return (float)(g->seed / 2147483646.0);
 }
 static float orc_F(float v) {
-    return (float)(-2 * (double)v * exp(-(double)v * (double)v) / sqrt(ORC_PI) + erf((double)v));
+    /* float products as in src/Particle.c:227-230 */
+    float m2v = -2 * v, mvv = -v * v;
+    return (float)((double)m2v * exp((double)mvv) / sqrt(ORC_PI) + erf((double)v));
 }
 static void maxwell_table(orc_case *c) {
-    int nvel = (int)(1. / (1 - (double)orc_F((float)NVTS)));
+    int nvel = (int)(1. / (double)(1 - orc_F((float)NVTS)));
     float dv = (float)(sqrt(ORC_PI) / (4.0 * nvel));
     c->nvel = nvel;
     c->vsave = xcalloc(nvel, sizeof(real));
     int i = 0, n;
     for (n = 0; n < nvel; ++n) {
-        double rr = (1.0 * n) / nvel;
-        while ((double)orc_F((float)i * dv) < rr) i++;
+        float rr = (float)((1.0 * n) / nvel);
+        while (orc_F((float)i * dv) < rr) i++;
         c->vsave[n] = (real)(float)(sqrt(2.0) * i * (double)dv);
     }
 }

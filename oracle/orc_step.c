@@ -795,3 +795,35 @@ void orc_set_np(orc_case *c, int isp, int n) { c->sp[isp].n = n; }
 void orc_set_tstep(orc_case *c, long tstep) { c->tstep = tstep; c->t = (double)tstep * (double)c->dt; }
 int orc_sizeof_deck(void) { return (int)sizeof(orc_deck); }
 int orc_sizeof_real(void) { return (int)sizeof(real); }
+
+/* ================= helpers exported for the pinning tests ================= */
+void orc_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { orc_philox4x32_10(ctr, key, out); }
+
+double orc_detmath(int fn, double x, double y) {
+    switch (fn) {
+    case 0: return dm_exp(x);
+    case 1: return dm_log(x);
+    case 2: return dm_log10(x);
+    case 3: return dm_sin(x);
+    case 4: return dm_cos(x);
+    case 5: return dm_tan(x);
+    case 6: return dm_atan(x);
+    case 7: return dm_pow(x, y);
+    default: return 0.0;
+    }
+}
+
+/* n draws of the Park-Miller generator from seed (src/Particle.c:211-226) */
+void orc_lcg_seq(long seed, int n, long *out) {
+    long a = 16807, m = 2147483647, q = 127773, r = 2836;
+    int i;
+    for (i = 0; i < n; ++i) {
+        long hi = seed / q, lo = seed - q * hi;
+        seed = a * lo - r * hi;
+        if (seed <= 0) seed += m;
+        out[i] = seed;
+    }
+}
+
+/* argon cross section lookup, for the collision-statistics tests */
+double orc_ar_sigma(orc_case *c, int r, double engy) { return (double)ar_sigma(c, r, (real)engy); }

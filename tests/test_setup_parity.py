@@ -1,0 +1,83 @@
+"""Host setup of the product (deck -> geometry, operator, Laplace and gas
+fields, tables, initial load) equals the oracle's bit for bit (no GPU)."""
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from picmcc import FP32_REFINED, FP64_PLAIN, HostSetup, PicError, decks
+
+SETUP_ARRAYS = ["Boundary", "CondID", "Face", "DensRegion", "Area", "Temp", "PlasmaRegion", "eps_r", "eps",
+                "StructIdx", "A_idx", "Ai", "Aj", "Aval", "Atemp", "Mdiag", "cond_b", "temp_b", "lap_phi",
+                "BackDen1", "BackVel1", "FluidN", "xe", "xee", "cx", "sigv", "P1", "P2", "Pion", "vsave"]
+
+
+def deck_variants():
+    base = decks.small(ncx=12, ncy=9, ppc=10)
+    out = {"small": base}
+    d = copy.deepcopy(base)   # thick electrodes (conductor cells) and a dielectric block
+    g = d["Geometry"]
+    g["ConductorSpec"][0].update({"Y0": 7, "Y1": 9})
+    g["ConductorSpec"][1].update({"Y0": 0, "Y1": 1})
+    g["DielectricSpec"] = [{"M_ID": 3, "X0": 9, "X1": 12, "Y0": 1, "Y1": 7, "Epsilon": 4.0}]
+    out["thick_electrodes_dielectric"] = d
+    d = copy.deepcopy(base)   # Dirichlet right wall, extra grounded conductor inside
+    g = d["Geometry"]
+    g["BoundaryCondition"][3]["B.C"] = 1
+    g["ConductorSpec"].append({"M_ID": 3, "X0": 5, "X1": 7, "Y0": 3, "Y1": 5, "Temp": 350})
+    out["inner_conductor"] = d
+    d = copy.deepcopy(base)   # anisotropic cells, phase and DC on the source
+    d["Geometry"]["SystemSpec"]["X_length(m)"] = 12 * 7e-4
+    d["Geometry"]["Source"][0].update({"DC(V)": -20.0, "Phase(deg)": 30.0})
+    out["anisotropic"] = d
+    out["config1"] = decks.config1()
+    return out
+
+
+@pytest.mark.parametrize("name", list(deck_variants()))
+@pytest.mark.parametrize("prec,pn", [(FP32_REFINED, "f32"), (FP64_PLAIN, "f64")])
+def test_setup_matches_oracle(name, prec, pn):
+    d = deck_variants()[name]
+    h = HostSetup(d, prec)
+    o = Oracle(d, pn)
+    for a in SETUP_ARRAYS:
+        x, y = h.array(a), o.arr(a)
+        assert x.shape == y.shape, a
+        assert np.array_equal(x, y.astype(x.dtype)), f"{name}/{pn}: {a}"
+    for s in range(2):
+        for k in ("x", "y", "vx", "vy", "vz", "id", "cell_start"):
+            x = h.array(f"sp{s}.{k}")
+            y = o.arr(f"sp{s}.{k}")[:len(x)]
+            assert np.array_equal(x, y), f"{name}/{pn}: sp{s}.{k}"
+    for sc in ("dt", "dx", "dy", "MCCn", "dt_mcc", "nvel", "A_size", "CondNUMR"):
+        assert h.scalar(sc) == o.scalar(sc), sc
+
+
+def test_missing_keys_read_as_zero():
+    """parson semantics: a missing key reads as 0 (src/parson.c:1319-1321); the
+    reference's own validation then rejects a zero TimeStep_PIC."""
+    d = decks.small()
+    del d["SimulationMethod"]["TimeStep_PIC"]
+    with pytest.raises(PicError) as e:
+        HostSetup(d)
+    assert "TimeStep_PIC" in str(e.value)
+
+
+@pytest.mark.parametrize("mutate,word", [
+    (lambda d: d["Geometry"].__setitem__("BoundaryCondition", d["Geometry"]["BoundaryCondition"][:3]), "too small"),
+    (lambda d: d["Geometry"]["ConductorSpec"][0].__setitem__("M_ID", 2), "M_ID"),
+    (lambda d: d["Geometry"]["Source"][0].__setitem__("M_ID", 9), "Source"),
+    (lambda d: d["GasSpecies"].__setitem__("Type(0:Ar,1:O2,2:Ar/O2)", 7), "Type"),
+])
+def test_deck_validation_errors(mutate, word):
+    d = decks.small()
+    mutate(d)
+    with pytest.raises(PicError) as e:
+        HostSetup(d)
+    assert word in str(e.value)
+
+
+def test_malformed_json_is_an_error():
+    with pytest.raises(PicError):
+        HostSetup('{"Geometry": {')
