@@ -28,6 +28,8 @@ def run_gpu(deck, precision, steps, warmup, timing=True):
     from picmcc import Case
     c = Case(deck, precision)
     c.step(warmup)
+    if os.environ.get("PIC_BENCH_DEBUG"):
+        print("debug: last field solve", c.last_field_solve(), file=sys.stderr, flush=True)
     c.sync()
     inf0 = c.info()
     if timing:

@@ -43,11 +43,16 @@ template <typename T>
 static hipError_t dalloc(T **p, size_t n) {
     return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
 }
+// allocate and copy on the given stream, then wait: the host vector may be
+// freed as soon as this returns
 template <typename T>
-static hipError_t upload(T **p, const std::vector<T> &v) {
+static hipError_t upload(T **p, const std::vector<T> &v, hipStream_t s) {
     hipError_t e = dalloc(p, v.size());
     if (e != hipSuccess) return e;
-    if (!v.empty()) e = hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (!v.empty()) {
+        e = hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
     return e;
 }
 
@@ -244,7 +249,7 @@ struct Case : CaseBase {
             HIPCHK(dalloc(&v->x, cap)); HIPCHK(dalloc(&v->y, cap)); HIPCHK(dalloc(&v->vx, cap));
             HIPCHK(dalloc(&v->vy, cap)); HIPCHK(dalloc(&v->vz, cap)); HIPCHK(dalloc(&v->id, cap));
             HIPCHK(dalloc(&v->cell_start, G + 1));
-            HIPCHK(hipMemset(v->cell_start, 0, (G + 1) * sizeof(int)));
+            HIPCHK(hipMemsetAsync(v->cell_start, 0, (G + 1) * sizeof(int), st));
             v->cap = cap;
         }
         HIPCHK(dalloc(&tag[s], cap));
@@ -269,6 +274,9 @@ struct Case : CaseBase {
         dt_pic = d.dt_pic;
         G = S.Gsize;
         nsp = S.nsp;
+        // every transfer and fill below is enqueued on the case stream, so
+        // kernels on that stream are ordered after them (a non-blocking
+        // stream is not ordered after null-stream work)
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         // node arrays
         HIPCHK(dalloc(&Ex, G)); HIPCHK(dalloc(&Ey, G)); HIPCHK(dalloc(&phi, G)); HIPCHK(dalloc(&tot, G));
@@ -278,23 +286,23 @@ struct Case : CaseBase {
         HIPCHK(dalloc(&brhs, S.A_size));
         const void *zero_targets[] = {};
         (void)zero_targets;
-        HIPCHK(hipMemset(Ex, 0, G * sizeof(R))); HIPCHK(hipMemset(Ey, 0, G * sizeof(R)));
-        HIPCHK(hipMemset(phi, 0, G * sizeof(R))); HIPCHK(hipMemset(tot, 0, G * sizeof(R)));
-        HIPCHK(hipMemset(Sigma, 0, G * sizeof(R))); HIPCHK(hipMemset(Source, 0, G * sizeof(R)));
-        HIPCHK(hipMemset(den, 0, (size_t)nsp * G * sizeof(R)));
-        HIPCHK(hipMemset(sigma_sp, 0, (size_t)nsp * G * sizeof(R)));
-        HIPCHK(hipMemset(part, 0, (size_t)nsp * G * 4 * sizeof(R)));
-        HIPCHK(hipMemset(brhs, 0, std::max(1, S.A_size) * sizeof(R)));
-        HIPCHK(upload(&P1, S.P1)); HIPCHK(upload(&P2, S.P2)); HIPCHK(upload(&Pion, S.Pion));
-        HIPCHK(upload(&BackVel1, S.BackVel1)); HIPCHK(upload(&vsave, S.vsave));
-        HIPCHK(upload(&xe, S.xe)); HIPCHK(upload(&cx, S.cx)); HIPCHK(upload(&lap, S.lap_phi));
-        HIPCHK(upload(&StructIdx, S.StructIdx)); HIPCHK(upload(&Boundary, S.Boundary));
-        HIPCHK(upload(&Face, S.Face)); HIPCHK(upload(&CondID, S.CondID)); HIPCHK(upload(&A_idx, S.A_idx));
-        HIPCHK(upload(&Area, S.Area)); HIPCHK(upload(&eps, S.eps));
+        HIPCHK(hipMemsetAsync(Ex, 0, G * sizeof(R), st)); HIPCHK(hipMemsetAsync(Ey, 0, G * sizeof(R), st));
+        HIPCHK(hipMemsetAsync(phi, 0, G * sizeof(R), st)); HIPCHK(hipMemsetAsync(tot, 0, G * sizeof(R), st));
+        HIPCHK(hipMemsetAsync(Sigma, 0, G * sizeof(R), st)); HIPCHK(hipMemsetAsync(Source, 0, G * sizeof(R), st));
+        HIPCHK(hipMemsetAsync(den, 0, (size_t)nsp * G * sizeof(R), st));
+        HIPCHK(hipMemsetAsync(sigma_sp, 0, (size_t)nsp * G * sizeof(R), st));
+        HIPCHK(hipMemsetAsync(part, 0, (size_t)nsp * G * 4 * sizeof(R), st));
+        HIPCHK(hipMemsetAsync(brhs, 0, std::max(1, S.A_size) * sizeof(R), st));
+        HIPCHK(upload(&P1, S.P1, st)); HIPCHK(upload(&P2, S.P2, st)); HIPCHK(upload(&Pion, S.Pion, st));
+        HIPCHK(upload(&BackVel1, S.BackVel1, st)); HIPCHK(upload(&vsave, S.vsave, st));
+        HIPCHK(upload(&xe, S.xe, st)); HIPCHK(upload(&cx, S.cx, st)); HIPCHK(upload(&lap, S.lap_phi, st));
+        HIPCHK(upload(&StructIdx, S.StructIdx, st)); HIPCHK(upload(&Boundary, S.Boundary, st));
+        HIPCHK(upload(&Face, S.Face, st)); HIPCHK(upload(&CondID, S.CondID, st)); HIPCHK(upload(&A_idx, S.A_idx, st));
+        HIPCHK(upload(&Area, S.Area, st)); HIPCHK(upload(&eps, S.eps, st));
         std::vector<Route> rt;
         std::vector<int> istart, ient;
         build_routes(rt, istart, ient);
-        HIPCHK(upload(&route, rt)); HIPCHK(upload(&inv_start, istart)); HIPCHK(upload(&inv_ent, ient));
+        HIPCHK(upload(&route, rt, st)); HIPCHK(upload(&inv_start, istart, st)); HIPCHK(upload(&inv_ent, ient, st));
         // field operator in padded 5-entry rows
         {
             const int N = S.A_size;
@@ -312,18 +320,18 @@ struct Case : CaseBase {
                 mf[r] = (float)S.Mdiag[r];
                 md[r] = (double)S.Mdiag[r];
             }
-            HIPCHK(upload(&pcol, col)); HIPCHK(upload(&pvalf, vf)); HIPCHK(upload(&pvald, vd));
-            HIPCHK(upload(&pmdf, mf)); HIPCHK(upload(&pmdd, md));
+            HIPCHK(upload(&pcol, col, st)); HIPCHK(upload(&pvalf, vf, st)); HIPCHK(upload(&pvald, vd, st));
+            HIPCHK(upload(&pmdf, mf, st)); HIPCHK(upload(&pmdd, md, st));
             HIPCHK(dalloc(&b64, N)); HIPCHK(dalloc(&x64, N));
-            HIPCHK(hipMemset(x64, 0, std::max(1, N) * sizeof(double)));
-            HIPCHK(hipMemset(b64, 0, std::max(1, N) * sizeof(double)));
+            HIPCHK(hipMemsetAsync(x64, 0, std::max(1, N) * sizeof(double), st));
+            HIPCHK(hipMemsetAsync(b64, 0, std::max(1, N) * sizeof(double), st));
             op.N = N; op.col = pcol; op.valf = pvalf; op.vald = pvald; op.mdiagf = pmdf; op.mdiagd = pmdd;
             int werr = 0;
             const char *nb_env = getenv("PIC_PCG_BLOCKS");
-            pw = pcg_work_create(N, &werr, nb_env ? atoi(nb_env) : 0);
+            pw = pcg_work_create(N, &werr, nb_env ? atoi(nb_env) : 0, st);
             if (!pw) return fail(-40, "field solver workspace allocation failed");
             HIPCHK(dalloc(&d_res, 1));
-            HIPCHK(hipMemset(d_res, 0, sizeof(PcgResult)));
+            HIPCHK(hipMemsetAsync(d_res, 0, sizeof(PcgResult), st));
             if (precision == PIC_FP32_REFINED && field_solver == 0) {
                 Spectral sp;
                 std::string why;
@@ -346,7 +354,7 @@ struct Case : CaseBase {
                     const float *dst[5];
                     for (int q = 0; q < 5; ++q) {
                         std::vector<float> f = f32(*srcs[q]);
-                        HIPCHK(upload(&p, f));
+                        HIPCHK(upload(&p, f, st));
                         spec_bufs.push_back(p);
                         dst[q] = p;
                     }
@@ -378,35 +386,36 @@ struct Case : CaseBase {
         HIPCHK(dalloc(&scan_tmp, (G + 1023) / 1024 + 2));
         HIPCHK(dalloc(&next_id, MAXSP));
         HIPCHK(dalloc(&mccr, (size_t)G * S.TnRct));
-        HIPCHK(hipMemset(mccr, 0, (size_t)G * S.TnRct * sizeof(unsigned long long)));
+        HIPCHK(hipMemsetAsync(mccr, 0, (size_t)G * S.TnRct * sizeof(unsigned long long), st));
         HIPCHK(dalloc(&condcharge, nsp * S.CondNUMR));
-        HIPCHK(hipMemset(condcharge, 0, nsp * S.CondNUMR * sizeof(int)));
+        HIPCHK(hipMemsetAsync(condcharge, 0, nsp * S.CondNUMR * sizeof(int), st));
         HIPCHK(dalloc(&d_err, 1));
-        HIPCHK(hipMemset(d_err, 0, sizeof(int)));
+        HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int), st));
         HIPCHK(dalloc(&sum_den, (size_t)nsp * G)); HIPCHK(dalloc(&sum_phi, G));
         HIPCHK(dalloc(&sum_ex, G)); HIPCHK(dalloc(&sum_ey, G));
-        HIPCHK(hipMemset(sum_den, 0, (size_t)nsp * G * sizeof(double)));
-        HIPCHK(hipMemset(sum_phi, 0, G * sizeof(double)));
-        HIPCHK(hipMemset(sum_ex, 0, G * sizeof(double)));
-        HIPCHK(hipMemset(sum_ey, 0, G * sizeof(double)));
+        HIPCHK(hipMemsetAsync(sum_den, 0, (size_t)nsp * G * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(sum_phi, 0, G * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(sum_ex, 0, G * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(sum_ey, 0, G * sizeof(double), st));
         {
             uint32_t nid[MAXSP] = {0};
             for (int s = 0; s < nsp; ++s) nid[s] = (uint32_t)S.sp[s].n;
-            HIPCHK(hipMemcpy(next_id, nid, sizeof nid, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(next_id, nid, sizeof nid, hipMemcpyHostToDevice, st));
         }
         for (int s = 0; s < nsp; ++s) {
             SpeciesInit<R> &si = S.sp[s];
             if (si.n > 0) {
-                HIPCHK(hipMemcpy(cur[s].x, si.x.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(cur[s].y, si.y.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(cur[s].vx, si.vx.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(cur[s].vy, si.vy.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(cur[s].vz, si.vz.data(), si.n * sizeof(R), hipMemcpyHostToDevice));
-                HIPCHK(hipMemcpy(cur[s].id, si.id.data(), si.n * sizeof(uint32_t), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpyAsync(cur[s].x, si.x.data(), si.n * sizeof(R), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(cur[s].y, si.y.data(), si.n * sizeof(R), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(cur[s].vx, si.vx.data(), si.n * sizeof(R), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(cur[s].vy, si.vy.data(), si.n * sizeof(R), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(cur[s].vz, si.vz.data(), si.n * sizeof(R), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(cur[s].id, si.id.data(), si.n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
             }
             if (!si.cell_start.empty())
-                HIPCHK(hipMemcpy(cur[s].cell_start, si.cell_start.data(), (G + 1) * sizeof(int), hipMemcpyHostToDevice));
-            // free the host copy of the load; the device holds the state now
+                HIPCHK(hipMemcpyAsync(cur[s].cell_start, si.cell_start.data(), (G + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+            // free the host copy of the load once the copies have landed
+            HIPCHK(hipStreamSynchronize(st));
             std::vector<R>().swap(si.x); std::vector<R>().swap(si.y); std::vector<R>().swap(si.vx);
             std::vector<R>().swap(si.vy); std::vector<R>().swap(si.vz); std::vector<uint32_t>().swap(si.id);
         }
@@ -596,17 +605,20 @@ struct Case : CaseBase {
     int sync() override {
         HIPCHK(hipStreamSynchronize(st));
         int e = 0;
-        HIPCHK(hipMemcpy(&e, d_err, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(&e, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         if (e) return fail(-30, "particle capacity exceeded (Max_np too small for this case)");
         PcgResult r{};
-        HIPCHK(hipMemcpy(&r, d_res, sizeof r, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(&r, d_res, sizeof r, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         if (r.status == 1) return fail(-31, "field solve hit the iteration limit (rel. residual %g)", r.rel_res);
         if (std::isnan(r.rel_res)) return fail(-32, "Field solver Error! (NaN residual)");
         return 0;
     }
     long long np_of(int s) {
         int v = 0;
-        hipMemcpy(&v, cur[s].cell_start + G, sizeof(int), hipMemcpyDeviceToHost);
+        hipMemcpyAsync(&v, cur[s].cell_start + G, sizeof(int), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
         return v;
     }
     int info(pic_info *o) override {
@@ -626,13 +638,20 @@ struct Case : CaseBase {
         HIPCHK(hipStreamSynchronize(st));
         long long n = np_of(s);
         const size_t b = (size_t)n * sizeof(R);
-        if (x) HIPCHK(hipMemcpy(x, cur[s].x, b, hipMemcpyDeviceToHost));
-        if (y) HIPCHK(hipMemcpy(y, cur[s].y, b, hipMemcpyDeviceToHost));
-        if (vx) HIPCHK(hipMemcpy(vx, cur[s].vx, b, hipMemcpyDeviceToHost));
-        if (vy) HIPCHK(hipMemcpy(vy, cur[s].vy, b, hipMemcpyDeviceToHost));
-        if (vz) HIPCHK(hipMemcpy(vz, cur[s].vz, b, hipMemcpyDeviceToHost));
-        if (id) HIPCHK(hipMemcpy(id, cur[s].id, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        if (cs) HIPCHK(hipMemcpy(cs, cur[s].cell_start, (G + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        if (x) HIPCHK(hipMemcpyAsync(x, cur[s].x, b, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (y) HIPCHK(hipMemcpyAsync(y, cur[s].y, b, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (vx) HIPCHK(hipMemcpyAsync(vx, cur[s].vx, b, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (vy) HIPCHK(hipMemcpyAsync(vy, cur[s].vy, b, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (vz) HIPCHK(hipMemcpyAsync(vz, cur[s].vz, b, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (id) HIPCHK(hipMemcpyAsync(id, cur[s].id, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (cs) HIPCHK(hipMemcpyAsync(cs, cur[s].cell_start, (G + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         return 0;
     }
     int set_particles(int s, long long n, const void *x, const void *y, const void *vx, const void *vy, const void *vz,
@@ -644,14 +663,15 @@ struct Case : CaseBase {
             if (cs[c + 1] < cs[c]) return fail(-3, "cell_start must be non-decreasing");
         HIPCHK(hipStreamSynchronize(st));
         const size_t b = (size_t)n * sizeof(R);
-        HIPCHK(hipMemcpy(cur[s].x, x, b, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cur[s].y, y, b, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cur[s].vx, vx, b, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cur[s].vy, vy, b, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cur[s].vz, vz, b, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cur[s].id, id, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(cur[s].cell_start, cs, (G + 1) * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(next_id + s, &nid, sizeof nid, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(cur[s].x, x, b, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cur[s].y, y, b, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cur[s].vx, vx, b, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cur[s].vy, vy, b, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cur[s].vz, vz, b, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cur[s].id, id, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(cur[s].cell_start, cs, (G + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(next_id + s, &nid, sizeof nid, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
         tag_valid = 0;
         return 0;
     }
@@ -732,7 +752,8 @@ struct Case : CaseBase {
         if (bytes < need) return fail(-5, "buffer too small for '%s' (%zu < %zu)", name, bytes, need);
         if (a.dev) {
             HIPCHK(hipStreamSynchronize(st));
-            HIPCHK(hipMemcpy(host, a.dev, need, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpyAsync(host, a.dev, need, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         } else if (need) {
             std::memcpy(host, a.host, need);
         }
@@ -745,7 +766,8 @@ struct Case : CaseBase {
         const size_t need = a.count * a.elsize;
         if (bytes != need) return fail(-5, "size mismatch for '%s' (%zu != %zu)", name, bytes, need);
         HIPCHK(hipStreamSynchronize(st));
-        HIPCHK(hipMemcpy(a.dev, host, need, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(a.dev, host, need, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
         return 0;
     }
     int set_step(long long s) override {
@@ -756,7 +778,8 @@ struct Case : CaseBase {
     int last_solve(int *iters, int *outer, double *rel) override {
         PcgResult r{};
         HIPCHK(hipStreamSynchronize(st));
-        HIPCHK(hipMemcpy(&r, d_res, sizeof r, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(&r, d_res, sizeof r, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         if (iters) *iters = r.iters;
         if (outer) *outer = r.outer;
         if (rel) *rel = r.rel_res;

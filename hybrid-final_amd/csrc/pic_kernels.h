@@ -88,7 +88,7 @@ struct PcgOperator {
 };
 
 struct PcgWork;   // device scratch owned by the context
-PcgWork *pcg_work_create(int N, int *err, int want_blocks = 0);
+PcgWork *pcg_work_create(int N, int *err, int want_blocks, hipStream_t stream);
 void pcg_work_destroy(PcgWork *);
 int pcg_grid_blocks(PcgWork *);
 
