@@ -73,6 +73,29 @@ def cpu_baseline(deck, steps):
     return 0.5 * (n0 + n1) * steps / dt, dt
 
 
+def copy_calibration(mb):
+    """Device-to-device copy of `mb` MiB through the HIP runtime's copy kernel.
+
+    Profiled under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, it gives the
+    counter reading of a kernel whose HBM traffic is known (mb MiB read and
+    mb MiB written), which scales the counters of the particle kernels.
+    """
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n}), file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,19 +104,25 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20, help="steps timed on the FP64 plain path")
     ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
+                    help="first run a device copy of MB MiB (a kernel of known traffic, for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs one discharge case on one GPU; --gpus must be 1 "
                                    "(ensembles shard one case per GPU with no communication)"}))
         sys.exit(2)
+    if args.copy_calib > 0:
+        copy_calibration(args.copy_calib)
 
     from picmcc import FP32_REFINED, FP64_PLAIN, decks
     deck = decks.config2() if args.config == "c2" else decks.config1()
 
     ref = run_gpu(deck, FP32_REFINED, args.steps, args.warmup)
     value = particles_avg(ref) * args.steps / ref["seconds"]
-    plain = run_gpu(deck, FP64_PLAIN, args.plain_steps, 2, timing=False)
-    plain_value = particles_avg(plain) * args.plain_steps / plain["seconds"]
+    plain, plain_value = None, 0.0
+    if args.plain_steps > 0:
+        plain = run_gpu(deck, FP64_PLAIN, args.plain_steps, 2, timing=False)
+        plain_value = particles_avg(plain) * args.plain_steps / plain["seconds"]
 
     # roofline of the dominant particle kernel (HIP events on the case stream)
     st = ref["stages_ms"]
@@ -134,7 +163,7 @@ def main():
         "fp64_plain": {"value": plain_value, "unit": "particle-steps/s",
                        "ms_per_step": plain["seconds"] / args.plain_steps * 1e3, "steps": args.plain_steps,
                        "field_solve": "FP64 Jacobi PCG (reference algorithm), tol 1e-10 relative",
-                       "field_iters_last": plain["field_iters"]},
+                       "field_iters_last": plain["field_iters"]} if plain else None,
         "refined_over_plain": value / plain_value if plain_value > 0 else None,
         "field_solve_last": {"cycles": ref["field_cycles"], "rel_residual": ref["field_rel_res"]},
         "stage_ms_per_step": {k: round(v, 5) for k, v in st.items()},

@@ -252,14 +252,14 @@ __global__ __launch_bounds__(256) void k_mcc_collide(DevConst<R> k, PView<R> p, 
             int ev;
             if (isp == 0) {
                 R evx = 0, evy = 0, evz = 0, ivx = 0, ivy = 0, ivz = 0;
-                ev = mcc_ar_electron<R>(k, c, p.id[kk], step, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
+                ev = mcc_ar_electron<R>(k, c, k.P1[c], k.P2[c], p.id[kk], step, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
                 if (ev == 3 || ev == 4) {
                     prod = true;
                     stage[kk] = evx; stage[cap + kk] = evy; stage[2 * cap + kk] = evz;
                     stage[3 * cap + kk] = ivx; stage[4 * cap + kk] = ivy; stage[5 * cap + kk] = ivz;
                 }
             } else {
-                ev = mcc_ar_ion<R>(k, c, p.id[kk], step, vx, vy, vz);
+                ev = mcc_ar_ion<R>(k, c, k.Pion[c], p.id[kk], step, vx, vy, vz);
             }
             if (ev >= 0) {
                 atomicAdd(&mccr[(size_t)c * k.TnRct + ev], 1ull);
@@ -310,52 +310,177 @@ __global__ __launch_bounds__(256) void k_mcc_insert(DevConst<R> k, PView<R> src,
 }
 
 // ============================================================ fused step
+// Both passes stage each wave's cell in chunks of FCHUNK particles through
+// LDS: every lane issues the loads of all its particles of the chunk at once
+// (one memory round trip per chunk instead of one per 64 particles), then
+// walks them 64 at a time.  The cell's 9 route entries and the collision
+// probabilities of their destination nodes are loaded once per wave into
+// lanes 0..8 and fetched per particle with a lane shuffle.
+//
+// Collisions are compacted: the walk runs the push and the null-collision
+// test on every particle and appends the (few) colliding ones, in particle
+// order, to a per-wave queue in LDS; the queue is then processed 64 entries
+// at a time, so the long collision code runs on full wavefronts instead of
+// on one or two lanes of every wavefront.
+constexpr int FCHUNK = 256;
+constexpr int FWAVES = 4;   // waves per 256-thread block
+enum { KIND_AR_ELECTRON = 0, KIND_AR_ION = 1 };
+
+template <typename R>
+struct FStage {
+    R x[FCHUNK], y[FCHUNK], vx[FCHUNK], vy[FCHUNK], vz[FCHUNK];
+    uint32_t id[FCHUNK];
+    int q[FCHUNK];      // collision queue: i | code << 8 | jcoll << 12
+    int qpos[FCHUNK];   // scatter pass: output slot of the queued particle
+};
+
+// Orders the LDS accesses of one wavefront (the hardware already executes a
+// wave's LDS instructions in order; this keeps the compiler from moving
+// them) between a phase that writes slots and one that reads other lanes'.
+__device__ inline void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// global -> LDS for particles [b0, b0+n), n >= 1.  Lanes past the end load the
+// last particle again (same cache line, never read back), so the 6*FCHUNK/64
+// loads of a lane form one straight-line block the compiler can batch.
+template <typename R>
+__device__ inline void stage_chunk(FStage<R> &S, const PView<R> &p, int b0, int n, int lane) {
+    R tx[FCHUNK / 64], ty[FCHUNK / 64], tvx[FCHUNK / 64], tvy[FCHUNK / 64], tvz[FCHUNK / 64];
+    uint32_t tid[FCHUNK / 64];
+#pragma unroll
+    for (int j = 0; j < FCHUNK / 64; ++j) {
+        int i = b0 + min(j * 64 + lane, n - 1);
+        tx[j] = p.x[i]; ty[j] = p.y[i]; tvx[j] = p.vx[i]; tvy[j] = p.vy[i]; tvz[j] = p.vz[i]; tid[j] = p.id[i];
+    }
+#pragma unroll
+    for (int j = 0; j < FCHUNK / 64; ++j) {
+        int i = j * 64 + lane;
+        S.x[i] = tx[j]; S.y[i] = ty[j]; S.vx[i] = tvx[j]; S.vy[i] = tvy[j]; S.vz[i] = tvz[j]; S.id[i] = tid[j];
+    }
+}
+
+// route entry `lane` of cell c and the collision probabilities of its
+// destination (electrons: P1 in pa, P2 in pb; ions: Pion in pb)
+template <typename R>
+struct WaveRoutes {
+    int dest, info;
+    R pa, pb;
+    __device__ void load(const DevConst<R> &k, int c, int lane, int kind) {
+        dest = -1; info = 0; pa = 0; pb = 0;
+        if (lane < ROUTE_SLOTS) {
+            Route r = k.route[c * ROUTE_SLOTS + lane];
+            dest = r.dest; info = r.info;
+            if (dest >= 0) {
+                if (kind == KIND_AR_ELECTRON) { pa = k.P1[dest]; pb = k.P2[dest]; }
+                else pb = k.Pion[dest];
+            }
+        }
+    }
+};
+
+// null-collision test of one particle at its destination node: the index of
+// the colliding draw, or -1
+template <typename R, int KIND>
+__device__ inline int null_test(const DevConst<R> &k, R pb, uint32_t id, uint32_t step) {
+    if (KIND == KIND_AR_ELECTRON) return mcc_ar_electron_null<R>(k, pb, id, step);
+    return mcc_ar_ion_null<R>(k, pb, id, step) ? 0 : -1;
+}
+
+// collision of a queued particle; returns the reaction index or -1
+template <typename R, int KIND>
+__device__ inline int collide(const DevConst<R> &k, int dest, R pa, uint32_t id, uint32_t step, int jc, R &vx, R &vy,
+                              R &vz, R &evx, R &evy, R &evz, R &ivx, R &ivy, R &ivz) {
+    if (KIND == KIND_AR_ELECTRON)
+        return mcc_ar_electron_collide<R>(k, dest, pa, id, step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
+    Stream st;
+    st.init(k.seed, id, step, mcc_tag(1));
+    st.seek(1);
+    return mcc_ar_ion_collide<R>(k, dest, st, vx, vy, vz);
+}
+
 // Pass 1: push + walls + collision, counting survivors and products per
 // (cell, slot).  Side effects (wall counters, surface charge, event counts)
 // happen here only.
-template <typename R>
+template <typename R, int KIND>
 __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, uint32_t step, int *cnt,
                                                      int *pcnt, unsigned long long *mccr, int *condcharge,
                                                      R *sigma_sp) {
+    __shared__ FStage<R> stage_all[FWAVES];
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
+    FStage<R> &S = stage_all[threadIdx.x >> 6];
+    const unsigned long long lt = lanes_below(lane);
     int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
-    CellE<R> e;
-    e.load(k.Ex, k.Ey, c, k.ngy);
-    const R As = k.Ascale[isp];
     int run[ROUTE_SLOTS], prun[ROUTE_SLOTS];
 #pragma unroll
     for (int s = 0; s < ROUTE_SLOTS; ++s) { run[s] = 0; prun[s] = 0; }
-    for (int base = k0; base < k1; base += 64) {
-        int kk = base + lane;
-        int slot = -1;
-        bool prod = false;
-        if (kk < k1) {
-            R x = p.x[kk], y = p.y[kk], vx = p.vx[kk], vy = p.vy[kk], vz = p.vz[kk];
-            int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx,
-                                        k.dt_dy, x, y, vx, vy);
-            Route rt = k.route[c * ROUTE_SLOTS + code];
-            if (rt.dest >= 0) {
-                apply_reflect<R>(rt.info, x, y, vx, vy);
-                slot = rt.info & 15;
-                int ev;
-                if (isp == 0) {
-                    R evx, evy, evz, ivx, ivy, ivz;
-                    ev = mcc_ar_electron<R>(k, rt.dest, p.id[kk], step, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
-                    prod = (ev == 3 || ev == 4);
-                } else {
-                    ev = mcc_ar_ion<R>(k, rt.dest, p.id[kk], step, vx, vy, vz);
+    if (k0 < k1) {
+        CellE<R> e;
+        e.load(k.Ex, k.Ey, c, k.ngy);
+        WaveRoutes<R> w;
+        w.load(k, c, lane, KIND);
+        const R As = k.Ascale[isp];
+        for (int b0 = k0; b0 < k1; b0 += FCHUNK) {
+            const int n = min(FCHUNK, k1 - b0);
+            stage_chunk<R>(S, p, b0, n, lane);
+            int qn = 0;
+            for (int j = 0; j * 64 < n; ++j) {
+                const int i = j * 64 + lane;
+                int slot = -1, jc = -1;
+                R x = S.x[i], y = S.y[i], vx = S.vx[i], vy = S.vy[i];
+                const uint32_t id = S.id[i];
+                const int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As,
+                                                  k.dt_dx, k.dt_dy, x, y, vx, vy);
+                const int dest = __shfl(w.dest, code, 64);
+                const int info = __shfl(w.info, code, 64);
+                const R pb = __shfl(w.pb, code, 64);
+                if (i < n) {
+                    if (dest >= 0) {
+                        apply_reflect<R>(info, x, y, vx, vy);
+                        slot = info & 15;
+                        jc = null_test<R, KIND>(k, pb, id, step);
+                    } else {
+                        wall_effects<R>(k, Route{dest, info}, isp, c, code, x, y, vx, vy, condcharge, sigma_sp);
+                    }
                 }
-                if (ev >= 0) atomicAdd(&mccr[(size_t)rt.dest * k.TnRct + ev], 1ull);
-            } else {
-                wall_effects<R>(k, rt, isp, c, code, x, y, vx, vy, condcharge, sigma_sp);
-            }
-        }
 #pragma unroll
-        for (int s = 0; s < ROUTE_SLOTS; ++s) {
-            run[s] += __popcll(__ballot(slot == s));
-            prun[s] += __popcll(__ballot(slot == s && prod));
+                for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] += __popcll(__ballot(slot == s));
+                const unsigned long long cm = __ballot(jc >= 0);
+                if (jc >= 0) {
+                    S.vx[i] = vx;   // post-push velocity (vz is unchanged by the push)
+                    S.vy[i] = vy;
+                    S.q[qn + __popcll(cm & lt)] = i | (code << 8) | (jc << 12);
+                }
+                qn += __popcll(cm);
+            }
+            wave_lds_fence();
+            for (int qb = 0; qb < qn; qb += 64) {
+                const bool valid = qb + lane < qn;
+                const int qe = valid ? S.q[qb + lane] : 0;
+                const int i = qe & 255, code = (qe >> 8) & 15, jc = qe >> 12;
+                const int dest = __shfl(w.dest, code, 64);
+                const int info = __shfl(w.info, code, 64);
+                const R pa = __shfl(w.pa, code, 64);
+                int slot = -1;
+                bool prod = false;
+                if (valid) {
+                    R vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+                    R evx, evy, evz, ivx, ivy, ivz;
+                    const int ev = collide<R, KIND>(k, dest, pa, S.id[i], step, jc, vx, vy, vz, evx, evy, evz, ivx,
+                                                    ivy, ivz);
+                    if (ev >= 0) atomicAdd(&mccr[(size_t)dest * k.TnRct + ev], 1ull);
+                    prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
+                    slot = info & 15;
+                }
+                if (KIND == KIND_AR_ELECTRON) {
+#pragma unroll
+                    for (int s = 0; s < ROUTE_SLOTS; ++s) prun[s] += __popcll(__ballot(prod && slot == s));
+                }
+            }
+            wave_lds_fence();
         }
     }
     if (lane == 0) {
@@ -370,83 +495,127 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
 // Pass 2: recompute (same bits) and write each particle and product to its
 // final slot.  For electrons, ionisation products go to both the electron
 // and the ion arrays.
-template <typename R>
+template <typename R, int KIND>
 __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p, PView<R> q, PView<R> qion, int isp,
                                                        uint32_t step, const int *base, const int *pbase_e,
                                                        const int *pbase_i, const int *pid_off, const uint32_t *next_id,
                                                        int cap, int cap_ion, int *err) {
+    __shared__ FStage<R> stage_all[FWAVES];
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
     int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
     if (k0 == k1) return;
+    FStage<R> &S = stage_all[threadIdx.x >> 6];
     CellE<R> e;
     e.load(k.Ex, k.Ey, c, k.ngy);
+    WaveRoutes<R> w;
+    w.load(k, c, lane, KIND);
+    // per-slot output bases of this cell, held by lane = slot
+    const int my_base = lane < ROUTE_SLOTS ? base[c * ROUTE_SLOTS + lane] : 0;
     const R As = k.Ascale[isp];
     const unsigned long long lt = lanes_below(lane);
     int run[ROUTE_SLOTS], prun[ROUTE_SLOTS];
 #pragma unroll
     for (int s = 0; s < ROUTE_SLOTS; ++s) { run[s] = 0; prun[s] = 0; }
-    for (int b0 = k0; b0 < k1; b0 += 64) {
-        int kk = b0 + lane;
-        int slot = -1;
-        bool prod = false;
-        R x = 0, y = 0, vx = 0, vy = 0, vz = 0, evx = 0, evy = 0, evz = 0, ivx = 0, ivy = 0, ivz = 0;
-        uint32_t id = 0;
-        if (kk < k1) {
-            x = p.x[kk]; y = p.y[kk]; vx = p.vx[kk]; vy = p.vy[kk]; vz = p.vz[kk]; id = p.id[kk];
-            int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx,
-                                        k.dt_dy, x, y, vx, vy);
-            Route rt = k.route[c * ROUTE_SLOTS + code];
-            if (rt.dest >= 0) {
-                apply_reflect<R>(rt.info, x, y, vx, vy);
-                slot = rt.info & 15;
-                if (isp == 0) {
-                    int ev = mcc_ar_electron<R>(k, rt.dest, id, step, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
-                    prod = (ev == 3 || ev == 4);
-                } else {
-                    mcc_ar_ion<R>(k, rt.dest, id, step, vx, vy, vz);
-                }
+    for (int b0 = k0; b0 < k1; b0 += FCHUNK) {
+        const int n = min(FCHUNK, k1 - b0);
+        stage_chunk<R>(S, p, b0, n, lane);
+        int qn = 0;
+        for (int j = 0; j * 64 < n; ++j) {
+            const int i = j * 64 + lane;
+            int slot = -1, jc = -1;
+            R x = S.x[i], y = S.y[i], vx = S.vx[i], vy = S.vy[i];
+            const uint32_t id = S.id[i];
+            const int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx,
+                                              k.dt_dy, x, y, vx, vy);
+            const int dest = __shfl(w.dest, code, 64);
+            const int info = __shfl(w.info, code, 64);
+            const R pb = __shfl(w.pb, code, 64);
+            if (i < n && dest >= 0) {
+                apply_reflect<R>(info, x, y, vx, vy);
+                slot = info & 15;
+                jc = null_test<R, KIND>(k, pb, id, step);
             }
-        }
-        int pos = -1, ppos = -1, prank = 0;
+            int pos = -1;
 #pragma unroll
-        for (int s = 0; s < ROUTE_SLOTS; ++s) {
-            unsigned long long m = __ballot(slot == s);
-            unsigned long long pm = __ballot(slot == s && prod);
-            if (slot == s) {
-                pos = run[s] + __popcll(m & lt);
-                prank = prun[s] + __popcll(pm & lt);
+            for (int s = 0; s < ROUTE_SLOTS; ++s) {
+                unsigned long long m = __ballot(slot == s);
+                if (slot == s) pos = run[s] + __popcll(m & lt);
+                run[s] += __popcll(m);
             }
-            run[s] += __popcll(m);
-            prun[s] += __popcll(pm);
-        }
-        if (slot >= 0) {
-            int ent = c * ROUTE_SLOTS + slot;
-            pos += base[ent];
-            if (pos < cap) {
-                q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = vz; q.id[pos] = id;
-            } else {
-                atomicOr(err, 1);
-            }
-            if (prod) {
-                int off = pid_off[ent] + prank;
-                ppos = pbase_e[ent] + prank;
-                if (ppos < cap) {
-                    q.x[ppos] = x; q.y[ppos] = y; q.vx[ppos] = evx; q.vy[ppos] = evy; q.vz[ppos] = evz;
-                    q.id[ppos] = next_id[0] + (uint32_t)off;
-                } else {
-                    atomicOr(err, 1);
-                }
-                int ipos = pbase_i[ent] + prank;
-                if (ipos < cap_ion) {
-                    qion.x[ipos] = x; qion.y[ipos] = y; qion.vx[ipos] = ivx; qion.vy[ipos] = ivy; qion.vz[ipos] = ivz;
-                    qion.id[ipos] = next_id[1] + (uint32_t)off;
+            const int sbase = __shfl(my_base, slot < 0 ? 0 : slot, 64);
+            pos += sbase;
+            if (slot >= 0 && jc < 0) {
+                if (pos < cap) {
+                    q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = S.vz[i]; q.id[pos] = id;
                 } else {
                     atomicOr(err, 1);
                 }
             }
+            const unsigned long long cm = __ballot(jc >= 0);
+            if (jc >= 0) {
+                const int qi = qn + __popcll(cm & lt);
+                S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy;
+                S.q[qi] = i | (code << 8) | (jc << 12);
+                S.qpos[qi] = pos;
+            }
+            qn += __popcll(cm);
         }
+        wave_lds_fence();
+        for (int qb = 0; qb < qn; qb += 64) {
+            const bool valid = qb + lane < qn;
+            const int qe = valid ? S.q[qb + lane] : 0;
+            const int i = qe & 255, code = (qe >> 8) & 15, jc = qe >> 12;
+            const int dest = __shfl(w.dest, code, 64);
+            const int info = __shfl(w.info, code, 64);
+            const R pa = __shfl(w.pa, code, 64);
+            int slot = -1;
+            bool prod = false;
+            R x = 0, y = 0, vx = 0, vy = 0, vz = 0, evx = 0, evy = 0, evz = 0, ivx = 0, ivy = 0, ivz = 0;
+            uint32_t id = 0;
+            int pos = 0;
+            if (valid) {
+                x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i]; id = S.id[i];
+                pos = S.qpos[qb + lane];
+                const int ev = collide<R, KIND>(k, dest, pa, id, step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
+                prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
+                slot = info & 15;
+                if (pos < cap) {
+                    q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = vz; q.id[pos] = id;
+                } else {
+                    atomicOr(err, 1);
+                }
+            }
+            if (KIND == KIND_AR_ELECTRON) {
+                int prank = 0;
+#pragma unroll
+                for (int s = 0; s < ROUTE_SLOTS; ++s) {
+                    unsigned long long pm = __ballot(prod && slot == s);
+                    if (prod && slot == s) prank = prun[s] + __popcll(pm & lt);
+                    prun[s] += __popcll(pm);
+                }
+                if (prod) {
+                    const int ent = c * ROUTE_SLOTS + slot;
+                    const int off = pid_off[ent] + prank;
+                    const int ppos = pbase_e[ent] + prank;
+                    if (ppos < cap) {
+                        q.x[ppos] = x; q.y[ppos] = y; q.vx[ppos] = evx; q.vy[ppos] = evy; q.vz[ppos] = evz;
+                        q.id[ppos] = next_id[0] + (uint32_t)off;
+                    } else {
+                        atomicOr(err, 1);
+                    }
+                    const int ipos = pbase_i[ent] + prank;
+                    if (ipos < cap_ion) {
+                        qion.x[ipos] = x; qion.y[ipos] = y; qion.vx[ipos] = ivx; qion.vy[ipos] = ivy;
+                        qion.vz[ipos] = ivz; qion.id[ipos] = next_id[1] + (uint32_t)off;
+                    } else {
+                        atomicOr(err, 1);
+                    }
+                }
+            }
+        }
+        wave_lds_fence();
     }
 }
 
@@ -746,15 +915,23 @@ void launch_mcc_insert(const DevConst<R> &k, const PView<R> &src, const PView<R>
 template <typename R>
 void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, uint32_t step, int *cnt, int *pcnt,
                         unsigned long long *mccr, int *cc, R *sg, hipStream_t s) {
-    hipLaunchKernelGGL(k_fused_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, step, cnt, pcnt, mccr,
-                       cc, sg);
+    if (isp == 0)
+        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
+                           step, cnt, pcnt, mccr, cc, sg);
+    else
+        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
+                           step, cnt, pcnt, mccr, cc, sg);
 }
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
                           uint32_t step, const int *base, const int *pbase_e, const int *pbase_i, const int *pid_off,
                           const uint32_t *next_id, int cap, int cap_ion, int *err, hipStream_t s) {
-    hipLaunchKernelGGL(k_fused_scatter<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, q, qion, isp, step, base,
-                       pbase_e, pbase_i, pid_off, next_id, cap, cap_ion, err);
+    if (isp == 0)
+        hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p,
+                           q, qion, isp, step, base, pbase_e, pbase_i, pid_off, next_id, cap, cap_ion, err);
+    else
+        hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, q,
+                           qion, isp, step, base, pbase_e, pbase_i, pid_off, next_id, cap, cap_ion, err);
 }
 template <typename R>
 void launch_deposit(const DevConst<R> &k, const NodeConst<R> &n, const PView<R> *ps, R *part, R *den, R *sigma_sp,

@@ -34,24 +34,50 @@ __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// uniform in (0,1) from one word: (w>>9 + 1/2) / 2^23.  The sum needs 24
+// significant bits and the scale is a power of two, so the value is exact in
+// float and in double (both evaluations give the same bits).
+template <typename R> __host__ __device__ inline R philox_uniform(uint32_t w);
+template <> __host__ __device__ inline float philox_uniform<float>(uint32_t w) {
+    return ((float)(w >> 9) + 0.5f) * (1.0f / 8388608.0f);
+}
+template <> __host__ __device__ inline double philox_uniform<double>(uint32_t w) {
+    return ((double)(w >> 9) + 0.5) * (1.0 / 8388608.0);
+}
+
 struct Stream {
     uint32_t k0, k1, id, step, tag, block;
-    uint32_t buf[4];
+    uint32_t b0, b1, b2, b3;   // current Philox block, consumed b0 first
     int used;
     __device__ void init(uint64_t seed, uint32_t pid, uint32_t st, uint32_t tg) {
         k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
         id = pid; step = st; tag = tg; block = 0; used = 4;
     }
-    // uniform in (0,1): (w>>9 + 1/2) / 2^23, exact in float
+    // position the stream after n draws (n >= 0), as if u() had been called n times
+    __device__ void seek(int n) {
+        block = (uint32_t)(n >> 2);
+        used = 4;
+        if (n & 3) {
+            uint32_t o[4];
+            philox4x32_10(id, step, tag, block, k0, k1, o);
+            b0 = o[0]; b1 = o[1]; b2 = o[2]; b3 = o[3];
+            ++block;
+            used = n & 3;
+        }
+    }
     template <typename R>
     __device__ R u() {
         if (used == 4) {
-            philox4x32_10(id, step, tag, block, k0, k1, buf);
+            uint32_t o[4];
+            philox4x32_10(id, step, tag, block, k0, k1, o);
+            b0 = o[0]; b1 = o[1]; b2 = o[2]; b3 = o[3];
             ++block;
             used = 0;
         }
-        uint32_t w = buf[used++];
-        return (R)(((double)(w >> 9) + 0.5) * (1.0 / 8388608.0));
+        // register select instead of an indexed buffer (keeps it out of LDS/scratch)
+        uint32_t w = used == 0 ? b0 : used == 1 ? b1 : used == 2 ? b2 : b3;
+        ++used;
+        return philox_uniform<R>(w);
     }
 };
 
@@ -264,13 +290,24 @@ __device__ inline void maxwell_draw(R &vx, R &vy, R &vz, R vsaven, R vti, R rphi
     vz = vti * vsaven * costheta;
 }
 
-// Ion collision (Direct_Argon_ArIon).  Returns the reaction index (5, 6) or -1.
+// The collision functions come in two parts so kernels can run the cheap
+// null-collision test on every particle and the collision itself only on the
+// (few) particles that collide, gathered into full wavefronts.  The stream
+// of a particle is fixed by (seed, id, step, tag); a collision resumes it
+// with Stream::seek at the draw that follows the null test.
+
+// Ion null test (Direct_Argon_ArIon, first draw): true when it collides.
 template <typename R>
-__device__ inline int mcc_ar_ion(const DevConst<R> &k, int node, uint32_t pid, uint32_t step, R &vx, R &vy, R &vz) {
+__device__ inline bool mcc_ar_ion_null(const DevConst<R> &k, R pion, uint32_t pid, uint32_t step) {
     Stream st;
     st.init(k.seed, pid, step, mcc_tag(1));
-    R R1 = st.u<R>();
-    if (R1 > k.Pion[node]) return -1;
+    return !(st.u<R>() > pion);
+}
+
+// Ion collision after a positive null test; st is positioned after draw 0.
+// Returns the reaction index (5, 6) or -1.
+template <typename R>
+__device__ inline int mcc_ar_ion_collide(const DevConst<R> &k, int node, Stream &st, R &vx, R &vy, R &vz) {
     int n = (int)((R)(k.nvel - 1) * st.u<R>());
     R rphi = st.u<R>(), rthe = st.u<R>();
     R vnx, vny, vnz;
@@ -279,7 +316,7 @@ __device__ inline int mcc_ar_ion(const DevConst<R> &k, int node, uint32_t pid, u
     R dum = VX * VX + VY * VY + VZ * VZ;
     R engy = k.Escale[1] * dum;
     R vel = rsqrt_ieee<R>(dum);
-    R1 = st.u<R>() * k.sigv[2] / vel;
+    R R1 = st.u<R>() * k.sigv[2] / vel;
     R sum = ar_sigma<R>(k, 5, engy);
     if (R1 <= sum) {
         vx = vnx; vy = vny; vz = vnz;
@@ -294,21 +331,40 @@ __device__ inline int mcc_ar_ion(const DevConst<R> &k, int node, uint32_t pid, u
     return -1;
 }
 
-// Electron collision (Direct_Argon_Electron).  Returns the reaction index
-// (0..4) or -1; for ionization (3, 4) fills the new electron and ion.
+// Ion collision (Direct_Argon_ArIon).  Returns the reaction index (5, 6) or -1.
+// pion = k.Pion[node].
 template <typename R>
-__device__ inline int mcc_ar_electron(const DevConst<R> &k, int node, uint32_t pid, uint32_t step,
-                                      R &vx, R &vy, R &vz, R &evx, R &evy, R &evz, R &ivx, R &ivy, R &ivz) {
+__device__ inline int mcc_ar_ion(const DevConst<R> &k, int node, R pion, uint32_t pid, uint32_t step, R &vx, R &vy,
+                                 R &vz) {
+    if (!mcc_ar_ion_null<R>(k, pion, pid, step)) return -1;
+    Stream st;
+    st.init(k.seed, pid, step, mcc_tag(1));
+    st.seek(1);
+    return mcc_ar_ion_collide<R>(k, node, st, vx, vy, vz);
+}
+
+// Electron null test (Direct_Argon_Electron, up to MCCn draws against P2):
+// returns the index of the draw that collided, or -1.
+template <typename R>
+__device__ inline int mcc_ar_electron_null(const DevConst<R> &k, R P2, uint32_t pid, uint32_t step) {
     Stream st;
     st.init(k.seed, pid, step, mcc_tag(0));
-    const R P1 = k.P1[node], P2 = k.P2[node];
-    int coll = 0;
-    R R1 = 0;
-    for (int j = 0; j < k.MCCn; ++j) {
-        R1 = st.u<R>();
-        if (R1 < P2) { coll = 1; break; }
-    }
-    if (!coll) return -1;
+    for (int j = 0; j < k.MCCn; ++j)
+        if (st.u<R>() < P2) return j;
+    return -1;
+}
+
+// Electron collision after a positive null test at draw jcoll.  Returns the
+// reaction index (0..4) or -1; for ionization (3, 4) fills the new electron
+// and ion.  P1 = k.P1[node].
+template <typename R>
+__device__ inline int mcc_ar_electron_collide(const DevConst<R> &k, int node, R P1, uint32_t pid, uint32_t step,
+                                              int jcoll, R &vx, R &vy, R &vz, R &evx, R &evy, R &evz, R &ivx,
+                                              R &ivy, R &ivz) {
+    Stream st;
+    st.init(k.seed, pid, step, mcc_tag(0));
+    st.seek(jcoll);
+    const R R1 = st.u<R>();
     int target = (R1 <= P1) ? 0 : 1;
     R VX = vx, VY = vy, VZ = vz;
     R dum = VX * VX + VY * VY + VZ * VZ;
@@ -362,6 +418,15 @@ __device__ inline int mcc_ar_electron(const DevConst<R> &k, int node, uint32_t p
         vx = VX; vy = VY; vz = VZ;
     }
     return ev;
+}
+
+// Electron collision (Direct_Argon_Electron): null test + collision.
+template <typename R>
+__device__ inline int mcc_ar_electron(const DevConst<R> &k, int node, R P1, R P2, uint32_t pid, uint32_t step,
+                                      R &vx, R &vy, R &vz, R &evx, R &evy, R &evz, R &ivx, R &ivy, R &ivz) {
+    const int jc = mcc_ar_electron_null<R>(k, P2, pid, step);
+    if (jc < 0) return -1;
+    return mcc_ar_electron_collide<R>(k, node, P1, pid, step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
 }
 
 }  // namespace pic

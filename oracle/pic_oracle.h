@@ -32,7 +32,7 @@ typedef REAL real;
 #define ORC_MAXRCT 100
 
 /* Flat copy of the JSON case deck (src/start.c:19-1431).  Filled from the
- * deck by oracle/deck.py with parson's "missing key reads as 0" rule. */
+ * deck by oracle/oracle.py with parson's "missing key reads as 0" rule. */
 typedef struct {
     double xlength, ylength, zlength;
     int ngx, ngy;
