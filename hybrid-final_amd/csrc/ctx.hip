@@ -116,14 +116,15 @@ struct Case : CaseBase {
     SpecDev spec{};
     SpecState *spec_state = nullptr;
     double *spec_part = nullptr;
-    float *spec_rf = nullptr;
-    std::vector<float *> spec_bufs;
+    std::vector<double *> spec_bufs;
+    int spec_cycles = 1;
     // particles: two buffers per species
     PView<R> cur[MAXSP]{}, nxt[MAXSP]{};
     uint8_t *tag[MAXSP] = {nullptr};
     int *key[MAXSP] = {nullptr}, *prank = nullptr;
     int *cnt[MAXSP] = {nullptr}, *base[MAXSP] = {nullptr}, *pbase[MAXSP] = {nullptr};
     int *pcnt = nullptr, *pid_off = nullptr, *total = nullptr, *alive = nullptr, *ptotal = nullptr;
+    int *total2 = nullptr, *alive2 = nullptr;
     int *pstart = nullptr, *scan_tmp = nullptr, *cellp = nullptr;
     uint32_t *next_id = nullptr;
     unsigned long long *mccr = nullptr;
@@ -149,7 +150,7 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey};
+                        sum_ey, total2, alive2};
         for (void *p : ptrs)
             if (p) hipFree(p);
         for (int s = 0; s < MAXSP; ++s) {
@@ -168,7 +169,7 @@ struct Case : CaseBase {
         pcg_work_destroy(pw);
         for (int k = 0; k <= NT; ++k)
             if (ev[k]) hipEventDestroy(ev[k]);
-        for (float *p : spec_bufs) hipFree(p);
+        for (double *p : spec_bufs) hipFree(p);
         if (spec_state) hipFree(spec_state);
         if (spec_part) hipFree(spec_part);
         if (st) hipStreamDestroy(st);
@@ -337,7 +338,6 @@ struct Case : CaseBase {
                 std::string why;
                 if (build_spectral<R>(S, sp, why)) {
                     const int nx = sp.nx, ny = sp.ny;
-                    auto f32 = [&](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
                     std::vector<double> VyiT((size_t)ny * ny), VyT((size_t)ny * ny), dinv((size_t)nx * ny);
                     for (int a = 0; a < ny; ++a)
                         for (int b2 = 0; b2 < ny; ++b2) {
@@ -349,12 +349,11 @@ struct Case : CaseBase {
                             double den = sp.cx * sp.lx[a] + sp.cy * sp.ly[b2];
                             dinv[(size_t)a * ny + b2] = den > 1e-300 ? 1.0 / den : 0.0;
                         }
-                    float *p;
+                    double *p;
                     const std::vector<double> *srcs[5] = {&sp.Vxi, &VyiT, &sp.Vx, &VyT, &dinv};
-                    const float *dst[5];
+                    const double *dst[5];
                     for (int q = 0; q < 5; ++q) {
-                        std::vector<float> f = f32(*srcs[q]);
-                        HIPCHK(upload(&p, f, st));
+                        HIPCHK(upload(&p, *srcs[q], st));
                         spec_bufs.push_back(p);
                         dst[q] = p;
                     }
@@ -362,9 +361,12 @@ struct Case : CaseBase {
                     spec.Vxi = dst[0]; spec.VyiT = dst[1]; spec.Vx = dst[2]; spec.VyT = dst[3]; spec.dinv = dst[4];
                     HIPCHK(dalloc(&p, (size_t)nx * ny)); spec_bufs.push_back(p); spec.T1 = p;
                     HIPCHK(dalloc(&p, (size_t)nx * ny)); spec_bufs.push_back(p); spec.T2 = p;
-                    HIPCHK(dalloc(&spec_rf, N)); spec_bufs.push_back(spec_rf);
+                    HIPCHK(dalloc(&p, (size_t)nx * ny)); spec_bufs.push_back(p); spec.r = p;
                     HIPCHK(dalloc(&spec_state, 1));
                     HIPCHK(dalloc(&spec_part, 2 * ((N + 255) / 256) + 2));
+                    // an exact separable model converges in one FP64 cycle; otherwise
+                    // the model error (~1e-7 relative) needs a few refinement cycles
+                    spec_cycles = sp.exact ? 1 : 4;
                     spectral = true;
                     solver_kind = 1;
                 }
@@ -383,7 +385,8 @@ struct Case : CaseBase {
         HIPCHK(dalloc(&pid_off, (size_t)G * ROUTE_SLOTS));
         HIPCHK(dalloc(&total, G)); HIPCHK(dalloc(&alive, G)); HIPCHK(dalloc(&ptotal, G));
         HIPCHK(dalloc(&pstart, G + 1)); HIPCHK(dalloc(&cellp, G));
-        HIPCHK(dalloc(&scan_tmp, (G + 1023) / 1024 + 2));
+        HIPCHK(dalloc(&total2, G)); HIPCHK(dalloc(&alive2, G));
+        HIPCHK(dalloc(&scan_tmp, scan_tmp_ints(G)));
         HIPCHK(dalloc(&next_id, MAXSP));
         HIPCHK(dalloc(&mccr, (size_t)G * S.TnRct));
         HIPCHK(hipMemsetAsync(mccr, 0, (size_t)G * S.TnRct * sizeof(unsigned long long), st));
@@ -466,7 +469,7 @@ struct Case : CaseBase {
         launch_rhs_to_fp64<R>(N, brhs, b64, st);
         const double tol = S.deck.pcg_tol > 0 ? S.deck.pcg_tol : 1e-10;
         if (precision == PIC_FP64_PLAIN) pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st);
-        else if (spectral) spectral_solve_refined(op, spec, b64, x64, tol, 4, spec_state, spec_part, spec_rf, d_res, st);
+        else if (spectral) spectral_solve_refined(op, spec, b64, x64, tol, spec_cycles, spec_state, spec_part, d_res, st);
         else pcg_solve_refined(op, b64, x64, tol, inner_tol, pcg_maxit, pw, d_res, st);
         HIPCHK(hipGetLastError());
         launch_x_to_phi<R>(G, A_idx, x64, phi, st);
@@ -531,18 +534,17 @@ struct Case : CaseBase {
         launch_fused_count<R>(kc, cur[0], 0, step, cnt[0], pcnt, mccr, condcharge, sigma_sp, st);
         launch_fused_count<R>(kc, cur[1], 1, step, cnt[1], nullptr, mccr, condcharge, sigma_sp, st);
         mark(3);
-        // electrons: survivors + ionisation electrons; global product order
-        launch_totals(G, inv_start, inv_ent, cnt[0], pcnt, total, alive, ptotal, st);
-        launch_scan(total, nxt[0].cell_start, G, scan_tmp, st);
-        launch_scan(ptotal, pstart, G, scan_tmp, st);
-        launch_assign_bases(G, inv_start, inv_ent, cnt[0], pcnt, nxt[0].cell_start, alive, pstart, base[0], pbase[0],
-                            pid_off, st);
-        // ions: survivors + ionisation ions (same per-entry product counts)
-        launch_totals(G, inv_start, inv_ent, cnt[1], pcnt, total, alive, nullptr, st);
-        launch_scan(total, nxt[1].cell_start, G, scan_tmp, st);
-        launch_assign_bases(G, inv_start, inv_ent, cnt[1], pcnt, nxt[1].cell_start, alive, pstart, base[1], pbase[1],
-                            pid_off, st);
-        // pid_off recomputed identically (same product counts and order)
+        // survivors + ionisation products per destination cell for both
+        // species, their three scans (electron and ion cell_start, global
+        // product order), then the base of every (cell, slot) entry
+        launch_totals_pair(G, inv_start, inv_ent, cnt[0], cnt[1], pcnt, total, alive, total2, alive2, ptotal, st);
+        {
+            const int *ins[3] = {total, total2, ptotal};
+            int *outs[3] = {nxt[0].cell_start, nxt[1].cell_start, pstart};
+            launch_scan_multi(3, ins, outs, G, scan_tmp, st);
+        }
+        launch_assign_bases_pair(G, inv_start, inv_ent, cnt[0], cnt[1], pcnt, nxt[0].cell_start, nxt[1].cell_start,
+                                 alive, alive2, pstart, base[0], base[1], pbase[0], pbase[1], pid_off, st);
         mark(4);
         launch_fused_scatter<R>(kc, cur[0], nxt[0], nxt[1], 0, step, base[0], pbase[0], pbase[1], pid_off, next_id,
                                 nxt[0].cap, nxt[1].cap, d_err, st);

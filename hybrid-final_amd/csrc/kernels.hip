@@ -11,6 +11,8 @@
 //                 ++ collision products in parent order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pic_device.h"
 #include "pic_kernels.h"
 #include "pic_setup.h"
@@ -163,50 +165,126 @@ __global__ void k_assign_bases(int Gsize, const int *inv_start, const int *inv_e
     }
 }
 
-// exclusive scan, three phases, 1024 elements per block
-__global__ __launch_bounds__(1024) void k_scan_blocks(const int *in, int *out, int n, int *block_sums) {
-    __shared__ int sh[1024];
-    int i = blockIdx.x * 1024 + threadIdx.x;
-    int v = i < n ? in[i] : 0;
-    sh[threadIdx.x] = v;
+// Electrons and ions of the fused step in one pass per destination cell d:
+// survivors of each species, plus the ionisation products of d (the same
+// count for both species: one electron and one ion per event).
+__global__ void k_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                              const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i,
+                              int *prod_total) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= Gsize) return;
+    int ae = 0, ai = 0, pr = 0;
+    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) {
+        int ent = inv_ent[e];
+        ae += cnt_e[ent];
+        ai += cnt_i[ent];
+        pr += prod_cnt[ent];
+    }
+    total_e[d] = ae + pr; alive_e[d] = ae;
+    total_i[d] = ai + pr; alive_i[d] = ai;
+    prod_total[d] = pr;
+}
+
+__global__ void k_assign_bases_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e,
+                                    const int *cnt_i, const int *prod_cnt, const int *start_e, const int *start_i,
+                                    const int *alive_e, const int *alive_i, const int *prod_start, int *base_e,
+                                    int *base_i, int *pbase_e, int *pbase_i, int *pid_off) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= Gsize) return;
+    int re = start_e[d], ri = start_i[d];
+    int pe = start_e[d] + alive_e[d], pi = start_i[d] + alive_i[d];
+    int irun = prod_start[d];
+    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) {
+        int ent = inv_ent[e];
+        base_e[ent] = re; re += cnt_e[ent];
+        base_i[ent] = ri; ri += cnt_i[ent];
+        pbase_e[ent] = pe; pbase_i[ent] = pi; pid_off[ent] = irun;
+        const int pc = prod_cnt[ent];
+        pe += pc; pi += pc; irun += pc;
+    }
+}
+
+// Exclusive scans of up to 3 int arrays of length n (out[n] = total) in two
+// launches: per-tile sums, then each tile adds the sums of the tiles before
+// it and scans locally.  Tiles are SCAN_TILE = 4 * 1024 elements.
+constexpr int SCAN_TILE = 4096;
+struct ScanJobs {
+    const int *in[3];
+    int *out[3];
+    int *sums[3];
+};
+
+__device__ inline int block_sum_1024(int v, int *sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) sh[w] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
+    int t = 0;
+    if (threadIdx.x < 64) {
+        t = threadIdx.x < 16 ? sh[threadIdx.x] : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     }
-    if (i < n) out[i] = sh[threadIdx.x] - v;
-    if (threadIdx.x == 1023) block_sums[blockIdx.x] = sh[1023];
+    return t;   // valid in thread 0
 }
 
-__global__ __launch_bounds__(1024) void k_scan_sums(int *block_sums, int nblocks, int *total_out) {
-    __shared__ int sh[1024];
-    // serial over chunks of 1024 block sums (nblocks is small)
-    int carry = 0;
-    for (int base = 0; base < nblocks; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = i < nblocks ? block_sums[i] : 0;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(ScanJobs J, int n) {
+    __shared__ int sh[16];
+    const int a = blockIdx.y;
+    const int *in = J.in[a];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int v = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (base + q < n) v += in[base + q];
+    v = block_sum_1024(v, sh);
+    if (threadIdx.x == 0) J.sums[a][blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(ScanJobs J, int n, int ntiles) {
+    __shared__ int wtot[16];
+    __shared__ int sh_off, sh_tot;
+    const int a = blockIdx.y;
+    const int *in = J.in[a];
+    int *out = J.out[a];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w == 0) {
+        int before = 0, all = 0;
+        for (int t = lane; t < ntiles; t += 64) {
+            const int v = J.sums[a][t];
+            all += v;
+            if (t < (int)blockIdx.x) before += v;
         }
-        if (i < nblocks) block_sums[i] = carry + sh[threadIdx.x] - v;
-        int add = sh[1023];
-        __syncthreads();
-        carry += add;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            before += __shfl_xor(before, off, 64);
+            all += __shfl_xor(all, off, 64);
+        }
+        if (lane == 0) { sh_off = before; sh_tot = all; }
     }
-    if (threadIdx.x == 0) *total_out = carry;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_add(int *out, int n, const int *block_sums, const int *total, int *end_slot) {
-    int i = blockIdx.x * 1024 + threadIdx.x;
-    if (i < n) out[i] += block_sums[blockIdx.x];
-    if (i == 0 && end_slot) *end_slot = *total;
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = base + q < n ? in[base + q] : 0;
+    const int tsum = v[0] + v[1] + v[2] + v[3];
+    int incl = tsum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    int wbefore = 0;
+    for (int q = 0; q < w; ++q) wbefore += wtot[q];
+    int run = sh_off + wbefore + incl - tsum;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (base + q < n) out[base + q] = run;
+        run += v[q];
+    }
+    if (blockIdx.x == ntiles - 1 && threadIdx.x == 0) out[n] = sh_tot;
 }
 
 // ============================================================ staged: Sort scatter
@@ -987,13 +1065,32 @@ void launch_assign_bases(int Gsize, const int *inv_start, const int *inv_ent, co
     hipLaunchKernelGGL(k_assign_bases, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent, cnt,
                        prod_cnt, start, alive_total, prod_start, base, pbase, pid_off);
 }
-// exclusive scan of n ints into out[0..n]; out[n] = total.  tmp needs
-// ceil(n/1024)+1 ints.
+void launch_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                        const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i, int *prod_total,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_totals_pair, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent, cnt_e,
+                       cnt_i, prod_cnt, total_e, alive_e, total_i, alive_i, prod_total);
+}
+void launch_assign_bases_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                              const int *prod_cnt, const int *start_e, const int *start_i, const int *alive_e,
+                              const int *alive_i, const int *prod_start, int *base_e, int *base_i, int *pbase_e,
+                              int *pbase_i, int *pid_off, hipStream_t s) {
+    hipLaunchKernelGGL(k_assign_bases_pair, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent,
+                       cnt_e, cnt_i, prod_cnt, start_e, start_i, alive_e, alive_i, prod_start, base_e, base_i, pbase_e,
+                       pbase_i, pid_off);
+}
+int scan_tmp_ints(int n) { return 3 * ((n + SCAN_TILE - 1) / SCAN_TILE) + 3; }
+// exclusive scans of k <= 3 arrays of n ints: out[a][0..n], out[a][n] = total;
+// tmp holds scan_tmp_ints(n) ints
+void launch_scan_multi(int k, const int *const *in, int *const *out, int n, int *tmp, hipStream_t s) {
+    const int nt = std::max(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+    ScanJobs J{};
+    for (int a = 0; a < k; ++a) { J.in[a] = in[a]; J.out[a] = out[a]; J.sums[a] = tmp + a * nt; }
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt, k), dim3(1024), 0, s, J, n);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt, k), dim3(1024), 0, s, J, n, nt);
+}
 void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
-    int nb = (n + 1023) / 1024;
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, s, in, out, n, tmp);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, tmp, nb, tmp + nb);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, out, n, (const int *)tmp, (const int *)(tmp + nb), out + n);
+    launch_scan_multi(1, &in, &out, n, tmp, s);
 }
 
 #define INST(R)                                                                                                         \

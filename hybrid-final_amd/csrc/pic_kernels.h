@@ -64,6 +64,15 @@ void launch_totals(int, const int *, const int *, const int *, const int *, int 
 void launch_assign_bases(int, const int *, const int *, const int *, const int *, const int *, const int *, const int *,
                          int *, int *, int *, hipStream_t);
 void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s);
+int scan_tmp_ints(int n);
+void launch_scan_multi(int k, const int *const *in, int *const *out, int n, int *tmp, hipStream_t s);
+void launch_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                        const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i, int *prod_total,
+                        hipStream_t s);
+void launch_assign_bases_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                              const int *prod_cnt, const int *start_e, const int *start_i, const int *alive_e,
+                              const int *alive_i, const int *prod_start, int *base_e, int *base_i, int *pbase_e,
+                              int *pbase_i, int *pid_off, hipStream_t s);
 void launch_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total, hipStream_t s);
 void launch_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp, hipStream_t s);
 template <typename R> void launch_rhs_to_fp64(int N, const R *b, double *b64, hipStream_t s);
@@ -99,18 +108,17 @@ void pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, doub
 void pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol, int maxit,
                        PcgWork *w, PcgResult *res_dev, hipStream_t s);
 
-// ---- separable fast solve (spectral.hip)
+// ---- separable fast solve (spectral.hip), FP64 throughout
 struct SpecDev {
     int nx, ny;
-    const float *Vxi, *VyiT, *Vx, *VyT, *dinv;   // dinv = 1/(cx lx_a + cy ly_b), 0 for a null mode
-    float *T1, *T2;
+    const double *Vxi, *VyiT, *Vx, *VyT, *dinv;   // dinv = 1/(cx lx_a + cy ly_b), 0 for a null mode
+    double *T1, *T2, *r;                            // nx*ny work matrices, r = residual
 };
 struct SpecState {
     int done, outer, status, pad;
     double rel;
 };
 void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const double *b, double *x, double tol,
-                            int max_cycles, SpecState *state, double *partials, float *rf, PcgResult *res,
-                            hipStream_t st);
+                            int max_cycles, SpecState *state, double *partials, PcgResult *res, hipStream_t st);
 
 }  // namespace pic

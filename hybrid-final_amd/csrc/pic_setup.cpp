@@ -815,6 +815,7 @@ bool build_spectral(const Setup<R> &S, Spectral &sp, std::string &why) {
     tridiag_matrix(ny, yl, yr, Ty);
     // every matrix entry must match cx Tx (x) I + I (x) cy Ty
     const double tol = std::is_same<R, float>::value ? 1e-6 : 1e-13;
+    bool exact = true;
     for (int a = 0; a < nx; ++a)
         for (int b = 0; b < ny; ++b) {
             const int row = a * ny + b;
@@ -828,12 +829,13 @@ bool build_spectral(const Setup<R> &S, Spectral &sp, std::string &why) {
                 else { why = "coupling outside the 5-point stencil"; return false; }
                 const double v = (double)S.Aval[q];
                 if (std::fabs(v - model) > tol * (std::fabs(model) + cx + cy)) { why = "coefficients differ from the separable model"; return false; }
+                if (v != model) exact = false;
                 ++seen;
             }
             int expect = 1 + (a > 0) + (a < nx - 1) + (b > 0) + (b < ny - 1);
             if (seen != expect) { why = "stencil entry count differs from the separable model"; return false; }
         }
-    sp.nx = nx; sp.ny = ny; sp.cx = cx; sp.cy = cy;
+    sp.nx = nx; sp.ny = ny; sp.cx = cx; sp.cy = cy; sp.exact = exact;
     tridiag_eigen(nx, xl, xr, sp.Vx, sp.lx);
     tridiag_eigen(ny, yl, yr, sp.Vy, sp.ly);
     if (!invert(nx, sp.Vx, sp.Vxi) || !invert(ny, sp.Vy, sp.Vyi)) { why = "singular eigenbasis"; return false; }

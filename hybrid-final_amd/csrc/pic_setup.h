@@ -94,6 +94,7 @@ int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool load = true);
 // Row-major nx x nx / ny x ny FP64 matrices.
 struct Spectral {
     bool ok = false;
+    bool exact = false;   // every operator entry equals the separable model bit for bit
     int nx = 0, ny = 0;
     double cx = 0, cy = 0;
     std::vector<double> Vx, Vxi, Vy, Vyi, lx, ly;
