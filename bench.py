@@ -128,14 +128,15 @@ def main():
     st = ref["stages_ms"]
     ne, ni = [0.5 * (a + b) for a, b in zip(ref["np0"], ref["np1"])]
     kernels = {
-        # name: (algorithmic bytes per launch, ms per launch); FP32 record = x,y,vx,vy,vz,id = 24 B
-        "scatter_e": (48.0 * ne, st.get("scatter_e", 0.0)),
-        "scatter_i": (48.0 * ni, st.get("scatter_i", 0.0)),
-        "count": (24.0 * (ne + ni), st.get("count", 0.0)),
-        "deposit": (8.0 * (ne + ni), st.get("deposit", 0.0)),
+        # stage: (kernel, algorithmic bytes per launch, ms per launch); one
+        # launch per stage; FP32 record = x, y, vx, vy, vz, id = 24 B
+        "scatter_e": ("k_fused_scatter<float, electron>", 48.0 * ne, st.get("scatter_e", 0.0)),
+        "scatter_i": ("k_fused_scatter<float, ion>", 48.0 * ni, st.get("scatter_i", 0.0)),
+        "count_e": ("k_fused_count<float, electron>", 24.0 * ne, st.get("count_e", 0.0)),
+        "count_i": ("k_fused_count<float, ion>", 24.0 * ni, st.get("count_i", 0.0)),
     }
-    dom = max(kernels, key=lambda k: kernels[k][1])
-    bytes_, ms = kernels[dom]
+    dom = max(kernels, key=lambda k: kernels[k][2])
+    kname, bytes_, ms = kernels[dom]
     achieved = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
 
     line = {
@@ -167,7 +168,7 @@ def main():
         "refined_over_plain": value / plain_value if plain_value > 0 else None,
         "field_solve_last": {"cycles": ref["field_cycles"], "rel_residual": ref["field_rel_res"]},
         "stage_ms_per_step": {k: round(v, 5) for k, v in st.items()},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"kernel": kname, "stage": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": bytes_, "ms_per_launch": ms},
     }

@@ -125,6 +125,7 @@ struct Case : CaseBase {
     int *cnt[MAXSP] = {nullptr}, *base[MAXSP] = {nullptr}, *pbase[MAXSP] = {nullptr};
     int *pcnt = nullptr, *pid_off = nullptr, *total = nullptr, *alive = nullptr, *ptotal = nullptr;
     int *total2 = nullptr, *alive2 = nullptr;
+    long long stage_stride = 0;   // stride of the 6 arrays of `stage`
     int *pstart = nullptr, *scan_tmp = nullptr, *cellp = nullptr;
     uint32_t *next_id = nullptr;
     unsigned long long *mccr = nullptr;
@@ -134,14 +135,33 @@ struct Case : CaseBase {
     long long tstep = 0;
     double t = 0.0;
     int tag_valid = 0;
-    // optional stage timers (HIP events on the case stream)
-    static constexpr int NT = 8;   // field, efield, count, bases, scatter, deposit, diag, (spare)
+    // optional stage timers: HIP events on the case stream, recorded into a
+    // ring of RING steps so that timing never stalls the launch pipeline; a
+    // slot is read back only when the ring wraps or the times are requested.
+    // Stage k of a step = events k .. k+1: field, efield, count_e, count_i,
+    // bases, scatter_e, scatter_i, deposit, accumulate.
+    static constexpr int NT = 9, RING = 32;
     bool timing = false;
-    hipEvent_t ev[NT + 1] = {};
+    hipEvent_t ev[RING][NT + 1] = {};
+    bool ev_pending[RING] = {};
+    int ev_slot = 0;
+    long long ev_rec = 0;
     double t_ms[NT] = {0};
     long long t_n = 0;
     void mark(int k) {
-        if (timing) hipEventRecord(ev[k], st);
+        if (timing) hipEventRecord(ev[ev_slot][k], st);
+    }
+    int flush_slot(int sl) {
+        if (!ev_pending[sl]) return 0;
+        HIPCHK(hipEventSynchronize(ev[sl][NT]));
+        for (int k = 0; k < NT; ++k) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[sl][k], ev[sl][k + 1]));
+            t_ms[k] += ms;
+        }
+        ++t_n;
+        ev_pending[sl] = false;
+        return 0;
     }
 
     ~Case() override {
@@ -167,8 +187,9 @@ struct Case : CaseBase {
             if (pbase[s]) hipFree(pbase[s]);
         }
         pcg_work_destroy(pw);
-        for (int k = 0; k <= NT; ++k)
-            if (ev[k]) hipEventDestroy(ev[k]);
+        for (int r = 0; r < RING; ++r)
+            for (int k = 0; k <= NT; ++k)
+                if (ev[r][k]) hipEventDestroy(ev[r][k]);
         for (double *p : spec_bufs) hipFree(p);
         if (spec_state) hipFree(spec_state);
         if (spec_part) hipFree(spec_part);
@@ -380,6 +401,7 @@ struct Case : CaseBase {
             if (int e2 = alloc_species(s, cap)) return e2;
         }
         HIPCHK(dalloc(&stage, (size_t)6 * capmax));
+        stage_stride = capmax;
         HIPCHK(dalloc(&prank, capmax));
         HIPCHK(dalloc(&pcnt, (size_t)G * ROUTE_SLOTS));
         HIPCHK(dalloc(&pid_off, (size_t)G * ROUTE_SLOTS));
@@ -531,9 +553,16 @@ struct Case : CaseBase {
     }
     int particle_stage_fused() {
         const uint32_t step = (uint32_t)tstep;
-        launch_fused_count<R>(kc, cur[0], 0, step, cnt[0], pcnt, mccr, condcharge, sigma_sp, st);
-        launch_fused_count<R>(kc, cur[1], 1, step, cnt[1], nullptr, mccr, condcharge, sigma_sp, st);
+        // flag bytes reuse the staged path's per-particle move tags; the
+        // ionisation stash reuses its product staging buffer (stride = the
+        // largest capacity)
+        const size_t sstride = (size_t)stage_stride;
+        launch_fused_count<R>(kc, cur[0], 0, step, cnt[0], pcnt, mccr, condcharge, sigma_sp, tag[0], stage, sstride,
+                              st);
         mark(3);
+        launch_fused_count<R>(kc, cur[1], 1, step, cnt[1], nullptr, mccr, condcharge, sigma_sp, tag[1], stage,
+                              sstride, st);
+        mark(4);
         // survivors + ionisation products per destination cell for both
         // species, their three scans (electron and ion cell_start, global
         // product order), then the base of every (cell, slot) entry
@@ -545,12 +574,12 @@ struct Case : CaseBase {
         }
         launch_assign_bases_pair(G, inv_start, inv_ent, cnt[0], cnt[1], pcnt, nxt[0].cell_start, nxt[1].cell_start,
                                  alive, alive2, pstart, base[0], base[1], pbase[0], pbase[1], pid_off, st);
-        mark(4);
-        launch_fused_scatter<R>(kc, cur[0], nxt[0], nxt[1], 0, step, base[0], pbase[0], pbase[1], pid_off, next_id,
-                                nxt[0].cap, nxt[1].cap, d_err, st);
         mark(5);
-        launch_fused_scatter<R>(kc, cur[1], nxt[1], nxt[1], 1, step, base[1], pbase[1], pbase[1], pid_off, next_id,
-                                nxt[1].cap, nxt[1].cap, d_err, st);
+        launch_fused_scatter<R>(kc, cur[0], nxt[0], nxt[1], 0, tag[0], stage, sstride, base[0], pbase[0], pbase[1],
+                                pid_off, next_id, nxt[0].cap, nxt[1].cap, d_err, st);
+        mark(6);
+        launch_fused_scatter<R>(kc, cur[1], nxt[1], nxt[1], 1, tag[1], stage, sstride, base[1], pbase[1], pbase[1],
+                                pid_off, next_id, nxt[1].cap, nxt[1].cap, d_err, st);
         launch_bump_ids(next_id, pstart + G, nsp, st);
         swap_species(0);
         swap_species(1);
@@ -559,6 +588,10 @@ struct Case : CaseBase {
     }
     int step_fused(int n) override {
         for (int i = 0; i < n; ++i) {
+            if (timing) {
+                ev_slot = (int)(ev_rec % RING);
+                if (int e = flush_slot(ev_slot)) return e;
+            }
             advance_time();
             mark(0);
             if (int e = field_solve()) return e;
@@ -566,40 +599,33 @@ struct Case : CaseBase {
             if (int e = efield()) return e;
             mark(2);
             if (int e = particle_stage_fused()) return e;
-            mark(6);
-            if (int e = deposit()) return e;
             mark(7);
-            if (int e = diagnostic()) return e;
+            if (int e = deposit()) return e;
             mark(8);
+            if (int e = diagnostic()) return e;
+            mark(9);
             if (timing) {
-                HIPCHK(hipEventSynchronize(ev[8]));
-                for (int k = 0; k < 8; ++k) {
-                    float ms = 0.f;
-                    if (k == 5) continue;
-                    int a = k, b = k + 1;
-                    if (k >= 2 && k <= 4) continue;
-                    hipEventElapsedTime(&ms, ev[a], ev[b]);
-                    t_ms[k] += ms;
-                }
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, ev[2], ev[3]); t_ms[2] += ms;
-                hipEventElapsedTime(&ms, ev[3], ev[4]); t_ms[3] += ms;
-                hipEventElapsedTime(&ms, ev[4], ev[5]); t_ms[4] += ms;
-                hipEventElapsedTime(&ms, ev[5], ev[6]); t_ms[5] += ms;
-                ++t_n;
+                ev_pending[ev_slot] = true;
+                ++ev_rec;
             }
         }
         return 0;
     }
     int set_timing(int on) override {
-        if (on && !ev[0])
-            for (int k = 0; k <= NT; ++k) HIPCHK(hipEventCreate(&ev[k]));
+        if (on && !ev[0][0])
+            for (int r = 0; r < RING; ++r)
+                for (int k = 0; k <= NT; ++k) HIPCHK(hipEventCreate(&ev[r][k]));
+        for (int r = 0; r < RING; ++r) ev_pending[r] = false;
         timing = on != 0;
         for (int k = 0; k < NT; ++k) t_ms[k] = 0;
         t_n = 0;
+        ev_rec = 0;
         return 0;
     }
     int get_timing(double *ms, int n, long long *steps) override {
+        if (timing)
+            for (long long q = 0; q < RING; ++q)
+                if (int e = flush_slot((int)((ev_rec + q) % RING))) return e;
         for (int k = 0; k < n && k < NT; ++k) ms[k] = t_ms[k];
         if (steps) *steps = t_n;
         return 0;

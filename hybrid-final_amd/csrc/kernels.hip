@@ -23,10 +23,10 @@ namespace {
 
 __device__ inline int wave_cell(int ncy, int ngy, int Csize, int &lane) {
     lane = threadIdx.x & 63;
-    int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    // wave-uniform: computed in scalar registers
+    const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     if (w >= Csize) return -1;
-    int c = (w / ncy) * ngy + (w % ncy);
-    return __builtin_amdgcn_readfirstlane(c);
+    return (w / ncy) * ngy + (w % ncy);
 }
 
 __device__ inline unsigned long long lanes_below(int lane) {
@@ -388,28 +388,32 @@ __global__ __launch_bounds__(256) void k_mcc_insert(DevConst<R> k, PView<R> src,
 }
 
 // ============================================================ fused step
+// Pass 1 (count) pushes every particle, applies the walls and the
+// null-collision test, and counts survivors per (cell, slot).  Colliding
+// particles are pooled across the 4 waves of a block and collided together,
+// so the long collision code runs on nearly full wavefronts; their final
+// state is written back in place, and a per-particle flag byte records the
+// move code, whether the particle collided and whether it made an
+// ionisation pair (whose velocities go to a stash indexed like the
+// particle).  Pass 2 (scatter) re-pushes only the particles that did not
+// collide (same bits), ranks everything with ballots in particle order and
+// writes the new, sorted arrays; it draws no random numbers.
+//
 // Both passes stage each wave's cell in chunks of FCHUNK particles through
 // LDS: every lane issues the loads of all its particles of the chunk at once
-// (one memory round trip per chunk instead of one per 64 particles), then
-// walks them 64 at a time.  The cell's 9 route entries and the collision
-// probabilities of their destination nodes are loaded once per wave into
-// lanes 0..8 and fetched per particle with a lane shuffle.
-//
-// Collisions are compacted: the walk runs the push and the null-collision
-// test on every particle and appends the (few) colliding ones, in particle
-// order, to a per-wave queue in LDS; the queue is then processed 64 entries
-// at a time, so the long collision code runs on full wavefronts instead of
-// on one or two lanes of every wavefront.
+// (one memory round trip per chunk), then walks them 64 at a time.  The
+// cell's 9 route entries and the collision probabilities of their
+// destination nodes are loaded once per wave and fetched per particle with a
+// lane shuffle.
 constexpr int FCHUNK = 256;
 constexpr int FWAVES = 4;   // waves per 256-thread block
 enum { KIND_AR_ELECTRON = 0, KIND_AR_ION = 1 };
+enum { FLAG_CODE = 15, FLAG_DONE = 16, FLAG_PROD = 32 };
 
 template <typename R>
 struct FStage {
     R x[FCHUNK], y[FCHUNK], vx[FCHUNK], vy[FCHUNK], vz[FCHUNK];
     uint32_t id[FCHUNK];
-    int q[FCHUNK];      // collision queue: i | code << 8 | jcoll << 12
-    int qpos[FCHUNK];   // scatter pass: output slot of the queued particle
 };
 
 // Orders the LDS accesses of one wavefront (the hardware already executes a
@@ -419,6 +423,18 @@ __device__ inline void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
+
+// collision pool of one count block
+template <typename R>
+struct FPool {
+    int q[FWAVES * FCHUNK];   // wave << 28 | jcoll << 12 | code << 8 | i
+    int qn;
+    int b0[FWAVES];           // first particle of each wave's current chunk
+    int dest[FWAVES][ROUTE_SLOTS], info[FWAVES][ROUTE_SLOTS];
+    R pa[FWAVES][ROUTE_SLOTS];
+    int prun[FWAVES][ROUTE_SLOTS];
+    int rounds[FWAVES];
+};
 
 // global -> LDS for particles [b0, b0+n), n >= 1.  Lanes past the end load the
 // last particle again (same cache line, never read back), so the 6*FCHUNK/64
@@ -447,7 +463,7 @@ struct WaveRoutes {
     R pa, pb;
     __device__ void load(const DevConst<R> &k, int c, int lane, int kind) {
         dest = -1; info = 0; pa = 0; pb = 0;
-        if (lane < ROUTE_SLOTS) {
+        if (c >= 0 && lane < ROUTE_SLOTS) {
             Route r = k.route[c * ROUTE_SLOTS + lane];
             dest = r.dest; info = r.info;
             if (dest >= 0) {
@@ -478,33 +494,48 @@ __device__ inline int collide(const DevConst<R> &k, int dest, R pa, uint32_t id,
     return mcc_ar_ion_collide<R>(k, dest, st, vx, vy, vz);
 }
 
-// Pass 1: push + walls + collision, counting survivors and products per
-// (cell, slot).  Side effects (wall counters, surface charge, event counts)
-// happen here only.
+// Pass 1.  flag[k] = move code | FLAG_DONE (collided: final state written
+// back in place) | FLAG_PROD (made an ionisation pair; the new electron's
+// and ion's velocities in stash[q * stash_stride + k], q = 0..5).  The waves
+// of a block walk their cells chunk by chunk in step; after each round the
+// block collides the pooled particles of its 4 cells together.
 template <typename R, int KIND>
 __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, uint32_t step, int *cnt,
                                                      int *pcnt, unsigned long long *mccr, int *condcharge,
-                                                     R *sigma_sp) {
+                                                     R *sigma_sp, uint8_t *flag, R *stash, size_t stash_stride) {
     __shared__ FStage<R> stage_all[FWAVES];
+    __shared__ FPool<R> pool;
+    const int wv = threadIdx.x >> 6;
     int lane;
-    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
-    if (c < 0) return;
-    FStage<R> &S = stage_all[threadIdx.x >> 6];
+    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    FStage<R> &S = stage_all[wv];
     const unsigned long long lt = lanes_below(lane);
-    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
-    int run[ROUTE_SLOTS], prun[ROUTE_SLOTS];
+    int k0 = 0, k1 = 0;
+    if (c >= 0) { k0 = p.cell_start[c]; k1 = p.cell_start[c + 1]; }
+    WaveRoutes<R> w;
+    w.load(k, c, lane, KIND);
+    if (lane < ROUTE_SLOTS) {
+        pool.dest[wv][lane] = w.dest; pool.info[wv][lane] = w.info; pool.pa[wv][lane] = w.pa;
+        pool.prun[wv][lane] = 0;
+    }
+    if (lane == 0) pool.rounds[wv] = (k1 - k0 + FCHUNK - 1) / FCHUNK;
+    if (threadIdx.x == 0) pool.qn = 0;
+    CellE<R> e;
+    if (k0 < k1) e.load(k.Ex, k.Ey, c, k.ngy);
+    const R As = k.Ascale[isp];
+    int run[ROUTE_SLOTS];
 #pragma unroll
-    for (int s = 0; s < ROUTE_SLOTS; ++s) { run[s] = 0; prun[s] = 0; }
-    if (k0 < k1) {
-        CellE<R> e;
-        e.load(k.Ex, k.Ey, c, k.ngy);
-        WaveRoutes<R> w;
-        w.load(k, c, lane, KIND);
-        const R As = k.Ascale[isp];
-        for (int b0 = k0; b0 < k1; b0 += FCHUNK) {
-            const int n = min(FCHUNK, k1 - b0);
+    for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] = 0;
+    __syncthreads();
+    int nrounds = 0;
+#pragma unroll
+    for (int q = 0; q < FWAVES; ++q) nrounds = max(nrounds, pool.rounds[q]);
+    for (int round = 0; round < nrounds; ++round) {
+        const int b0 = k0 + round * FCHUNK;
+        const int n = min(FCHUNK, k1 - b0);
+        if (lane == 0) pool.b0[wv] = b0;
+        if (n > 0) {
             stage_chunk<R>(S, p, b0, n, lane);
-            int qn = 0;
             for (int j = 0; j * 64 < n; ++j) {
                 const int i = j * 64 + lane;
                 int slot = -1, jc = -1;
@@ -527,68 +558,80 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
 #pragma unroll
                 for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] += __popcll(__ballot(slot == s));
                 const unsigned long long cm = __ballot(jc >= 0);
+                int qbase = 0;
+                if (lane == 0 && cm) qbase = atomicAdd(&pool.qn, __popcll(cm));
+                qbase = __shfl(qbase, 0, 64);
                 if (jc >= 0) {
-                    S.vx[i] = vx;   // post-push velocity (vz is unchanged by the push)
-                    S.vy[i] = vy;
-                    S.q[qn + __popcll(cm & lt)] = i | (code << 8) | (jc << 12);
-                }
-                qn += __popcll(cm);
-            }
-            wave_lds_fence();
-            for (int qb = 0; qb < qn; qb += 64) {
-                const bool valid = qb + lane < qn;
-                const int qe = valid ? S.q[qb + lane] : 0;
-                const int i = qe & 255, code = (qe >> 8) & 15, jc = qe >> 12;
-                const int dest = __shfl(w.dest, code, 64);
-                const int info = __shfl(w.info, code, 64);
-                const R pa = __shfl(w.pa, code, 64);
-                int slot = -1;
-                bool prod = false;
-                if (valid) {
-                    R vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-                    R evx, evy, evz, ivx, ivy, ivz;
-                    const int ev = collide<R, KIND>(k, dest, pa, S.id[i], step, jc, vx, vy, vz, evx, evy, evz, ivx,
-                                                    ivy, ivz);
-                    if (ev >= 0) atomicAdd(&mccr[(size_t)dest * k.TnRct + ev], 1ull);
-                    prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
-                    slot = info & 15;
-                }
-                if (KIND == KIND_AR_ELECTRON) {
-#pragma unroll
-                    for (int s = 0; s < ROUTE_SLOTS; ++s) prun[s] += __popcll(__ballot(prod && slot == s));
+                    S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy;   // post-push state (vz unchanged)
+                    pool.q[qbase + __popcll(cm & lt)] = (wv << 28) | (jc << 12) | (code << 8) | i;
+                } else if (i < n) {
+                    flag[b0 + i] = (uint8_t)code;
                 }
             }
-            wave_lds_fence();
         }
+        __syncthreads();
+        const int qn = pool.qn;
+        __syncthreads();
+        if (threadIdx.x == 0) pool.qn = 0;   // every thread has read qn; the next walk appends after the barrier below
+        for (int qb = 0; qb < qn; qb += FWAVES * 64) {
+            const int ei = qb + (int)threadIdx.x;
+            if (ei < qn) {
+                const int qe = pool.q[ei];
+                const int ww = qe >> 28, jc = (qe >> 12) & 0xffff, code = (qe >> 8) & 15, i = qe & 255;
+                const int dest = pool.dest[ww][code], info = pool.info[ww][code];
+                const R pa = pool.pa[ww][code];
+                FStage<R> &T = stage_all[ww];
+                R vx = T.vx[i], vy = T.vy[i], vz = T.vz[i];
+                R evx, evy, evz, ivx, ivy, ivz;
+                const int ev = collide<R, KIND>(k, dest, pa, T.id[i], step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy,
+                                                ivz);
+                if (ev >= 0) atomicAdd(&mccr[(size_t)dest * k.TnRct + ev], 1ull);
+                const bool prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
+                const int kk = pool.b0[ww] + i;
+                p.x[kk] = T.x[i]; p.y[kk] = T.y[i]; p.vx[kk] = vx; p.vy[kk] = vy; p.vz[kk] = vz;
+                flag[kk] = (uint8_t)(code | FLAG_DONE | (prod ? FLAG_PROD : 0));
+                if (prod) {
+                    atomicAdd(&pool.prun[ww][info & 15], 1);
+                    stash[kk] = evx; stash[stash_stride + kk] = evy; stash[2 * stash_stride + kk] = evz;
+                    stash[3 * stash_stride + kk] = ivx; stash[4 * stash_stride + kk] = ivy;
+                    stash[5 * stash_stride + kk] = ivz;
+                }
+            }
+        }
+        __syncthreads();
     }
-    if (lane == 0) {
+    if (c >= 0 && lane == 0) {
 #pragma unroll
         for (int s = 0; s < ROUTE_SLOTS; ++s) {
             cnt[c * ROUTE_SLOTS + s] = run[s];
-            if (pcnt) pcnt[c * ROUTE_SLOTS + s] = prun[s];
+            if (pcnt) pcnt[c * ROUTE_SLOTS + s] = pool.prun[wv][s];
         }
     }
 }
 
-// Pass 2: recompute (same bits) and write each particle and product to its
-// final slot.  For electrons, ionisation products go to both the electron
-// and the ion arrays.
+// Pass 2: write each particle and product to its final slot.  For
+// electrons, ionisation products go to both the electron and the ion arrays.
+// Particles that did not collide are pushed again (same bits as pass 1);
+// collided ones already hold their final state.  The flag bytes are staged
+// with the particles.
 template <typename R, int KIND>
 __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p, PView<R> q, PView<R> qion, int isp,
-                                                       uint32_t step, const int *base, const int *pbase_e,
-                                                       const int *pbase_i, const int *pid_off, const uint32_t *next_id,
-                                                       int cap, int cap_ion, int *err) {
+                                                       const uint8_t *flag, const R *stash, size_t stash_stride,
+                                                       const int *base, const int *pbase_e, const int *pbase_i,
+                                                       const int *pid_off, const uint32_t *next_id, int cap,
+                                                       int cap_ion, int *err) {
     __shared__ FStage<R> stage_all[FWAVES];
+    __shared__ uint8_t flag_all[FWAVES][FCHUNK];
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
     int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
     if (k0 == k1) return;
     FStage<R> &S = stage_all[threadIdx.x >> 6];
+    uint8_t *F = flag_all[threadIdx.x >> 6];
     CellE<R> e;
     e.load(k.Ex, k.Ey, c, k.ngy);
-    WaveRoutes<R> w;
-    w.load(k, c, lane, KIND);
+    const Route myr = lane < ROUTE_SLOTS ? k.route[c * ROUTE_SLOTS + lane] : Route{-1, 0};
     // per-slot output bases of this cell, held by lane = slot
     const int my_base = lane < ROUTE_SLOTS ? base[c * ROUTE_SLOTS + lane] : 0;
     const R As = k.Ascale[isp];
@@ -598,102 +641,71 @@ __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p
     for (int s = 0; s < ROUTE_SLOTS; ++s) { run[s] = 0; prun[s] = 0; }
     for (int b0 = k0; b0 < k1; b0 += FCHUNK) {
         const int n = min(FCHUNK, k1 - b0);
-        stage_chunk<R>(S, p, b0, n, lane);
-        int qn = 0;
+        {
+            uint8_t tf[FCHUNK / 64];
+#pragma unroll
+            for (int j = 0; j < FCHUNK / 64; ++j) tf[j] = flag[b0 + min(j * 64 + lane, n - 1)];
+            stage_chunk<R>(S, p, b0, n, lane);
+#pragma unroll
+            for (int j = 0; j < FCHUNK / 64; ++j) F[j * 64 + lane] = tf[j];
+        }
         for (int j = 0; j * 64 < n; ++j) {
             const int i = j * 64 + lane;
-            int slot = -1, jc = -1;
+            const int f = F[i];
+            const int code = f & FLAG_CODE;
+            const int dest = __shfl(myr.dest, code, 64);
+            const int info = __shfl(myr.info, code, 64);
             R x = S.x[i], y = S.y[i], vx = S.vx[i], vy = S.vy[i];
-            const uint32_t id = S.id[i];
-            const int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx,
-                                              k.dt_dy, x, y, vx, vy);
-            const int dest = __shfl(w.dest, code, 64);
-            const int info = __shfl(w.info, code, 64);
-            const R pb = __shfl(w.pb, code, 64);
-            if (i < n && dest >= 0) {
-                apply_reflect<R>(info, x, y, vx, vy);
-                slot = info & 15;
-                jc = null_test<R, KIND>(k, pb, id, step);
+            if (!(f & FLAG_DONE)) {
+                push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx, k.dt_dy, x, y,
+                                 vx, vy);
+                if (dest >= 0) apply_reflect<R>(info, x, y, vx, vy);
             }
-            int pos = -1;
+            const int slot = (i < n && dest >= 0) ? (info & 15) : -1;
+            const bool prod = (KIND == KIND_AR_ELECTRON) && slot >= 0 && (f & FLAG_PROD);
+            int pos = -1, prank = 0;
 #pragma unroll
             for (int s = 0; s < ROUTE_SLOTS; ++s) {
-                unsigned long long m = __ballot(slot == s);
+                const unsigned long long m = __ballot(slot == s);
                 if (slot == s) pos = run[s] + __popcll(m & lt);
                 run[s] += __popcll(m);
-            }
-            const int sbase = __shfl(my_base, slot < 0 ? 0 : slot, 64);
-            pos += sbase;
-            if (slot >= 0 && jc < 0) {
-                if (pos < cap) {
-                    q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = S.vz[i]; q.id[pos] = id;
-                } else {
-                    atomicOr(err, 1);
-                }
-            }
-            const unsigned long long cm = __ballot(jc >= 0);
-            if (jc >= 0) {
-                const int qi = qn + __popcll(cm & lt);
-                S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy;
-                S.q[qi] = i | (code << 8) | (jc << 12);
-                S.qpos[qi] = pos;
-            }
-            qn += __popcll(cm);
-        }
-        wave_lds_fence();
-        for (int qb = 0; qb < qn; qb += 64) {
-            const bool valid = qb + lane < qn;
-            const int qe = valid ? S.q[qb + lane] : 0;
-            const int i = qe & 255, code = (qe >> 8) & 15, jc = qe >> 12;
-            const int dest = __shfl(w.dest, code, 64);
-            const int info = __shfl(w.info, code, 64);
-            const R pa = __shfl(w.pa, code, 64);
-            int slot = -1;
-            bool prod = false;
-            R x = 0, y = 0, vx = 0, vy = 0, vz = 0, evx = 0, evy = 0, evz = 0, ivx = 0, ivy = 0, ivz = 0;
-            uint32_t id = 0;
-            int pos = 0;
-            if (valid) {
-                x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i]; id = S.id[i];
-                pos = S.qpos[qb + lane];
-                const int ev = collide<R, KIND>(k, dest, pa, id, step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
-                prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
-                slot = info & 15;
-                if (pos < cap) {
-                    q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = vz; q.id[pos] = id;
-                } else {
-                    atomicOr(err, 1);
-                }
-            }
-            if (KIND == KIND_AR_ELECTRON) {
-                int prank = 0;
-#pragma unroll
-                for (int s = 0; s < ROUTE_SLOTS; ++s) {
-                    unsigned long long pm = __ballot(prod && slot == s);
+                if (KIND == KIND_AR_ELECTRON) {
+                    const unsigned long long pm = __ballot(prod && slot == s);
                     if (prod && slot == s) prank = prun[s] + __popcll(pm & lt);
                     prun[s] += __popcll(pm);
                 }
-                if (prod) {
-                    const int ent = c * ROUTE_SLOTS + slot;
-                    const int off = pid_off[ent] + prank;
-                    const int ppos = pbase_e[ent] + prank;
-                    if (ppos < cap) {
-                        q.x[ppos] = x; q.y[ppos] = y; q.vx[ppos] = evx; q.vy[ppos] = evy; q.vz[ppos] = evz;
-                        q.id[ppos] = next_id[0] + (uint32_t)off;
-                    } else {
-                        atomicOr(err, 1);
-                    }
-                    const int ipos = pbase_i[ent] + prank;
-                    if (ipos < cap_ion) {
-                        qion.x[ipos] = x; qion.y[ipos] = y; qion.vx[ipos] = ivx; qion.vy[ipos] = ivy;
-                        qion.vz[ipos] = ivz; qion.id[ipos] = next_id[1] + (uint32_t)off;
-                    } else {
-                        atomicOr(err, 1);
-                    }
+            }
+            const int sbase = __shfl(my_base, slot < 0 ? 0 : slot, 64);
+            if (slot >= 0) {
+                pos += sbase;
+                if (pos < cap) {
+                    q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = S.vz[i]; q.id[pos] = S.id[i];
+                } else {
+                    atomicOr(err, 1);
+                }
+            }
+            if (prod) {
+                const int kk = b0 + i;
+                const int ent = c * ROUTE_SLOTS + slot;
+                const int off = pid_off[ent] + prank;
+                const int ppos = pbase_e[ent] + prank;
+                if (ppos < cap) {
+                    q.x[ppos] = x; q.y[ppos] = y;
+                    q.vx[ppos] = stash[kk]; q.vy[ppos] = stash[stash_stride + kk]; q.vz[ppos] = stash[2 * stash_stride + kk];
+                    q.id[ppos] = next_id[0] + (uint32_t)off;
+                } else {
+                    atomicOr(err, 1);
+                }
+                const int ipos = pbase_i[ent] + prank;
+                if (ipos < cap_ion) {
+                    qion.x[ipos] = x; qion.y[ipos] = y; qion.vx[ipos] = stash[3 * stash_stride + kk];
+                    qion.vy[ipos] = stash[4 * stash_stride + kk]; qion.vz[ipos] = stash[5 * stash_stride + kk];
+                    qion.id[ipos] = next_id[1] + (uint32_t)off;
+                } else {
+                    atomicOr(err, 1);
                 }
             }
         }
-        wave_lds_fence();
     }
 }
 
@@ -992,24 +1004,28 @@ void launch_mcc_insert(const DevConst<R> &k, const PView<R> &src, const PView<R>
 }
 template <typename R>
 void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, uint32_t step, int *cnt, int *pcnt,
-                        unsigned long long *mccr, int *cc, R *sg, hipStream_t s) {
+                        unsigned long long *mccr, int *cc, R *sg, uint8_t *flag, R *stash, size_t stash_stride,
+                        hipStream_t s) {
     if (isp == 0)
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           step, cnt, pcnt, mccr, cc, sg);
+                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride);
     else
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           step, cnt, pcnt, mccr, cc, sg);
+                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride);
 }
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
-                          uint32_t step, const int *base, const int *pbase_e, const int *pbase_i, const int *pid_off,
-                          const uint32_t *next_id, int cap, int cap_ion, int *err, hipStream_t s) {
+                          const uint8_t *flag, const R *stash, size_t stash_stride, const int *base,
+                          const int *pbase_e, const int *pbase_i, const int *pid_off, const uint32_t *next_id,
+                          int cap, int cap_ion, int *err, hipStream_t s) {
     if (isp == 0)
         hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p,
-                           q, qion, isp, step, base, pbase_e, pbase_i, pid_off, next_id, cap, cap_ion, err);
+                           q, qion, isp, flag, stash, stash_stride, base, pbase_e, pbase_i, pid_off, next_id, cap,
+                           cap_ion, err);
     else
         hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, q,
-                           qion, isp, step, base, pbase_e, pbase_i, pid_off, next_id, cap, cap_ion, err);
+                           qion, isp, flag, stash, stash_stride, base, pbase_e, pbase_i, pid_off, next_id, cap,
+                           cap_ion, err);
 }
 template <typename R>
 void launch_deposit(const DevConst<R> &k, const NodeConst<R> &n, const PView<R> *ps, R *part, R *den, R *sigma_sp,
@@ -1105,10 +1121,10 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
                                        const int *, const PView<R> &, const int *, const R *, int, const uint32_t *,   \
                                        int, int *, hipStream_t);                                                       \
     template void launch_fused_count<R>(const DevConst<R> &, const PView<R> &, int, uint32_t, int *, int *,            \
-                                        unsigned long long *, int *, R *, hipStream_t);                                \
+                                        unsigned long long *, int *, R *, uint8_t *, R *, size_t, hipStream_t);        \
     template void launch_fused_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const PView<R> &,   \
-                                          int, uint32_t, const int *, const int *, const int *, const int *,           \
-                                          const uint32_t *, int, int, int *, hipStream_t);                             \
+                                          int, const uint8_t *, const R *, size_t, const int *, const int *,           \
+                                          const int *, const int *, const uint32_t *, int, int, int *, hipStream_t);   \
     template void launch_deposit<R>(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *, R *, R *, R *,   \
                                     R *, R *, hipStream_t);                                                            \
     template void launch_efield<R>(const NodeConst<R> &, const VoltConst &, double, const R *, R *, const R *, R *,    \

@@ -44,13 +44,18 @@ void launch_mcc_collide(const DevConst<R> &, const PView<R> &, int, uint32_t, un
 template <typename R>
 void launch_mcc_insert(const DevConst<R> &, const PView<R> &, const PView<R> &, const int *, const int *,
                        const PView<R> &, const int *, const R *, int, const uint32_t *, int, int *, hipStream_t);
+// fused step: count pass (push, walls, MCC; collided particles written back
+// in place, flag byte per particle, ionisation products in stash, 6 arrays
+// of stash_stride) and scatter pass (sorted write of survivors and products)
 template <typename R>
-void launch_fused_count(const DevConst<R> &, const PView<R> &, int, uint32_t, int *, int *, unsigned long long *, int *,
-                        R *, hipStream_t);
+void launch_fused_count(const DevConst<R> &, const PView<R> &, int isp, uint32_t step, int *cnt, int *pcnt,
+                        unsigned long long *mccr, int *condcharge, R *sigma_sp, uint8_t *flag, R *stash,
+                        size_t stash_stride, hipStream_t);
 template <typename R>
-void launch_fused_scatter(const DevConst<R> &, const PView<R> &, const PView<R> &, const PView<R> &, int, uint32_t,
-                          const int *, const int *, const int *, const int *, const uint32_t *, int, int, int *,
-                          hipStream_t);
+void launch_fused_scatter(const DevConst<R> &, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
+                          const uint8_t *flag, const R *stash, size_t stash_stride, const int *base,
+                          const int *pbase_e, const int *pbase_i, const int *pid_off, const uint32_t *next_id, int cap,
+                          int cap_ion, int *err, hipStream_t);
 template <typename R>
 void launch_deposit(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *, R *, R *, R *, R *, R *,
                     hipStream_t);

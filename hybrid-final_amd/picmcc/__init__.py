@@ -218,15 +218,15 @@ class Case:
     def set_step(self, tstep):
         _check(lib().pic_set_step(self.h, tstep))
 
-    STAGES = ("field", "efield", "count", "bases", "scatter_e", "scatter_i", "deposit", "accumulate")
+    STAGES = ("field", "efield", "count_e", "count_i", "bases", "scatter_e", "scatter_i", "deposit", "accumulate")
 
     def stage_timing(self, enable=True):
         _check(lib().pic_stage_timing(self.h, int(enable)))
 
     def stage_times(self):
-        ms = (C.c_double * 8)()
+        ms = (C.c_double * len(self.STAGES))()
         n = C.c_longlong()
-        _check(lib().pic_stage_times(self.h, ms, 8, C.byref(n)))
+        _check(lib().pic_stage_times(self.h, ms, len(self.STAGES), C.byref(n)))
         return dict(zip(self.STAGES, list(ms))), n.value
 
     def field_solver_kind(self):

@@ -88,9 +88,9 @@ const char *pic_array_names(void);
 int pic_set_step(pic_ctx *ctx, long long tstep);
 int pic_last_field_solve(pic_ctx *ctx, int *iters, int *outer, double *rel_res);
 /* per-stage GPU time of pic_step, from HIP events on the case's stream:
- * ms[0] field solve, [1] E field, [2] count pass (push+walls+MCC),
- * [3] scan/bases, [4] electron scatter, [5] ion scatter, [6] deposit,
- * [7] accumulators; summed over *steps steps since enabling */
+ * ms[0] field solve, [1] E field, [2] electron count pass (push+walls+MCC),
+ * [3] ion count pass, [4] scan/bases, [5] electron scatter, [6] ion scatter,
+ * [7] deposit, [8] accumulators; summed over *steps steps since enabling */
 int pic_stage_timing(pic_ctx *ctx, int enable);
 int pic_stage_times(pic_ctx *ctx, double *ms, int n, long long *steps);
 /* 0 = Jacobi PCG, 1 = separable fast solve (FP32 spectral inverse in FP64 refinement) */
