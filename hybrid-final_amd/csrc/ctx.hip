@@ -125,6 +125,7 @@ struct Case : CaseBase {
     int *cnt[MAXSP] = {nullptr}, *base[MAXSP] = {nullptr}, *pbase[MAXSP] = {nullptr};
     int *pcnt = nullptr, *pid_off = nullptr, *total = nullptr, *alive = nullptr, *ptotal = nullptr;
     int *total2 = nullptr, *alive2 = nullptr;
+    int *ent_dest = nullptr, *ent_pos = nullptr;
     long long stage_stride = 0;   // stride of the 6 arrays of `stage`
     int *pstart = nullptr, *scan_tmp = nullptr, *cellp = nullptr;
     uint32_t *next_id = nullptr;
@@ -170,7 +171,7 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey, total2, alive2};
+                        sum_ey, total2, alive2, ent_dest, ent_pos};
         for (void *p : ptrs)
             if (p) hipFree(p);
         for (int s = 0; s < MAXSP; ++s) {
@@ -325,6 +326,17 @@ struct Case : CaseBase {
         std::vector<int> istart, ient;
         build_routes(rt, istart, ient);
         HIPCHK(upload(&route, rt, st)); HIPCHK(upload(&inv_start, istart, st)); HIPCHK(upload(&inv_ent, ient, st));
+        {
+            // per (cell, slot) entry: its destination cell and its place in
+            // that cell's canonical list of incoming entries
+            std::vector<int> edest((size_t)G * ROUTE_SLOTS, -1), epos((size_t)G * ROUTE_SLOTS, 0);
+            for (int d = 0; d < G; ++d)
+                for (int q = istart[d]; q < istart[d + 1]; ++q) {
+                    edest[ient[q]] = d;
+                    epos[ient[q]] = q - istart[d];
+                }
+            HIPCHK(upload(&ent_dest, edest, st)); HIPCHK(upload(&ent_pos, epos, st));
+        }
         // field operator in padded 5-entry rows
         {
             const int N = S.A_size;
@@ -486,15 +498,34 @@ struct Case : CaseBase {
         t = (double)tstep * (double)S.dt;
         return 0;
     }
-    int field_solve() override {
+    // the solve itself: FP64 solution in x64 (warm start) from the rhs brhs
+    int solve_x64() {
         const int N = S.A_size;
-        launch_rhs_to_fp64<R>(N, brhs, b64, st);
         const double tol = S.deck.pcg_tol > 0 ? S.deck.pcg_tol : 1e-10;
-        if (precision == PIC_FP64_PLAIN) pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st);
-        else if (spectral) spectral_solve_refined(op, spec, b64, x64, tol, spec_cycles, spec_state, spec_part, d_res, st);
-        else pcg_solve_refined(op, b64, x64, tol, inner_tol, pcg_maxit, pw, d_res, st);
+        if (spectral) {
+            // the first residual converts the FP32 rhs (spectral implies R = float)
+            spectral_solve_refined(op, spec, (const float *)(const void *)brhs, b64, x64, tol, spec_cycles,
+                                   spec_state, spec_part, d_res, st);
+        } else {
+            launch_rhs_to_fp64<R>(N, brhs, b64, st);
+            if (precision == PIC_FP64_PLAIN) pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st);
+            else pcg_solve_refined(op, b64, x64, tol, inner_tol, pcg_maxit, pw, d_res, st);
+        }
         HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int field_solve() override {
+        if (int e = solve_x64()) return e;
         launch_x_to_phi<R>(G, A_idx, x64, phi, st);
+        return 0;
+    }
+    // fused step: solve, then phi + voltages + potential in one kernel and E
+    int field_efield_fused() {
+        if (int e = solve_x64()) return e;
+        mark(1);
+        launch_phi_potential<R>(nc, vc, t, x64, lap, V, phi, tot, st);
+        launch_efield_only<R>(nc, phi, tot, den, Sigma, Ex, Ey, st);
+        HIPCHK(hipGetLastError());
         return 0;
     }
     int efield() override {
@@ -541,7 +572,7 @@ struct Case : CaseBase {
         return 0;
     }
     int deposit() override {
-        launch_deposit<R>(kc, nc, cur, part, den, sigma_sp, Sigma, Source, brhs, st);
+        launch_deposit<R>(kc, nc, cur, part, den, sigma_sp, Sigma, Source, brhs, nullptr, st);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -572,15 +603,16 @@ struct Case : CaseBase {
             int *outs[3] = {nxt[0].cell_start, nxt[1].cell_start, pstart};
             launch_scan_multi(3, ins, outs, G, scan_tmp, st);
         }
-        launch_assign_bases_pair(G, inv_start, inv_ent, cnt[0], cnt[1], pcnt, nxt[0].cell_start, nxt[1].cell_start,
-                                 alive, alive2, pstart, base[0], base[1], pbase[0], pbase[1], pid_off, st);
+        launch_assign_bases_pair(G, ent_dest, ent_pos, inv_start, inv_ent, cnt[0], cnt[1], pcnt, nxt[0].cell_start,
+                                 nxt[1].cell_start, alive, alive2, pstart, base[0], base[1], pbase[0], pbase[1],
+                                 pid_off, st);
         mark(5);
         launch_fused_scatter<R>(kc, cur[0], nxt[0], nxt[1], 0, tag[0], stage, sstride, base[0], pbase[0], pbase[1],
                                 pid_off, next_id, nxt[0].cap, nxt[1].cap, d_err, st);
         mark(6);
         launch_fused_scatter<R>(kc, cur[1], nxt[1], nxt[1], 1, tag[1], stage, sstride, base[1], pbase[1], pbase[1],
                                 pid_off, next_id, nxt[1].cap, nxt[1].cap, d_err, st);
-        launch_bump_ids(next_id, pstart + G, nsp, st);
+        // the id counters advance in the deposit that follows (after both scatters)
         swap_species(0);
         swap_species(1);
         HIPCHK(hipGetLastError());
@@ -594,15 +626,18 @@ struct Case : CaseBase {
             }
             advance_time();
             mark(0);
-            if (int e = field_solve()) return e;
-            mark(1);
-            if (int e = efield()) return e;
+            if (int e = field_efield_fused()) return e;   // marks 1 between solve and E field
             mark(2);
             if (int e = particle_stage_fused()) return e;
             mark(7);
-            if (int e = deposit()) return e;
+            {
+                // deposit + profile sums + id counter advance in one node kernel
+                DepositExtras<R> ex{1, sum_den, sum_phi, sum_ex, sum_ey, tot, Ex, Ey, next_id, pstart + G};
+                launch_deposit<R>(kc, nc, cur, part, den, sigma_sp, Sigma, Source, brhs, &ex, st);
+                ++n_accum;
+                HIPCHK(hipGetLastError());
+            }
             mark(8);
-            if (int e = diagnostic()) return e;
             mark(9);
             if (timing) {
                 ev_pending[ev_slot] = true;

@@ -49,6 +49,18 @@ struct CellE {
     }
 };
 
+// voltage of conductor i at time t (src/cuda_Field.cu:10-20)
+template <typename R>
+__device__ inline R cond_voltage(const VoltConst &v, int i, double t) {
+    R acc = 0;
+    if (v.flag[i]) {
+        for (int j = 0; j < v.nsrc; ++j)
+            if (v.mid[j] == i + 1)
+                acc = (R)((double)acc + ((double)(R)v.dc[j] + (double)(R)v.ac[j] * dm_sin((double)(R)v.w[j] * t + (double)(R)v.ph[j])));
+    }
+    return acc;
+}
+
 }  // namespace
 
 // ============================================================ staged: Move
@@ -185,23 +197,32 @@ __global__ void k_totals_pair(int Gsize, const int *inv_start, const int *inv_en
     prod_total[d] = pr;
 }
 
-__global__ void k_assign_bases_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e,
-                                    const int *cnt_i, const int *prod_cnt, const int *start_e, const int *start_i,
-                                    const int *alive_e, const int *alive_i, const int *prod_start, int *base_e,
-                                    int *base_i, int *pbase_e, int *pbase_i, int *pid_off) {
-    int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= Gsize) return;
-    int re = start_e[d], ri = start_i[d];
-    int pe = start_e[d] + alive_e[d], pi = start_i[d] + alive_i[d];
-    int irun = prod_start[d];
-    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) {
-        int ent = inv_ent[e];
-        base_e[ent] = re; re += cnt_e[ent];
-        base_i[ent] = ri; ri += cnt_i[ent];
-        pbase_e[ent] = pe; pbase_i[ent] = pi; pid_off[ent] = irun;
-        const int pc = prod_cnt[ent];
-        pe += pc; pi += pc; irun += pc;
+// One thread per (source cell, slot) entry: its base in the destination's
+// new bucket is the destination's start plus the counts of the entries
+// before it in the destination's canonical list (same sums as walking the
+// list per destination, but every output is written by its own thread).
+__global__ void k_assign_bases_pair(int nent, const int *ent_dest, const int *ent_pos, const int *inv_start,
+                                    const int *inv_ent, const int *cnt_e, const int *cnt_i, const int *prod_cnt,
+                                    const int *start_e, const int *start_i, const int *alive_e, const int *alive_i,
+                                    const int *prod_start, int *base_e, int *base_i, int *pbase_e, int *pbase_i,
+                                    int *pid_off) {
+    const int ent = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ent >= nent) return;
+    const int d = ent_dest[ent];
+    if (d < 0) return;
+    const int e0 = inv_start[d], pos = ent_pos[ent];
+    int se = 0, si = 0, sp = 0;
+    for (int q = 0; q < pos; ++q) {
+        const int e2 = inv_ent[e0 + q];
+        se += cnt_e[e2];
+        si += cnt_i[e2];
+        sp += prod_cnt[e2];
     }
+    base_e[ent] = start_e[d] + se;
+    base_i[ent] = start_i[d] + si;
+    pbase_e[ent] = start_e[d] + alive_e[d] + sp;
+    pbase_i[ent] = start_i[d] + alive_i[d] + sp;
+    pid_off[ent] = prod_start[d] + sp;
 }
 
 // Exclusive scans of up to 3 int arrays of length n (out[n] = total) in two
@@ -710,34 +731,59 @@ __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p
 }
 
 // ============================================================ deposit
-// CIC weights summed per cell with a wave reduction (src/cuda_Deposit.cu:289-317)
+// CIC weights summed per cell with a wave reduction (src/cuda_Deposit.cu:289-317);
+// blockIdx.y = species.  A lane's particles of a cell are loaded at once
+// (up to 4 per lane per chunk) and summed in the same order as a plain
+// 64-strided walk.
 template <typename R>
-__global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PView<R> p, R *part) {
+__global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PSet<R> ps, R *part) {
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
+    const PView<R> &p = ps.s[blockIdx.y];
+    R *pp = part + (size_t)blockIdx.y * k.Gsize * 4;
     int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
     R WS = 0, WN = 0, ES = 0, EN = 0;
-    for (int kk = k0 + lane; kk < k1; kk += 64) {
-        R lx = p.x[kk], ly = p.y[kk];
-        WS += (1 - lx) * (1 - ly);
-        WN += (1 - lx) * ly;
-        ES += lx * (1 - ly);
-        EN += lx * ly;
+    for (int b0 = k0; b0 < k1; b0 += 256) {
+        const int n = min(256, k1 - b0);
+        R lx[4], ly[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = b0 + min(j * 64 + lane, n - 1);
+            lx[j] = p.x[i];
+            ly[j] = p.y[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j * 64 + lane < n) {
+                WS += (1 - lx[j]) * (1 - ly[j]);
+                WN += (1 - lx[j]) * ly[j];
+                ES += lx[j] * (1 - ly[j]);
+                EN += lx[j] * ly[j];
+            }
+        }
     }
     WS = wave_sum(WS); WN = wave_sum(WN); ES = wave_sum(ES); EN = wave_sum(EN);
     if (lane == 0) {
-        part[(size_t)c * 4 + 0] = WS; part[(size_t)c * 4 + 1] = WN;
-        part[(size_t)c * 4 + 2] = ES; part[(size_t)c * 4 + 3] = EN;
+        pp[(size_t)c * 4 + 0] = WS; pp[(size_t)c * 4 + 1] = WN;
+        pp[(size_t)c * 4 + 2] = ES; pp[(size_t)c * 4 + 3] = EN;
     }
 }
 
 // Node density from the four adjacent cells, boundary factors
 // (src/cuda_Deposit.cu:277-288), then charge density, source and PCG rhs
 // (src/cuda_Deposit.cu:30-104).  part holds nsp x Gsize x 4.
+// With ex.on (fused step) the same thread also adds this step's node fields
+// to the FP64 profile sums (src/cuda_Diagnostic.cu:505-547) and thread 0
+// advances the particle id counters past this step's ionisation products.
 template <typename R>
-__global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_sp, R *Sigma, R *Source, R *b) {
+__global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_sp, R *Sigma, R *Source, R *b,
+                                DepositExtras<R> ex) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ex.on && g == 0) {
+        const uint32_t add = (uint32_t)*ex.prod_total_end;
+        for (int s = 0; s < n.nsp; ++s) ex.next_id[s] += add;
+    }
     if (g >= n.Gsize) return;
     const int ngy = n.ngy, ngx = n.ngx;
     const int x = g / ngy, y = g % ngy;
@@ -782,6 +828,33 @@ __global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_
     Source[g] = src;
     int ai = n.A_idx[g];
     if (ai) b[ai - 1] = src;
+    if (ex.on) {
+        for (int i = 0; i < n.nsp; ++i) ex.sden[(size_t)i * n.Gsize + g] += (double)den[(size_t)i * n.Gsize + g];
+        ex.sphi[g] += (double)ex.tot[g];
+        ex.sex[g] += (double)ex.Ex[g];
+        ex.sey[g] += (double)ex.Ey[g];
+    }
+}
+
+// Fused step: node potential straight from the field solution, with the
+// electrode voltages of this time evaluated per thread:
+// phi = x[A_idx-1] (0 on fixed nodes), tot = sum_c V_c(t) lap_c + phi.
+template <typename R>
+__global__ void k_phi_potential(NodeConst<R> n, VoltConst v, double t, const double *x, const R *lap, R *V, R *phi,
+                                R *tot) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n.Gsize) return;
+    const int a = n.A_idx[g];
+    const R ph = a ? (R)x[a - 1] : (R)0;
+    phi[g] = ph;
+    R tp = 0;
+    for (int i = 0; i < n.CondNUMR; ++i) {
+        const R vi = cond_voltage<R>(v, i, t);
+        if (g == 0) V[i] = vi;
+        tp += vi * lap[(size_t)i * n.Gsize + g];
+    }
+    tp += (R)1 * ph;
+    tot[g] = tp;
 }
 
 // ============================================================ E field
@@ -919,13 +992,7 @@ template <typename R>
 __global__ void k_voltages(VoltConst v, double t, R *V) {
     int i = threadIdx.x;
     if (i >= v.ncond) return;
-    R acc = 0;
-    if (v.flag[i]) {
-        for (int j = 0; j < v.nsrc; ++j)
-            if (v.mid[j] == i + 1)
-                acc = (R)((double)acc + ((double)(R)v.dc[j] + (double)(R)v.ac[j] * dm_sin((double)(R)v.w[j] * t + (double)(R)v.ph[j])));
-    }
-    V[i] = acc;
+    V[i] = cond_voltage<R>(v, i, t);
 }
 
 // profile accumulators in FP64 (src/cuda_Diagnostic.cu:505-547)
@@ -1029,12 +1096,24 @@ void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R
 }
 template <typename R>
 void launch_deposit(const DevConst<R> &k, const NodeConst<R> &n, const PView<R> *ps, R *part, R *den, R *sigma_sp,
-                    R *Sigma, R *Source, R *b, hipStream_t s) {
-    for (int isp = 0; isp < n.nsp; ++isp)
-        hipLaunchKernelGGL(k_deposit_cells<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, ps[isp],
-                           part + (size_t)isp * n.Gsize * 4);
+                    R *Sigma, R *Source, R *b, const DepositExtras<R> *ex, hipStream_t s) {
+    PSet<R> set{};
+    for (int isp = 0; isp < n.nsp; ++isp) set.s[isp] = ps[isp];
+    hipLaunchKernelGGL(k_deposit_cells<R>, dim3(cell_blocks(k.Csize), n.nsp), dim3(256), 0, s, k, set, part);
+    DepositExtras<R> e{};
+    if (ex) e = *ex;
     hipLaunchKernelGGL(k_deposit_nodes<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, (const R *)part, den,
-                       sigma_sp, Sigma, Source, b);
+                       sigma_sp, Sigma, Source, b, e);
+}
+template <typename R>
+void launch_phi_potential(const NodeConst<R> &n, const VoltConst &v, double t, const double *x, const R *lap, R *V,
+                          R *phi, R *tot, hipStream_t s) {
+    hipLaunchKernelGGL(k_phi_potential<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, v, t, x, lap, V, phi, tot);
+}
+template <typename R>
+void launch_efield_only(const NodeConst<R> &n, const R *phi, const R *tot, const R *den, R *Sigma, R *Ex, R *Ey,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_efield<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, phi, tot, den, Sigma, Ex, Ey);
 }
 template <typename R>
 void launch_efield(const NodeConst<R> &n, const VoltConst &v, double t, const R *lap, R *V, const R *phi, R *tot,
@@ -1087,13 +1166,15 @@ void launch_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, con
     hipLaunchKernelGGL(k_totals_pair, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent, cnt_e,
                        cnt_i, prod_cnt, total_e, alive_e, total_i, alive_i, prod_total);
 }
-void launch_assign_bases_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
-                              const int *prod_cnt, const int *start_e, const int *start_i, const int *alive_e,
-                              const int *alive_i, const int *prod_start, int *base_e, int *base_i, int *pbase_e,
-                              int *pbase_i, int *pid_off, hipStream_t s) {
-    hipLaunchKernelGGL(k_assign_bases_pair, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent,
-                       cnt_e, cnt_i, prod_cnt, start_e, start_i, alive_e, alive_i, prod_start, base_e, base_i, pbase_e,
-                       pbase_i, pid_off);
+void launch_assign_bases_pair(int Gsize, const int *ent_dest, const int *ent_pos, const int *inv_start,
+                              const int *inv_ent, const int *cnt_e, const int *cnt_i, const int *prod_cnt,
+                              const int *start_e, const int *start_i, const int *alive_e, const int *alive_i,
+                              const int *prod_start, int *base_e, int *base_i, int *pbase_e, int *pbase_i,
+                              int *pid_off, hipStream_t s) {
+    const int nent = Gsize * ROUTE_SLOTS;
+    hipLaunchKernelGGL(k_assign_bases_pair, dim3((nent + 255) / 256), dim3(256), 0, s, nent, ent_dest, ent_pos,
+                       inv_start, inv_ent, cnt_e, cnt_i, prod_cnt, start_e, start_i, alive_e, alive_i, prod_start,
+                       base_e, base_i, pbase_e, pbase_i, pid_off);
 }
 int scan_tmp_ints(int n) { return 3 * ((n + SCAN_TILE - 1) / SCAN_TILE) + 3; }
 // exclusive scans of k <= 3 arrays of n ints: out[a][0..n], out[a][n] = total;
@@ -1126,7 +1207,11 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
                                           int, const uint8_t *, const R *, size_t, const int *, const int *,           \
                                           const int *, const int *, const uint32_t *, int, int, int *, hipStream_t);   \
     template void launch_deposit<R>(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *, R *, R *, R *,   \
-                                    R *, R *, hipStream_t);                                                            \
+                                    R *, R *, const DepositExtras<R> *, hipStream_t);                                  \
+    template void launch_phi_potential<R>(const NodeConst<R> &, const VoltConst &, double, const double *, const R *,  \
+                                          R *, R *, R *, hipStream_t);                                                 \
+    template void launch_efield_only<R>(const NodeConst<R> &, const R *, const R *, const R *, R *, R *, R *,          \
+                                        hipStream_t);                                                            \
     template void launch_efield<R>(const NodeConst<R> &, const VoltConst &, double, const R *, R *, const R *, R *,    \
                                    const R *, R *, R *, R *, hipStream_t);                                             \
     template void launch_accumulate<R>(int, int, const R *, const R *, const R *, const R *, double *, double *,       \

@@ -25,6 +25,23 @@ struct NodeConst {
     const R *Area, *eps;
 };
 
+// particle views of all species of a case (kernel argument)
+template <typename R>
+struct PSet {
+    PView<R> s[5];
+};
+
+// fused-step extras of the node deposit kernel: FP64 profile sums and the
+// particle id counters (advanced by the step's product count)
+template <typename R>
+struct DepositExtras {
+    int on;
+    double *sden, *sphi, *sex, *sey;
+    const R *tot, *Ex, *Ey;
+    uint32_t *next_id;
+    const int *prod_total_end;
+};
+
 constexpr int MAX_SRC = 16;
 struct VoltConst {
     int ncond, nsrc;
@@ -57,8 +74,15 @@ void launch_fused_scatter(const DevConst<R> &, const PView<R> &p, const PView<R>
                           const int *pbase_e, const int *pbase_i, const int *pid_off, const uint32_t *next_id, int cap,
                           int cap_ion, int *err, hipStream_t);
 template <typename R>
-void launch_deposit(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *, R *, R *, R *, R *, R *,
-                    hipStream_t);
+void launch_deposit(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *part, R *den, R *sigma_sp,
+                    R *Sigma, R *Source, R *b, const DepositExtras<R> *ex, hipStream_t);
+// fused step: phi from the FP64 solution, electrode voltages and total potential in one kernel
+template <typename R>
+void launch_phi_potential(const NodeConst<R> &n, const VoltConst &v, double t, const double *x, const R *lap, R *V,
+                          R *phi, R *tot, hipStream_t s);
+template <typename R>
+void launch_efield_only(const NodeConst<R> &n, const R *phi, const R *tot, const R *den, R *Sigma, R *Ex, R *Ey,
+                        hipStream_t s);
 template <typename R>
 void launch_efield(const NodeConst<R> &, const VoltConst &, double, const R *, R *, const R *, R *, const R *, R *, R *,
                    R *, hipStream_t);
@@ -74,10 +98,11 @@ void launch_scan_multi(int k, const int *const *in, int *const *out, int n, int 
 void launch_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
                         const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i, int *prod_total,
                         hipStream_t s);
-void launch_assign_bases_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
-                              const int *prod_cnt, const int *start_e, const int *start_i, const int *alive_e,
-                              const int *alive_i, const int *prod_start, int *base_e, int *base_i, int *pbase_e,
-                              int *pbase_i, int *pid_off, hipStream_t s);
+void launch_assign_bases_pair(int Gsize, const int *ent_dest, const int *ent_pos, const int *inv_start,
+                              const int *inv_ent, const int *cnt_e, const int *cnt_i, const int *prod_cnt,
+                              const int *start_e, const int *start_i, const int *alive_e, const int *alive_i,
+                              const int *prod_start, int *base_e, int *base_i, int *pbase_e, int *pbase_i,
+                              int *pid_off, hipStream_t s);
 void launch_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total, hipStream_t s);
 void launch_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp, hipStream_t s);
 template <typename R> void launch_rhs_to_fp64(int N, const R *b, double *b64, hipStream_t s);
@@ -123,7 +148,9 @@ struct SpecState {
     int done, outer, status, pad;
     double rel;
 };
-void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const double *b, double *x, double tol,
-                            int max_cycles, SpecState *state, double *partials, PcgResult *res, hipStream_t st);
+// b32 = FP32 right-hand side, converted into b (FP64) by the first residual
+void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const float *b32, double *b, double *x,
+                            double tol, int max_cycles, SpecState *state, double *partials, PcgResult *res,
+                            hipStream_t st);
 
 }  // namespace pic

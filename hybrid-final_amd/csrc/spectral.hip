@@ -15,10 +15,10 @@
 // flag, so the solve is a fixed launch sequence (graph-capturable) that idles
 // once converged.
 //
-// At these sizes (n <= ~1000 per side) the solve is bound by launch and L2
-// latency, not by FLOPs: the GEMMs read their operands straight from L2 with
-// independent accumulators for latency hiding, one 16x16 output tile per
-// 256-thread block so that a 256^2 problem fills the chip.
+// At these sizes (n <= ~1000 per side) the solve is bound by launch and
+// memory latency, not by FLOPs: each GEMM block stages its operand panels
+// through LDS in one round trip and computes one 16x16 output tile, so a
+// 256^2 problem fills the chip with ~256 blocks.
 #include <hip/hip_runtime.h>
 
 #include "pic_kernels.h"
@@ -30,18 +30,21 @@ namespace {
 constexpr int RES_BLOCK = 256;
 constexpr int GT = 16;   // GEMM output tile edge
 
-__global__ void k_spec_begin(SpecState *s) {
-    if (threadIdx.x == 0) { s->done = 0; s->outer = 0; s->rel = 0.0; s->status = 0; }
-}
-
-// r = b - A x in FP64; per-block partial sums of r.r and b.b
-__global__ __launch_bounds__(RES_BLOCK) void k_residual64(PcgOperator A, const double *b, const double *x, double *r,
-                                                          double *partials, const SpecState *s) {
-    if (s->done) return;
+// r = b - A x in FP64; per-block partial sums of r.r and b.b.  The first
+// residual of a solve (b32 != null) also starts the solve: it converts the
+// FP32 right-hand side into b and resets the solve state.
+__global__ __launch_bounds__(RES_BLOCK) void k_residual64(PcgOperator A, const float *b32, double *b, const double *x,
+                                                          double *r, double *partials, SpecState *s) {
+    if (b32) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { s->done = 0; s->outer = 0; s->rel = 0.0; s->status = 0; }
+    } else if (s->done) {
+        return;
+    }
     __shared__ double sh[2 * RES_BLOCK];
     const int i = blockIdx.x * RES_BLOCK + threadIdx.x;
     double rr = 0.0, bb = 0.0;
     if (i < A.N) {
+        if (b32) b[i] = (double)b32[i];
         const int o = 5 * i;
         double ax = A.vald[o] * x[A.col[o]];
         ax += A.vald[o + 1] * x[A.col[o + 1]];
@@ -108,30 +111,52 @@ __global__ __launch_bounds__(RES_BLOCK) void k_spec_finalize(const double *parti
     }
 }
 
-// C = A B (row-major, A: M x K, B: K x N) in FP64, one output per thread.
+// C = A B (row-major, A: M x K, B: K x N) in FP64, one 16x16 output tile per
+// 256-thread block, one output per thread.  The block's A rows and B columns
+// are staged through LDS in K-panels of GK: every thread issues all its
+// panel loads at once (one memory round trip per panel), then accumulates
+// from LDS (A broadcast along a tile row, B conflict-free along a row).
 // MODE 0: store; MODE 1: store C .* scale; MODE 2: x64 += C.
+constexpr int GK = 272;   // GT * GK is a multiple of the block size
 template <int MODE>
 __global__ __launch_bounds__(GT *GT) void k_dgemm(int M, int N, int K, const double *__restrict__ A,
                                                  const double *__restrict__ B, double *__restrict__ C,
                                                  const double *__restrict__ scale, double *__restrict__ x64,
                                                  const SpecState *s) {
-    if (s->done) return;
+    if (s->done) return;   // uniform over the grid: no thread reaches a barrier alone
+    __shared__ double As[GT][GK + 1];
+    __shared__ double Bs[GK][GT];
     const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
-    const int r = blockIdx.y * GT + ty, c = blockIdx.x * GT + tx;
-    if (r >= M || c >= N) return;
-    const double *a = A + (size_t)r * K;
-    const double *b = B + c;
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
-    int k = 0;
-#pragma unroll 4
-    for (; k + 4 <= K; k += 4) {
-        acc0 = __builtin_fma(a[k], b[(size_t)k * N], acc0);
-        acc1 = __builtin_fma(a[k + 1], b[(size_t)(k + 1) * N], acc1);
-        acc2 = __builtin_fma(a[k + 2], b[(size_t)(k + 2) * N], acc2);
-        acc3 = __builtin_fma(a[k + 3], b[(size_t)(k + 3) * N], acc3);
+    const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int kb = 0; kb < K; kb += GK) {
+        const int kn = min(GK, K - kb);
+        // A panel: GT rows x kn, B panel: kn rows x GT (zero outside the matrix)
+        static_assert((GT * GK) % (GT * GT) == 0, "panel size");
+#pragma unroll
+        for (int e = threadIdx.x; e < GT * GK; e += GT * GT) {
+            const int rr = e / GK, kk = e % GK;
+            const int gr = r0 + rr;
+            As[rr][kk] = (kk < kn && gr < M) ? A[(size_t)gr * K + kb + kk] : 0.0;
+        }
+#pragma unroll
+        for (int e = threadIdx.x; e < GK * GT; e += GT * GT) {
+            const int kk = e / GT, cc = e % GT;
+            const int gc = c0 + cc;
+            Bs[kk][cc] = (kk < kn && gc < N) ? B[(size_t)(kb + kk) * N + gc] : 0.0;
+        }
+        __syncthreads();
+        int kk = 0;
+        for (; kk + 2 <= kn; kk += 2) {
+            acc0 = __builtin_fma(As[ty][kk], Bs[kk][tx], acc0);
+            acc1 = __builtin_fma(As[ty][kk + 1], Bs[kk + 1][tx], acc1);
+        }
+        if (kk < kn) acc0 = __builtin_fma(As[ty][kk], Bs[kk][tx], acc0);
+        __syncthreads();
     }
-    for (; k < K; ++k) acc0 = __builtin_fma(a[k], b[(size_t)k * N], acc0);
-    const double v = (acc0 + acc1) + (acc2 + acc3);
+    const int r = r0 + ty, c = c0 + tx;
+    if (r >= M || c >= N) return;
+    const double v = acc0 + acc1;
     const size_t o = (size_t)r * N + c;
     if (MODE == 0) C[o] = v;
     else if (MODE == 1) C[o] = v * scale[o];
@@ -140,16 +165,17 @@ __global__ __launch_bounds__(GT *GT) void k_dgemm(int M, int N, int K, const dou
 
 }  // namespace
 
-void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const double *b, double *x, double tol,
-                            int max_cycles, SpecState *state, double *partials, PcgResult *res, hipStream_t st) {
+void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const float *b32, double *b, double *x,
+                            double tol, int max_cycles, SpecState *state, double *partials, PcgResult *res,
+                            hipStream_t st) {
     const int N = A.N;
     const int nblk = (N + RES_BLOCK - 1) / RES_BLOCK;
     const dim3 g((sp.ny + GT - 1) / GT, (sp.nx + GT - 1) / GT);
     const dim3 blk(GT * GT);
     const SpecState *cs = state;
-    hipLaunchKernelGGL(k_spec_begin, dim3(1), dim3(64), 0, st, state);
     for (int k = 0; k <= max_cycles; ++k) {
-        hipLaunchKernelGGL(k_residual64, dim3(nblk), dim3(RES_BLOCK), 0, st, A, b, (const double *)x, sp.r, partials, cs);
+        hipLaunchKernelGGL(k_residual64, dim3(nblk), dim3(RES_BLOCK), 0, st, A, k == 0 ? b32 : (const float *)nullptr,
+                           b, (const double *)x, sp.r, partials, state);
         hipLaunchKernelGGL(k_spec_finalize, dim3(1), dim3(RES_BLOCK), 0, st, (const double *)partials, nblk, tol,
                            k == max_cycles ? 1 : 0, state, res);
         if (k == max_cycles) break;
