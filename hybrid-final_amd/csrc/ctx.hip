@@ -331,6 +331,9 @@ struct Case : CaseBase {
             // that cell's canonical list of incoming entries
             std::vector<int> edest((size_t)G * ROUTE_SLOTS, -1), epos((size_t)G * ROUTE_SLOTS, 0);
             for (int d = 0; d < G; ++d)
+                if (istart[d + 1] - istart[d] > ROUTE_SLOTS)
+                    return fail(-43, "a cell receives particles from more than 9 source entries");
+            for (int d = 0; d < G; ++d)
                 for (int q = istart[d]; q < istart[d + 1]; ++q) {
                     edest[ient[q]] = d;
                     epos[ient[q]] = q - istart[d];
@@ -438,6 +441,7 @@ struct Case : CaseBase {
             uint32_t nid[MAXSP] = {0};
             for (int s = 0; s < nsp; ++s) nid[s] = (uint32_t)S.sp[s].n;
             HIPCHK(hipMemcpyAsync(next_id, nid, sizeof nid, hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));   // nid lives on this stack frame
         }
         for (int s = 0; s < nsp; ++s) {
             SpeciesInit<R> &si = S.sp[s];
@@ -589,10 +593,10 @@ struct Case : CaseBase {
         // largest capacity)
         const size_t sstride = (size_t)stage_stride;
         launch_fused_count<R>(kc, cur[0], 0, step, cnt[0], pcnt, mccr, condcharge, sigma_sp, tag[0], stage, sstride,
-                              st);
+                              d_err, st);
         mark(3);
         launch_fused_count<R>(kc, cur[1], 1, step, cnt[1], nullptr, mccr, condcharge, sigma_sp, tag[1], stage,
-                              sstride, st);
+                              sstride, d_err, st);
         mark(4);
         // survivors + ionisation products per destination cell for both
         // species, their three scans (electron and ion cell_start, global
@@ -670,7 +674,8 @@ struct Case : CaseBase {
         int e = 0;
         HIPCHK(hipMemcpyAsync(&e, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (e) return fail(-30, "particle capacity exceeded (Max_np too small for this case)");
+        if (e & 1) return fail(-30, "particle capacity exceeded (Max_np too small for this case)");
+        if (e & 2) return fail(-33, "corrupt particle record (position far outside its cell)");
         PcgResult r{};
         HIPCHK(hipMemcpyAsync(&r, d_res, sizeof r, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));

@@ -185,12 +185,18 @@ __global__ void k_totals_pair(int Gsize, const int *inv_start, const int *inv_en
                               int *prod_total) {
     int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= Gsize) return;
+    const int e0 = inv_start[d], ne = inv_start[d + 1] - e0;   // ne <= 9, checked at set-up
+    int ent[ROUTE_SLOTS];
+#pragma unroll
+    for (int q = 0; q < ROUTE_SLOTS; ++q) ent[q] = q < ne ? inv_ent[e0 + q] : -1;
     int ae = 0, ai = 0, pr = 0;
-    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) {
-        int ent = inv_ent[e];
-        ae += cnt_e[ent];
-        ai += cnt_i[ent];
-        pr += prod_cnt[ent];
+#pragma unroll
+    for (int q = 0; q < ROUTE_SLOTS; ++q) {
+        if (ent[q] >= 0) {
+            ae += cnt_e[ent[q]];
+            ai += cnt_i[ent[q]];
+            pr += prod_cnt[ent[q]];
+        }
     }
     total_e[d] = ae + pr; alive_e[d] = ae;
     total_i[d] = ai + pr; alive_i[d] = ai;
@@ -211,12 +217,19 @@ __global__ void k_assign_bases_pair(int nent, const int *ent_dest, const int *en
     const int d = ent_dest[ent];
     if (d < 0) return;
     const int e0 = inv_start[d], pos = ent_pos[ent];
+    // a destination has at most 9 incoming entries (its 3x3 neighbourhood):
+    // unrolled so that the list loads, then the count loads, issue together
+    int e2[ROUTE_SLOTS];
+#pragma unroll
+    for (int q = 0; q < ROUTE_SLOTS; ++q) e2[q] = q < pos ? inv_ent[e0 + q] : -1;
     int se = 0, si = 0, sp = 0;
-    for (int q = 0; q < pos; ++q) {
-        const int e2 = inv_ent[e0 + q];
-        se += cnt_e[e2];
-        si += cnt_i[e2];
-        sp += prod_cnt[e2];
+#pragma unroll
+    for (int q = 0; q < ROUTE_SLOTS; ++q) {
+        if (e2[q] >= 0) {
+            se += cnt_e[e2[q]];
+            si += cnt_i[e2[q]];
+            sp += prod_cnt[e2[q]];
+        }
     }
     base_e[ent] = start_e[d] + se;
     base_i[ent] = start_i[d] + si;
@@ -523,7 +536,8 @@ __device__ inline int collide(const DevConst<R> &k, int dest, R pa, uint32_t id,
 template <typename R, int KIND>
 __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, uint32_t step, int *cnt,
                                                      int *pcnt, unsigned long long *mccr, int *condcharge,
-                                                     R *sigma_sp, uint8_t *flag, R *stash, size_t stash_stride) {
+                                                     R *sigma_sp, uint8_t *flag, R *stash, size_t stash_stride,
+                                                     int *err) {
     __shared__ FStage<R> stage_all[FWAVES];
     __shared__ FPool<R> pool;
     const int wv = threadIdx.x >> 6;
@@ -562,6 +576,10 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
                 int slot = -1, jc = -1;
                 R x = S.x[i], y = S.y[i], vx = S.vx[i], vy = S.vy[i];
                 const uint32_t id = S.id[i];
+                // a stored in-cell offset is in [0,1), or just outside after a
+                // move of more than one cell (not handled, as in the reference);
+                // anything else is a corrupt particle record
+                if (i < n && !(x > (R)-4 && x < (R)4 && y > (R)-4 && y < (R)4)) atomicOr(err, 2);
                 const int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As,
                                                   k.dt_dx, k.dt_dy, x, y, vx, vy);
                 const int dest = __shfl(w.dest, code, 64);
@@ -1072,13 +1090,13 @@ void launch_mcc_insert(const DevConst<R> &k, const PView<R> &src, const PView<R>
 template <typename R>
 void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, uint32_t step, int *cnt, int *pcnt,
                         unsigned long long *mccr, int *cc, R *sg, uint8_t *flag, R *stash, size_t stash_stride,
-                        hipStream_t s) {
+                        int *err, hipStream_t s) {
     if (isp == 0)
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride);
+                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
     else
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride);
+                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
 }
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
@@ -1202,7 +1220,7 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
                                        const int *, const PView<R> &, const int *, const R *, int, const uint32_t *,   \
                                        int, int *, hipStream_t);                                                       \
     template void launch_fused_count<R>(const DevConst<R> &, const PView<R> &, int, uint32_t, int *, int *,            \
-                                        unsigned long long *, int *, R *, uint8_t *, R *, size_t, hipStream_t);        \
+                                        unsigned long long *, int *, R *, uint8_t *, R *, size_t, int *, hipStream_t); \
     template void launch_fused_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const PView<R> &,   \
                                           int, const uint8_t *, const R *, size_t, const int *, const int *,           \
                                           const int *, const int *, const uint32_t *, int, int, int *, hipStream_t);   \

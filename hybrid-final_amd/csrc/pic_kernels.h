@@ -67,7 +67,7 @@ void launch_mcc_insert(const DevConst<R> &, const PView<R> &, const PView<R> &, 
 template <typename R>
 void launch_fused_count(const DevConst<R> &, const PView<R> &, int isp, uint32_t step, int *cnt, int *pcnt,
                         unsigned long long *mccr, int *condcharge, R *sigma_sp, uint8_t *flag, R *stash,
-                        size_t stash_stride, hipStream_t);
+                        size_t stash_stride, int *err, hipStream_t);
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
                           const uint8_t *flag, const R *stash, size_t stash_stride, const int *base,

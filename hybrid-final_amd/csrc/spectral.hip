@@ -131,19 +131,25 @@ __global__ __launch_bounds__(GT *GT) void k_dgemm(int M, int N, int K, const dou
     double acc0 = 0.0, acc1 = 0.0;
     for (int kb = 0; kb < K; kb += GK) {
         const int kn = min(GK, K - kb);
-        // A panel: GT rows x kn, B panel: kn rows x GT (zero outside the matrix)
+        // A panel: GT rows x kn, B panel: kn rows x GT (zero outside the
+        // matrix); fixed trip counts so that all loads issue before the
+        // first LDS store
         static_assert((GT * GK) % (GT * GT) == 0, "panel size");
+        constexpr int PER = (GT * GK) / (GT * GT);
+        double ta[PER], tb[PER];
 #pragma unroll
-        for (int e = threadIdx.x; e < GT * GK; e += GT * GT) {
-            const int rr = e / GK, kk = e % GK;
-            const int gr = r0 + rr;
-            As[rr][kk] = (kk < kn && gr < M) ? A[(size_t)gr * K + kb + kk] : 0.0;
+        for (int m = 0; m < PER; ++m) {
+            const int e = (int)threadIdx.x + m * GT * GT;
+            const int rr = e / GK, kk = e % GK, gr = r0 + rr;
+            ta[m] = (kk < kn && gr < M) ? A[(size_t)gr * K + kb + kk] : 0.0;
+            const int kb2 = e / GT, cc = e % GT, gc = c0 + cc;
+            tb[m] = (kb2 < kn && gc < N) ? B[(size_t)(kb + kb2) * N + gc] : 0.0;
         }
 #pragma unroll
-        for (int e = threadIdx.x; e < GK * GT; e += GT * GT) {
-            const int kk = e / GT, cc = e % GT;
-            const int gc = c0 + cc;
-            Bs[kk][cc] = (kk < kn && gc < N) ? B[(size_t)(kb + kk) * N + gc] : 0.0;
+        for (int m = 0; m < PER; ++m) {
+            const int e = (int)threadIdx.x + m * GT * GT;
+            As[e / GK][e % GK] = ta[m];
+            Bs[e / GT][e % GT] = tb[m];
         }
         __syncthreads();
         int kk = 0;
