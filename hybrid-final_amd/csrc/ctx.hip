@@ -66,8 +66,10 @@ struct CaseBase {
     virtual int mcc() = 0;
     virtual int deposit() = 0;
     virtual int diagnostic() = 0;
+    virtual int eedf_sample(double emax) = 0;
     virtual int step_fused(int n) = 0;
     virtual int sync() = 0;
+    virtual int query() = 0;
     virtual int info(pic_info *) = 0;
     virtual int get_particles(int, void *, void *, void *, void *, void *, uint32_t *, int32_t *) = 0;
     virtual int set_particles(int, long long, const void *, const void *, const void *, const void *, const void *,
@@ -126,6 +128,15 @@ struct Case : CaseBase {
     int *pcnt = nullptr, *pid_off = nullptr, *total = nullptr, *alive = nullptr, *ptotal = nullptr;
     int *total2 = nullptr, *alive2 = nullptr;
     int *ent_dest = nullptr, *ent_pos = nullptr;
+    StepClock *clk = nullptr;   // device clock of the fused step (the step about to run)
+    unsigned long long *eedf = nullptr;   // electron energy histogram (EEDF_BINS counts)
+    double eedf_emax = 0;
+    long long eedf_n = 0;
+    bool clk_dirty = true;      // host tstep changed outside the fused path
+    // captured fused steps, one per buffer parity (key = electron x buffer in use)
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    const void *gkey[2] = {nullptr, nullptr};
+    bool use_graph = true;
     long long stage_stride = 0;   // stride of the 6 arrays of `stage`
     int *pstart = nullptr, *scan_tmp = nullptr, *cellp = nullptr;
     uint32_t *next_id = nullptr;
@@ -171,7 +182,7 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey, total2, alive2, ent_dest, ent_pos};
+                        sum_ey, total2, alive2, ent_dest, ent_pos, clk, eedf};
         for (void *p : ptrs)
             if (p) hipFree(p);
         for (int s = 0; s < MAXSP; ++s) {
@@ -188,6 +199,8 @@ struct Case : CaseBase {
             if (pbase[s]) hipFree(pbase[s]);
         }
         pcg_work_destroy(pw);
+        for (int q = 0; q < 2; ++q)
+            if (gexec[q]) hipGraphExecDestroy(gexec[q]);
         for (int r = 0; r < RING; ++r)
             for (int k = 0; k <= NT; ++k)
                 if (ev[r][k]) hipEventDestroy(ev[r][k]);
@@ -431,6 +444,9 @@ struct Case : CaseBase {
         HIPCHK(hipMemsetAsync(condcharge, 0, nsp * S.CondNUMR * sizeof(int), st));
         HIPCHK(dalloc(&d_err, 1));
         HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int), st));
+        HIPCHK(dalloc(&clk, 1));
+        clk_dirty = true;
+        if (const char *ng = getenv("PIC_NO_GRAPH")) use_graph = atoi(ng) == 0;
         HIPCHK(dalloc(&sum_den, (size_t)nsp * G)); HIPCHK(dalloc(&sum_phi, G));
         HIPCHK(dalloc(&sum_ex, G)); HIPCHK(dalloc(&sum_ey, G));
         HIPCHK(hipMemsetAsync(sum_den, 0, (size_t)nsp * G * sizeof(double), st));
@@ -497,9 +513,22 @@ struct Case : CaseBase {
     void swap_species(int s) { std::swap(cur[s], nxt[s]); }
 
     // ------------------------------------------------------------ stages
-    int advance_time() override {
+    void host_tick() {
         ++tstep;
         t = (double)tstep * (double)S.dt;
+    }
+    int advance_time() override {
+        host_tick();
+        clk_dirty = true;
+        return 0;
+    }
+    // bring the device clock to the host's step count before fused steps
+    int ensure_clock() {
+        if (!clk_dirty) return 0;
+        StepClock h{(uint32_t)(tstep + 1), 0u, (double)(tstep + 1) * (double)S.dt};
+        HIPCHK(hipMemcpyAsync(clk, &h, sizeof h, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));   // h lives on this stack frame
+        clk_dirty = false;
         return 0;
     }
     // the solve itself: FP64 solution in x64 (warm start) from the rhs brhs
@@ -527,7 +556,7 @@ struct Case : CaseBase {
     int field_efield_fused() {
         if (int e = solve_x64()) return e;
         mark(1);
-        launch_phi_potential<R>(nc, vc, t, x64, lap, V, phi, tot, st);
+        launch_phi_potential<R>(nc, vc, clk, x64, lap, V, phi, tot, st);
         launch_efield_only<R>(nc, phi, tot, den, Sigma, Ex, Ey, st);
         HIPCHK(hipGetLastError());
         return 0;
@@ -580,6 +609,24 @@ struct Case : CaseBase {
         HIPCHK(hipGetLastError());
         return 0;
     }
+    // adds the current electron energy histogram to the EEDF accumulator;
+    // a new emax starts a new accumulation
+    int eedf_sample(double emax) override {
+        if (!(emax > 0)) return fail(-2, "EEDF energy range must be positive");
+        if (!eedf) {
+            HIPCHK(dalloc(&eedf, EEDF_BINS));
+            eedf_emax = 0;
+        }
+        if (emax != eedf_emax) {
+            HIPCHK(hipMemsetAsync(eedf, 0, EEDF_BINS * sizeof(unsigned long long), st));
+            eedf_emax = emax;
+            eedf_n = 0;
+        }
+        launch_eedf<R>(cur[0], G, S.Escale[0], (R)emax, eedf, st);
+        HIPCHK(hipGetLastError());
+        ++eedf_n;
+        return 0;
+    }
     int diagnostic() override {
         launch_accumulate<R>(G, nsp, den, tot, Ex, Ey, sum_den, sum_phi, sum_ex, sum_ey, st);
         ++n_accum;
@@ -587,15 +634,14 @@ struct Case : CaseBase {
         return 0;
     }
     int particle_stage_fused() {
-        const uint32_t step = (uint32_t)tstep;
         // flag bytes reuse the staged path's per-particle move tags; the
         // ionisation stash reuses its product staging buffer (stride = the
         // largest capacity)
         const size_t sstride = (size_t)stage_stride;
-        launch_fused_count<R>(kc, cur[0], 0, step, cnt[0], pcnt, mccr, condcharge, sigma_sp, tag[0], stage, sstride,
+        launch_fused_count<R>(kc, cur[0], 0, clk, cnt[0], pcnt, mccr, condcharge, sigma_sp, tag[0], stage, sstride,
                               d_err, st);
         mark(3);
-        launch_fused_count<R>(kc, cur[1], 1, step, cnt[1], nullptr, mccr, condcharge, sigma_sp, tag[1], stage,
+        launch_fused_count<R>(kc, cur[1], 1, clk, cnt[1], nullptr, mccr, condcharge, sigma_sp, tag[1], stage,
                               sstride, d_err, st);
         mark(4);
         // survivors + ionisation products per destination cell for both
@@ -622,27 +668,67 @@ struct Case : CaseBase {
         HIPCHK(hipGetLastError());
         return 0;
     }
+    // one fused step enqueued on the stream (no host synchronisation); the
+    // host buffer parity flips, the host clock does not move
+    int enqueue_step() {
+        if (int e = field_efield_fused()) return e;   // marks 1 between solve and E field
+        mark(2);
+        if (int e = particle_stage_fused()) return e;
+        mark(7);
+        // deposit + profile sums + id counter and clock advance in one node kernel
+        DepositExtras<R> ex{1, sum_den, sum_phi, sum_ex, sum_ey, tot, Ex, Ey, next_id, pstart + G, clk,
+                            (double)S.dt};
+        launch_deposit<R>(kc, nc, cur, part, den, sigma_sp, Sigma, Source, brhs, &ex, st);
+        HIPCHK(hipGetLastError());
+        mark(8);
+        mark(9);
+        return 0;
+    }
+    // the graph of one fused step for the current buffer parity (captured on
+    // first use; the capture itself runs nothing)
+    int step_graph(hipGraphExec_t &out) {
+        const void *key = cur[0].x;
+        for (int q = 0; q < 2; ++q)
+            if (gexec[q] && gkey[q] == key) { out = gexec[q]; return 0; }
+        const int q = gexec[0] ? 1 : 0;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int e = enqueue_step();
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        // the capture flipped the host parity without running: flip back
+        for (int s = 0; s < nsp; ++s) swap_species(s);
+        if (e || ce != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            if (e) return e;
+            HIPCHK(ce);
+        }
+        const hipError_t ie = hipGraphInstantiate(&gexec[q], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        HIPCHK(ie);
+        gkey[q] = key;
+        out = gexec[q];
+        return 0;
+    }
     int step_fused(int n) override {
+        if (int e = ensure_clock()) return e;
+        // graphs need a fixed launch sequence: the separable field solve, no timers
+        const bool graph = use_graph && spectral && !timing;
         for (int i = 0; i < n; ++i) {
             if (timing) {
                 ev_slot = (int)(ev_rec % RING);
                 if (int e = flush_slot(ev_slot)) return e;
             }
-            advance_time();
-            mark(0);
-            if (int e = field_efield_fused()) return e;   // marks 1 between solve and E field
-            mark(2);
-            if (int e = particle_stage_fused()) return e;
-            mark(7);
-            {
-                // deposit + profile sums + id counter advance in one node kernel
-                DepositExtras<R> ex{1, sum_den, sum_phi, sum_ex, sum_ey, tot, Ex, Ey, next_id, pstart + G};
-                launch_deposit<R>(kc, nc, cur, part, den, sigma_sp, Sigma, Source, brhs, &ex, st);
-                ++n_accum;
-                HIPCHK(hipGetLastError());
+            host_tick();
+            if (graph) {
+                hipGraphExec_t ge = nullptr;
+                if (int e = step_graph(ge)) return e;
+                HIPCHK(hipGraphLaunch(ge, st));
+                for (int s = 0; s < nsp; ++s) swap_species(s);
+            } else {
+                mark(0);
+                if (int e = enqueue_step()) return e;
             }
-            mark(8);
-            mark(9);
+            ++n_accum;
             if (timing) {
                 ev_pending[ev_slot] = true;
                 ++ev_rec;
@@ -683,11 +769,18 @@ struct Case : CaseBase {
         if (std::isnan(r.rel_res)) return fail(-32, "Field solver Error! (NaN residual)");
         return 0;
     }
+    // 1 when all work enqueued on the case stream has finished, 0 while it runs
+    int query() override {
+        const hipError_t q = hipStreamQuery(st);
+        if (q == hipSuccess) return 1;
+        if (q == hipErrorNotReady) return 0;
+        return fail(-1, "HIP error %d (%s) at stream query", (int)q, hipGetErrorString(q));
+    }
     long long np_of(int s) {
         int v = 0;
         hipMemcpyAsync(&v, cur[s].cell_start + G, sizeof(int), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
-        return v;
+        return std::min<long long>(v, cur[s].cap);   // after an overflow only cap slots exist
     }
     int info(pic_info *o) override {
         HIPCHK(hipStreamSynchronize(st));
@@ -775,6 +868,10 @@ struct Case : CaseBase {
         if (n == "sum_ex") return D(sum_ex, G, 8, 1);
         if (n == "sum_ey") return D(sum_ey, G, 8, 1);
         if (n == "next_id") return D(next_id, nsp, 4, 0);
+        if (n == "eedf") {
+            if (!eedf) return false;
+            return D(eedf, EEDF_BINS, 8, 0, false);
+        }
         if (n == "Boundary") return H(S.Boundary.data(), G, 4, 0);
         if (n == "CondID") return H(S.CondID.data(), G, 4, 0);
         if (n == "Face") return H(S.Face.data(), G, 4, 0);
@@ -840,6 +937,7 @@ struct Case : CaseBase {
     }
     int set_step(long long s) override {
         tstep = s;
+        clk_dirty = true;
         t = (double)s * (double)S.dt;
         return 0;
     }
@@ -1011,6 +1109,8 @@ int pic_info_get(pic_ctx *ctx, pic_info *info) {
     return ctx->c->info(info);
 }
 int pic_sync(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->sync(); }
+int pic_query(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->query(); }
+int pic_eedf_sample(pic_ctx *ctx, double emax_ev) { CTX_OR_FAIL(ctx); return ctx->c->eedf_sample(emax_ev); }
 int pic_advance_time(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->advance_time(); }
 int pic_field_solve(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->field_solve(); }
 int pic_efield(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->efield(); }
@@ -1063,7 +1163,7 @@ int pic_set_array(pic_ctx *ctx, const char *name, const void *host, size_t bytes
 }
 const char *pic_array_names(void) {
     return "Ex Ey phi TotPot Sigma Source den sigma_sp b xsol V_cond mccr CondCharge sum_den sum_phi sum_ex sum_ey "
-           "next_id Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai Aj Aval "
+           "eedf next_id Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai Aj Aval "
            "Atemp Mdiag cond_b temp_b lap_phi BackDen1 BackVel1 FluidN xe xee cx sigv P1 P2 Pion vsave";
 }
 int pic_set_step(pic_ctx *ctx, long long tstep) { CTX_OR_FAIL(ctx); return ctx->c->set_step(tstep); }

@@ -29,6 +29,16 @@ __device__ inline int wave_cell(int ncy, int ngy, int Csize, int &lane) {
     return (w / ncy) * ngy + (w % ncy);
 }
 
+// [k0, k1) of cell c, clamped to the array capacity: after a capacity
+// overflow (reported through the error flag) the offsets may point past the
+// arrays, and no kernel may follow them there
+template <typename R>
+__device__ inline void cell_range(const PView<R> &p, int c, int &k0, int &k1) {
+    k0 = min(p.cell_start[c], p.cap);
+    k1 = min(p.cell_start[c + 1], p.cap);
+    if (k1 < k0) k1 = k0;
+}
+
 __device__ inline unsigned long long lanes_below(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
@@ -69,7 +79,8 @@ __global__ __launch_bounds__(256) void k_push(DevConst<R> k, PView<R> p, int isp
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
     if (k0 == k1) return;
     CellE<R> e;
     e.load(k.Ex, k.Ey, c, k.ngy);
@@ -108,7 +119,8 @@ __global__ __launch_bounds__(256) void k_sort_count(DevConst<R> k, PView<R> p, i
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
     const unsigned long long lt = lanes_below(lane);
     int run[ROUTE_SLOTS];
 #pragma unroll
@@ -328,7 +340,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(DevConst<R> k, PView<R> p,
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
     for (int kk = k0 + lane; kk < k1; kk += 64) {
         int ky = key[kk];
         if (ky < 0) continue;
@@ -352,7 +365,8 @@ __global__ __launch_bounds__(256) void k_mcc_collide(DevConst<R> k, PView<R> p, 
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
     const unsigned long long lt = lanes_below(lane);
     int run = 0;
     const size_t cap = (size_t)p.cap;
@@ -395,7 +409,8 @@ __global__ __launch_bounds__(256) void k_mcc_insert(DevConst<R> k, PView<R> src,
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    int k0 = src.cell_start[c], k1 = src.cell_start[c + 1];
+    int k0, k1;
+    cell_range(src, c, k0, k1);
     int ns = new_start[c];
     for (int kk = k0 + lane; kk < k1; kk += 64) {
         int pos = ns + (kk - k0);
@@ -405,7 +420,8 @@ __global__ __launch_bounds__(256) void k_mcc_insert(DevConst<R> k, PView<R> src,
     }
     if (!prank) return;
     int m = k1 - k0;
-    int q0 = parent.cell_start[c], q1 = parent.cell_start[c + 1];
+    int q0, q1;
+    cell_range(parent, c, q0, q1);
     const size_t pcap = (size_t)parent.cap;
     for (int kk = q0 + lane; kk < q1; kk += 64) {
         int r = prank[kk];
@@ -534,19 +550,20 @@ __device__ inline int collide(const DevConst<R> &k, int dest, R pa, uint32_t id,
 // of a block walk their cells chunk by chunk in step; after each round the
 // block collides the pooled particles of its 4 cells together.
 template <typename R, int KIND>
-__global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, uint32_t step, int *cnt,
+__global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, const StepClock *clk, int *cnt,
                                                      int *pcnt, unsigned long long *mccr, int *condcharge,
                                                      R *sigma_sp, uint8_t *flag, R *stash, size_t stash_stride,
                                                      int *err) {
     __shared__ FStage<R> stage_all[FWAVES];
     __shared__ FPool<R> pool;
+    const uint32_t step = clk->step;
     const int wv = threadIdx.x >> 6;
     int lane;
     const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     FStage<R> &S = stage_all[wv];
     const unsigned long long lt = lanes_below(lane);
     int k0 = 0, k1 = 0;
-    if (c >= 0) { k0 = p.cell_start[c]; k1 = p.cell_start[c + 1]; }
+    if (c >= 0) cell_range(p, c, k0, k1);
     WaveRoutes<R> w;
     w.load(k, c, lane, KIND);
     if (lane < ROUTE_SLOTS) {
@@ -664,7 +681,8 @@ __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
     if (k0 == k1) return;
     FStage<R> &S = stage_all[threadIdx.x >> 6];
     uint8_t *F = flag_all[threadIdx.x >> 6];
@@ -760,7 +778,8 @@ __global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PSet<R> ps
     if (c < 0) return;
     const PView<R> &p = ps.s[blockIdx.y];
     R *pp = part + (size_t)blockIdx.y * k.Gsize * 4;
-    int k0 = p.cell_start[c], k1 = p.cell_start[c + 1];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
     R WS = 0, WN = 0, ES = 0, EN = 0;
     for (int b0 = k0; b0 < k1; b0 += 256) {
         const int n = min(256, k1 - b0);
@@ -801,6 +820,10 @@ __global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_
     if (ex.on && g == 0) {
         const uint32_t add = (uint32_t)*ex.prod_total_end;
         for (int s = 0; s < n.nsp; ++s) ex.next_id[s] += add;
+        // the next step: same arithmetic as the host's t = tstep * dt
+        const uint32_t nstep = ex.clk->step + 1u;
+        ex.clk->step = nstep;
+        ex.clk->t = (double)nstep * ex.dt;
     }
     if (g >= n.Gsize) return;
     const int ngy = n.ngy, ngx = n.ngx;
@@ -858,10 +881,11 @@ __global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_
 // electrode voltages of this time evaluated per thread:
 // phi = x[A_idx-1] (0 on fixed nodes), tot = sum_c V_c(t) lap_c + phi.
 template <typename R>
-__global__ void k_phi_potential(NodeConst<R> n, VoltConst v, double t, const double *x, const R *lap, R *V, R *phi,
-                                R *tot) {
+__global__ void k_phi_potential(NodeConst<R> n, VoltConst v, const StepClock *clk, const double *x, const R *lap,
+                                R *V, R *phi, R *tot) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n.Gsize) return;
+    const double t = clk->t;
     const int a = n.A_idx[g];
     const R ph = a ? (R)x[a - 1] : (R)0;
     phi[g] = ph;
@@ -873,6 +897,29 @@ __global__ void k_phi_potential(NodeConst<R> n, VoltConst v, double t, const dou
     }
     tp += (R)1 * ph;
     tot[g] = tp;
+}
+
+// ============================================================ EEDF
+// Electron energy histogram: E = Escale v^2 (the MCC's energy, in eV) in
+// EEDF_BINS equal bins over [0, emax); energies at or above emax go to the
+// last bin.  Per-block LDS counts, then one 64-bit add per bin: the counts
+// are integers, so the result does not depend on the order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_eedf(PView<R> p, int ncell_nodes, R escale, R emax, unsigned long long *hist) {
+    __shared__ unsigned int h[EEDF_BINS];
+    for (int b = threadIdx.x; b < EEDF_BINS; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
+    const int np = min(p.cell_start[ncell_nodes], p.cap);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
+        const R vx = p.vx[k], vy = p.vy[k], vz = p.vz[k];
+        const R e = escale * (vx * vx + vy * vy + vz * vz);
+        int b = (int)(e / emax * (R)EEDF_BINS);
+        b = b < 0 ? 0 : (b >= EEDF_BINS ? EEDF_BINS - 1 : b);
+        atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < EEDF_BINS; b += blockDim.x)
+        if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
 }
 
 // ============================================================ E field
@@ -1088,15 +1135,15 @@ void launch_mcc_insert(const DevConst<R> &k, const PView<R> &src, const PView<R>
                        prod_start, (const int *)nullptr, parent, prank, stage, stage_off, next_id, cap, err);
 }
 template <typename R>
-void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, uint32_t step, int *cnt, int *pcnt,
+void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, const StepClock *clk, int *cnt, int *pcnt,
                         unsigned long long *mccr, int *cc, R *sg, uint8_t *flag, R *stash, size_t stash_stride,
                         int *err, hipStream_t s) {
     if (isp == 0)
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
+                           clk, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
     else
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           step, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
+                           clk, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
 }
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
@@ -1124,9 +1171,13 @@ void launch_deposit(const DevConst<R> &k, const NodeConst<R> &n, const PView<R> 
                        sigma_sp, Sigma, Source, b, e);
 }
 template <typename R>
-void launch_phi_potential(const NodeConst<R> &n, const VoltConst &v, double t, const double *x, const R *lap, R *V,
-                          R *phi, R *tot, hipStream_t s) {
-    hipLaunchKernelGGL(k_phi_potential<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, v, t, x, lap, V, phi, tot);
+void launch_eedf(const PView<R> &p, int Gsize, R escale, R emax, unsigned long long *hist, hipStream_t s) {
+    hipLaunchKernelGGL(k_eedf<R>, dim3(1024), dim3(256), 0, s, p, Gsize, escale, emax, hist);
+}
+template <typename R>
+void launch_phi_potential(const NodeConst<R> &n, const VoltConst &v, const StepClock *clk, const double *x,
+                          const R *lap, R *V, R *phi, R *tot, hipStream_t s) {
+    hipLaunchKernelGGL(k_phi_potential<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, v, clk, x, lap, V, phi, tot);
 }
 template <typename R>
 void launch_efield_only(const NodeConst<R> &n, const R *phi, const R *tot, const R *den, R *Sigma, R *Ex, R *Ey,
@@ -1219,15 +1270,16 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
     template void launch_mcc_insert<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const int *,           \
                                        const int *, const PView<R> &, const int *, const R *, int, const uint32_t *,   \
                                        int, int *, hipStream_t);                                                       \
-    template void launch_fused_count<R>(const DevConst<R> &, const PView<R> &, int, uint32_t, int *, int *,            \
+    template void launch_fused_count<R>(const DevConst<R> &, const PView<R> &, int, const StepClock *, int *, int *,   \
                                         unsigned long long *, int *, R *, uint8_t *, R *, size_t, int *, hipStream_t); \
     template void launch_fused_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const PView<R> &,   \
                                           int, const uint8_t *, const R *, size_t, const int *, const int *,           \
                                           const int *, const int *, const uint32_t *, int, int, int *, hipStream_t);   \
     template void launch_deposit<R>(const DevConst<R> &, const NodeConst<R> &, const PView<R> *, R *, R *, R *, R *,   \
                                     R *, R *, const DepositExtras<R> *, hipStream_t);                                  \
-    template void launch_phi_potential<R>(const NodeConst<R> &, const VoltConst &, double, const double *, const R *,  \
-                                          R *, R *, R *, hipStream_t);                                                 \
+    template void launch_phi_potential<R>(const NodeConst<R> &, const VoltConst &, const StepClock *, const double *,  \
+                                          const R *, R *, R *, R *, hipStream_t);                                      \
+    template void launch_eedf<R>(const PView<R> &, int, R, R, unsigned long long *, hipStream_t);               \
     template void launch_efield_only<R>(const NodeConst<R> &, const R *, const R *, const R *, R *, R *, R *,          \
                                         hipStream_t);                                                            \
     template void launch_efield<R>(const NodeConst<R> &, const VoltConst &, double, const R *, R *, const R *, R *,    \

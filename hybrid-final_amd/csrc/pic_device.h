@@ -131,6 +131,10 @@ struct DevConst {
 };
 
 // ---------------------------------------------------------------- push
+template <typename R> __host__ __device__ constexpr R one_minus_ulp();
+template <> __host__ __device__ constexpr float one_minus_ulp<float>() { return 0.99999994f; }           // 1 - 2^-24
+template <> __host__ __device__ constexpr double one_minus_ulp<double>() { return 0.99999999999999989; } // 1 - 2^-53
+
 // Bilinear E gather and leapfrog push in cell units.  Returns the move code.
 template <typename R>
 __device__ inline int push_particle(R ews, R ewn, R ees, R een, R yws, R ywn, R yes, R yen, R As, R dtdx, R dtdy,
@@ -148,6 +152,12 @@ __device__ inline int push_particle(R ews, R ewn, R ees, R een, R yws, R ywn, R 
         else if (ly < 0) { dys = -1; ly += 1; }
         if (lx >= 1) { dxs = 1; lx -= 1; }
         else if (lx < 0) { dxs = -1; lx += 1; }
+        // at most one cell per axis and step, as in the reference; an offset
+        // still outside [0,1) (a faster move) is clamped into the destination
+        // cell instead of being left outside it
+        const R top = one_minus_ulp<R>();
+        if (lx >= 1) lx = top; else if (lx < 0) lx = 0;
+        if (ly >= 1) ly = top; else if (ly < 0) ly = 0;
     }
     x = lx; y = ly; vx = mx; vy = my;
     return (dxs + 1) * 3 + (dys + 1);

@@ -31,8 +31,17 @@ struct PSet {
     PView<R> s[5];
 };
 
-// fused-step extras of the node deposit kernel: FP64 profile sums and the
-// particle id counters (advanced by the step's product count)
+// Device-side clock of the fused step: the step number and time of the step
+// about to run.  Kept in device memory so that a captured step graph needs no
+// per-step kernel arguments; the deposit at the end of each step advances it.
+struct StepClock {
+    uint32_t step;
+    uint32_t pad;
+    double t;
+};
+
+// fused-step extras of the node deposit kernel: FP64 profile sums, the
+// particle id counters (advanced by the step's product count) and the clock
 template <typename R>
 struct DepositExtras {
     int on;
@@ -40,9 +49,12 @@ struct DepositExtras {
     const R *tot, *Ex, *Ey;
     uint32_t *next_id;
     const int *prod_total_end;
+    StepClock *clk;
+    double dt;
 };
 
 constexpr int MAX_SRC = 16;
+constexpr int EEDF_BINS = 256;
 struct VoltConst {
     int ncond, nsrc;
     int flag[MAX_SRC], mid[MAX_SRC];
@@ -65,7 +77,7 @@ void launch_mcc_insert(const DevConst<R> &, const PView<R> &, const PView<R> &, 
 // in place, flag byte per particle, ionisation products in stash, 6 arrays
 // of stash_stride) and scatter pass (sorted write of survivors and products)
 template <typename R>
-void launch_fused_count(const DevConst<R> &, const PView<R> &, int isp, uint32_t step, int *cnt, int *pcnt,
+void launch_fused_count(const DevConst<R> &, const PView<R> &, int isp, const StepClock *clk, int *cnt, int *pcnt,
                         unsigned long long *mccr, int *condcharge, R *sigma_sp, uint8_t *flag, R *stash,
                         size_t stash_stride, int *err, hipStream_t);
 template <typename R>
@@ -78,8 +90,11 @@ void launch_deposit(const DevConst<R> &, const NodeConst<R> &, const PView<R> *,
                     R *Sigma, R *Source, R *b, const DepositExtras<R> *ex, hipStream_t);
 // fused step: phi from the FP64 solution, electrode voltages and total potential in one kernel
 template <typename R>
-void launch_phi_potential(const NodeConst<R> &n, const VoltConst &v, double t, const double *x, const R *lap, R *V,
-                          R *phi, R *tot, hipStream_t s);
+void launch_phi_potential(const NodeConst<R> &n, const VoltConst &v, const StepClock *clk, const double *x,
+                          const R *lap, R *V, R *phi, R *tot, hipStream_t s);
+// electron energy histogram (EEDF_BINS bins over [0, emax) eV) added into hist
+template <typename R>
+void launch_eedf(const PView<R> &electrons, int Gsize, R escale, R emax, unsigned long long *hist, hipStream_t s);
 template <typename R>
 void launch_efield_only(const NodeConst<R> &n, const R *phi, const R *tot, const R *den, R *Sigma, R *Ex, R *Ey,
                         hipStream_t s);

@@ -47,6 +47,8 @@ EXPORTS = {
     "pic_last_error": (C.c_char_p, []),
     "pic_info_get": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "pic_sync": (C.c_int, [C.c_void_p]),
+    "pic_query": (C.c_int, [C.c_void_p]),
+    "pic_eedf_sample": (C.c_int, [C.c_void_p, C.c_double]),
     "pic_advance_time": (C.c_int, [C.c_void_p]),
     "pic_field_solve": (C.c_int, [C.c_void_p]),
     "pic_efield": (C.c_int, [C.c_void_p]),
@@ -70,6 +72,14 @@ EXPORTS = {
     "pic_field_solver_kind": (C.c_int, [C.c_void_p]),
     "pic_stage_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "pic_stage_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_longlong)]),
+    "pic_pbatch_create": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "pic_pbatch_destroy": (C.c_int, [C.c_void_p]),
+    "pic_pbatch_unknowns": (C.c_longlong, [C.c_void_p]),
+    "pic_pbatch_set_rhs": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pic_pbatch_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "pic_pbatch_sync": (C.c_int, [C.c_void_p]),
+    "pic_pbatch_result": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pic_pbatch_last_error": (C.c_char_p, []),
     "pic_setup_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "pic_setup_array_info": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
@@ -199,6 +209,17 @@ class Case:
     def sync(self):
         _check(lib().pic_sync(self.h))
 
+    def eedf_sample(self, emax_ev=40.0):
+        """Add the current electron energy histogram to the "eedf" array."""
+        _check(lib().pic_eedf_sample(self.h, float(emax_ev)))
+
+    def done(self):
+        """True when all enqueued work of this case has finished (non-blocking)."""
+        r = lib().pic_query(self.h)
+        if r < 0:
+            _check(r)
+        return r == 1
+
     # ---- stage table
     def advance_time(self): _check(lib().pic_advance_time(self.h))
     def field_solve(self): _check(lib().pic_field_solve(self.h))
@@ -271,3 +292,58 @@ class Case:
         _check(L.pic_array_info(self.h, name.encode(), C.byref(n), C.byref(es), C.byref(fl)))
         a = np.ascontiguousarray(values, dtype=_dtype(name, es.value, fl.value))
         _check(L.pic_set_array(self.h, name.encode(), _ptr(a), a.nbytes))
+
+
+class PoissonBatch:
+    """Independent 5-point Poisson problems solved together on the GPU
+    (include/picmcc.h, pic_pbatch_*).  sizes: [(nx, ny)], bcs: [(left, right,
+    bottom, top)] with 0 = Neumann, 1 = Dirichlet."""
+
+    def __init__(self, sizes, bcs, device=0):
+        L = lib()
+        n = len(sizes)
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        nx = np.array([s[0] for s in self.sizes], dtype=np.int32)
+        ny = np.array([s[1] for s in self.sizes], dtype=np.int32)
+        bc = np.ascontiguousarray(np.asarray(bcs, dtype=np.int32).reshape(n, 4))
+        h = C.c_void_p()
+        rc = L.pic_pbatch_create(n, _ptr(nx), _ptr(ny), _ptr(bc), device, C.byref(h))
+        if rc != 0:
+            raise PicError(rc, L.pic_pbatch_last_error().decode())
+        self.h = h
+        self.n = n
+        self.N = int(L.pic_pbatch_unknowns(self.h))
+        self.offsets = np.concatenate([[0], np.cumsum(nx.astype(np.int64) * ny)])
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise PicError(rc, lib().pic_pbatch_last_error().decode())
+        return rc
+
+    def set_rhs(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        assert b.size == self.N
+        self._chk(lib().pic_pbatch_set_rhs(self.h, _ptr(b)))
+
+    def solve(self, tol=1e-10, max_cycles=3):
+        self._chk(lib().pic_pbatch_solve(self.h, float(tol), int(max_cycles)))
+
+    def sync(self):
+        self._chk(lib().pic_pbatch_sync(self.h))
+
+    def result(self):
+        x = np.empty(self.N, dtype=np.float64)
+        rel = np.empty(self.n, dtype=np.float64)
+        bad = self._chk(lib().pic_pbatch_result(self.h, _ptr(x), _ptr(rel)))
+        return x, rel, bad
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pic_pbatch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,191 @@
+"""Continuous-batching ensemble server for discharge cases on one GPU.
+
+Requests (a deck and a step count, with an arrival time) are admitted while
+the device memory budget allows it.  The budget check uses a footprint
+estimate from the deck; the footprint of an admitted case is then measured
+(device free memory before and after its creation).  Every admitted case
+keeps its particles, fields and random-stream state resident in HBM on its
+own HIP stream.
+
+The loop feeds each active case a slice of fused steps whenever its
+previous slice has finished.  The FP32 separable-solver path replays each
+step as one captured graph, so the host cost of a slice is small and the
+slices of all active cases overlap on the device.  A case leaves when its
+last slice completes; its slot and memory go to the next arrival.
+
+Each case's result is independent of what else runs: the step is
+deterministic per case, so a case run alone and the same case run in the
+mix give the same bits.
+"""
+import collections
+import ctypes as C
+import math
+import random
+import time
+
+import numpy as np
+
+from . import FP32_REFINED, Case, decks
+
+_hip = None
+
+
+def device_free_bytes():
+    """Free and total device memory (hipMemGetInfo on the current device)."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+    f, t = C.c_size_t(), C.c_size_t()
+    if _hip.hipMemGetInfo(C.byref(f), C.byref(t)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return f.value, t.value
+
+
+def footprint_estimate(deck, precision=FP32_REFINED):
+    """Device bytes a case needs, from its deck (an upper estimate)."""
+    geo = deck["Geometry"]["SystemSpec"]
+    ngx, ngy = int(geo["NumGridx"]), int(geo["NumGridy"])
+    G = ngx * ngy
+    R = 4 if precision == FP32_REFINED else 8
+    caps = [int(v.get("Max_np", 0)) for k, v in deck["GasSpecies"]["ChargeSpecies"].items() if isinstance(v, dict)]
+    capmax = max(caps) if caps else 0
+    per_slot = 2 * (5 * R + 4) + 1 + 4 + 4          # two particle buffers, move tag, sort key, cell_start share
+    shared = capmax * (6 * R + 4)                    # product staging / stash, product ranks
+    nodes = G * (30 * R + 40 * 8 + 9 * 4 * 12)      # node fields, FP64 sums, route and counting tables
+    solve = 2 * (ngx * ngx + ngy * ngy + 3 * G) * 8  # separable-solver matrices and work arrays
+    return sum(caps) * per_slot + shared + nodes + solve + (16 << 20)
+
+
+Request = collections.namedtuple("Request", "rid arrival deck steps")
+
+
+def ar_ensemble(n, seed=7, rate_hz=40.0, grid=(32, 96), ppc=(20, 60), torr=(0.01, 0.5), volts=(50.0, 300.0),
+                cycles=(1, 3), steps_per_cycle=400):
+    """Poisson arrivals of argon CCP cases with log-uniform pressure and
+    uniform grid size, particles per cell, RF amplitude and cycle count."""
+    rng = random.Random(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.expovariate(rate_hz)
+        ncx = rng.randint(grid[0] // 8, grid[1] // 8) * 8
+        ncy = rng.randint(grid[0] // 8, grid[1] // 8) * 8
+        p = math.exp(rng.uniform(math.log(torr[0]), math.log(torr[1])))
+        d = decks.ccp_deck(ncx=ncx, ncy=ncy, dx=5e-4, volts=rng.uniform(*volts), pres_torr=p,
+                           ppc=rng.randint(*ppc), steps_per_cycle=steps_per_cycle, cycles=rng.randint(*cycles),
+                           seed=1000 + i, philox_seed=5000 + i)
+        steps = d["SimulationMethod"]["RFCycles"] * steps_per_cycle
+        out.append(Request(i, t, d, steps))
+    return out
+
+
+class _Active:
+    __slots__ = ("req", "case", "remaining", "inflight", "admit", "footprint", "np0")
+
+    def __init__(self, req, case, admit, footprint):
+        self.req, self.case, self.admit, self.footprint = req, case, admit, footprint
+        self.remaining = req.steps
+        self.inflight = False
+        inf = case.info()
+        self.np0 = sum(inf.np[:inf.nsp])
+
+
+class EnsembleServer:
+    """mem_fraction: share of the free device memory the server may fill;
+    slice_steps: steps enqueued per case per feed; max_active: slot count."""
+
+    def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=64):
+        self.precision = precision
+        self.slice = slice_steps
+        self.max_active = max_active
+        free, _ = device_free_bytes()
+        self.budget = int(free * mem_fraction)
+
+    def run(self, requests, keep_state=None):
+        """Serve `requests` in real time; returns one record per case.
+        keep_state(rid) -> bool: return that case's final particles and density."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        active, results = [], []
+        used = 0
+        t0 = time.perf_counter()
+        while pending or active:
+            now = time.perf_counter() - t0
+            # admission: arrivals that fit in the slot count and the memory budget
+            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                est = footprint_estimate(pending[0].deck, self.precision)
+                if active and used + est > self.budget:
+                    break
+                req = pending.popleft()
+                f0, _ = device_free_bytes()
+                case = Case(req.deck, self.precision)
+                f1, _ = device_free_bytes()
+                fp = max(f0 - f1, 0)
+                used += fp
+                active.append(_Active(req, case, time.perf_counter() - t0, fp))
+            # feed slices; retire finished cases
+            busy = False
+            for a in list(active):
+                if a.inflight:
+                    if not a.case.done():
+                        busy = True
+                        continue
+                    a.inflight = False
+                if a.remaining == 0:
+                    a.case.sync()
+                    inf = a.case.info()
+                    np1 = sum(inf.np[:inf.nsp])
+                    rec = {"rid": a.req.rid, "arrival": a.req.arrival, "admit": a.admit,
+                           "finish": time.perf_counter() - t0, "steps": a.req.steps,
+                           "particle_steps": 0.5 * (a.np0 + np1) * a.req.steps, "footprint": a.footprint,
+                           "cells": (inf.ngx - 1) * (inf.ngy - 1)}
+                    rec["latency"] = rec["finish"] - rec["arrival"]
+                    if keep_state and keep_state(a.req.rid):
+                        rec["state"] = {"particles": [a.case.particles(s) for s in range(inf.nsp)],
+                                        "den": a.case.array("den")}
+                    a.case.close()
+                    used -= a.footprint
+                    active.remove(a)
+                    results.append(rec)
+                    continue
+                k = min(self.slice, a.remaining)
+                a.case.step(k)
+                a.remaining -= k
+                a.inflight = True
+                busy = True
+            if not busy and pending:
+                time.sleep(max(0.0, min(0.002, pending[0].arrival - (time.perf_counter() - t0))))
+            elif busy:
+                time.sleep(0.0002)
+        return results
+
+
+def run_alone(req, precision=FP32_REFINED, keep_state=False):
+    """One case by itself: creation, all steps, synchronisation."""
+    t0 = time.perf_counter()
+    case = Case(req.deck, precision)
+    inf = case.info()
+    np0 = sum(inf.np[:inf.nsp])
+    case.step(req.steps)
+    case.sync()
+    inf = case.info()
+    np1 = sum(inf.np[:inf.nsp])
+    rec = {"rid": req.rid, "seconds": time.perf_counter() - t0, "particle_steps": 0.5 * (np0 + np1) * req.steps}
+    if keep_state:
+        rec["state"] = {"particles": [case.particles(s) for s in range(inf.nsp)], "den": case.array("den")}
+    case.close()
+    return rec
+
+
+def summarize(results, one_by_one=None):
+    lat = np.array([r["latency"] for r in results])
+    ps = sum(r["particle_steps"] for r in results)
+    span = max(r["finish"] for r in results) - min(r["arrival"] for r in results)
+    out = {"cases": len(results), "p50_latency_s": float(np.percentile(lat, 50)),
+           "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": span,
+           "aggregate_particle_steps_per_s": ps / span,
+           "peak_footprint_bytes": max(r["footprint"] for r in results)}
+    if one_by_one:
+        tot = sum(r["seconds"] for r in one_by_one)
+        out["one_by_one_seconds"] = tot
+        out["one_by_one_particle_steps_per_s"] = sum(r["particle_steps"] for r in one_by_one) / tot
+    return out

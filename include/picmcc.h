@@ -54,6 +54,10 @@ int pic_destroy(pic_ctx *ctx);
 const char *pic_last_error(void);
 int pic_info_get(pic_ctx *ctx, pic_info *info);
 int pic_sync(pic_ctx *ctx);   /* waits for the stream; reports device-side overflow */
+/* non-blocking: 1 when the case's enqueued work has finished, 0 while it
+ * runs, <0 on error (no counterpart in the reference, which synchronises
+ * after every stage; used by the ensemble server to overlap cases) */
+int pic_query(pic_ctx *ctx);
 
 /* ---------------- stage table (src/cuda_main.cu:112-235) ---------------- */
 int pic_advance_time(pic_ctx *ctx);  /* t += dt, tstep++ (src/cuda_main.cu:114-116) */
@@ -64,6 +68,11 @@ int pic_sort_boundary(pic_ctx *ctx); /* SortBounndary_cuda, src/cuda_Sortboundar
 int pic_mcc(pic_ctx *ctx);           /* MCC_Ar_cuda, src/cuda_mcc.cu:97-106 */
 int pic_deposit(pic_ctx *ctx);       /* Deposit_cuda / Deposit_Basic, src/cuda_Deposit.cu:3-29 */
 int pic_diagnostic(pic_ctx *ctx);    /* profile accumulators, src/cuda_Diagnostic.cu:505-547 */
+/* electron energy distribution: adds a 256-bin histogram of the electrons'
+ * energies (eV, bins over [0, emax_ev), the top bin also takes higher
+ * energies) to the "eedf" array; a different emax_ev restarts it.  The
+ * reference's EEDF diagnostic is commented out (src/start.c:934-967). */
+int pic_eedf_sample(pic_ctx *ctx, double emax_ev);
 /* one full timestep through the stage functions above, nsteps times */
 int pic_run_staged(pic_ctx *ctx, int nsteps);
 /* one full timestep on the fused path (push+walls+MCC counted, scanned and
@@ -95,6 +104,25 @@ int pic_stage_timing(pic_ctx *ctx, int enable);
 int pic_stage_times(pic_ctx *ctx, double *ms, int n, long long *steps);
 /* 0 = Jacobi PCG, 1 = separable fast solve (FP32 spectral inverse in FP64 refinement) */
 int pic_field_solver_kind(pic_ctx *ctx);
+
+/* ---------------- batched Poisson solves ---------------- */
+/* Many independent 5-point problems (unit spacing, operator = the reference
+ * field operator's stencil with eps = 1, src/load.c:121-245) solved together.
+ * Problem p has nx[p] x ny[p] unknowns, unknown (a, b) at a*ny + b of its
+ * block (blocks concatenated in problem order); bc[4p..4p+3] = left (a = 0),
+ * right, bottom (b = 0), top: 0 Neumann (mirror), 1 Dirichlet (0 V).
+ * No counterpart in the reference, which solves one problem per case. */
+typedef struct pic_pbatch pic_pbatch;
+int pic_pbatch_create(int nprob, const int *nx, const int *ny, const int *bc, int device, pic_pbatch **out);
+int pic_pbatch_destroy(pic_pbatch *b);
+long long pic_pbatch_unknowns(pic_pbatch *b);
+int pic_pbatch_set_rhs(pic_pbatch *b, const double *rhs);          /* host array of all unknowns */
+int pic_pbatch_solve(pic_pbatch *b, double tol, int max_cycles);   /* enqueued, from x = 0 */
+int pic_pbatch_sync(pic_pbatch *b);
+/* waits; copies x and per-problem relative residuals (either may be NULL);
+ * returns the number of problems above tol, or < 0 on error */
+int pic_pbatch_result(pic_pbatch *b, double *x, double *rel_res);
+const char *pic_pbatch_last_error(void);
 
 /* ---------------- host-only setup (no device needed) ---------------- */
 /* deck -> geometry maps, field operator, Laplace solutions, tables and the

@@ -225,7 +225,17 @@ void orc_efield(orc_case *c) {
 }
 
 /* ================= push: src/cuda_Move.cu:181-270 =================
- * tag = relative cell step (0 = stays).  Positions are offsets in the cell. */
+ * tag = relative cell step (0 = stays).  Positions are offsets in the cell.
+ * As in the reference, a particle moves at most one cell per axis and step.
+ * Deliberate change: an offset still outside [0, 1) after that single wrap
+ * (a move of more than one cell) is clamped into the destination cell; the
+ * reference leaves it outside, which corrupts the next CIC weights and can
+ * grow without bound. */
+#ifdef ORC_FLOAT
+#define ORC_ONE_MINUS 0.99999994f           /* 1 - 2^-24, largest float below 1 */
+#else
+#define ORC_ONE_MINUS 0.99999999999999989   /* 1 - 2^-53, largest double below 1 */
+#endif
 void orc_move(orc_case *c) {
     int isp, cell, ngy = c->ngy;
     for (isp = 0; isp < c->nsp; ++isp) {
@@ -251,6 +261,8 @@ void orc_move(orc_case *c) {
                     else if (ly < 0) { t -= 1; ly += 1; }
                     if (lx >= 1) { t += ngy; lx -= 1; }
                     else if (lx < 0) { t -= ngy; lx += 1; }
+                    if (lx >= 1) lx = ORC_ONE_MINUS; else if (lx < 0) lx = 0;
+                    if (ly >= 1) ly = ORC_ONE_MINUS; else if (ly < 0) ly = 0;
                 }
                 s->x[k] = lx; s->y[k] = ly; s->vx[k] = mx; s->vy[k] = my;
                 tag[k] = t;

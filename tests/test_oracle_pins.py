@@ -211,6 +211,32 @@ def test_push_uniform_field_closed_form():
     assert np.array_equal(s["vz"], vz)
 
 
+def test_push_moves_at_most_one_cell_per_axis():
+    """A particle fast enough to cross more than one cell in a step moves one
+    cell per axis (the reference's single wrap, src/cuda_Move.cu:231-254),
+    and its offset is clamped into that cell instead of staying outside
+    [0, 1); a slower crosser keeps its exact wrapped offset."""
+    d = decks.small(ncx=8, ncy=8)
+    for prec, top in (("f64", np.nextafter(1.0, 0.0)), ("f32", np.nextafter(np.float32(1), np.float32(0)))):
+        o = Oracle(d, prec)
+        ngy, G = 9, 81
+        o.arr("Ex")[:] = 0.0
+        o.arr("Ey")[:] = 0.0
+        cell = 3 * ngy + 4
+        dtdx = o.scalar("dt") / o.scalar("dx")
+        # cells per step: +2.5 and -1.7 in x (fast), +0.3 in y (slow crosser: y 0.9 -> 0.2)
+        xs = np.array([0.5, 0.5]); ys = np.array([0.9, 0.5])
+        vx = np.array([2.5, -1.7]) / dtdx; vy = np.array([0.3, 0.0]) / dtdx; vz = np.zeros(2)
+        o.set_species(0, _single_cell_state(o, ngy, cell, xs, ys, vx, vy, vz, G))
+        o.set_species(1, _single_cell_state(o, ngy, cell, xs[:0], ys[:0], vx[:0], vy[:0], vz[:0], G))
+        o.orc_move()
+        s = o.species(0)
+        tag = o.arr("sp0.tag")[:2]
+        assert list(tag) == [ngy + 1, -ngy], tag
+        assert s["x"][0] == top and s["x"][1] == 0.0
+        assert abs(s["y"][0] - 0.2) < 1e-6 and s["y"][1] == 0.5
+
+
 def test_deposit_weights_sum_to_particle_count():
     d = decks.small(ncx=8, ncy=8, ppc=30)
     o = Oracle(d, "f64")
@@ -325,3 +351,25 @@ def test_null_collision_fraction_matches_probability():
     mean = np.sum(p_any * p_target0 * np.minimum(sig * vel / o.arr("sigv")[0], 1.0))
     total = ev[0:4].sum()
     assert abs(total - mean) < 5 * math.sqrt(mean) + 5
+
+
+def test_poisson_oracle_operator_is_the_field_operator():
+    """For uniform permittivity and dx = dy, the batched-Poisson oracle's
+    operator equals the field operator restated from src/load.c:121-245
+    (unknown ordering a*ny + b), up to the operator's constant scale."""
+    from picmcc import FP64_PLAIN, HostSetup, decks
+    from poisson_oracle import operator
+    d = decks.small(ncx=6, ncy=5)
+    h = HostSetup(d, FP64_PLAIN, load=False)
+    Ai, Aj, Av = h.array("Ai"), h.array("Aj"), h.array("Aval")
+    N = len(Ai) - 1
+    A = np.zeros((N, N))
+    for r in range(N):
+        for q in range(Ai[r], Ai[r + 1]):
+            A[r, Aj[q]] = Av[q]
+    # CCP deck: Neumann sides in x, conductors on the bottom and top rows
+    nx, ny = 7, 4
+    assert N == nx * ny
+    B = operator(nx, ny, (0, 0, 1, 1)).toarray()
+    scale = A[0, 0] / B[0, 0]
+    np.testing.assert_allclose(A, scale * B, rtol=0, atol=1e-12 * abs(scale))
