@@ -152,8 +152,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--pbatch", type=int, default=256, metavar="N",
                     help="independent Poisson problems (64^2-512^2) in the batched-solve leg (0 = skip)")
-    ap.add_argument("--ensemble", type=int, default=32, metavar="N",
-                    help="cases in the served argon ensemble (Poisson arrivals; 0 = skip)")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="cases in the served argon ensemble (Poisson arrivals; 0 = skip; "
+                         "concurrent serving is currently switched off)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="first run a device copy of MB MiB (a kernel of known traffic, for counter calibration)")
     args = ap.parse_args()
@@ -226,6 +227,9 @@ def main():
         line["batched_poisson"] = run_pbatch(args.pbatch)
     if args.ensemble > 0:
         line["ensemble"] = run_ensemble(args.ensemble)
+    else:
+        from picmcc import serve
+        line["ensemble"] = {"status": "not run", "reason": serve.DISABLED_REASON}
     if args.cpu_steps > 0:
         cv, cdt = cpu_baseline(deck, args.cpu_steps)
         line["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",

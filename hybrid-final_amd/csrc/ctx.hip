@@ -177,7 +177,10 @@ struct Case : CaseBase {
     }
 
     ~Case() override {
-        if (st) hipStreamSynchronize(st);
+        if (st) {
+            const hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) fprintf(stderr, "picmcc: case teardown: %s\n", hipGetErrorString(e));
+        }
         void *ptrs[] = {Ex, Ey, P1, P2, Pion, BackVel1, vsave, xe, cx, lap, phi, tot, Sigma, Source, den, sigma_sp,
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
@@ -591,6 +594,9 @@ struct Case : CaseBase {
         const uint32_t step = (uint32_t)tstep;
         // electrons first in the event order is irrelevant: each particle owns its stream
         launch_mcc_collide<R>(kc, cur[1], 1, step, mccr, cellp, prank, stage, st);   // ions (no products)
+        // product counts are written per cell; the nodes that start no cell
+        // (last row and column) take part in the scan as zeros
+        launch_fill_int(pcnt, 0, G, st);
         launch_mcc_collide<R>(kc, cur[0], 0, step, mccr, pcnt, prank, stage, st);    // electrons
         launch_scan(pcnt, pstart, G, scan_tmp, st);
         for (int s = 0; s < nsp; ++s) {
@@ -1110,6 +1116,27 @@ int pic_info_get(pic_ctx *ctx, pic_info *info) {
 }
 int pic_sync(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->sync(); }
 int pic_query(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->query(); }
+int pic_device_sync(int device) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    HIPCHK(hipSetDevice(device));
+    const hipError_t e = hipDeviceSynchronize();
+    const hipError_t l = hipGetLastError();
+    HIPCHK(hipSetDevice(cur));
+    HIPCHK(e);
+    HIPCHK(l);
+    return 0;
+}
+int pic_device_memory(int device, size_t *free_bytes, size_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(-2, "pic_device_memory: null output");
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    HIPCHK(hipSetDevice(device));
+    const hipError_t e = hipMemGetInfo(free_bytes, total_bytes);
+    HIPCHK(hipSetDevice(cur));
+    HIPCHK(e);
+    return 0;
+}
 int pic_eedf_sample(pic_ctx *ctx, double emax_ev) { CTX_OR_FAIL(ctx); return ctx->c->eedf_sample(emax_ev); }
 int pic_advance_time(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->advance_time(); }
 int pic_field_solve(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->field_solve(); }

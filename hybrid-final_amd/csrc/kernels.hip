@@ -1080,6 +1080,12 @@ __global__ void k_cell_counts(int Gsize, const int *cell_start, const int *pcnt,
     total[d] = cell_start[d + 1] - cell_start[d] + (pcnt ? pcnt[d] : 0);
 }
 
+// v into n ints (a kernel, not a memset: usable where the launches are captured)
+__global__ void k_fill_int(int *p, int v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // next_id[s] += number of products created this step (same count for the
 // electron and the ion array: one of each per ionisation)
 __global__ void k_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp) {
@@ -1201,6 +1207,9 @@ void launch_accumulate(int Gsize, int nsp, const R *den, const R *tot, const R *
 
 void launch_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total, hipStream_t s) {
     hipLaunchKernelGGL(k_cell_counts, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, cell_start, pcnt, total);
+}
+void launch_fill_int(int *p, int v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_int, dim3(node_blocks(n)), dim3(256), 0, s, p, v, n);
 }
 void launch_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp, hipStream_t s) {
     hipLaunchKernelGGL(k_bump_ids, dim3(1), dim3(64), 0, s, next_id, prod_total_end, nsp);

@@ -119,6 +119,7 @@ void launch_assign_bases_pair(int Gsize, const int *ent_dest, const int *ent_pos
                               const int *prod_start, int *base_e, int *base_i, int *pbase_e, int *pbase_i,
                               int *pid_off, hipStream_t s);
 void launch_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total, hipStream_t s);
+void launch_fill_int(int *p, int v, int n, hipStream_t s);
 void launch_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp, hipStream_t s);
 template <typename R> void launch_rhs_to_fp64(int N, const R *b, double *b64, hipStream_t s);
 template <typename R> void launch_x_to_phi(int Gsize, const int *A_idx, const double *x, R *phi, hipStream_t s);

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <map>
 #include <string>
@@ -257,7 +258,10 @@ static hipError_t pb_alloc(pic_pbatch *B, T **p, size_t n) {
 
 static int pb_destroy(pic_pbatch *B) {
     if (!B) return 0;
-    if (B->st) hipStreamSynchronize(B->st);
+    if (B->st) {
+        const hipError_t e = hipStreamSynchronize(B->st);
+        if (e != hipSuccess) fprintf(stderr, "picmcc: Poisson batch teardown: %s\n", hipGetErrorString(e));
+    }
     for (void *p : B->owned) hipFree(p);
     if (B->st) hipStreamDestroy(B->st);
     delete B;

@@ -48,6 +48,8 @@ EXPORTS = {
     "pic_info_get": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "pic_sync": (C.c_int, [C.c_void_p]),
     "pic_query": (C.c_int, [C.c_void_p]),
+    "pic_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "pic_device_sync": (C.c_int, [C.c_int]),
     "pic_eedf_sample": (C.c_int, [C.c_void_p, C.c_double]),
     "pic_advance_time": (C.c_int, [C.c_void_p]),
     "pic_field_solve": (C.c_int, [C.c_void_p]),
@@ -117,6 +119,11 @@ def _check(code):
     if code != 0:
         raise PicError(code, lib().pic_last_error().decode())
     return code
+
+
+def device_sync(device=0):
+    """Wait for all work on the device; raises PicError on a device error."""
+    _check(lib().pic_device_sync(device))
 
 
 def _ptr(a):

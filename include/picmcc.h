@@ -58,6 +58,12 @@ int pic_sync(pic_ctx *ctx);   /* waits for the stream; reports device-side overf
  * runs, <0 on error (no counterpart in the reference, which synchronises
  * after every stage; used by the ensemble server to overlap cases) */
 int pic_query(pic_ctx *ctx);
+/* free and total device memory of a device, in bytes (no counterpart in the
+ * reference; the ensemble server sizes its admission budget with it) */
+int pic_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
+/* waits for all work on a device and reports any device error (no
+ * counterpart in the reference; test harnesses call it between cases) */
+int pic_device_sync(int device);
 
 /* ---------------- stage table (src/cuda_main.cu:112-235) ---------------- */
 int pic_advance_time(pic_ctx *ctx);  /* t += dt, tstep++ (src/cuda_main.cu:114-116) */

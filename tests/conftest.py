@@ -12,9 +12,12 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
 
 
-def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+@pytest.fixture(autouse=True)
+def _device_clean_after_gpu_test(request):
+    """After every GPU test, wait for the whole device and check it for
+    errors, so that an asynchronous device error is reported by the test
+    whose work caused it."""
+    yield
+    if request.node.get_closest_marker("gpu") is not None:
+        import picmcc
+        picmcc.device_sync(0)

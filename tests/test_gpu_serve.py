@@ -9,7 +9,7 @@ import pytest
 
 from picmcc import serve
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason=serve.DISABLED_REASON)]
 
 
 def test_results_same_alone_and_in_the_mix():
