@@ -148,7 +148,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=20, help="steps timed on the FP64 plain path")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--pbatch", type=int, default=256, metavar="N",
                     help="independent Poisson problems (64^2-512^2) in the batched-solve leg (0 = skip)")
@@ -189,6 +189,14 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k][2])
     kname, bytes_, ms = kernels[dom]
     achieved = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    # measured HBM bytes per launch of that kernel: rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE of the same workload, scaled by a copy kernel of known traffic
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_ar_ccp_256x256.json")
+    if args.config == "c2" and os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)["per_launch_bytes"].get(dom)
+        traffic = t["total"] if t else None
 
     line = {
         "metric": METRIC,
@@ -220,7 +228,7 @@ def main():
         "field_solve_last": {"cycles": ref["field_cycles"], "rel_residual": ref["field_rel_res"]},
         "stage_ms_per_step": {k: round(v, 5) for k, v in st.items()},
         "roofline": {"kernel": kname, "stage": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_, "ms_per_launch": ms},
     }
     if args.pbatch > 0:
