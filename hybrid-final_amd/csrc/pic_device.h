@@ -229,19 +229,47 @@ __host__ __device__ inline int padded_index(int ID, int MCID, int ngy) {
 }
 
 // ---------------------------------------------------------------- argon MCC
+// Cross section of reaction r at one energy (Argon_CrossSection,
+// src/cuda_mccAr.cu:707-777): log-spaced table, linear inside, a decaying
+// tail above it, the first row below it.  The table position depends only on
+// the energy, so it is found once (init) and shared by all the reactions a
+// collision compares; each sigma() is the same arithmetic as a lookup from
+// scratch.
+template <typename R>
+struct ArXs {
+    int mode;       // 0 zero energy, 1 below the table, 2 above it, 3 last row, 4 inside
+    int ee2;
+    R a1, a2;
+    double tail;    // dm_exp(-(lengy - xe[N-1])) above the table
+    __device__ void init(const DevConst<R> &k, R engy) {
+        const int N = k.nlogx;
+        if (engy == 0) { mode = 0; return; }
+        const R lengy = (R)dm_log10((double)engy);
+        const R ee1 = k.idLOGX * (lengy - k.xe[0]);
+        ee2 = (int)ee1;
+        a1 = ee1 - (R)ee2;
+        a2 = 1 - a1;
+        if (lengy < k.xe[0]) mode = 1;
+        else if (lengy > k.xe[N - 1]) { mode = 2; tail = dm_exp(-(double)(lengy - k.xe[N - 1])); }
+        else if (ee2 >= N - 1) mode = 3;
+        else mode = 4;
+    }
+    __device__ R sigma(const DevConst<R> &k, int r) const {
+        const int N = k.nlogx, T = k.TnRct;
+        switch (mode) {
+        case 0: return 0;
+        case 1: return k.cx[r];
+        case 2: return (R)((double)k.cx[(N - 1) * T + r] * 0.1 * tail);
+        case 3: return k.cx[(N - 1) * T + r];
+        default: return a2 * k.cx[ee2 * T + r] + a1 * k.cx[(ee2 + 1) * T + r];
+        }
+    }
+};
 template <typename R>
 __device__ inline R ar_sigma(const DevConst<R> &k, int r, R engy) {
-    if (engy == 0) return 0;
-    const int N = k.nlogx, T = k.TnRct;
-    R lengy = (R)dm_log10((double)engy);
-    R ee1 = k.idLOGX * (lengy - k.xe[0]);
-    int ee2 = (int)ee1;
-    R a1 = ee1 - (R)ee2;
-    R a2 = 1 - a1;
-    if (lengy < k.xe[0]) return k.cx[r];
-    if (lengy > k.xe[N - 1]) return (R)((double)k.cx[(N - 1) * T + r] * 0.1 * dm_exp(-(double)(lengy - k.xe[N - 1])));
-    if (ee2 >= N - 1) return k.cx[(N - 1) * T + r];
-    return a2 * k.cx[ee2 * T + r] + a1 * k.cx[(ee2 + 1) * T + r];
+    ArXs<R> x;
+    x.init(k, engy);
+    return x.sigma(k, r);
 }
 
 #define PIC_TWO_PI 6.28318530717958647692
@@ -327,12 +355,14 @@ __device__ inline int mcc_ar_ion_collide(const DevConst<R> &k, int node, Stream 
     R engy = k.Escale[1] * dum;
     R vel = rsqrt_ieee<R>(dum);
     R R1 = st.u<R>() * k.sigv[2] / vel;
-    R sum = ar_sigma<R>(k, 5, engy);
+    ArXs<R> xs;
+    xs.init(k, engy);
+    R sum = xs.sigma(k, 5);
     if (R1 <= sum) {
         vx = vnx; vy = vny; vz = vnz;
         return 5;
     }
-    if (R1 <= (sum += ar_sigma<R>(k, 6, engy))) {
+    if (R1 <= (sum += xs.sigma(k, 6))) {
         R a = st.u<R>(), b = st.u<R>();
         newvel_ionsc<R>(VX, VY, VZ, vel, a, b);
         vx = VX + vnx; vy = VY + vny; vz = VZ + vnz;
@@ -384,15 +414,17 @@ __device__ inline int mcc_ar_electron_collide(const DevConst<R> &k, int node, R 
     R R2, sum;
     int type = 1, ev = -1;
     const R Es = k.Escale[0];
+    ArXs<R> xs;
+    xs.init(k, engy);   // every comparison below uses this energy
     if (target == 0) {
         R2 = st.u<R>() * k.sigv[0] / vel;
-        sum = ar_sigma<R>(k, 0, engy);
+        sum = xs.sigma(k, 0);
         if (R2 <= sum) { type = 0; ev = 0; }
-        else if (engy > k.Th_e[1] && R2 <= (sum += ar_sigma<R>(k, 1, engy))) {
+        else if (engy > k.Th_e[1] && R2 <= (sum += xs.sigma(k, 1))) {
             type = 0; ev = 1; engy -= k.Th_e[1]; vel = rsqrt_ieee<R>(rabs<R>(engy) / Es);
-        } else if (engy > k.Th_e[2] && R2 <= (sum += ar_sigma<R>(k, 2, engy))) {
+        } else if (engy > k.Th_e[2] && R2 <= (sum += xs.sigma(k, 2))) {
             type = 0; ev = 2; engy -= k.Th_e[2]; vel = rsqrt_ieee<R>(rabs<R>(engy) / Es);
-        } else if (engy > k.Th_e[3] && R2 <= (sum += ar_sigma<R>(k, 3, engy))) {
+        } else if (engy > k.Th_e[3] && R2 <= (sum += xs.sigma(k, 3))) {
             type = 2; ev = 3; engy -= k.Th_e[3];
             R u = st.u<R>();
             rengy = (R)10 * (R)dm_tan((double)(u * (R)dm_atan((double)(engy / (R)20))));
@@ -402,7 +434,7 @@ __device__ inline int mcc_ar_electron_collide(const DevConst<R> &k, int node, R 
         }
     } else {
         R2 = st.u<R>() * k.sigv[1] / vel;
-        sum = ar_sigma<R>(k, 4, engy);
+        sum = xs.sigma(k, 4);
         if (engy > k.Th_e[4] && R2 <= sum) {
             type = 2; ev = 4; engy -= k.Th_e[4];
             R u = st.u<R>();
