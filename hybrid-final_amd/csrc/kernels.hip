@@ -456,14 +456,15 @@ __global__ __launch_bounds__(256) void k_mcc_insert(DevConst<R> k, PView<R> src,
 // destination nodes are loaded once per wave and fetched per particle with a
 // lane shuffle.
 constexpr int FCHUNK = 256;
+constexpr int SCHUNK = 256;   // pass 2 chunk (larger chunks measured faster: more loads in flight)
 constexpr int FWAVES = 4;   // waves per 256-thread block
 enum { KIND_AR_ELECTRON = 0, KIND_AR_ION = 1 };
 enum { FLAG_CODE = 15, FLAG_DONE = 16, FLAG_PROD = 32 };
 
-template <typename R>
+template <typename R, int C = FCHUNK>
 struct FStage {
-    R x[FCHUNK], y[FCHUNK], vx[FCHUNK], vy[FCHUNK], vz[FCHUNK];
-    uint32_t id[FCHUNK];
+    R x[C], y[C], vx[C], vy[C], vz[C];
+    uint32_t id[C];
 };
 
 // Orders the LDS accesses of one wavefront (the hardware already executes a
@@ -489,17 +490,17 @@ struct FPool {
 // global -> LDS for particles [b0, b0+n), n >= 1.  Lanes past the end load the
 // last particle again (same cache line, never read back), so the 6*FCHUNK/64
 // loads of a lane form one straight-line block the compiler can batch.
-template <typename R>
-__device__ inline void stage_chunk(FStage<R> &S, const PView<R> &p, int b0, int n, int lane) {
-    R tx[FCHUNK / 64], ty[FCHUNK / 64], tvx[FCHUNK / 64], tvy[FCHUNK / 64], tvz[FCHUNK / 64];
-    uint32_t tid[FCHUNK / 64];
+template <typename R, int C>
+__device__ inline void stage_chunk(FStage<R, C> &S, const PView<R> &p, int b0, int n, int lane) {
+    R tx[C / 64], ty[C / 64], tvx[C / 64], tvy[C / 64], tvz[C / 64];
+    uint32_t tid[C / 64];
 #pragma unroll
-    for (int j = 0; j < FCHUNK / 64; ++j) {
+    for (int j = 0; j < C / 64; ++j) {
         int i = b0 + min(j * 64 + lane, n - 1);
         tx[j] = p.x[i]; ty[j] = p.y[i]; tvx[j] = p.vx[i]; tvy[j] = p.vy[i]; tvz[j] = p.vz[i]; tid[j] = p.id[i];
     }
 #pragma unroll
-    for (int j = 0; j < FCHUNK / 64; ++j) {
+    for (int j = 0; j < C / 64; ++j) {
         int i = j * 64 + lane;
         S.x[i] = tx[j]; S.y[i] = ty[j]; S.vx[i] = tvx[j]; S.vy[i] = tvy[j]; S.vz[i] = tvz[j]; S.id[i] = tid[j];
     }
@@ -587,7 +588,7 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
         const int n = min(FCHUNK, k1 - b0);
         if (lane == 0) pool.b0[wv] = b0;
         if (n > 0) {
-            stage_chunk<R>(S, p, b0, n, lane);
+            stage_chunk<R, FCHUNK>(S, p, b0, n, lane);
             for (int j = 0; j * 64 < n; ++j) {
                 const int i = j * 64 + lane;
                 int slot = -1, jc = -1;
@@ -676,15 +677,15 @@ __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p
                                                        const int *base, const int *pbase_e, const int *pbase_i,
                                                        const int *pid_off, const uint32_t *next_id, int cap,
                                                        int cap_ion, int *err) {
-    __shared__ FStage<R> stage_all[FWAVES];
-    __shared__ uint8_t flag_all[FWAVES][FCHUNK];
+    __shared__ FStage<R, SCHUNK> stage_all[FWAVES];
+    __shared__ uint8_t flag_all[FWAVES][SCHUNK];
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
     int k0, k1;
     cell_range(p, c, k0, k1);
     if (k0 == k1) return;
-    FStage<R> &S = stage_all[threadIdx.x >> 6];
+    FStage<R, SCHUNK> &S = stage_all[threadIdx.x >> 6];
     uint8_t *F = flag_all[threadIdx.x >> 6];
     CellE<R> e;
     e.load(k.Ex, k.Ey, c, k.ngy);
@@ -696,15 +697,15 @@ __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p
     int run[ROUTE_SLOTS], prun[ROUTE_SLOTS];
 #pragma unroll
     for (int s = 0; s < ROUTE_SLOTS; ++s) { run[s] = 0; prun[s] = 0; }
-    for (int b0 = k0; b0 < k1; b0 += FCHUNK) {
-        const int n = min(FCHUNK, k1 - b0);
+    for (int b0 = k0; b0 < k1; b0 += SCHUNK) {
+        const int n = min(SCHUNK, k1 - b0);
         {
-            uint8_t tf[FCHUNK / 64];
+            uint8_t tf[SCHUNK / 64];
 #pragma unroll
-            for (int j = 0; j < FCHUNK / 64; ++j) tf[j] = flag[b0 + min(j * 64 + lane, n - 1)];
-            stage_chunk<R>(S, p, b0, n, lane);
+            for (int j = 0; j < SCHUNK / 64; ++j) tf[j] = flag[b0 + min(j * 64 + lane, n - 1)];
+            stage_chunk<R, SCHUNK>(S, p, b0, n, lane);
 #pragma unroll
-            for (int j = 0; j < FCHUNK / 64; ++j) F[j * 64 + lane] = tf[j];
+            for (int j = 0; j < SCHUNK / 64; ++j) F[j * 64 + lane] = tf[j];
         }
         for (int j = 0; j * 64 < n; ++j) {
             const int i = j * 64 + lane;
