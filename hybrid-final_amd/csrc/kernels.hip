@@ -727,7 +727,11 @@ __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p
                 const unsigned long long m = __ballot(slot == s);
                 if (slot == s) pos = run[s] + __popcll(m & lt);
                 run[s] += __popcll(m);
-                if (KIND == KIND_AR_ELECTRON) {
+            }
+            // product ranks: most groups of 64 hold no ionisation pair
+            if (KIND == KIND_AR_ELECTRON && __ballot(prod)) {
+#pragma unroll
+                for (int s = 0; s < ROUTE_SLOTS; ++s) {
                     const unsigned long long pm = __ballot(prod && slot == s);
                     if (prod && slot == s) prank = prun[s] + __popcll(pm & lt);
                     prun[s] += __popcll(pm);
