@@ -141,7 +141,7 @@ struct Case : CaseBase {
     int *pstart = nullptr, *scan_tmp = nullptr, *cellp = nullptr;
     uint32_t *next_id = nullptr;
     unsigned long long *mccr = nullptr;
-    int *condcharge = nullptr, *d_err = nullptr;
+    int *condcharge = nullptr, *d_err = nullptr, *lost = nullptr;
     double *sum_den = nullptr, *sum_phi = nullptr, *sum_ex = nullptr, *sum_ey = nullptr;
     long long n_accum = 0;
     long long tstep = 0;
@@ -185,7 +185,7 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey, total2, alive2, ent_dest, ent_pos, clk, eedf};
+                        sum_ey, total2, alive2, ent_dest, ent_pos, clk, eedf, lost};
         for (void *p : ptrs)
             if (p) hipFree(p);
         for (int s = 0; s < MAXSP; ++s) {
@@ -445,6 +445,8 @@ struct Case : CaseBase {
         HIPCHK(hipMemsetAsync(mccr, 0, (size_t)G * S.TnRct * sizeof(unsigned long long), st));
         HIPCHK(dalloc(&condcharge, nsp * S.CondNUMR));
         HIPCHK(hipMemsetAsync(condcharge, 0, nsp * S.CondNUMR * sizeof(int), st));
+        HIPCHK(dalloc(&lost, nsp));
+        HIPCHK(hipMemsetAsync(lost, 0, nsp * sizeof(int), st));
         HIPCHK(dalloc(&d_err, 1));
         HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int), st));
         HIPCHK(dalloc(&clk, 1));
@@ -488,7 +490,7 @@ struct Case : CaseBase {
         for (int r = 0; r < 16; ++r) kc.Th_e[r] = S.Th_e[r];
         kc.seed = (uint64_t)d.rng_seed;
         kc.Ex = Ex; kc.Ey = Ey; kc.P1 = P1; kc.P2 = P2; kc.Pion = Pion; kc.BackVel1 = BackVel1;
-        kc.vsave = vsave; kc.xe = xe; kc.cx = cx; kc.StructIdx = StructIdx; kc.route = route;
+        kc.vsave = vsave; kc.xe = xe; kc.cx = cx; kc.StructIdx = StructIdx; kc.route = route; kc.lost = lost;
         nc.Gsize = G; nc.ngx = S.ngx; nc.ngy = S.ngy; nc.nsp = nsp; nc.CondNUMR = S.CondNUMR;
         nc.hdx = S.hdx; nc.hdy = S.hdy; nc.idx = S.idx; nc.idy = S.idy; nc.idxy = S.idxy;
         nc.hxy = (R)(0.5 * (double)S.dx / (double)S.dy);
@@ -869,6 +871,7 @@ struct Case : CaseBase {
         if (n == "V_cond") return D(V, S.CondNUMR, rs, 1);
         if (n == "mccr") return D(mccr, (size_t)G * S.TnRct, 8, 0);
         if (n == "CondCharge") return D(condcharge, (size_t)nsp * S.CondNUMR, 4, 0);
+        if (n == "Lost") return D(lost, nsp, 4, 0);
         if (n == "sum_den") return D(sum_den, (size_t)nsp * G, 8, 1);
         if (n == "sum_phi") return D(sum_phi, G, 8, 1);
         if (n == "sum_ex") return D(sum_ex, G, 8, 1);
@@ -1190,7 +1193,7 @@ int pic_set_array(pic_ctx *ctx, const char *name, const void *host, size_t bytes
 }
 const char *pic_array_names(void) {
     return "Ex Ey phi TotPot Sigma Source den sigma_sp b xsol V_cond mccr CondCharge sum_den sum_phi sum_ex sum_ey "
-           "eedf next_id Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai Aj Aval "
+           "eedf Lost next_id Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai Aj Aval "
            "Atemp Mdiag cond_b temp_b lap_phi BackDen1 BackVel1 FluidN xe xee cx sigv P1 P2 Pion vsave";
 }
 int pic_set_step(pic_ctx *ctx, long long tstep) { CTX_OR_FAIL(ctx); return ctx->c->set_step(tstep); }

@@ -100,6 +100,8 @@ __device__ inline void wall_effects(const DevConst<R> &k, const Route &rt, int i
                                     R vy, int *condcharge, R *sigma_sp) {
     if (rt.dest == -2) {
         atomicAdd(&condcharge[isp * k.CondNUMR + rt.info], 1);
+    } else if (rt.dest == -1) {
+        atomicAdd(&k.lost[isp], 1);
     } else if (rt.dest == -3) {
         int mcid = code_mcid(code, k.ngy);
         int sm = padded_index(c, mcid, k.ngy);

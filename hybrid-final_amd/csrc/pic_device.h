@@ -128,6 +128,7 @@ struct DevConst {
     const R *P1, *P2, *Pion, *BackVel1, *vsave, *xe, *cx;
     const int *StructIdx;
     const Route *route;        // Csize-independent: indexed by node cell * 9 + code
+    int *lost;                 // per species: particles deleted at a wall (no conductor or dielectric there)
 };
 
 // ---------------------------------------------------------------- push
@@ -165,8 +166,10 @@ __device__ inline int push_particle(R ews, R ewn, R ees, R een, R yws, R ywn, R 
 
 template <typename R>
 __device__ inline void apply_reflect(int info, R &x, R &y, R &vx, R &vy) {
-    if (info & 16) x = 1 - x;
-    if (info & 32) y = 1 - y;
+    // a mirrored offset of 0 (a particle clamped onto the wall) would be 1:
+    // keep it inside the cell like every other offset
+    if (info & 16) { x = 1 - x; if (x >= 1) x = one_minus_ulp<R>(); }
+    if (info & 32) { y = 1 - y; if (y >= 1) y = one_minus_ulp<R>(); }
     if (info & 64) vx = -vx;
     if (info & 128) vy = -vy;
 }

@@ -131,7 +131,7 @@ def _ptr(a):
 
 
 _INT_ARRAYS = {"Boundary", "CondID", "Face", "DensRegion", "PlasmaRegion", "StructIdx", "A_idx", "Ai", "Aj",
-               "CondCharge", "next_id"}
+               "CondCharge", "next_id", "Lost"}
 
 
 def _dtype(name, elsize, is_float):

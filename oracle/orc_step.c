@@ -41,6 +41,7 @@ orc_case *orc_create(const orc_deck *d, int *err) {
     c->xsol = zalloc(c->A_size, sizeof(double));
     c->V_cond = zalloc(c->CondNUMR, sizeof(real));
     c->CondCharge = zalloc((size_t)c->nsp * c->CondNUMR, sizeof(real));
+    c->Lost = zalloc((size_t)c->nsp, sizeof(real));
     c->mccr = zalloc((size_t)G * c->TnRct, sizeof(int64_t));
     c->sum_den = zalloc((size_t)c->nsp * G, sizeof(double));
     c->sum_phi = zalloc(G, sizeof(double));
@@ -350,11 +351,12 @@ static int resolve_crosser(orc_case *c, int isp, int ID, int MCID, real *px, rea
     case -16: sum = 1 - ngy; fx = 0; fy = 0; fvx = -1; fvy = -1; break;
     case -20: sum = ngy - 1; fx = 0; fy = 0; fvx = -1; fvy = -1; break;
     case -24: sum = -ngy - 1; fx = 0; fy = 0; fvx = -1; fvy = -1; break;
-    default: return 0;   /* -1 Dirichlet, and undefined codes */
+    default: c->Lost[isp] += 1; return 0;   /* -1 Dirichlet, and undefined codes: deleted */
     }
     *dest = CID + sum;
-    if (!fx) *px = 1 - X;
-    if (!fy) *py = 1 - Y;
+    /* a mirrored offset of 0 (clamped onto the wall) would be 1: keep it inside */
+    if (!fx) { *px = 1 - X; if (*px >= 1) *px = ORC_ONE_MINUS; }
+    if (!fy) { *py = 1 - Y; if (*py >= 1) *py = ORC_ONE_MINUS; }
     *pvx = fvx * VX;
     *pvy = fvy * VY;
     return 1;
@@ -747,7 +749,7 @@ static int lookup(orc_case *c, const char *name, orc_arr *out) {
     ENT("Sigma", c->Sigma, G, rs) ENT("Source", c->Source, G, rs) ENT("Ex", c->Ex, G, rs) ENT("Ey", c->Ey, G, rs)
     ENT("den", c->den, (long)c->nsp * G, rs) ENT("sigma_sp", c->sigma_sp, (long)c->nsp * G, rs)
     ENT("b", c->b, c->A_size, rs) ENT("xsol", c->xsol, c->A_size, 8) ENT("V_cond", c->V_cond, c->CondNUMR, rs)
-    ENT("CondCharge", c->CondCharge, (long)c->nsp * c->CondNUMR, rs) ENT("mccr", c->mccr, G * c->TnRct, 8)
+    ENT("CondCharge", c->CondCharge, (long)c->nsp * c->CondNUMR, rs) ENT("Lost", c->Lost, c->nsp, rs) ENT("mccr", c->mccr, G * c->TnRct, 8)
     ENT("sum_den", c->sum_den, (long)c->nsp * G, 8) ENT("sum_phi", c->sum_phi, G, 8)
     ENT("sum_ex", c->sum_ex, G, 8) ENT("sum_ey", c->sum_ey, G, 8) ENT("lu", c->lu, (long)c->A_size * (2 * c->lu_bw + 1), 8)
     if (!strncmp(name, "sp", 2) && name[2] >= '0' && name[2] <= '9' && name[3] == '.') {

@@ -129,6 +129,7 @@ typedef struct {
     double *xsol;     /* last solution (warm start / diagnostics) */
     real *V_cond;     /* conductor voltages this step */
     real *CondCharge; /* nsp x CondNUMR */
+    real *Lost;       /* nsp: particles deleted at a wall with no conductor or dielectric */
     int64_t *mccr;    /* Gsize x TnRct event counts, cumulative */
     /* profile accumulators (FP64, src/cuda_Diagnostic.cu:505-547) */
     double *sum_den, *sum_phi, *sum_ex, *sum_ey;

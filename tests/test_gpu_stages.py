@@ -74,6 +74,9 @@ def test_stage_by_stage(prec, pn):
         assert orc.orc_sort() == 0
         for s in range(2):
             same_particles(case, orc, s, f"sort step {step}")
+        # particles absorbed by conductors and deleted at walls, per species
+        for nm in ("CondCharge", "Lost"):
+            assert np.array_equal(case.array(nm).astype(np.int64), orc.arr(nm).astype(np.int64)), f"{nm} differs"
         case.mcc()
         assert orc.orc_mcc() == 0
         for s in range(2):

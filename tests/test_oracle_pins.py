@@ -237,6 +237,27 @@ def test_push_moves_at_most_one_cell_per_axis():
         assert abs(s["y"][0] - 0.2) < 1e-6 and s["y"][1] == 0.5
 
 
+def test_mirrored_offset_stays_inside_the_cell():
+    """A fast particle driven into a Neumann wall is clamped onto the wall
+    (offset 0) and mirrored; the mirrored offset is kept below 1."""
+    d = decks.small(ncx=8, ncy=8)
+    for prec, top in (("f64", np.nextafter(1.0, 0.0)), ("f32", np.nextafter(np.float32(1), np.float32(0)))):
+        o = Oracle(d, prec)
+        ngy, G = 9, 81
+        o.arr("Ex")[:] = 0.0
+        o.arr("Ey")[:] = 0.0
+        cell = 0 * ngy + 4          # first column: the x = 0 wall is Neumann
+        dtdx = o.scalar("dt") / o.scalar("dx")
+        xs = np.array([0.5]); ys = np.array([0.5])
+        vx = np.array([-1.7]) / dtdx; vy = np.zeros(1); vz = np.zeros(1)
+        o.set_species(0, _single_cell_state(o, ngy, cell, xs, ys, vx, vy, vz, G))
+        o.set_species(1, _single_cell_state(o, ngy, cell, xs[:0], ys[:0], vx[:0], vy[:0], vz[:0], G))
+        o.orc_move()
+        assert o.orc_sort() == 0
+        s = o.species(0)
+        assert len(s["x"]) == 1 and s["x"][0] == top and s["vx"][0] > 0 and s["y"][0] == 0.5
+
+
 def test_deposit_weights_sum_to_particle_count():
     d = decks.small(ncx=8, ncy=8, ppc=30)
     o = Oracle(d, "f64")
@@ -373,3 +394,25 @@ def test_poisson_oracle_operator_is_the_field_operator():
     B = operator(nx, ny, (0, 0, 1, 1)).toarray()
     scale = A[0, 0] / B[0, 0]
     np.testing.assert_allclose(A, scale * B, rtol=0, atol=1e-12 * abs(scale))
+
+
+@pytest.mark.parametrize("pn", ["f64", "f32"])
+def test_particle_balance_closes(pn):
+    """Every particle that appears or disappears is counted: after any
+    number of steps N_s = N_s(0) + ionisations - absorbed_s - deleted_s for
+    both species (each ionisation adds one electron and one ion)."""
+    d = decks.ccp_deck(ncx=32, ncy=24, dx=5e-4, volts=250.0, pres_torr=0.3, ppc=30, steps_per_cycle=400,
+                       cycles=1, seed=11, philox_seed=22)
+    o = Oracle(d, pn)
+    o.orc_deposit()
+    n0 = [int(o.scalar("np0")), int(o.scalar("np1"))]
+    assert o.orc_run(60) == 0
+    n1 = [int(o.scalar("np0")), int(o.scalar("np1"))]
+    m = o.arr("mccr").reshape(-1, int(o.scalar("TnRct"))).astype(np.int64)
+    pairs = int(m[:, 3].sum() + m[:, 4].sum())
+    absorbed = o.arr("CondCharge").reshape(2, -1).sum(axis=1)
+    lost = o.arr("Lost")
+    assert pairs > 0 and lost[0] > 0   # the case ionises and loses electrons at the walls
+    for s in range(2):
+        assert n1[s] == n0[s] + pairs - int(absorbed[s]) - int(lost[s])
+
