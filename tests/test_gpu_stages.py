@@ -33,9 +33,16 @@ def deck_small():
     return decks.small(ncx=16, ncy=12, ppc=40, volts=150.0)
 
 
+def deck_dense():
+    """More than 256 particles per cell: several LDS chunks per cell in both
+    passes of the fused step."""
+    return decks.small(ncx=8, ncy=6, ppc=420, volts=200.0, pres_torr=0.3)
+
+
+@pytest.mark.parametrize("deck", [deck_small, deck_dense], ids=["small", "dense"])
 @pytest.mark.parametrize("prec,pn", PRECS)
-def test_stage_by_stage(prec, pn):
-    d = deck_small()
+def test_stage_by_stage(prec, pn, deck):
+    d = deck()
     case = Case(d, prec)
     orc = Oracle(d, pn)
     for s in range(2):
@@ -91,10 +98,11 @@ def test_stage_by_stage(prec, pn):
     case.sync()
 
 
+@pytest.mark.parametrize("deck", [deck_small, deck_dense], ids=["small", "dense"])
 @pytest.mark.parametrize("prec,pn", PRECS)
-def test_fused_step_equals_staged(prec, pn):
+def test_fused_step_equals_staged(prec, pn, deck):
     """pic_step (fused two-pass) and the staged stage functions give the same bits."""
-    d = deck_small()
+    d = deck()
     a = Case(d, prec)
     b = Case(d, prec)
     for step in range(1, 6):
