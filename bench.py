@@ -227,6 +227,12 @@ def main():
         "refined_over_plain": value / plain_value if plain_value > 0 else None,
         "field_solve_last": {"cycles": ref["field_cycles"], "rel_residual": ref["field_rel_res"]},
         "stage_ms_per_step": {k: round(v, 5) for k, v in st.items()},
+        # algorithmic HBM bytes of each particle stage over its time (particle
+        # records only: 24 B read by a count pass, 24 B read + 24 B written by
+        # a scatter, x and y of both species by the deposit)
+        "stage_hbm_gbs": {k: round(b / (st[k] * 1e-3) / 1e9, 1) for k, b in
+                          (("count_e", 24.0 * ne), ("count_i", 24.0 * ni), ("scatter_e", 48.0 * ne),
+                           ("scatter_i", 48.0 * ni), ("deposit", 8.0 * (ne + ni))) if st.get(k, 0) > 0},
         "roofline": {"kernel": kname, "stage": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_, "ms_per_launch": ms},
