@@ -161,7 +161,7 @@ struct Case : CaseBase {
     double t_ms[NT] = {0};
     long long t_n = 0;
     void mark(int k) {
-        if (timing) hipEventRecord(ev[ev_slot][k], st);
+        if (timing) (void)hipEventRecord(ev[ev_slot][k], st);
     }
     int flush_slot(int sl) {
         if (!ev_pending[sl]) return 0;
@@ -187,30 +187,30 @@ struct Case : CaseBase {
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
                         sum_ey, total2, alive2, ent_dest, ent_pos, clk, eedf, lost};
         for (void *p : ptrs)
-            if (p) hipFree(p);
+            if (p) (void)hipFree(p);
         for (int s = 0; s < MAXSP; ++s) {
             PView<R> *vs[2] = {&cur[s], &nxt[s]};
             for (PView<R> *v : vs) {
                 void *q[] = {v->x, v->y, v->vx, v->vy, v->vz, v->id, v->cell_start};
                 for (void *p : q)
-                    if (p) hipFree(p);
+                    if (p) (void)hipFree(p);
             }
-            if (tag[s]) hipFree(tag[s]);
-            if (key[s]) hipFree(key[s]);
-            if (cnt[s]) hipFree(cnt[s]);
-            if (base[s]) hipFree(base[s]);
-            if (pbase[s]) hipFree(pbase[s]);
+            if (tag[s]) (void)hipFree(tag[s]);
+            if (key[s]) (void)hipFree(key[s]);
+            if (cnt[s]) (void)hipFree(cnt[s]);
+            if (base[s]) (void)hipFree(base[s]);
+            if (pbase[s]) (void)hipFree(pbase[s]);
         }
         pcg_work_destroy(pw);
         for (int q = 0; q < 2; ++q)
-            if (gexec[q]) hipGraphExecDestroy(gexec[q]);
+            if (gexec[q]) (void)hipGraphExecDestroy(gexec[q]);
         for (int r = 0; r < RING; ++r)
             for (int k = 0; k <= NT; ++k)
-                if (ev[r][k]) hipEventDestroy(ev[r][k]);
-        for (double *p : spec_bufs) hipFree(p);
-        if (spec_state) hipFree(spec_state);
-        if (spec_part) hipFree(spec_part);
-        if (st) hipStreamDestroy(st);
+                if (ev[r][k]) (void)hipEventDestroy(ev[r][k]);
+        for (double *p : spec_bufs) (void)hipFree(p);
+        if (spec_state) (void)hipFree(spec_state);
+        if (spec_part) (void)hipFree(spec_part);
+        if (st) (void)hipStreamDestroy(st);
     }
 
     // ------------------------------------------------------------ routing tables
@@ -546,8 +546,8 @@ struct Case : CaseBase {
                                    spec_state, spec_part, d_res, st);
         } else {
             launch_rhs_to_fp64<R>(N, brhs, b64, st);
-            if (precision == PIC_FP64_PLAIN) pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st);
-            else pcg_solve_refined(op, b64, x64, tol, inner_tol, pcg_maxit, pw, d_res, st);
+            if (precision == PIC_FP64_PLAIN) HIPCHK(pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st));
+            else HIPCHK(pcg_solve_refined(op, b64, x64, tol, inner_tol, pcg_maxit, pw, d_res, st));
         }
         HIPCHK(hipGetLastError());
         return 0;
@@ -706,12 +706,12 @@ struct Case : CaseBase {
         // the capture flipped the host parity without running: flip back
         for (int s = 0; s < nsp; ++s) swap_species(s);
         if (e || ce != hipSuccess) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             if (e) return e;
             HIPCHK(ce);
         }
         const hipError_t ie = hipGraphInstantiate(&gexec[q], g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         HIPCHK(ie);
         gkey[q] = key;
         out = gexec[q];
@@ -784,11 +784,12 @@ struct Case : CaseBase {
         if (q == hipErrorNotReady) return 0;
         return fail(-1, "HIP error %d (%s) at stream query", (int)q, hipGetErrorString(q));
     }
-    long long np_of(int s) {
+    int np_of(int s, long long &n) {
         int v = 0;
-        hipMemcpyAsync(&v, cur[s].cell_start + G, sizeof(int), hipMemcpyDeviceToHost, st);
-        hipStreamSynchronize(st);
-        return std::min<long long>(v, cur[s].cap);   // after an overflow only cap slots exist
+        HIPCHK(hipMemcpyAsync(&v, cur[s].cell_start + G, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        n = std::min<long long>(v, cur[s].cap);   // after an overflow only cap slots exist
+        return 0;
     }
     int info(pic_info *o) override {
         HIPCHK(hipStreamSynchronize(st));
@@ -797,7 +798,10 @@ struct Case : CaseBase {
         o->dt = (double)S.dt; o->dx = (double)S.dx; o->dy = (double)S.dy; o->t = t; o->tstep = tstep;
         o->precision = precision;
         for (int s = 0; s < 5; ++s) {
-            o->np[s] = s < nsp ? np_of(s) : 0;
+            long long n = 0;
+            if (s < nsp)
+                if (int e = np_of(s, n)) return e;
+            o->np[s] = n;
             o->cap[s] = s < nsp ? cur[s].cap : 0;
         }
         return 0;
@@ -805,7 +809,8 @@ struct Case : CaseBase {
     int get_particles(int s, void *x, void *y, void *vx, void *vy, void *vz, uint32_t *id, int32_t *cs) override {
         if (s < 0 || s >= nsp) return fail(-3, "bad species %d", s);
         HIPCHK(hipStreamSynchronize(st));
-        long long n = np_of(s);
+        long long n = 0;
+        if (int e = np_of(s, n)) return e;
         const size_t b = (size_t)n * sizeof(R);
         if (x) HIPCHK(hipMemcpyAsync(x, cur[s].x, b, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1297,7 +1302,7 @@ void pic_shim_close(void) {
 void info_Device(void) {
     int dev = 0;
     hipDeviceProp_t p;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     if (hipGetDeviceProperties(&p, dev) == hipSuccess)
         printf(" Device %d: %s, %d CUs, %.1f GB\n", dev, p.name, p.multiProcessorCount, p.totalGlobalMem / 1e9);
 }

@@ -281,12 +281,17 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
 PcgWork *pcg_work_create(int N, int *err, int want_blocks, hipStream_t stream) {
     PcgWork *w = new PcgWork();
     w->N = N;
-    int dev = 0, ncu = 0, per_cu = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_pcg_refined), PCG_BLOCK, 0);
-    int per_cu2 = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, reinterpret_cast<const void *>(k_pcg_plain), PCG_BLOCK, 0);
+    int dev = 0, ncu = 0, per_cu = 0, per_cu2 = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_pcg_refined),
+                                                     PCG_BLOCK, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, reinterpret_cast<const void *>(k_pcg_plain), PCG_BLOCK,
+                                                     0) != hipSuccess) {
+        if (err) *err = -40;
+        delete w;
+        return nullptr;
+    }
     if (per_cu2 < per_cu) per_cu = per_cu2;
     int resident = ncu * (per_cu > 0 ? per_cu : 1);
     int need = (N + PCG_BLOCK - 1) / PCG_BLOCK;
@@ -330,28 +335,29 @@ PcgWork *pcg_work_create(int N, int *err, int want_blocks, hipStream_t stream) {
 
 void pcg_work_destroy(PcgWork *w) {
     if (!w) return;
-    hipFree(w->r); hipFree(w->z); hipFree(w->p); hipFree(w->q);
-    hipFree(w->rf); hipFree(w->zf); hipFree(w->pf); hipFree(w->qf); hipFree(w->df);
-    hipFree(w->partials); hipFree(w->gs);
+    void *bufs[] = {w->r, w->z, w->p, w->q, w->rf, w->zf, w->pf, w->qf, w->df, w->partials, w->gs};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
     delete w;
 }
 
 int pcg_grid_blocks(PcgWork *w) { return w ? w->nblk : 0; }
 
-void pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, double tol, int maxit, PcgWork *w,
-                          PcgResult *res, hipStream_t s) {
+hipError_t pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, double tol, int maxit, PcgWork *w,
+                                PcgResult *res, hipStream_t s) {
     void *args[] = {(void *)&A, (void *)&b, (void *)&x, (void *)&w->r, (void *)&w->z, (void *)&w->p, (void *)&w->q,
                     (void *)&w->partials, (void *)&w->gs, (void *)&tol, (void *)&maxit, (void *)&res};
-    hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_plain), dim3(w->nblk), dim3(PCG_BLOCK), args, 0, s);
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_plain), dim3(w->nblk), dim3(PCG_BLOCK), args,
+                                      0, s);
 }
 
-void pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol, int maxit,
-                       PcgWork *w, PcgResult *res, hipStream_t s) {
+hipError_t pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol,
+                             int maxit, PcgWork *w, PcgResult *res, hipStream_t s) {
     void *args[] = {(void *)&A, (void *)&b, (void *)&x, (void *)&w->r, (void *)&w->rf, (void *)&w->zf,
                     (void *)&w->pf, (void *)&w->qf, (void *)&w->df, (void *)&w->partials, (void *)&w->gs,
                     (void *)&tol, (void *)&inner_tol, (void *)&maxit, (void *)&res};
-    hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_refined), dim3(w->nblk), dim3(PCG_BLOCK), args, 0,
-                               s);
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_refined), dim3(w->nblk), dim3(PCG_BLOCK),
+                                      args, 0, s);
 }
 
 }  // namespace pic

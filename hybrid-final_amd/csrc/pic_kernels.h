@@ -149,10 +149,10 @@ int pcg_grid_blocks(PcgWork *);
 
 // x (FP64, warm start, in/out), b (FP64).  Writes the result to *res (device
 // memory) so the solve can run inside a captured graph.
-void pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, double tol, int maxit, PcgWork *w,
-                          PcgResult *res_dev, hipStream_t s);
-void pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol, int maxit,
-                       PcgWork *w, PcgResult *res_dev, hipStream_t s);
+hipError_t pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, double tol, int maxit, PcgWork *w,
+                                PcgResult *res, hipStream_t s);
+hipError_t pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol,
+                             int maxit, PcgWork *w, PcgResult *res, hipStream_t s);
 
 // ---- separable fast solve (spectral.hip), FP64 throughout
 struct SpecDev {

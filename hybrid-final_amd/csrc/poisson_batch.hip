@@ -262,8 +262,8 @@ static int pb_destroy(pic_pbatch *B) {
         const hipError_t e = hipStreamSynchronize(B->st);
         if (e != hipSuccess) fprintf(stderr, "picmcc: Poisson batch teardown: %s\n", hipGetErrorString(e));
     }
-    for (void *p : B->owned) hipFree(p);
-    if (B->st) hipStreamDestroy(B->st);
+    for (void *p : B->owned) (void)hipFree(p);
+    if (B->st) (void)hipStreamDestroy(B->st);
     delete B;
     return 0;
 }
