@@ -827,6 +827,37 @@ double orc_detmath(int fn, double x, double y) {
     }
 }
 
+/* Test hook: one call of a collision kinematics routine of the MCC above,
+ * with its random numbers given.  which = 0 anewvel {E, vel, ux, uy, uz, r1,
+ * r2} -> {vx, vy, vz}; 1 newvel_ionsc {vx, vy, vz, vel, r1, r2} -> {vx, vy,
+ * vz}; 2 dmaxwellv {vsave_n, vti, Rphi, Rthe} -> {vx, vy, vz}; 3 the
+ * ionisation energy sharing {E, u} -> {secondary energy}. */
+int orc_kinematics(int which, const double *in, double *out) {
+    real a = 0, b = 0, cc = 0;
+    switch (which) {
+    case 0:
+        a = (real)in[2]; b = (real)in[3]; cc = (real)in[4];
+        anewvel((real)in[0], (real)in[1], &a, &b, &cc, (real)in[5], (real)in[6]);
+        break;
+    case 1:
+        a = (real)in[0]; b = (real)in[1]; cc = (real)in[2];
+        newvel_ionsc(&a, &b, &cc, (real)in[3], (real)in[4], (real)in[5]);
+        break;
+    case 2:
+        dmaxwellv(&a, &b, &cc, (real)in[0], (real)in[1], (real)in[2], (real)in[3]);
+        break;
+    case 3: {
+        real engy = (real)in[0], u = (real)in[1];
+        a = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+        break;
+    }
+    default:
+        return -1;
+    }
+    out[0] = (double)a; out[1] = (double)b; out[2] = (double)cc;
+    return 0;
+}
+
 /* n draws of the Park-Miller generator from seed (src/Particle.c:211-226) */
 void orc_lcg_seq(long seed, int n, long *out) {
     long a = 16807, m = 2147483647, q = 127773, r = 2836;

@@ -16,7 +16,14 @@
  *      collision products in parent order (parent species ascending).
  * Everything else follows the reference; each function cites the lines it
  * restates.  Reference defects that are undefined behaviour are resolved as
- * noted at the function; the others are preserved.
+ * noted at the function.  Three more are fixed, as on the GPU path: an
+ * offset left outside its cell by a move of more than one cell is clamped
+ * into it, a mirrored offset of 1 is kept below 1, and particles deleted at
+ * walls are counted (Lost).  The others are preserved.
+ *
+ * How each part is pinned (reference-built fixtures, known answers, closed
+ * forms, statistics of the collision channels) is listed with the tests in
+ * tests/test_oracle_pins.py and tests/test_setup_parity.py.
  */
 #ifndef PIC_ORACLE_H
 #define PIC_ORACLE_H

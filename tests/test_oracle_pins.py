@@ -416,3 +416,165 @@ def test_particle_balance_closes(pn):
     for s in range(2):
         assert n1[s] == n0[s] + pairs - int(absorbed[s]) - int(lost[s])
 
+
+
+# ---------------------------------------------------------------- collision kinematics and channels
+def _kin(o, which, vals):
+    L = o.lib
+    L.orc_kinematics.restype = C.c_int
+    L.orc_kinematics.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    inp = (C.c_double * len(vals))(*vals)
+    out = (C.c_double * 3)()
+    assert L.orc_kinematics(which, inp, out) == 0
+    return np.array(out[:])
+
+
+@pytest.mark.parametrize("pn,rtol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_electron_scattering_angle_closed_form(pn, rtol):
+    """anewvel keeps the speed, turns the velocity by cos(chi) =
+    (2 + E - 2 (1+E)^R1) / E about the incident direction and spreads the
+    azimuth over 2 pi R2 (R2 and R2 + 1/2 give opposite transverse parts);
+    a zero-energy electron keeps its direction."""
+    o = Oracle(decks.small(ncx=4, ncy=4), pn)
+    rng = np.random.default_rng(5)
+    dirs = [np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0])]
+    for _ in range(6):
+        u = rng.normal(size=3)
+        dirs.append(u / np.linalg.norm(u))
+    for E in (0.3, 4.0, 60.0):
+        for u in dirs:
+            for r1 in (0.05, 0.5, 0.93):
+                vel = 1.7e6
+                a = _kin(o, 0, [E, vel, *u, r1, 0.2])
+                b = _kin(o, 0, [E, vel, *u, r1, 0.7])
+                assert abs(np.linalg.norm(a) - vel) <= rtol * vel * 10
+                cos_chi = (2 + E - 2 * (1 + E) ** r1) / E
+                assert abs(a @ u / vel - cos_chi) <= rtol * 50
+                ta, tb = a - (a @ u) * u, b - (b @ u) * u
+                assert np.allclose(ta, -tb, atol=rtol * 50 * vel)
+    u = dirs[1]
+    assert np.allclose(_kin(o, 0, [0.0, 2.0, *u, 0.3, 0.4]), 2.0 * u)
+
+
+@pytest.mark.parametrize("pn,rtol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_ion_scattering_closed_form(pn, rtol):
+    """Equal-mass hard spheres (newvel_ionsc): the relative speed is scaled by
+    cos(chi) = sqrt(R1) and the velocity turned by chi."""
+    o = Oracle(decks.small(ncx=4, ncy=4), pn)
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        u = rng.normal(size=3)
+        vel = 3.1e3
+        v = vel * u / np.linalg.norm(u)
+        r1, r2 = rng.random(), rng.random()
+        w = _kin(o, 1, [*v, vel, r1, r2])
+        c = np.sqrt(r1)
+        assert abs(np.linalg.norm(w) - vel * c) <= rtol * vel * 10
+        if c > 1e-3:
+            assert abs(w @ v / (np.linalg.norm(w) * vel) - c) <= rtol * 100
+
+
+@pytest.mark.parametrize("pn,rtol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_neutral_velocity_draw_is_isotropic(pn, rtol):
+    """dmaxwellv: speed vti * vsave_n, polar cosine 1 - 2 Rthe, azimuth
+    2 pi Rphi (uniform on the sphere for uniform Rthe, Rphi)."""
+    o = Oracle(decks.small(ncx=4, ncy=4), pn)
+    for vs, vti, rphi, rthe in [(1.3, 400.0, 0.1, 0.2), (0.7, 250.0, 0.6, 0.9), (2.2, 300.0, 0.35, 0.5)]:
+        v = _kin(o, 2, [vs, vti, rphi, rthe])
+        s = vs * vti
+        assert abs(np.linalg.norm(v) - s) <= rtol * s * 10
+        assert abs(v[2] / s - (1 - 2 * rthe)) <= rtol * 10
+        phi = math.atan2(v[1], v[0]) % (2 * math.pi)
+        assert abs(phi - 2 * math.pi * rphi) <= rtol * 100
+
+
+@pytest.mark.parametrize("pn,rtol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_ionisation_energy_sharing_inverts(pn, rtol):
+    """The secondary electron's energy 10 tan(u atan(E/20)) has the CDF
+    atan(e/10) / atan(E/20) on [0, E/2]: inverting it returns u."""
+    o = Oracle(decks.small(ncx=4, ncy=4), pn)
+    for E in (1.0, 20.0, 150.0):
+        prev = -1.0
+        for u in np.linspace(0.0, 1.0, 11):
+            e = _kin(o, 3, [E, u])[0]
+            assert 0 <= e <= E / 2 * (1 + rtol * 10) and e >= prev
+            prev = e
+            assert abs(math.atan(e / 10) / math.atan(E / 20) - u) <= rtol * 100
+
+
+def _beam(o, G, cell, n, v, species):
+    st = {"x": np.full(n, 0.5), "y": np.full(n, 0.5), "vx": np.full(n, v[0]), "vy": np.full(n, v[1]),
+          "vz": np.full(n, v[2]), "id": np.arange(n, dtype=np.uint32)}
+    cs = np.zeros(G + 1, dtype=np.int32)
+    cs[cell + 1:] = n
+    st["cell_start"] = cs
+    empty = {k: (val[:0] if k != "cell_start" else np.zeros(G + 1, np.int32)) for k, val in st.items()}
+    o.set_species(species, st)
+    o.set_species(1 - species, empty)
+
+
+def test_electron_channel_fractions():
+    """A mono-energetic electron beam that always collides (P2 = 1): on the
+    argon target (P1 = 1) reaction r happens with probability
+    sigma_r(E) v / nu_max[Ar], on the metastable target (P1 = 0) reaction 4
+    with sigma_4(E) v / nu_max[Ar*] (binomial, 5 sigma per channel)."""
+    d = decks.small(ncx=4, ncy=4, capacity_factor=400.0)
+    o = Oracle(d, "f64")
+    ngy, G, cell, n = 5, 25, 2 * 5 + 2, 60000
+    L = o.lib
+    L.orc_ar_sigma.restype = C.c_double
+    L.orc_ar_sigma.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    Es = o.scalar("Escale0")
+    for E in (8.0, 30.0, 120.0):
+        vel = math.sqrt(E / Es)
+        for target, p1 in ((0, 1.0), (1, 0.0)):
+            _beam(o, G, cell, n, (0.0, 0.0, vel), 0)
+            o.arr("P2")[:] = 1.0
+            o.arr("P1")[:] = p1
+            o.arr("mccr")[:] = 0
+            o.orc_set_tstep(3)
+            assert o.orc_mcc() == 0
+            ev = o.arr("mccr").reshape(G, 10)[cell]
+            sigv = o.arr("sigv")
+            th = {1: 11.55, 2: 11.548, 3: 15.76, 4: 4.22}
+            rs = (0, 1, 2, 3) if target == 0 else (4,)
+            for r in rs:
+                sig = L.orc_ar_sigma(o.h, r, E) if E > th.get(r, 0.0) else 0.0
+                p = sig * vel / sigv[target]
+                mean = n * p
+                assert abs(ev[r] - mean) <= 5 * math.sqrt(mean * (1 - p)) + 3, (E, r, ev[r], mean)
+            other = (4,) if target == 0 else (0, 1, 2, 3)
+            assert all(ev[r] == 0 for r in other)
+
+
+def test_ion_channel_fractions_and_charge_exchange():
+    """A mono-energetic ion beam through a cold background (BackVel1 = 0),
+    always colliding (Pion = 1): charge exchange (5) and elastic scattering
+    (6) happen with probabilities sigma_r(E) v / nu_max[ion]; after a charge
+    exchange the ion has the neutral's velocity (zero here) and after an
+    elastic collision its speed is below the incident one."""
+    d = decks.small(ncx=4, ncy=4, capacity_factor=400.0)
+    o = Oracle(d, "f64")
+    ngy, G, cell, n = 5, 25, 2 * 5 + 2, 60000
+    L = o.lib
+    L.orc_ar_sigma.restype = C.c_double
+    L.orc_ar_sigma.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    Ei = o.scalar("Escale1")
+    for E in (0.5, 5.0, 50.0):
+        vel = math.sqrt(E / Ei)
+        _beam(o, G, cell, n, (vel, 0.0, 0.0), 1)
+        o.arr("Pion")[:] = 1.0
+        o.arr("BackVel1")[:] = 0.0
+        o.arr("mccr")[:] = 0
+        o.orc_set_tstep(2)
+        assert o.orc_mcc() == 0
+        ev = o.arr("mccr").reshape(G, 10)[cell]
+        sigv2 = o.arr("sigv")[2]
+        for r in (5, 6):
+            p = L.orc_ar_sigma(o.h, r, E) * vel / sigv2
+            mean = n * p
+            assert abs(ev[r] - mean) <= 5 * math.sqrt(mean * (1 - p)) + 3, (E, r, ev[r], mean)
+        s = o.species(1)
+        speed = np.sqrt(s["vx"] ** 2 + s["vy"] ** 2 + s["vz"] ** 2)
+        assert np.count_nonzero(speed == 0.0) == ev[5]
+        assert np.count_nonzero((speed > 0) & (speed < vel * (1 - 1e-12))) == ev[6]
