@@ -142,6 +142,8 @@ struct Case : CaseBase {
     uint32_t *next_id = nullptr;
     unsigned long long *mccr = nullptr;
     int *condcharge = nullptr, *d_err = nullptr, *lost = nullptr;
+    unsigned long long *sigma_hits = nullptr;   // nsp x G, this step's dielectric hits in fixed point
+    bool diel_routes = false;                   // some cell border leads into a dielectric
     double *sum_den = nullptr, *sum_phi = nullptr, *sum_ex = nullptr, *sum_ey = nullptr;
     long long n_accum = 0;
     long long tstep = 0;
@@ -185,7 +187,7 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey, total2, alive2, ent_dest, ent_pos, clk, eedf, lost};
+                        sum_ey, total2, alive2, ent_dest, ent_pos, clk, eedf, lost, sigma_hits};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (int s = 0; s < MAXSP; ++s) {
@@ -238,7 +240,7 @@ struct Case : CaseBase {
                     const int cid = c + mcid;
                     int flags = 0;
                     if (SI == 0) r.dest = cid;
-                    else if (SI >= 1 && SI < 100) r.dest = -3;
+                    else if (SI >= 1 && SI < 100) { r.dest = -3; diel_routes = true; }
                     else if (SI >= 100) { r.dest = -2; r.info = SI - 101; }
                     else {
                         int sum = 0, ok = 1;
@@ -447,6 +449,8 @@ struct Case : CaseBase {
         HIPCHK(hipMemsetAsync(condcharge, 0, nsp * S.CondNUMR * sizeof(int), st));
         HIPCHK(dalloc(&lost, nsp));
         HIPCHK(hipMemsetAsync(lost, 0, nsp * sizeof(int), st));
+        HIPCHK(dalloc(&sigma_hits, (size_t)nsp * G));
+        HIPCHK(hipMemsetAsync(sigma_hits, 0, (size_t)nsp * G * sizeof(unsigned long long), st));
         HIPCHK(dalloc(&d_err, 1));
         HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int), st));
         HIPCHK(dalloc(&clk, 1));
@@ -491,12 +495,14 @@ struct Case : CaseBase {
         kc.seed = (uint64_t)d.rng_seed;
         kc.Ex = Ex; kc.Ey = Ey; kc.P1 = P1; kc.P2 = P2; kc.Pion = Pion; kc.BackVel1 = BackVel1;
         kc.vsave = vsave; kc.xe = xe; kc.cx = cx; kc.StructIdx = StructIdx; kc.route = route; kc.lost = lost;
+        kc.sigma_hits = sigma_hits;
         nc.Gsize = G; nc.ngx = S.ngx; nc.ngy = S.ngy; nc.nsp = nsp; nc.CondNUMR = S.CondNUMR;
         nc.hdx = S.hdx; nc.hdy = S.hdy; nc.idx = S.idx; nc.idy = S.idy; nc.idxy = S.idxy;
         nc.hxy = (R)(0.5 * (double)S.dx / (double)S.dy);
         nc.hyx = (R)(0.5 * (double)S.dy / (double)S.dx);
         for (int s = 0; s < MAXSP; ++s) nc.q_density[s] = S.q_density[s];
         nc.Boundary = Boundary; nc.Face = Face; nc.CondID = CondID; nc.A_idx = A_idx; nc.Area = Area; nc.eps = eps;
+        nc.sigma_hits = diel_routes ? sigma_hits : nullptr;
         if (S.CondNUMR > MAX_SRC || (int)d.src.size() > MAX_SRC) return fail(-41, "too many conductors or sources");
         vc.ncond = S.CondNUMR;
         vc.nsrc = (int)d.src.size();
@@ -580,7 +586,7 @@ struct Case : CaseBase {
     int sort_boundary() override {
         if (!tag_valid) return fail(-42, "SortBounndary_cuda called without a preceding Move_cuda");
         for (int s = 0; s < nsp; ++s) {
-            launch_sort_count<R>(kc, cur[s], s, tag[s], key[s], cnt[s], condcharge, sigma_sp, st);
+            launch_sort_count<R>(kc, cur[s], s, tag[s], key[s], cnt[s], condcharge, st);
             launch_totals(G, inv_start, inv_ent, cnt[s], nullptr, total, alive, nullptr, st);
             launch_scan(total, nxt[s].cell_start, G, scan_tmp, st);
             launch_assign_bases(G, inv_start, inv_ent, cnt[s], nullptr, nxt[s].cell_start, alive, nullptr, base[s],
@@ -646,11 +652,10 @@ struct Case : CaseBase {
         // ionisation stash reuses its product staging buffer (stride = the
         // largest capacity)
         const size_t sstride = (size_t)stage_stride;
-        launch_fused_count<R>(kc, cur[0], 0, clk, cnt[0], pcnt, mccr, condcharge, sigma_sp, tag[0], stage, sstride,
-                              d_err, st);
+        launch_fused_count<R>(kc, cur[0], 0, clk, cnt[0], pcnt, mccr, condcharge, tag[0], stage, sstride, d_err, st);
         mark(3);
-        launch_fused_count<R>(kc, cur[1], 1, clk, cnt[1], nullptr, mccr, condcharge, sigma_sp, tag[1], stage,
-                              sstride, d_err, st);
+        launch_fused_count<R>(kc, cur[1], 1, clk, cnt[1], nullptr, mccr, condcharge, tag[1], stage, sstride, d_err,
+                              st);
         mark(4);
         // survivors + ionisation products per destination cell for both
         // species, their three scans (electron and ion cell_start, global
@@ -877,6 +882,7 @@ struct Case : CaseBase {
         if (n == "mccr") return D(mccr, (size_t)G * S.TnRct, 8, 0);
         if (n == "CondCharge") return D(condcharge, (size_t)nsp * S.CondNUMR, 4, 0);
         if (n == "Lost") return D(lost, nsp, 4, 0);
+        if (n == "sigma_hits") return D(sigma_hits, (size_t)nsp * G, 8, 0);
         if (n == "sum_den") return D(sum_den, (size_t)nsp * G, 8, 1);
         if (n == "sum_phi") return D(sum_phi, G, 8, 1);
         if (n == "sum_ex") return D(sum_ex, G, 8, 1);
@@ -1197,7 +1203,7 @@ int pic_set_array(pic_ctx *ctx, const char *name, const void *host, size_t bytes
     return ctx->c->set_array(name, host, bytes);
 }
 const char *pic_array_names(void) {
-    return "Ex Ey phi TotPot Sigma Source den sigma_sp b xsol V_cond mccr CondCharge sum_den sum_phi sum_ex sum_ey "
+    return "Ex Ey phi TotPot Sigma Source den sigma_sp sigma_hits b xsol V_cond mccr CondCharge sum_den sum_phi sum_ex sum_ey "
            "eedf Lost next_id Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai Aj Aval "
            "Atemp Mdiag cond_b temp_b lap_phi BackDen1 BackVel1 FluidN xe xee cx sigv P1 P2 Pion vsave";
 }

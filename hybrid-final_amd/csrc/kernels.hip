@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_push(DevConst<R> k, PView<R> p, int isp
 // side effects of a particle that leaves through a wall
 template <typename R>
 __device__ inline void wall_effects(const DevConst<R> &k, const Route &rt, int isp, int c, int code, R x, R y, R vx,
-                                    R vy, int *condcharge, R *sigma_sp) {
+                                    R vy, int *condcharge) {
     if (rt.dest == -2) {
         atomicAdd(&condcharge[isp * k.CondNUMR + rt.info], 1);
     } else if (rt.dest == -1) {
@@ -108,8 +108,8 @@ __device__ inline void wall_effects(const DevConst<R> &k, const Route &rt, int i
         int pa, pb;
         R a, b;
         dielectric_split<R>(k.StructIdx, k.ngy, c, mcid, sm, x, y, vx, vy, pa, a, pb, b);
-        atomicAdd(&sigma_sp[(size_t)isp * k.Gsize + pa], a);
-        atomicAdd(&sigma_sp[(size_t)isp * k.Gsize + pb], b);
+        atomicAdd(&k.sigma_hits[(size_t)isp * k.Gsize + pa], sigma_fixed((double)a));
+        atomicAdd(&k.sigma_hits[(size_t)isp * k.Gsize + pb], sigma_fixed((double)b));
     }
 }
 
@@ -117,7 +117,7 @@ __device__ inline void wall_effects(const DevConst<R> &k, const Route &rt, int i
 // key[k] = slot << 24 | rank within (cell, slot); -1 when the particle leaves
 template <typename R>
 __global__ __launch_bounds__(256) void k_sort_count(DevConst<R> k, PView<R> p, int isp, const uint8_t *tag, int *key,
-                                                    int *cnt, int *condcharge, R *sigma_sp) {
+                                                    int *cnt, int *condcharge) {
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_sort_count(DevConst<R> k, PView<R> p, i
             int code = tag[kk];
             Route rt = k.route[c * ROUTE_SLOTS + code];
             if (rt.dest >= 0) slot = rt.info & 15;
-            else wall_effects<R>(k, rt, isp, c, code, p.x[kk], p.y[kk], p.vx[kk], p.vy[kk], condcharge, sigma_sp);
+            else wall_effects<R>(k, rt, isp, c, code, p.x[kk], p.y[kk], p.vx[kk], p.vy[kk], condcharge);
         }
         int rank = -1;
 #pragma unroll
@@ -555,8 +555,7 @@ __device__ inline int collide(const DevConst<R> &k, int dest, R pa, uint32_t id,
 template <typename R, int KIND>
 __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, const StepClock *clk, int *cnt,
                                                      int *pcnt, unsigned long long *mccr, int *condcharge,
-                                                     R *sigma_sp, uint8_t *flag, R *stash, size_t stash_stride,
-                                                     int *err) {
+                                                     uint8_t *flag, R *stash, size_t stash_stride, int *err) {
     __shared__ FStage<R> stage_all[FWAVES];
     __shared__ FPool<R> pool;
     const uint32_t step = clk->step;
@@ -611,7 +610,7 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
                         slot = info & 15;
                         jc = null_test<R, KIND>(k, pb, id, step);
                     } else {
-                        wall_effects<R>(k, Route{dest, info}, isp, c, code, x, y, vx, vy, condcharge, sigma_sp);
+                        wall_effects<R>(k, Route{dest, info}, isp, c, code, x, y, vx, vy, condcharge);
                     }
                 }
 #pragma unroll
@@ -816,7 +815,9 @@ __global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PSet<R> ps
 
 // Node density from the four adjacent cells, boundary factors
 // (src/cuda_Deposit.cu:277-288), then charge density, source and PCG rhs
-// (src/cuda_Deposit.cu:30-104).  part holds nsp x Gsize x 4.
+// (src/cuda_Deposit.cu:30-104).  part holds nsp x Gsize x 4.  This step's
+// dielectric hits (fixed point, see sigma_fixed) are first added to the
+// per-species surface charge and cleared.
 // With ex.on (fused step) the same thread also adds this step's node fields
 // to the FP64 profile sums (src/cuda_Diagnostic.cu:505-547) and thread 0
 // advances the particle id counters past this step's ionisation products.
@@ -833,6 +834,16 @@ __global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_
         ex.clk->t = (double)nstep * ex.dt;
     }
     if (g >= n.Gsize) return;
+    if (n.sigma_hits) {
+        for (int i = 0; i < n.nsp; ++i) {
+            const size_t at = (size_t)i * n.Gsize + g;
+            const long long h = (long long)n.sigma_hits[at];
+            if (h) {
+                sigma_sp[at] += (R)((double)h * (1.0 / SIGMA_FIX));
+                n.sigma_hits[at] = 0;
+            }
+        }
+    }
     const int ngy = n.ngy, ngx = n.ngx;
     const int x = g / ngy, y = g % ngy;
     const bool c00 = x < ngx - 1 && y < ngy - 1;   // cell g
@@ -1126,8 +1137,8 @@ void launch_push(const DevConst<R> &k, const PView<R> &p, int isp, uint8_t *tag,
 }
 template <typename R>
 void launch_sort_count(const DevConst<R> &k, const PView<R> &p, int isp, const uint8_t *tag, int *key, int *cnt,
-                       int *cc, R *sg, hipStream_t s) {
-    hipLaunchKernelGGL(k_sort_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag, key, cnt, cc, sg);
+                       int *cc, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag, key, cnt, cc);
 }
 template <typename R>
 void launch_sort_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const uint8_t *tag,
@@ -1149,14 +1160,14 @@ void launch_mcc_insert(const DevConst<R> &k, const PView<R> &src, const PView<R>
 }
 template <typename R>
 void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, const StepClock *clk, int *cnt, int *pcnt,
-                        unsigned long long *mccr, int *cc, R *sg, uint8_t *flag, R *stash, size_t stash_stride,
-                        int *err, hipStream_t s) {
+                        unsigned long long *mccr, int *cc, uint8_t *flag, R *stash, size_t stash_stride, int *err,
+                        hipStream_t s) {
     if (isp == 0)
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           clk, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
+                           clk, cnt, pcnt, mccr, cc, flag, stash, stash_stride, err);
     else
         hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           clk, cnt, pcnt, mccr, cc, sg, flag, stash, stash_stride, err);
+                           clk, cnt, pcnt, mccr, cc, flag, stash, stash_stride, err);
 }
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
@@ -1278,7 +1289,7 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
 #define INST(R)                                                                                                         \
     template void launch_push<R>(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);                 \
     template void launch_sort_count<R>(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *,      \
-                                       int *, R *, hipStream_t);                                                       \
+                                       int *, hipStream_t);                                                            \
     template void launch_sort_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const uint8_t *,     \
                                          const int *, const int *, int, int *, hipStream_t);                           \
     template void launch_mcc_collide<R>(const DevConst<R> &, const PView<R> &, int, uint32_t, unsigned long long *,    \
@@ -1287,7 +1298,7 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
                                        const int *, const PView<R> &, const int *, const R *, int, const uint32_t *,   \
                                        int, int *, hipStream_t);                                                       \
     template void launch_fused_count<R>(const DevConst<R> &, const PView<R> &, int, const StepClock *, int *, int *,   \
-                                        unsigned long long *, int *, R *, uint8_t *, R *, size_t, int *, hipStream_t); \
+                                        unsigned long long *, int *, uint8_t *, R *, size_t, int *, hipStream_t);      \
     template void launch_fused_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const PView<R> &,   \
                                           int, const uint8_t *, const R *, size_t, const int *, const int *,           \
                                           const int *, const int *, const uint32_t *, int, int, int *, hipStream_t);   \

@@ -129,7 +129,21 @@ struct DevConst {
     const int *StructIdx;
     const Route *route;        // Csize-independent: indexed by node cell * 9 + code
     int *lost;                 // per species: particles deleted at a wall (no conductor or dielectric there)
+    unsigned long long *sigma_hits;   // nsp x Gsize: this step's dielectric hits, fixed point (sigma_fixed)
 };
+
+// Dielectric surface hits are summed in signed 64-bit fixed point, 2^-40 of a
+// particle per unit.  Integer adds commute, so the per-node sums, and every
+// bit of the surface charge folded from them, do not depend on the order in
+// which the atomics land: repeated runs and the fused and staged steps agree
+// bit for bit.  A node takes fewer than 2^23 hits per step: a particle moves
+// at most one cell per step, so only particles of the four cells around the
+// node can reach it.  The folding kernel adds the sums to the per-species
+// surface charge once per step and clears them.
+constexpr double SIGMA_FIX = 1099511627776.0;   // 2^40
+__device__ inline unsigned long long sigma_fixed(double w) {
+    return (unsigned long long)__double2ll_rn(w * SIGMA_FIX);
+}
 
 // ---------------------------------------------------------------- push
 template <typename R> __host__ __device__ constexpr R one_minus_ulp();

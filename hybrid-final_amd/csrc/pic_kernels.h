@@ -23,6 +23,7 @@ struct NodeConst {
     R q_density[5];
     const int *Boundary, *Face, *CondID, *A_idx;
     const R *Area, *eps;
+    unsigned long long *sigma_hits;   // null when no cell border leads into a dielectric
 };
 
 // particle views of all species of a case (kernel argument)
@@ -63,7 +64,7 @@ struct VoltConst {
 
 template <typename R> void launch_push(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);
 template <typename R>
-void launch_sort_count(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *, int *, R *, hipStream_t);
+void launch_sort_count(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *, int *, hipStream_t);
 template <typename R>
 void launch_sort_scatter(const DevConst<R> &, const PView<R> &, const PView<R> &, const uint8_t *, const int *,
                          const int *, int, int *, hipStream_t);
@@ -78,8 +79,8 @@ void launch_mcc_insert(const DevConst<R> &, const PView<R> &, const PView<R> &, 
 // of stash_stride) and scatter pass (sorted write of survivors and products)
 template <typename R>
 void launch_fused_count(const DevConst<R> &, const PView<R> &, int isp, const StepClock *clk, int *cnt, int *pcnt,
-                        unsigned long long *mccr, int *condcharge, R *sigma_sp, uint8_t *flag, R *stash,
-                        size_t stash_stride, int *err, hipStream_t);
+                        unsigned long long *mccr, int *condcharge, uint8_t *flag, R *stash, size_t stash_stride,
+                        int *err, hipStream_t);
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
                           const uint8_t *flag, const R *stash, size_t stash_stride, const int *base,

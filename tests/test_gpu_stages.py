@@ -4,7 +4,10 @@ Both sides start from the same deck, hence the same loaded particles; each
 stage is then run on both and compared.  Integer/ordering results and the
 FP32/FP64 arithmetic of push, walls and collisions must be bit-exact; node
 density is compared within 1e-5 relative (different summation order); the
-potential within the field-solve tolerance.
+potential within the field-solve tolerance.  Dielectric surface hits are summed
+on the GPU in 64-bit fixed point (2^-40 of a particle per unit, order-free) and
+in particle order by the oracle, so the per-species surface charge and the
+Sigma derived from it are compared within 1e-5 (FP32) / 1e-10 (FP64) relative.
 """
 import numpy as np
 import pytest
@@ -29,6 +32,16 @@ def same_particles(case, orc, s, what):
                                  f"{a[k][bad[:3]]} vs {b[k][bad[:3]]}")
 
 
+def close(a, b, tol, what):
+    """|a - b| <= tol * (|b| + max|b|) elementwise: relative where the charge
+    is large, absolute at the scale of the array where species cancel."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.max(np.abs(b)) if b.size else 0.0
+    bad = np.abs(a - b) > tol * (np.abs(b) + scale)
+    assert not np.any(bad), f"{what}: {np.count_nonzero(bad)} values differ, first at {np.nonzero(bad)[0][:5]}"
+
+
 def deck_small():
     return decks.small(ncx=16, ncy=12, ppc=40, volts=150.0)
 
@@ -39,12 +52,37 @@ def deck_dense():
     return decks.small(ncx=8, ncy=6, ppc=420, volts=200.0, pres_torr=0.3)
 
 
-@pytest.mark.parametrize("deck", [deck_small, deck_dense], ids=["small", "dense"])
+def _variant(name):
+    from test_setup_parity import deck_variants
+    d = deck_variants()[name]
+    d["Geometry"]["Source"][0]["AC(V)"] = 150.0
+    return d
+
+
+def deck_dielectric():
+    """Thick electrodes (conductor cells absorb) and a dielectric block
+    (surface charge split onto nodes): the operator is not separable, so the
+    FP32 field solve is the Jacobi PCG in FP64 refinement."""
+    return _variant("thick_electrodes_dielectric")
+
+
+def deck_inner_conductor():
+    """A Dirichlet right wall (particles deleted there) and a grounded
+    conductor inside the plasma."""
+    return _variant("inner_conductor")
+
+
+DECKS = [deck_small, deck_dense, deck_dielectric, deck_inner_conductor]
+DECK_IDS = ["small", "dense", "dielectric", "inner_conductor"]
+
+
+@pytest.mark.parametrize("deck", DECKS, ids=DECK_IDS)
 @pytest.mark.parametrize("prec,pn", PRECS)
 def test_stage_by_stage(prec, pn, deck):
     d = deck()
     case = Case(d, prec)
     orc = Oracle(d, pn)
+    tol = 1e-5 if pn == "f32" else 1e-10
     for s in range(2):
         same_particles(case, orc, s, "load")
     orc.orc_deposit()
@@ -64,11 +102,12 @@ def test_stage_by_stage(prec, pn, deck):
         # Jacobi PCG stops at 1e-10 relative residual (error <= cond(A) * 1e-10)
         x_g, x_o = case.array("xsol"), orc.arr("xsol")
         xs = np.max(np.abs(x_o)) + 1e-300
-        bound = 1e-8 if prec == FP32_REFINED else 1e-6
+        bound = 1e-8 if case.field_solver_kind() == 1 else 1e-6
         assert np.max(np.abs(x_g - x_o)) <= bound * xs, f"FP64 solution error {np.max(np.abs(x_g - x_o)) / xs:.2e}"
-        # from here on both use the oracle's potential and density
+        # from here on both use the oracle's potential, density and surface charge
         case.set_array("phi", orc.arr("phi"))
         case.set_array("den", orc.arr("den"))
+        case.set_array("Sigma", orc.arr("Sigma"))
         case.efield()
         orc.orc_efield()
         for nm in ("TotPot", "Sigma", "Ex", "Ey"):
@@ -84,6 +123,9 @@ def test_stage_by_stage(prec, pn, deck):
         # particles absorbed by conductors and deleted at walls, per species
         for nm in ("CondCharge", "Lost"):
             assert np.array_equal(case.array(nm).astype(np.int64), orc.arr(nm).astype(np.int64)), f"{nm} differs"
+        # surface charge including this step's hits, which the deposit folds in
+        sig_g = case.array("sigma_sp").astype(np.float64) + case.array("sigma_hits").astype(np.float64) * 2.0**-40
+        close(sig_g, orc.arr("sigma_sp"), tol, f"sigma_sp after sort, step {step}")
         case.mcc()
         assert orc.orc_mcc() == 0
         for s in range(2):
@@ -95,10 +137,13 @@ def test_stage_by_stage(prec, pn, deck):
         orc.orc_deposit()
         np.testing.assert_allclose(case.array("den"), orc.arr("den"), rtol=1e-5, atol=1e-4)
         np.testing.assert_allclose(case.array("b"), orc.arr("b"), rtol=1e-5, atol=1e-3 * np.max(np.abs(orc.arr("b"))))
+        assert not np.any(case.array("sigma_hits")), "the deposit leaves surface hits unfolded"
+        close(case.array("sigma_sp"), orc.arr("sigma_sp"), tol, f"sigma_sp after deposit, step {step}")
+        close(case.array("Sigma"), orc.arr("Sigma"), tol, f"Sigma after deposit, step {step}")
     case.sync()
 
 
-@pytest.mark.parametrize("deck", [deck_small, deck_dense], ids=["small", "dense"])
+@pytest.mark.parametrize("deck", DECKS, ids=DECK_IDS)
 @pytest.mark.parametrize("prec,pn", PRECS)
 def test_fused_step_equals_staged(prec, pn, deck):
     """pic_step (fused two-pass) and the staged stage functions give the same bits."""
