@@ -1008,7 +1008,12 @@ static bool setup_lookup(Setup<R> &S, const std::string &n, const void *&p, size
     if (n == "xe") return H(S.xe.data(), S.xe.size(), rs, 1);
     if (n == "xee") return H(S.xee.data(), S.xee.size(), rs, 1);
     if (n == "cx") return H(S.cx.data(), S.cx.size(), rs, 1);
-    if (n == "sigv") return H(S.sigv, 3, rs, 1);
+    if (n == "sigv") return H(S.sigv, S.nsigv, rs, 1);
+    if (n == "Th_e") return H(S.Th_e, S.TnRct, rs, 1);
+    if (n == "Flag") return H(S.Flag, S.TnRct, rs, 1);
+    if (n == "mofM") return H(S.mofM, S.TnRct, rs, 1);
+    if (n == "RR") return H(S.RR, S.TnRct, rs, 1);
+    if (n == "colprob") return H(S.colprob, S.nsp, rs, 1);
     if (n == "P1") return H(S.P1.data(), S.P1.size(), rs, 1);
     if (n == "P2") return H(S.P2.data(), S.P2.size(), rs, 1);
     if (n == "Pion") return H(S.Pion.data(), S.Pion.size(), rs, 1);
@@ -1040,6 +1045,10 @@ static bool setup_scalar(Setup<R> &S, const std::string &n, double &v) {
     if (n == "dt_mcc") { v = (double)S.dt_mcc; return true; }
     if (n == "nvel") { v = S.nvel; return true; }
     if (n == "nsp") { v = S.nsp; return true; }
+    if (n == "TnRct") { v = S.TnRct; return true; }
+    if (n == "nbg") { v = S.nbg; return true; }
+    if (n == "bg_dens0") { v = (double)S.bg_dens[0]; return true; }
+    if (n == "bg_dens1") { v = (double)S.bg_dens[1]; return true; }
     if (n.compare(0, 2, "np") == 0 && n.size() == 3) { v = S.sp[n[2] - '0'].n; return true; }
     if (n.compare(0, 6, "Ascale") == 0 && n.size() == 7) { v = (double)S.Ascale[n[6] - '0']; return true; }
     if (n.compare(0, 6, "Escale") == 0 && n.size() == 7) { v = (double)S.Escale[n[6] - '0']; return true; }
@@ -1083,6 +1092,9 @@ int pic_create(const char *deck_json, const pic_options *opt, pic_ctx **out) {
     Deck d;
     std::string err;
     if (int e = parse_deck(deck_json, d, err)) return fail(e, "%s", err.c_str());
+    if (d.gas != 0)
+        return fail(-19, "the O2 and Ar/O2 collision step is not built: gas type %d has host setup only "
+                         "(pic_setup_open)", d.gas);
     std::unique_ptr<pic_ctx> h(new pic_ctx());
     int e = 0;
     try {
@@ -1276,6 +1288,11 @@ int pic_setup_scalar(pic_setup_h *h, const char *name, double *v) {
     return 0;
 }
 void pic_setup_close(pic_setup_h *h) { delete h; }
+const char *pic_reaction_names(int gas) {
+    static const std::string names[3] = {reaction_names(0), reaction_names(1), reaction_names(2)};
+    if (gas < 0 || gas > 2) return nullptr;
+    return names[gas].c_str();
+}
 
 // ------------------------------------------------------------------ shims
 static pic_ctx *g_default = nullptr;

@@ -9,6 +9,8 @@
 #include "detmath.h"
 #include "json.h"
 #include "xsec_data.inc"
+#include "coll_keys.inc"
+#include "xsec_oxygen.h"
 
 namespace pic {
 
@@ -20,10 +22,19 @@ constexpr double kNperTorr = 8.3221e20;
 constexpr double kDblMin = 1e-200;
 constexpr double kPi = 3.14159265358979323846;
 
-const char *kArKeys[AR_NRCT] = {"0.e+Ar>e+Ar", "1.e+Ar>e+Ar*", "2.e+Ar>e+Ar*m", "3.e+Ar>2e+Ar^",
-                                "4.e+Ar*m>2e+Ar^", "5.Ar+Ar^>Ar^+Ar", "6.Ar+Ar^>Ar+Ar^",
-                                "7.e+Ar*m>e+Ar", "8.Ar*m+Ar>e+Ar^+Ar", "9.Ar+Ar*m>Ar+Ar"};
+// per gas type: the deck object with the reaction switches, their names
+// and count, and each reaction's mass-ratio operands
+const char *const kCollObj[3] = {pic_coll_obj0, pic_coll_obj1, pic_coll_obj2};
+const char *const *const kCollKeys[3] = {pic_coll_keys0, pic_coll_keys1, pic_coll_keys2};
+const int kCollN[3] = {pic_coll_n0, pic_coll_n1, pic_coll_n2};
+const signed char (*const kCollMofM[3])[4] = {pic_coll_mofm0, pic_coll_mofm1, pic_coll_mofm2};
 }  // namespace
+
+std::string reaction_names(int gas) {
+    std::string out = kCollObj[gas];
+    for (int r = 0; r < kCollN[gas]; ++r) out += std::string("\n") + kCollKeys[gas][r];
+    return out;
+}
 
 int parse_deck(const std::string &text, Deck &d, std::string &err) {
     JValue root;
@@ -106,9 +117,9 @@ int parse_deck(const std::string &text, Deck &d, std::string &err) {
     if (d.seed == 0) d.seed = 1;
     d.rng_seed = (int64_t)sm.number("PhiloxSeed");
     d.rf_cycles = (int)sm.number("RFCycles");
-    if (d.gas == 0) {
-        const JValue &ac = root.object("Collision_On_Off").object("ArgonCase");
-        for (int r = 0; r < AR_NRCT; ++r) d.coll_flag[r] = ac.number(kArKeys[r]);
+    if (d.gas >= 0 && d.gas <= 2) {
+        const JValue &ac = root.object("Collision_On_Off").object(kCollObj[d.gas]);
+        for (int r = 0; r < kCollN[d.gas]; ++r) d.coll_flag[r] = ac.number(kCollKeys[d.gas][r]);
     }
     return 0;
 }
@@ -207,7 +218,6 @@ int constants(const Deck &d, Setup<R> &s, std::string &err) {
     if (max_freq <= 0.0f) { err = "no source frequency"; return -16; }
     if (d.dt_pic == 0) { err = "ERROR! - \"TimeStep_PIC\" = 0"; return -17; }
     if (d.gas < 0 || d.gas > 2) { err = "\"Type\" is error in GasSpecies."; return -18; }
-    if (d.gas != 0) { err = "O2 and Ar/O2 chemistry is not available in this build"; return -19; }
 
     s.Gsize = s.ngx * s.ngy;
     s.ncx = s.ngx - 1; s.ncy = s.ngy - 1;
@@ -226,14 +236,44 @@ int constants(const Deck &d, Setup<R> &s, std::string &err) {
     s.dt_dx = s.dt / s.dx;
     s.dt_dy = s.dt / s.dy;
 
-    s.nsp = 2; s.nfsp = 1;
-    s.bg_mass[0] = (R)(39.950 * kAMU);
-    s.bg_pres[0] = (R)d.total_pres;
-    s.bg_dens[0] = (R)(kNperTorr * (double)s.bg_pres[0] / ((double)(R)d.bg_temp[0] + kDblMin));
-    s.fl_mass[0] = (R)(39.950 * kAMU);
-    s.fl_dens[0] = (R)d.fl_dens[0];
-    s.mass[0] = (R)(5.485799486e-4 * kAMU); s.q[0] = (R)(-1.0 * kCQ);
-    s.mass[1] = (R)(39.9500 * kAMU);        s.q[1] = (R)(1.0 * kCQ);
+    // gas species (src/start.c:303-728)
+    const R me = (R)(5.485799486e-4 * kAMU), mAr = (R)(39.950 * kAMU), mO2 = (R)(32.0000 * kAMU),
+            mO = (R)(16.000 * kAMU), qe = (R)(1.0 * kCQ);
+    auto bg = [&](int b, R mass, R pres, double temp) {
+        s.bg_mass[b] = mass;
+        s.bg_pres[b] = pres;
+        s.bg_dens[b] = (R)(kNperTorr * (double)pres / ((double)(R)temp + kDblMin));
+    };
+    if (d.gas == 0) {
+        s.nsp = 2; s.nfsp = 1; s.nbg = 1;
+        bg(0, mAr, (R)d.total_pres, d.bg_temp[0]);
+        s.fl_mass[0] = mAr;
+        s.mass[0] = me; s.q[0] = -qe;
+        s.mass[1] = mAr; s.q[1] = qe;
+    } else if (d.gas == 1) {
+        s.nsp = 4; s.nfsp = 4; s.nbg = 1;
+        bg(0, mO2, (R)d.total_pres, d.bg_temp[1]);
+        s.fl_mass[0] = mO2; s.fl_mass[1] = mO2; s.fl_mass[2] = mO; s.fl_mass[3] = mO;
+        s.mass[0] = me; s.q[0] = -qe;
+        s.mass[1] = mO2; s.q[1] = qe;
+        s.mass[2] = mO; s.q[2] = qe;
+        s.mass[3] = mO; s.q[3] = -qe;
+    } else {
+        // background ratios: Ar takes 1 - (the O2 ratio) of the total pressure
+        s.nsp = 5; s.nfsp = 5; s.nbg = 2;
+        const R o2 = (R)d.bg_ratio[1];
+        const R ar = (R)(1.0 - (double)o2);
+        const R tot = (R)d.total_pres;
+        bg(0, mAr, tot * ar, d.bg_temp[0]);
+        bg(1, mO2, tot * o2, d.bg_temp[1]);
+        s.fl_mass[0] = mAr; s.fl_mass[1] = mO2; s.fl_mass[2] = mO2; s.fl_mass[3] = mO; s.fl_mass[4] = mO;
+        s.mass[0] = me; s.q[0] = -qe;
+        s.mass[1] = mAr; s.q[1] = qe;
+        s.mass[2] = mO2; s.q[2] = qe;
+        s.mass[3] = mO; s.q[3] = qe;
+        s.mass[4] = mO; s.q[4] = -qe;
+    }
+    for (int f = 0; f < s.nfsp; ++f) s.fl_dens[f] = (R)d.fl_dens[f];
     for (int i = 0; i < s.nsp; ++i) {
         s.np2c[i] = (R)d.sp_np2c[i];
         R temp = (R)d.sp_temp[i];
@@ -557,11 +597,107 @@ void argon_tables(Setup<R> &s) {
             s.cx[i * AR_NRCT + r] = v;
         }
     }
-    R me = s.mass[0], mi = s.mass[1], mAr = s.bg_mass[0], mAs = s.fl_mass[0];
-    s.mofM[0] = s.mofM[1] = s.mofM[2] = s.mofM[3] = me / mAr;
-    s.mofM[4] = me / mAs; s.mofM[5] = s.mofM[6] = mAr / mi; s.mofM[7] = me / mAs;
-    s.mofM[8] = mAs / mAr; s.mofM[9] = mAr / mAs;
     s.RR[7] = s.Flag[7] * (R)2.0e-13; s.RR[8] = s.Flag[8] * (R)6.2e-16; s.RR[9] = s.Flag[9] * (R)3.0e-21;
+}
+
+// O2 and Ar/O2 tables (src/PhysicsData.c:1719-3275): the same grid as
+// argon, 58 or 68 tabulated reactions from the fits and data sets of
+// xsec_oxygen.h, then the rate constants of the remaining reactions.
+template <typename R>
+void oxygen_tables(Setup<R> &s) {
+    using namespace oxy;
+    const bool o2 = s.deck.gas == 1;
+    const Row *rows = o2 ? kO2Rows : kArO2Rows;
+    const int ntab = o2 ? 58 : 68;
+    const double *rates = o2 ? kO2Rates : kArO2Rates;
+    s.TnRct = kCollN[s.deck.gas];
+    s.idLOGX = is_f32<R>() ? (R)((R)1 / (R)0.1f) : (R)10.0;
+    s.xe.assign(N_LOGX, 0); s.xee.assign(N_LOGX, 0); s.cx.assign((size_t)N_LOGX * s.TnRct, 0);
+    for (int i = 0; i < N_LOGX; ++i) {
+        if (is_f32<R>()) s.xe[i] = (R)(0.1f * (float)(i - 40));
+        else s.xe[i] = (R)(0.1 * (double)(i - 40));
+        s.xee[i] = (R)std::pow(10.0, (double)s.xe[i]);
+    }
+    for (int r = 0; r < s.TnRct; ++r) { s.Flag[r] = (R)s.deck.coll_flag[r]; if (s.Flag[r] < 0) s.Flag[r] = 0; }
+    const double shift_d[4] = {0.0, 0.977, 1.627, 1.96};
+    for (int r = 0; r < ntab; ++r) {
+        const Row &w = rows[r];
+        const R sh = is_f32<R>() ? (R)kShift[w.shift] : (R)shift_d[w.shift];
+        // threshold: the constant minus the float shift, in double, rounded to R
+        s.Th_e[r] = w.shift ? (R)(w.th - (double)sh) : (R)w.th;
+        for (int i = 0; i < N_LOGX; ++i) {
+            const R E = s.xee[i];
+            R v = 0;
+            if (!(E < s.Th_e[r])) {
+                switch (w.kind) {
+                case K_FIT: v = s.Flag[r] * fit<R>(w.a, (R)(E + sh)); break;
+                case K_FIT_X35: v = (R)((double)s.Flag[r] * 3.5 * (double)fit<R>(w.a, (R)(E + sh))); break;
+                case K_VIB: v = s.Flag[r] * (fit<R>(F_VIB1, E) + fit<R>(F_VIB2, E) + fit<R>(F_VIB3, E) + fit<R>(F_VIB4, E)); break;
+                case K_DATA: v = s.Flag[r] * func_decay<R>(pic_xsec_x[w.a], pic_xsec_y[w.a], pic_xsec_n[w.a], E); break;
+                default:
+                    v = s.Flag[r] * func_decay<R>(pic_xsec_x[w.a], pic_xsec_y[w.a], pic_xsec_n[w.a], E);
+                    v = v - s.Flag[r] * func_decay<R>(pic_xsec_x[w.b], pic_xsec_y[w.b], pic_xsec_n[w.b], E);
+                }
+            }
+            s.cx[(size_t)i * s.TnRct + r] = v;
+        }
+    }
+    for (int r = ntab; r < s.TnRct; ++r) s.RR[r] = (R)((double)s.Flag[r] * rates[r - ntab]);
+}
+
+// mass ratio target/projectile of every reaction (src/start.c:1003-1420)
+template <typename R>
+void mass_ratios(Setup<R> &s) {
+    const signed char(*mm)[4] = kCollMofM[s.deck.gas];
+    auto mass = [&](int kind, int i) { return kind == 0 ? s.mass[i] : kind == 1 ? s.bg_mass[i] : s.fl_mass[i]; };
+    for (int r = 0; r < kCollN[s.deck.gas]; ++r) s.mofM[r] = mass(mm[r][0], mm[r][1]) / mass(mm[r][2], mm[r][3]);
+}
+
+// Null-collision constants for O2 and Ar/O2 (src/cuda_mcc.cu:291-501): the
+// largest sigma*v over the grid per (projectile, target) pair, the per-step
+// collision probability of each projectile species from the initial target
+// densities, and the electron MCC sub-cycle count from the electrons'.
+template <typename R>
+void oxygen_null_collision(Setup<R> &s) {
+    using namespace oxy;
+    const bool o2 = s.deck.gas == 1;
+    const Pair *pairs = o2 ? kO2Pairs : kArO2Pairs;
+    const int np = o2 ? 20 : 25;
+    s.nsigv = np;
+    for (int k = 0; k < np; ++k) s.sigv[k] = 0;
+    for (int i = 0; i < N_LOGX; ++i) {
+        const R *row = s.cx.data() + (size_t)i * s.TnRct;
+        const double e = (double)s.xee[i];
+        for (int k = 0; k < np; ++k) {
+            R sig = 0, grp = 0;
+            bool first = true, open = false;
+            for (const int *c = pairs[k].cols; *c != -2; ++c) {
+                if (*c == -1) { sig = first ? grp : sig + grp; first = false; open = false; continue; }
+                grp = open ? grp + row[*c] : row[*c];
+                open = true;
+            }
+            if (open) sig = first ? grp : sig + grp;
+            const R v = (R)(std::sqrt(2 * 1.602e-19 * e / (double)s.mass[pairs[k].proj]) * (double)sig);
+            if (v > s.sigv[k]) s.sigv[k] = v;
+        }
+    }
+    for (int p = 0; p < MAXSP; ++p) s.colprob[p] = 0;
+    for (int k = 0; k < np; ++k) {
+        const Pair &w = pairs[k];
+        const double dens = w.tkind == T_BG ? (double)s.bg_dens[w.tidx]
+                          : w.tkind == T_FG ? (double)s.fl_dens[w.tidx] : (double)(R)s.deck.sp_dens[w.tidx];
+        const R t = (R)(1.0 - std::exp(-1 * (double)s.dt * (double)s.sigv[k] * dens));
+        s.colprob[w.proj] = s.colprob[w.proj] + t;
+    }
+    const R colprob = s.colprob[0];
+    if (colprob > (R)0.08) {
+        s.MCCn = (int)std::ceil((double)(colprob / (R)0.08));
+        R ratio = (R)1 / (R)s.MCCn;
+        s.dt_mcc = s.dt * ratio;
+    } else {
+        s.MCCn = 1;
+        s.dt_mcc = s.dt;
+    }
 }
 
 template <typename R>
@@ -581,9 +717,12 @@ void null_collision(Setup<R> &s) {
         if (v2 > sv[2]) sv[2] = v2;
     }
     s.sigv[0] = sv[0]; s.sigv[1] = sv[1]; s.sigv[2] = sv[2];
+    s.nsigv = 3;
     double t0 = 1.0 - std::exp(-1 * (double)s.dt * (double)sv[0] * (double)s.bg_dens[0]);
     double t1 = 1.0 - std::exp(-1 * (double)s.dt * (double)sv[1] * (double)s.fl_dens[0]);
     R colprob = (R)t0 + (R)t1;
+    s.colprob[0] = colprob;
+    s.colprob[1] = (R)(1.0 - std::exp(-1 * (double)s.dt * (double)sv[2] * (double)s.bg_dens[0]));
     if (colprob > (R)0.08) {
         s.MCCn = (int)std::ceil((double)(colprob / (R)0.08));
         R ratio = (R)1 / (R)s.MCCn;
@@ -710,8 +849,14 @@ int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool do_load) {
     field_matrix(s);
     e = presolves(s, err);
     if (e) return e;
-    argon_tables(s);
-    null_collision(s);
+    if (d.gas == 0) {
+        argon_tables(s);
+        null_collision(s);
+    } else {
+        oxygen_tables(s);
+        oxygen_null_collision(s);
+    }
+    mass_ratios(s);
     maxwell_table(s);
     if (do_load) {
         e = load(s, err);

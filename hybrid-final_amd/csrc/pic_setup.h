@@ -2,8 +2,8 @@
 // that is computed once (geometry maps, field operator, Laplace solutions,
 // gas fields, cross-section tables, null-collision constants, Maxwellian
 // table, initial particles).  Restates src/start.c:19-2052, src/load.c:121-245,
-// src/Efield.c:3-14, src/PhysicsData.c:638-705,1579-1718,
-// src/cuda_mcc.cu:202-291, src/Particle.c:3-230 and
+// src/Efield.c:3-14, src/PhysicsData.c:638-705,1579-3275,
+// src/cuda_mcc.cu:202-501, src/Particle.c:3-230 and
 // src/cuda_Field.cu:182-236,646-678.
 #pragma once
 #include <cstdint>
@@ -45,6 +45,9 @@ struct Deck {
 
 // Parses deck text; returns 0 or a negative code with a message.
 int parse_deck(const std::string &text, Deck &d, std::string &err);
+// "Collision_On_Off" object name, then the reaction keys of gas type 0-2,
+// newline-separated
+std::string reaction_names(int gas);
 
 template <typename R>
 struct SpeciesInit {
@@ -57,7 +60,7 @@ struct SpeciesInit {
 template <typename R>
 struct Setup {
     Deck deck;
-    int ngx = 0, ngy = 0, ncx = 0, ncy = 0, Gsize = 0, Csize = 0, nsp = 0, nfsp = 0, CondNUMR = 0;
+    int ngx = 0, ngy = 0, ncx = 0, ncy = 0, Gsize = 0, Csize = 0, nsp = 0, nfsp = 0, nbg = 1, CondNUMR = 0;
     R dx = 0, dy = 0, zl = 0, dt = 0, dt_dx = 0, dt_dy = 0, hdx = 0, hdy = 0, idx = 0, idy = 0, idxy = 0;
     R mass[MAXSP] = {0}, q[MAXSP] = {0}, np2c[MAXSP] = {0}, q_density[MAXSP] = {0}, vti[MAXSP] = {0};
     R qm[MAXSP] = {0}, Escale[MAXSP] = {0}, Ascale[MAXSP] = {0}, Denscale[MAXSP] = {0};
@@ -77,7 +80,9 @@ struct Setup {
     R idLOGX = 0, dt_mcc = 0;
     std::vector<R> xe, xee, cx;      // cx: N_LOGX x TnRct
     R Th_e[100] = {0}, Flag[100] = {0}, mofM[100] = {0}, RR[100] = {0};
-    R sigv[32] = {0};
+    R sigv[32] = {0};                // max sigma*v per (projectile, target) pair
+    int nsigv = 0;                   // pairs: 3 (Ar), 20 (O2), 25 (Ar/O2)
+    R colprob[MAXSP] = {0};          // collision probability per step, per projectile species
     std::vector<R> P1, P2, Pion;
     int nvel = 0;
     std::vector<R> vsave;

@@ -88,6 +88,7 @@ EXPORTS = {
     "pic_setup_get_array": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_size_t]),
     "pic_setup_scalar": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     "pic_setup_close": (None, [C.c_void_p]),
+    "pic_reaction_names": (C.c_char_p, [C.c_int]),
     "pic_shim_open": (C.c_int, [C.c_char_p, C.c_int]),
     "pic_shim_close": (None, []),
 }

@@ -80,6 +80,78 @@ def ccp_deck(ncx=64, ncy=64, dx=5e-4, volts=100.0, freq=13.56e6, pres_torr=0.1,
     return deck
 
 
+def reaction_keys(gas):
+    """(object name under "Collision_On_Off", reaction keys in order) of gas
+    type 0 (Ar), 1 (O2) or 2 (Ar/O2), as the library reads them."""
+    from . import lib
+    txt = lib().pic_reaction_names(gas)
+    if txt is None:
+        raise ValueError(f"unknown gas type {gas}")
+    obj, *keys = txt.decode().split("\n")
+    return obj, keys
+
+
+OXYGEN_NEUTRALS = {"O2A": 1e19, "O2B": 5e18, "OP": 5e19, "OD": 1e18}
+
+
+def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None, **kw):
+    """O2 (gas=1) or Ar/O2 (gas=2) CCP deck on the ccp_deck geometry.
+
+    ``o2_ratio`` is the O2 fraction of the background pressure (gas 2; the
+    rest is Ar).  Charged-species densities default to a quasi-neutral mix
+    (e 1e15, O- 2e14; positive ions make up the rest) with ppc macro-particles
+    per cell for e and the molecular/atomic majority ions and ppc/5 for O+
+    and O-.  Every reaction switch is on.  Gas types 1 and 2 have host setup
+    only in this build (pic_setup_open); pic_create rejects them."""
+    d = ccp_deck(**kw)
+    ppc = kw.get("ppc", 50)
+    dx = kw.get("dx", 5e-4)
+    ncx, ncy = kw.get("ncx", 64), kw.get("ncy", 64)
+    g = d["GasSpecies"]
+    g["Type(0:Ar,1:O2,2:Ar/O2)"] = gas
+    temp = g["Background"]["Argon"]["Temp(eV)"]
+    if gas == 1:
+        g["Background"] = {"Oxygen": {"Ratio": 1.0, "Temp(eV)": temp}}
+        dens = {"Electron": 1e15, "O2+": 1.1e15, "O+": 1e14, "O-": 2e14}
+        neut = dict(OXYGEN_NEUTRALS)
+    elif gas == 2:
+        g["Background"] = {"Argon": {"Ratio": 1.0 - o2_ratio, "Temp(eV)": temp},
+                           "Oxygen": {"Ratio": o2_ratio, "Temp(eV)": temp}}
+        dens = {"Electron": 1e15, "Ar+": 8e14, "O2+": 3e14, "O+": 1e14, "O-": 2e14}
+        neut = {"Ar*": 1e15, **OXYGEN_NEUTRALS}
+    else:
+        raise ValueError("oxygen_deck builds gas types 1 and 2")
+    dens.update(ion_densities or {})
+    neut.update(neutral_densities or {})
+    ns = g["NeutralSpecies"]
+    ns.pop("Ar*", None)
+    for name, n in neut.items():
+        ns[name] = {"Density": n, "Temp(eV)": 0.026, "Quenching": 1.0}
+    cs = g["ChargeSpecies"]
+    te, ti = cs["Electron"]["Temp(eV)"], cs["Ar+"]["Temp(eV)"]
+    for name in ("Electron", "Ar+"):
+        cs.pop(name)
+    for sid, (name, n) in enumerate(dens.items()):
+        per_cell = ppc if name in ("Electron", "Ar+", "O2+") else max(1, ppc // 5)
+        np2c = n * dx * dx * 1.0 / per_cell
+        nmax = int(3 * per_cell * ncx * ncy)
+        cs[name] = {"S_ID": sid, "Density": n, "Temp(eV)": te if name == "Electron" else ti,
+                    "np2c": np2c, "Ratio": 1, "Max_np": nmax}
+    d["SimulationMethod"]["ParticleLimit"] = max(c["Max_np"] for c in cs.values() if isinstance(c, dict)
+                                                 and "Max_np" in c)
+    obj, keys = reaction_keys(gas)
+    d["Collision_On_Off"] = {obj: {k: 1.0 for k in keys}}
+    return d
+
+
+def config3():
+    """512x256 Ar/O2 CCP (O2 20 %), dx=0.2 mm, 150 V, 30 mTorr, 100 /cell for e,
+    Ar+ and O2+ and 20 /cell for O+ and O-, dt=T/800, 5 cycles.  Host setup
+    only in this build."""
+    return oxygen_deck(gas=2, o2_ratio=0.2, ncx=512, ncy=256, dx=2e-4, volts=150.0, pres_torr=0.03,
+                       ppc=100, steps_per_cycle=800, cycles=5)
+
+
 def config1():
     """64x64 Ar CCP, dx=0.5 mm, 100 V, 100 mTorr, 50 /cell/species, dt=T/400, 10 cycles."""
     return ccp_deck(ncx=64, ncy=64, dx=5e-4, volts=100.0, pres_torr=0.1, ppc=50,

@@ -139,6 +139,12 @@ int pic_setup_array_info(pic_setup_h *h, const char *name, size_t *count, int *e
 int pic_setup_get_array(pic_setup_h *h, const char *name, void *host, size_t bytes);
 int pic_setup_scalar(pic_setup_h *h, const char *name, double *value);
 void pic_setup_close(pic_setup_h *h);
+/* Reaction switches of gas type 0 (Ar), 1 (O2) or 2 (Ar/O2): the name of
+ * the deck's object under "Collision_On_Off", then every reaction key in
+ * reaction order, newline-separated (src/start.c:1003-1420); NULL for an
+ * unknown type.  Gas types 1 and 2 have host setup only: pic_create
+ * rejects them (-19) because their collision step is not built. */
+const char *pic_reaction_names(int gas);
 
 /* ---------------- reference-named zero-argument shims ---------------- */
 int pic_shim_open(const char *deck_path, int precision);

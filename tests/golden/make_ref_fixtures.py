@@ -9,10 +9,16 @@ target of oracle/Makefile), and stores its output as data:
   maxwell_vsave.json    size, every 37th entry and the sum of the
                         Maxwellian inverse-CDF table (maxwellv)
   xsec_raw_checksum.json  size and sums of each tabulated data set
+  o2_xsec_table.json, aro2_xsec_table.json
+                        the 81 x 58 oxygen and 81 x 68 argon/oxygen tables and
+                        their thresholds, every reaction switch on
+                        (Oxygen_CrossSectionSET / ArO2_CrossSectionSET
+                        arithmetic on the reference's own fit functions)
 
 Only runs where the reference tree is present:
     make -C oracle ref && python tests/golden/make_ref_fixtures.py
 """
+import base64
 import json
 import os
 import subprocess
@@ -61,6 +67,20 @@ def main():
                    "sum": float(np.sum(np.array(vs, dtype=np.float32).astype(np.float64)))}, f)
     with open(os.path.join(HERE, "xsec_raw_checksum.json"), "w") as f:
         json.dump({str(k): v for k, v in sets.items()}, f, indent=0)
+    # oxygen and argon/oxygen tables (float32, little-endian, base64) and thresholds
+    for tag, th_tag, name in (("O2TABLE", "O2TH", "o2_xsec_table.json"), ("AO2TABLE", "AO2TH", "aro2_xsec_table.json")):
+        k = next(i for i, ln in enumerate(lines) if ln.startswith(tag + " "))
+        n, m = (int(v) for v in lines[k].split()[1:])
+        tab = np.array([[float(v) for v in lines[k + 1 + r].split()] for r in range(n)], dtype=np.float32)
+        t = lines.index(th_tag)
+        th = np.array([float(v) for v in lines[t + 1].split()], dtype=np.float32)
+        assert tab.shape == (n, m) and th.shape == (m,)
+        with open(os.path.join(HERE, name), "w") as f:
+            json.dump({"source": "oracle/_ref/ref_dump (reference O2sigma* fits, CrossSection, FUNC_decay; "
+                                 "every reaction switch 1)",
+                       "rows": n, "cols": m,
+                       "table_f32_b64": base64.b64encode(tab.astype("<f4").tobytes()).decode(),
+                       "th_e_f32_b64": base64.b64encode(th.astype("<f4").tobytes()).decode()}, f, indent=0)
     print("fixtures written")
 
 
