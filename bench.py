@@ -104,18 +104,20 @@ def run_pbatch(n, reps=3):
     return out
 
 
-def run_ensemble(n):
+def run_ensemble(n, rate_hz):
     """A served ensemble of n argon CCP cases (grids 32^2-96^2, 20-60
     particles/cell, 10-500 mTorr, 50-300 V, 1-3 RF cycles of 400 steps,
-    Poisson arrivals at 40 /s), then the same cases one by one."""
+    Poisson arrivals), served first come, first served with one case
+    resident at a time (the concurrent mode is switched off)."""
     from picmcc import serve
-    reqs = serve.ar_ensemble(n)
-    res = serve.EnsembleServer().run(reqs)
-    alone = [serve.run_alone(r) for r in reqs]
-    out = serve.summarize(res, alone)
+    reqs = serve.ar_ensemble(n, rate_hz=rate_hz)
+    res = serve.SequentialServer().run(reqs)
+    out = serve.summarize(res)
+    out["mode"] = "sequential first-come first-served, one case resident (concurrent mode switched off)"
+    out["concurrent_mode"] = serve.DISABLED_REASON
     out["workload"] = (f"{n} Ar CCP cases, grids 32^2-96^2, 20-60 particles/cell, 10-500 mTorr log-uniform, "
-                       "50-300 V, 1-3 RF cycles of 400 steps, Poisson arrivals 40/s (Ar/O2 cases need the O2 "
-                       "chemistry, not built)")
+                       f"50-300 V, 1-3 RF cycles of 400 steps, Poisson arrivals {rate_hz:g}/s, latency = finish - "
+                       "arrival incl. setup and queueing (Ar/O2 cases need the O2 collision step, not built)")
     return out
 
 
@@ -152,9 +154,11 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--pbatch", type=int, default=256, metavar="N",
                     help="independent Poisson problems (64^2-512^2) in the batched-solve leg (0 = skip)")
-    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
-                    help="cases in the served argon ensemble (Poisson arrivals; 0 = skip; "
-                         "concurrent serving is currently switched off)")
+    ap.add_argument("--ensemble", type=int, default=16, metavar="N",
+                    help="cases in the served argon ensemble (Poisson arrivals, served first come first "
+                         "served with one case resident; 0 = skip)")
+    ap.add_argument("--ensemble-rate", type=float, default=4.0, metavar="HZ",
+                    help="mean arrival rate of the ensemble's cases")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="first run a device copy of MB MiB (a kernel of known traffic, for counter calibration)")
     args = ap.parse_args()
@@ -240,10 +244,9 @@ def main():
     if args.pbatch > 0:
         line["batched_poisson"] = run_pbatch(args.pbatch)
     if args.ensemble > 0:
-        line["ensemble"] = run_ensemble(args.ensemble)
+        line["ensemble"] = run_ensemble(args.ensemble, args.ensemble_rate)
     else:
-        from picmcc import serve
-        line["ensemble"] = {"status": "not run", "reason": serve.DISABLED_REASON}
+        line["ensemble"] = {"status": "not run (--ensemble 0)"}
     if args.cpu_steps > 0:
         cv, cdt = cpu_baseline(deck, args.cpu_steps)
         line["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",

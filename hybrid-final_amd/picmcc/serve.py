@@ -16,6 +16,10 @@ last slice completes; its slot and memory go to the next arrival.
 Each case's result is independent of what else runs: the step is
 deterministic per case, so a case run alone and the same case run in the
 mix give the same bits.
+
+The concurrent mode above is switched off (DISABLED_REASON).
+SequentialServer serves the same requests first come, first served, with
+one case resident at a time.
 """
 import collections
 import ctypes as C
@@ -162,6 +166,48 @@ class EnsembleServer:
         return results
 
 
+class SequentialServer:
+    """First-come first-served, one case resident at a time.
+
+    Each request waits until the device is free, then its case is created,
+    stepped to the end with fused steps (captured graphs on the FP32
+    separable-solver path), synchronised, read and destroyed.  Latency is
+    finish minus arrival, so it includes the queueing behind earlier cases.
+    This is the serving mode in use while the concurrent one is switched off
+    (DISABLED_REASON)."""
+
+    def __init__(self, precision=FP32_REFINED):
+        self.precision = precision
+
+    def run(self, requests, keep_state=None):
+        results = []
+        t0 = time.perf_counter()
+        for req in sorted(requests, key=lambda r: r.arrival):
+            wait = req.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            admit = time.perf_counter() - t0
+            f0, _ = device_free_bytes()
+            case = Case(req.deck, self.precision)
+            f1, _ = device_free_bytes()
+            inf = case.info()
+            np0 = sum(inf.np[:inf.nsp])
+            case.step(req.steps)
+            case.sync()
+            inf = case.info()
+            np1 = sum(inf.np[:inf.nsp])
+            rec = {"rid": req.rid, "arrival": req.arrival, "admit": admit, "finish": time.perf_counter() - t0,
+                   "steps": req.steps, "particle_steps": 0.5 * (np0 + np1) * req.steps,
+                   "footprint": max(f0 - f1, 0), "cells": (inf.ngx - 1) * (inf.ngy - 1)}
+            rec["latency"] = rec["finish"] - rec["arrival"]
+            rec["service"] = rec["finish"] - rec["admit"]
+            if keep_state and keep_state(req.rid):
+                rec["state"] = {"particles": [case.particles(s) for s in range(inf.nsp)], "den": case.array("den")}
+            case.close()
+            results.append(rec)
+        return results
+
+
 def run_alone(req, precision=FP32_REFINED, keep_state=False):
     """One case by itself: creation, all steps, synchronisation."""
     t0 = time.perf_counter()
@@ -185,6 +231,7 @@ def summarize(results, one_by_one=None):
     span = max(r["finish"] for r in results) - min(r["arrival"] for r in results)
     out = {"cases": len(results), "p50_latency_s": float(np.percentile(lat, 50)),
            "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": span,
+           "busy_fraction": (sum(r["service"] for r in results) / span) if "service" in results[0] else None,
            "aggregate_particle_steps_per_s": ps / span,
            "peak_footprint_bytes": max(r["footprint"] for r in results)}
     if one_by_one:
