@@ -437,7 +437,10 @@ void geometry(const Deck &d, Setup<R> &s) {
     SI(0, 0) = -B[0];
     SI(ncx + 1, 0) = -B[(ncx - 1) * ngy];
     SI(0, ncy + 1) = -B[ngy + ncy];
-    SI(ncx + 1, ncy + 1) = -B[(ncy - 1) * ngy + ncy];
+    // top-right: the reference reads node (ncy-1, ncy) (src/start.c:1848), a
+    // top-wall node while ncy <= ncx and past the node array when ncy > ncx;
+    // there the top-wall node (ncx-1, ncy) is used instead
+    SI(ncx + 1, ncy + 1) = -B[(size_t)(ncy <= ncx ? ncy - 1 : ncx - 1) * ngy + ncy];
     for (int i = 0; i < s.Csize; ++i) s.eps[i] = (R)(kEps0 * (double)s.eps_r[i]);
 }
 

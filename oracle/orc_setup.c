@@ -292,7 +292,9 @@ static void geometry(orc_case *c) {
     SI(0, 0) = -c->Boundary[0];
     SI(ncx + 1, 0) = -c->Boundary[(ncx - 1) * ngy];
     SI(0, ncy + 1) = -c->Boundary[ngy + ncy];
-    SI(ncx + 1, ncy + 1) = -c->Boundary[(ncy - 1) * ngy + ncy];
+    /* src/start.c:1848 reads node (ncy-1, ncy), past the node array when
+     * ncy > ncx; there the top-wall node (ncx-1, ncy) is used */
+    SI(ncx + 1, ncy + 1) = -c->Boundary[(size_t)(ncy <= ncx ? ncy - 1 : ncx - 1) * ngy + ncy];
 #undef SI
     for (i = 0; i < c->Csize; ++i) c->eps[i] = (real)(EPS0 * (double)c->eps_r[i]);
 }

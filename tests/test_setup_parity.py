@@ -32,7 +32,32 @@ def deck_variants():
     d["Geometry"]["Source"][0].update({"DC(V)": -20.0, "Phase(deg)": 30.0})
     out["anisotropic"] = d
     out["config1"] = decks.config1()
+    # taller than wide: the reference's top-right structure-map corner reads
+    # past its node array here (src/start.c:1848)
+    out["tall"] = decks.small(ncx=6, ncy=20, ppc=4)
     return out
+
+
+@pytest.mark.parametrize("ncx,ncy", [(6, 20), (12, 9), (10, 10)])
+def test_structure_map_top_right_corner(ncx, ncy):
+    """The padded map's top-right corner holds minus the boundary code of a
+    top-wall node: node (ncy-1, ncy) as the reference reads it while that
+    node exists (ncy <= ncx), node (ncx-1, ncy) otherwise."""
+    d = decks.small(ncx=ncx, ncy=ncy, ppc=2)
+    # the top wall is Dirichlet up to x = ncx - 3 and Neumann beyond, so
+    # top-wall nodes differ and the corner shows which one was read
+    bc = d["Geometry"]["BoundaryCondition"]
+    bc[1]["X1"] = ncx - 3
+    bc.append({"X0": ncx - 2, "X1": ncx, "Y0": ncy, "Y1": ncy, "B.C": 2, "Temp": 300.0})
+    h = HostSetup(d, FP32_REFINED, load=False)
+    ngy = ncy + 1
+    B = h.array("Boundary")
+    SI = h.array("StructIdx").reshape(ncx + 2, ncy + 2)
+    node = (ncy - 1 if ncy <= ncx else ncx - 1) * ngy + ncy
+    assert SI[ncx + 1, ncy + 1] == -B[node]
+    if ncy < ncx:   # the two candidate nodes really differ here
+        assert B[node] != B[(ncx - 1) * ngy + ncy]
+    assert np.array_equal(SI, Oracle(d, "f32").arr("StructIdx").reshape(ncx + 2, ncy + 2))
 
 
 @pytest.mark.parametrize("name", list(deck_variants()))
