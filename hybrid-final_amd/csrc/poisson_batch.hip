@@ -24,12 +24,12 @@
 #include <vector>
 
 #include "../../include/picmcc.h"
+#include "dgemm_mfma.h"
 
 namespace pic {
 namespace {
 
 constexpr int GT = 16;    // GEMM output tile edge
-constexpr int GK = 272;   // K panel (GT * GK is a multiple of GT * GT)
 constexpr int RB = 256;   // residual block
 
 struct GemmTask {          // one 16x16 output tile of one problem's GEMM
@@ -105,7 +105,9 @@ __global__ void k_pb_finalize(int nprob, const int *bstart, const double *part, 
     if (conv || last) { done[p] = 1; status[p] = conv ? 0 : 1; }
 }
 
-// batched C = A B per problem; MODE selects the operands:
+// batched C = A B per problem, one 16x16 tile per block on the FP64 matrix
+// cores (dgemm_tile_mfma, K split over the block's 4 waves); MODE selects
+// the operands:
 // 0: T1 = Vx^-1 R, 1: T2 = (T1 Vy^-T) .* dinv, 2: T1 = Vx T2, 3: x += T1 Vy^T
 template <int MODE>
 __global__ __launch_bounds__(GT *GT) void k_pb_gemm(const ProbDev *P, const GemmTask *tasks, const double *R,
@@ -117,40 +119,17 @@ __global__ __launch_bounds__(GT *GT) void k_pb_gemm(const ProbDev *P, const Gemm
     const int K = (MODE == 0 || MODE == 2) ? q.nx : q.ny;
     const double *A = MODE == 0 ? q.Vxi : MODE == 1 ? T1 + q.off : MODE == 2 ? q.Vx : T1 + q.off;
     const double *B = MODE == 0 ? R + q.off : MODE == 1 ? q.VyiT : MODE == 2 ? T2 + q.off : q.VyT;
-    __shared__ double As[GT][GK + 1];
-    __shared__ double Bs[GK][GT];
-    const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
-    double acc0 = 0.0, acc1 = 0.0;
-    constexpr int PER = (GT * GK) / (GT * GT);
-    for (int kb = 0; kb < K; kb += GK) {
-        const int kn = min(GK, K - kb);
-        double ta[PER], tb[PER];
+    __shared__ double part[4][256];
+    const v4d acc = dgemm_tile_mfma(M, N, K, A, B, tk.r0, tk.c0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = (int)threadIdx.x + m * GT * GT;
-            const int rr = e / GK, kk = e % GK, gr = tk.r0 + rr;
-            ta[m] = (kk < kn && gr < M) ? A[(size_t)gr * K + kb + kk] : 0.0;
-            const int kb2 = e / GT, cc = e % GT, gc = tk.c0 + cc;
-            tb[m] = (kb2 < kn && gc < N) ? B[(size_t)(kb + kb2) * N + gc] : 0.0;
-        }
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = (int)threadIdx.x + m * GT * GT;
-            As[e / GK][e % GK] = ta[m];
-            Bs[e / GT][e % GT] = tb[m];
-        }
-        __syncthreads();
-        int kk = 0;
-        for (; kk + 2 <= kn; kk += 2) {
-            acc0 = __builtin_fma(As[ty][kk], Bs[kk][tx], acc0);
-            acc1 = __builtin_fma(As[ty][kk + 1], Bs[kk + 1][tx], acc1);
-        }
-        if (kk < kn) acc0 = __builtin_fma(As[ty][kk], Bs[kk][tx], acc0);
-        __syncthreads();
-    }
-    const int r = tk.r0 + ty, c = tk.c0 + tx;
+    for (int j = 0; j < 4; ++j) part[wv][lane * 4 + j] = acc[j];
+    __syncthreads();
+    // thread t adds the four waves' accumulator slot t (lane t/4, register t%4)
+    const int t = threadIdx.x, l = t >> 2;
+    const double v = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+    const int r = tk.r0 + 4 * (t & 3) + (l >> 4), c = tk.c0 + (l & 15);
     if (r >= M || c >= N) return;
-    const double v = acc0 + acc1;
     const size_t o = (size_t)q.off + (size_t)r * N + c;
     if (MODE == 0) T1[o] = v;
     else if (MODE == 1) T2[o] = v * q.dinv[(size_t)r * N + c];

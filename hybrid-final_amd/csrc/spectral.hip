@@ -16,11 +16,12 @@
 // once converged.
 //
 // At these sizes (n <= ~1000 per side) the solve is bound by launch and
-// memory latency, not by FLOPs: each GEMM block stages its operand panels
-// through LDS in one round trip and computes one 16x16 output tile, so a
-// 256^2 problem fills the chip with ~256 blocks.
+// memory latency, not by FLOPs: each GEMM block computes one 16x16 output
+// tile on the FP64 matrix cores with K split over its 4 waves, so a 256^2
+// problem runs ~256 blocks of 4 waves from L2-resident operands.
 #include <hip/hip_runtime.h>
 
+#include "dgemm_mfma.h"
 #include "pic_kernels.h"
 
 namespace pic {
@@ -111,62 +112,33 @@ __global__ __launch_bounds__(RES_BLOCK) void k_spec_finalize(const double *parti
     }
 }
 
-// C = A B (row-major, A: M x K, B: K x N) in FP64, one 16x16 output tile per
-// 256-thread block, one output per thread.  The block's A rows and B columns
-// are staged through LDS in K-panels of GK: every thread issues all its
-// panel loads at once (one memory round trip per panel), then accumulates
-// from LDS (A broadcast along a tile row, B conflict-free along a row).
-// MODE 0: store; MODE 1: store C .* scale; MODE 2: x64 += C.
-constexpr int GK = 272;   // GT * GK is a multiple of the block size
+// C = A B (row-major, A: M x K, B: K x N) in FP64 on the matrix cores
+// (dgemm_tile_mfma), one 16x16 output tile per 256-thread block with K split
+// over its 4 waves.  MODE 0: store; MODE 1: store C .* scale; MODE 2:
+// x64 += C.
 template <int MODE>
-__global__ __launch_bounds__(GT *GT) void k_dgemm(int M, int N, int K, const double *__restrict__ A,
-                                                 const double *__restrict__ B, double *__restrict__ C,
-                                                 const double *__restrict__ scale, double *__restrict__ x64,
-                                                 const SpecState *s) {
+__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double *__restrict__ A,
+                                               const double *__restrict__ B, double *__restrict__ C,
+                                               const double *__restrict__ scale, double *__restrict__ x64,
+                                               const SpecState *s) {
     if (s->done) return;   // uniform over the grid: no thread reaches a barrier alone
-    __shared__ double As[GT][GK + 1];
-    __shared__ double Bs[GK][GT];
-    const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
+    __shared__ double part[4][256];
     const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
-    double acc0 = 0.0, acc1 = 0.0;
-    for (int kb = 0; kb < K; kb += GK) {
-        const int kn = min(GK, K - kb);
-        // A panel: GT rows x kn, B panel: kn rows x GT (zero outside the
-        // matrix); fixed trip counts so that all loads issue before the
-        // first LDS store
-        static_assert((GT * GK) % (GT * GT) == 0, "panel size");
-        constexpr int PER = (GT * GK) / (GT * GT);
-        double ta[PER], tb[PER];
+    const v4d acc = dgemm_tile_mfma(M, N, K, A, B, r0, c0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = (int)threadIdx.x + m * GT * GT;
-            const int rr = e / GK, kk = e % GK, gr = r0 + rr;
-            ta[m] = (kk < kn && gr < M) ? A[(size_t)gr * K + kb + kk] : 0.0;
-            const int kb2 = e / GT, cc = e % GT, gc = c0 + cc;
-            tb[m] = (kb2 < kn && gc < N) ? B[(size_t)(kb + kb2) * N + gc] : 0.0;
-        }
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int e = (int)threadIdx.x + m * GT * GT;
-            As[e / GK][e % GK] = ta[m];
-            Bs[e / GT][e % GT] = tb[m];
-        }
-        __syncthreads();
-        int kk = 0;
-        for (; kk + 2 <= kn; kk += 2) {
-            acc0 = __builtin_fma(As[ty][kk], Bs[kk][tx], acc0);
-            acc1 = __builtin_fma(As[ty][kk + 1], Bs[kk + 1][tx], acc1);
-        }
-        if (kk < kn) acc0 = __builtin_fma(As[ty][kk], Bs[kk][tx], acc0);
-        __syncthreads();
-    }
-    const int r = r0 + ty, c = c0 + tx;
+    for (int v = 0; v < 4; ++v) part[wv][lane * 4 + v] = acc[v];
+    __syncthreads();
+    // thread t adds the four waves' accumulator slot t (lane t/4, register t%4)
+    const int t = threadIdx.x;
+    const int l = t >> 2, v = t & 3;
+    const double sum = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+    const int r = r0 + 4 * v + (l >> 4), c = c0 + (l & 15);
     if (r >= M || c >= N) return;
-    const double v = acc0 + acc1;
     const size_t o = (size_t)r * N + c;
-    if (MODE == 0) C[o] = v;
-    else if (MODE == 1) C[o] = v * scale[o];
-    else x64[o] += v;
+    if (MODE == 0) C[o] = sum;
+    else if (MODE == 1) C[o] = sum * scale[o];
+    else x64[o] += sum;
 }
 
 }  // namespace
