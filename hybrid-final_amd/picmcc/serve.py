@@ -87,6 +87,13 @@ class _Active:
         self.np0 = sum(inf.np[:inf.nsp])
 
 
+# Concurrent serving was switched off after two GPU memory faults in runs
+# that stepped several cases at once.  Their cause: the host setup read past
+# the node array for the structure map's top-right corner on grids taller
+# than wide (the served decks had such grids), so a route of the corner cell
+# could carry an arbitrary conductor, dielectric or cell code into the
+# kernels.  That read is fixed; CONCURRENT is on again.
+CONCURRENT = True
 DISABLED_REASON = (
     "concurrent ensemble serving is switched off: both GPU runs that stepped several cases at once "
     "(graph-captured steps on up to four streams) were followed by a GPU memory fault whose cause is "
@@ -110,7 +117,8 @@ class EnsembleServer:
     def run(self, requests, keep_state=None):
         """Serve `requests` in real time; returns one record per case.
         keep_state(rid) -> bool: return that case's final particles and density."""
-        raise RuntimeError(DISABLED_REASON)
+        if not CONCURRENT:
+            raise RuntimeError(DISABLED_REASON)
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         active, results = [], []
         used = 0

@@ -36,7 +36,7 @@ def test_sequential_serving_is_fifo_and_exact():
                 assert np.array_equal(alone["state"]["particles"][s_][k], got["particles"][s_][k])
 
 
-@pytest.mark.skip(reason=serve.DISABLED_REASON)
+@pytest.mark.skipif(not serve.CONCURRENT, reason=serve.DISABLED_REASON)
 def test_results_same_alone_and_in_the_mix():
     # dt = T_RF/400: these decks stay within their particle capacity (checked
     # with the CPU oracle); a much coarser dt makes them run away
