@@ -107,14 +107,16 @@ def run_pbatch(n, reps=3):
 def run_ensemble(n, rate_hz):
     """A served ensemble of n argon CCP cases (grids 32^2-96^2, 20-60
     particles/cell, 10-500 mTorr, 50-300 V, 1-3 RF cycles of 400 steps,
-    Poisson arrivals), served first come, first served with one case
-    resident at a time (the concurrent mode is switched off)."""
+    Poisson arrivals): the concurrent server (continuous admission, each
+    case on its own stream), the same requests served first come first
+    served with one case resident, and the same cases run one by one."""
     from picmcc import serve
     reqs = serve.ar_ensemble(n, rate_hz=rate_hz)
-    res = serve.SequentialServer().run(reqs)
-    out = serve.summarize(res)
-    out["mode"] = "sequential first-come first-served, one case resident (concurrent mode switched off)"
-    out["concurrent_mode"] = serve.DISABLED_REASON
+    out = serve.summarize(serve.EnsembleServer().run(reqs), [serve.run_alone(r) for r in reqs])
+    out["mode"] = "concurrent: continuous admission by device-memory budget, one stream per case, graph replays"
+    seq = serve.summarize(serve.SequentialServer().run(reqs))
+    out["sequential"] = {k: seq[k] for k in ("p50_latency_s", "p99_latency_s", "makespan_s", "busy_fraction",
+                                             "aggregate_particle_steps_per_s")}
     out["workload"] = (f"{n} Ar CCP cases, grids 32^2-96^2, 20-60 particles/cell, 10-500 mTorr log-uniform, "
                        f"50-300 V, 1-3 RF cycles of 400 steps, Poisson arrivals {rate_hz:g}/s, latency = finish - "
                        "arrival incl. setup and queueing (Ar/O2 cases need the O2 collision step, not built)")
@@ -155,9 +157,9 @@ def main():
     ap.add_argument("--pbatch", type=int, default=256, metavar="N",
                     help="independent Poisson problems (64^2-512^2) in the batched-solve leg (0 = skip)")
     ap.add_argument("--ensemble", type=int, default=16, metavar="N",
-                    help="cases in the served argon ensemble (Poisson arrivals, served first come first "
-                         "served with one case resident; 0 = skip)")
-    ap.add_argument("--ensemble-rate", type=float, default=4.0, metavar="HZ",
+                    help="cases in the served argon ensemble (Poisson arrivals; concurrent server, first-come "
+                         "first-served server and one by one; 0 = skip)")
+    ap.add_argument("--ensemble-rate", type=float, default=8.0, metavar="HZ",
                     help="mean arrival rate of the ensemble's cases")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="first run a device copy of MB MiB (a kernel of known traffic, for counter calibration)")
