@@ -17,7 +17,6 @@ Each case's result is independent of what else runs: the step is
 deterministic per case, so a case run alone and the same case run in the
 mix give the same bits.
 
-The concurrent mode above is switched off (DISABLED_REASON).
 SequentialServer serves the same requests first come, first served, with
 one case resident at a time.
 """
@@ -94,18 +93,14 @@ class _Active:
 # could carry an arbitrary conductor, dielectric or cell code into the
 # kernels.  That read is fixed; CONCURRENT is on again.
 CONCURRENT = True
-DISABLED_REASON = (
-    "concurrent ensemble serving is switched off: both GPU runs that stepped several cases at once "
-    "(graph-captured steps on up to four streams) were followed by a GPU memory fault whose cause is "
-    "not yet shown (a CPU emulation of the same cases, with AddressSanitizer and interleaved stream "
-    "work, finds no out-of-bounds access)")
+DISABLED_REASON = "concurrent ensemble serving is switched off (serve.CONCURRENT)"
 
 
 class EnsembleServer:
     """mem_fraction: share of the free device memory the server may fill;
     slice_steps: steps enqueued per case per feed; max_active: slot count.
 
-    run() raises while concurrent serving is switched off (DISABLED_REASON)."""
+    run() raises if CONCURRENT is off (DISABLED_REASON)."""
 
     def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=64):
         self.precision = precision
@@ -180,9 +175,7 @@ class SequentialServer:
     Each request waits until the device is free, then its case is created,
     stepped to the end with fused steps (captured graphs on the FP32
     separable-solver path), synchronised, read and destroyed.  Latency is
-    finish minus arrival, so it includes the queueing behind earlier cases.
-    This is the serving mode in use while the concurrent one is switched off
-    (DISABLED_REASON)."""
+    finish minus arrival, so it includes the queueing behind earlier cases."""
 
     def __init__(self, precision=FP32_REFINED):
         self.precision = precision
