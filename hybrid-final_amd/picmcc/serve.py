@@ -1,9 +1,11 @@
 """Continuous-batching ensemble server for discharge cases on one GPU.
 
 Requests (a deck and a step count, with an arrival time) are admitted while
-the device memory budget allows it.  The budget check uses a footprint
-estimate from the deck; the footprint of an admitted case is then measured
-(device free memory before and after its creation).  Every admitted case
+the device memory budget allows it, charged with a footprint estimate from
+the deck.  An admitted case is created (host setup, uploads) on a worker
+thread while the loop keeps feeding the running cases; the library's calls
+release the interpreter lock, and step graphs are captured in thread-local
+mode, so creations on other threads do not disturb a capture.  Every case
 keeps its particles, fields and random-stream state resident in HBM on its
 own HIP stream.
 
@@ -22,6 +24,7 @@ one case resident at a time.
 """
 import collections
 import ctypes as C
+from concurrent.futures import ThreadPoolExecutor
 import math
 import random
 import time
@@ -98,14 +101,22 @@ DISABLED_REASON = "concurrent ensemble serving is switched off (serve.CONCURRENT
 
 class EnsembleServer:
     """mem_fraction: share of the free device memory the server may fill;
-    slice_steps: steps enqueued per case per feed; max_active: slot count.
+    slice_steps: steps enqueued per case per feed; max_active: slot count;
+    setup_threads: host threads that create admitted cases (deck parsing,
+    geometry, operator, Laplace solves, particle loading, uploads) while the
+    loop keeps feeding the running cases; 0 creates them in the loop.
+
+    Admission charges the budget with the deck's footprint estimate (device
+    memory is allocated by several creations at once, so a free-memory
+    difference would not belong to one case).
 
     run() raises if CONCURRENT is off (DISABLED_REASON)."""
 
-    def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=64):
+    def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=64, setup_threads=2):
         self.precision = precision
         self.slice = slice_steps
         self.max_active = max_active
+        self.setup_threads = setup_threads
         free, _ = device_free_bytes()
         self.budget = int(free * mem_fraction)
 
@@ -115,58 +126,73 @@ class EnsembleServer:
         if not CONCURRENT:
             raise RuntimeError(DISABLED_REASON)
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        active, results = [], []
+        active, creating, results = [], [], []
         used = 0
+        pool = ThreadPoolExecutor(self.setup_threads) if self.setup_threads > 0 else None
         t0 = time.perf_counter()
-        while pending or active:
-            now = time.perf_counter() - t0
-            # admission: arrivals that fit in the slot count and the memory budget
-            while pending and pending[0].arrival <= now and len(active) < self.max_active:
-                est = footprint_estimate(pending[0].deck, self.precision)
-                if active and used + est > self.budget:
-                    break
-                req = pending.popleft()
-                f0, _ = device_free_bytes()
-                case = Case(req.deck, self.precision)
-                f1, _ = device_free_bytes()
-                fp = max(f0 - f1, 0)
-                used += fp
-                active.append(_Active(req, case, time.perf_counter() - t0, fp))
-            # feed slices; retire finished cases
-            busy = False
-            for a in list(active):
-                if a.inflight:
-                    if not a.case.done():
-                        busy = True
-                        continue
-                    a.inflight = False
-                if a.remaining == 0:
-                    a.case.sync()
-                    inf = a.case.info()
-                    np1 = sum(inf.np[:inf.nsp])
-                    rec = {"rid": a.req.rid, "arrival": a.req.arrival, "admit": a.admit,
-                           "finish": time.perf_counter() - t0, "steps": a.req.steps,
-                           "particle_steps": 0.5 * (a.np0 + np1) * a.req.steps, "footprint": a.footprint,
-                           "cells": (inf.ngx - 1) * (inf.ngy - 1)}
-                    rec["latency"] = rec["finish"] - rec["arrival"]
-                    if keep_state and keep_state(a.req.rid):
-                        rec["state"] = {"particles": [a.case.particles(s) for s in range(inf.nsp)],
-                                        "den": a.case.array("den")}
-                    a.case.close()
-                    used -= a.footprint
-                    active.remove(a)
-                    results.append(rec)
-                    continue
-                k = min(self.slice, a.remaining)
-                a.case.step(k)
-                a.remaining -= k
-                a.inflight = True
-                busy = True
-            if not busy and pending:
-                time.sleep(max(0.0, min(0.002, pending[0].arrival - (time.perf_counter() - t0))))
-            elif busy:
-                time.sleep(0.0002)
+        try:
+            while pending or active or creating:
+                now = time.perf_counter() - t0
+                # admission: arrivals that fit in the slot count and the memory budget
+                while pending and pending[0].arrival <= now and len(active) + len(creating) < self.max_active:
+                    est = footprint_estimate(pending[0].deck, self.precision)
+                    if (active or creating) and used + est > self.budget:
+                        break
+                    req = pending.popleft()
+                    used += est
+                    admit = time.perf_counter() - t0
+                    if pool is None:
+                        active.append(_Active(req, Case(req.deck, self.precision), admit, est))
+                    else:
+                        creating.append((req, pool.submit(Case, req.deck, self.precision), admit, est))
+                # cases whose host setup has finished join the running set
+                for item in list(creating):
+                    req, fut, admit, est = item
+                    if fut.done():
+                        creating.remove(item)
+                        active.append(_Active(req, fut.result(), admit, est))
+                used -= self._feed(active, results, keep_state, t0)
+                busy = any(a.inflight for a in active) or bool(creating)
+                if not busy and pending:
+                    time.sleep(max(0.0, min(0.002, pending[0].arrival - (time.perf_counter() - t0))))
+                elif busy:
+                    time.sleep(0.0002)
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True)
         return results
+
+    def _feed(self, active, results, keep_state, t0):
+        """Gives every idle case its next slice of steps and retires the cases
+        that have finished; returns the footprint freed by the retired ones."""
+        freed = 0
+        for a in list(active):
+            if a.inflight:
+                if not a.case.done():
+                    continue
+                a.inflight = False
+            if a.remaining == 0:
+                a.case.sync()
+                inf = a.case.info()
+                np1 = sum(inf.np[:inf.nsp])
+                rec = {"rid": a.req.rid, "arrival": a.req.arrival, "admit": a.admit,
+                       "finish": time.perf_counter() - t0, "steps": a.req.steps,
+                       "particle_steps": 0.5 * (a.np0 + np1) * a.req.steps, "footprint": a.footprint,
+                       "cells": (inf.ngx - 1) * (inf.ngy - 1)}
+                rec["latency"] = rec["finish"] - rec["arrival"]
+                if keep_state and keep_state(a.req.rid):
+                    rec["state"] = {"particles": [a.case.particles(s) for s in range(inf.nsp)],
+                                    "den": a.case.array("den")}
+                a.case.close()
+                freed += a.footprint
+                active.remove(a)
+                results.append(rec)
+                continue
+            k = min(self.slice, a.remaining)
+            a.case.step(k)
+            a.remaining -= k
+            a.inflight = True
+        return freed
 
 
 class SequentialServer:
