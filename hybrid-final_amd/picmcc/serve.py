@@ -112,7 +112,7 @@ class EnsembleServer:
 
     run() raises if CONCURRENT is off (DISABLED_REASON)."""
 
-    def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=64, setup_threads=2):
+    def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=16, setup_threads=2):
         self.precision = precision
         self.slice = slice_steps
         self.max_active = max_active
