@@ -72,8 +72,19 @@ def deck_inner_conductor():
     return _variant("inner_conductor")
 
 
-DECKS = [deck_small, deck_dense, deck_dielectric, deck_inner_conductor]
-DECK_IDS = ["small", "dense", "dielectric", "inner_conductor"]
+def deck_tall():
+    """Taller than wide (6 x 20 cells): the top-right corner cell's diagonal
+    route goes through the structure map's padded corner."""
+    return _variant("tall")
+
+
+def deck_anisotropic():
+    """dx != dy (0.7 mm x 0.5 mm cells), a DC bias and a phase on the source."""
+    return _variant("anisotropic")
+
+
+DECKS = [deck_small, deck_dense, deck_dielectric, deck_inner_conductor, deck_tall, deck_anisotropic]
+DECK_IDS = ["small", "dense", "dielectric", "inner_conductor", "tall", "anisotropic"]
 
 
 @pytest.mark.parametrize("deck", DECKS, ids=DECK_IDS)
