@@ -462,6 +462,11 @@ constexpr int SCHUNK = 256;   // pass 2 chunk (larger chunks measured faster: mo
 constexpr int FWAVES = 4;   // waves per 256-thread block
 enum { KIND_AR_ELECTRON = 0, KIND_AR_ION = 1 };
 enum { FLAG_CODE = 15, FLAG_DONE = 16, FLAG_PROD = 32 };
+// Collision events are counted per (wave, route code, reaction) in LDS and
+// added to the per-node counters once per wave at the end of the pass
+// (integer sums: the totals do not depend on the order); reactions beyond
+// EV_MAX go to the node counters directly.
+constexpr int EV_MAX = 10;
 
 template <typename R, int C = FCHUNK>
 struct FStage {
@@ -487,6 +492,7 @@ struct FPool {
     R pa[FWAVES][ROUTE_SLOTS];
     int prun[FWAVES][ROUTE_SLOTS];
     int rounds[FWAVES];
+    int ev[FWAVES][ROUTE_SLOTS][EV_MAX];   // collision events per route slot, flushed at the end
 };
 
 // global -> LDS for particles [b0, b0+n), n >= 1.  Lanes past the end load the
@@ -571,6 +577,8 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
     if (lane < ROUTE_SLOTS) {
         pool.dest[wv][lane] = w.dest; pool.info[wv][lane] = w.info; pool.pa[wv][lane] = w.pa;
         pool.prun[wv][lane] = 0;
+#pragma unroll
+        for (int r = 0; r < EV_MAX; ++r) pool.ev[wv][lane][r] = 0;
     }
     if (lane == 0) pool.rounds[wv] = (k1 - k0 + FCHUNK - 1) / FCHUNK;
     if (threadIdx.x == 0) pool.qn = 0;
@@ -643,7 +651,8 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
                 R evx, evy, evz, ivx, ivy, ivz;
                 const int ev = collide<R, KIND>(k, dest, pa, T.id[i], step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy,
                                                 ivz);
-                if (ev >= 0) atomicAdd(&mccr[(size_t)dest * k.TnRct + ev], 1ull);
+                if (ev >= 0 && ev < EV_MAX) atomicAdd(&pool.ev[ww][code][ev], 1);
+                else if (ev >= 0) atomicAdd(&mccr[(size_t)dest * k.TnRct + ev], 1ull);
                 const bool prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
                 const int kk = pool.b0[ww] + i;
                 p.x[kk] = T.x[i]; p.y[kk] = T.y[i]; p.vx[kk] = vx; p.vy[kk] = vy; p.vz[kk] = vz;
@@ -664,6 +673,12 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
             cnt[c * ROUTE_SLOTS + s] = run[s];
             if (pcnt) pcnt[c * ROUTE_SLOTS + s] = pool.prun[wv][s];
         }
+    }
+    // event counts of this wave's cell, one lane per (route code, reaction)
+    for (int e = lane; e < ROUTE_SLOTS * EV_MAX; e += 64) {
+        const int code = e / EV_MAX, r = e % EV_MAX;
+        const int n = pool.ev[wv][code][r];
+        if (n) atomicAdd(&mccr[(size_t)pool.dest[wv][code] * k.TnRct + r], (unsigned long long)n);
     }
 }
 
