@@ -127,7 +127,6 @@ struct Case : CaseBase {
     int *cnt[MAXSP] = {nullptr}, *base[MAXSP] = {nullptr}, *pbase[MAXSP] = {nullptr};
     int *pcnt = nullptr, *pid_off = nullptr, *total = nullptr, *alive = nullptr, *ptotal = nullptr;
     int *total2 = nullptr, *alive2 = nullptr;
-    int *ent_dest = nullptr, *ent_pos = nullptr;
     StepClock *clk = nullptr;   // device clock of the fused step (the step about to run)
     unsigned long long *eedf = nullptr;   // electron energy histogram (EEDF_BINS counts)
     double eedf_emax = 0;
@@ -187,7 +186,7 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey, total2, alive2, ent_dest, ent_pos, clk, eedf, lost, sigma_hits};
+                        sum_ey, total2, alive2, clk, eedf, lost, sigma_hits};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (int s = 0; s < MAXSP; ++s) {
@@ -344,20 +343,10 @@ struct Case : CaseBase {
         std::vector<int> istart, ient;
         build_routes(rt, istart, ient);
         HIPCHK(upload(&route, rt, st)); HIPCHK(upload(&inv_start, istart, st)); HIPCHK(upload(&inv_ent, ient, st));
-        {
-            // per (cell, slot) entry: its destination cell and its place in
-            // that cell's canonical list of incoming entries
-            std::vector<int> edest((size_t)G * ROUTE_SLOTS, -1), epos((size_t)G * ROUTE_SLOTS, 0);
-            for (int d = 0; d < G; ++d)
-                if (istart[d + 1] - istart[d] > ROUTE_SLOTS)
-                    return fail(-43, "a cell receives particles from more than 9 source entries");
-            for (int d = 0; d < G; ++d)
-                for (int q = istart[d]; q < istart[d + 1]; ++q) {
-                    edest[ient[q]] = d;
-                    epos[ient[q]] = q - istart[d];
-                }
-            HIPCHK(upload(&ent_dest, edest, st)); HIPCHK(upload(&ent_pos, epos, st));
-        }
+        // the bases kernels walk at most 9 incoming entries per cell
+        for (int d = 0; d < G; ++d)
+            if (istart[d + 1] - istart[d] > ROUTE_SLOTS)
+                return fail(-43, "a cell receives particles from more than 9 source entries");
         // field operator in padded 5-entry rows
         {
             const int N = S.A_size;
@@ -660,15 +649,9 @@ struct Case : CaseBase {
         // survivors + ionisation products per destination cell for both
         // species, their three scans (electron and ion cell_start, global
         // product order), then the base of every (cell, slot) entry
-        launch_totals_pair(G, inv_start, inv_ent, cnt[0], cnt[1], pcnt, total, alive, total2, alive2, ptotal, st);
-        {
-            const int *ins[3] = {total, total2, ptotal};
-            int *outs[3] = {nxt[0].cell_start, nxt[1].cell_start, pstart};
-            launch_scan_multi(3, ins, outs, G, scan_tmp, st);
-        }
-        launch_assign_bases_pair(G, ent_dest, ent_pos, inv_start, inv_ent, cnt[0], cnt[1], pcnt, nxt[0].cell_start,
-                                 nxt[1].cell_start, alive, alive2, pstart, base[0], base[1], pbase[0], pbase[1],
-                                 pid_off, st);
+        launch_bases_pair(G, inv_start, inv_ent, cnt[0], cnt[1], pcnt, total, alive, total2, alive2, ptotal,
+                          nxt[0].cell_start, nxt[1].cell_start, pstart, base[0], base[1], pbase[0], pbase[1], pid_off,
+                          scan_tmp, st);
         mark(5);
         launch_fused_scatter<R>(kc, cur[0], nxt[0], nxt[1], 0, tag[0], stage, sstride, base[0], pbase[0], pbase[1],
                                 pid_off, next_id, nxt[0].cap, nxt[1].cap, d_err, st);

@@ -191,67 +191,6 @@ __global__ void k_assign_bases(int Gsize, const int *inv_start, const int *inv_e
     }
 }
 
-// Electrons and ions of the fused step in one pass per destination cell d:
-// survivors of each species, plus the ionisation products of d (the same
-// count for both species: one electron and one ion per event).
-__global__ void k_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
-                              const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i,
-                              int *prod_total) {
-    int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= Gsize) return;
-    const int e0 = inv_start[d], ne = inv_start[d + 1] - e0;   // ne <= 9, checked at set-up
-    int ent[ROUTE_SLOTS];
-#pragma unroll
-    for (int q = 0; q < ROUTE_SLOTS; ++q) ent[q] = q < ne ? inv_ent[e0 + q] : -1;
-    int ae = 0, ai = 0, pr = 0;
-#pragma unroll
-    for (int q = 0; q < ROUTE_SLOTS; ++q) {
-        if (ent[q] >= 0) {
-            ae += cnt_e[ent[q]];
-            ai += cnt_i[ent[q]];
-            pr += prod_cnt[ent[q]];
-        }
-    }
-    total_e[d] = ae + pr; alive_e[d] = ae;
-    total_i[d] = ai + pr; alive_i[d] = ai;
-    prod_total[d] = pr;
-}
-
-// One thread per (source cell, slot) entry: its base in the destination's
-// new bucket is the destination's start plus the counts of the entries
-// before it in the destination's canonical list (same sums as walking the
-// list per destination, but every output is written by its own thread).
-__global__ void k_assign_bases_pair(int nent, const int *ent_dest, const int *ent_pos, const int *inv_start,
-                                    const int *inv_ent, const int *cnt_e, const int *cnt_i, const int *prod_cnt,
-                                    const int *start_e, const int *start_i, const int *alive_e, const int *alive_i,
-                                    const int *prod_start, int *base_e, int *base_i, int *pbase_e, int *pbase_i,
-                                    int *pid_off) {
-    const int ent = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ent >= nent) return;
-    const int d = ent_dest[ent];
-    if (d < 0) return;
-    const int e0 = inv_start[d], pos = ent_pos[ent];
-    // a destination has at most 9 incoming entries (its 3x3 neighbourhood):
-    // unrolled so that the list loads, then the count loads, issue together
-    int e2[ROUTE_SLOTS];
-#pragma unroll
-    for (int q = 0; q < ROUTE_SLOTS; ++q) e2[q] = q < pos ? inv_ent[e0 + q] : -1;
-    int se = 0, si = 0, sp = 0;
-#pragma unroll
-    for (int q = 0; q < ROUTE_SLOTS; ++q) {
-        if (e2[q] >= 0) {
-            se += cnt_e[e2[q]];
-            si += cnt_i[e2[q]];
-            sp += prod_cnt[e2[q]];
-        }
-    }
-    base_e[ent] = start_e[d] + se;
-    base_i[ent] = start_i[d] + si;
-    pbase_e[ent] = start_e[d] + alive_e[d] + sp;
-    pbase_i[ent] = start_i[d] + alive_i[d] + sp;
-    pid_off[ent] = prod_start[d] + sp;
-}
-
 // Exclusive scans of up to 3 int arrays of length n (out[n] = total) in two
 // launches: per-tile sums, then each tile adds the sums of the tiles before
 // it and scans locally.  Tiles are SCAN_TILE = 4 * 1024 elements.
@@ -333,6 +272,146 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(ScanJobs J, int n, int ntil
         run += v[q];
     }
     if (blockIdx.x == ntiles - 1 && threadIdx.x == 0) out[n] = sh_tot;
+}
+
+// ============================================================ fused step: bases of both species
+// Between the two passes of the fused step, for electrons and ions at once:
+// each destination cell's total (survivors routed to it, plus its ionisation
+// products: one electron and one ion per event), the exclusive scans of the
+// three totals (electron and ion cell_start, global product order), and the
+// base of every incoming (source cell, slot) entry in the destination's new
+// bucket (the destination's start plus the counts of the entries before it
+// in canonical order; products follow the survivors).  Two launches over
+// tiles of BASES_TILE destinations, one per thread.
+constexpr int BASES_TILE = 256;
+
+// a destination's (at most 9) incoming entries and their counts, all loads
+// issued together
+struct DestEntries {
+    int n;
+    int ent[ROUTE_SLOTS], ce[ROUTE_SLOTS], ci[ROUTE_SLOTS], cp[ROUTE_SLOTS];
+    __device__ void load(int d, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                         const int *prod_cnt) {
+        const int e0 = inv_start[d];
+        n = inv_start[d + 1] - e0;   // <= 9, checked at set-up
+#pragma unroll
+        for (int q = 0; q < ROUTE_SLOTS; ++q) ent[q] = q < n ? inv_ent[e0 + q] : -1;
+#pragma unroll
+        for (int q = 0; q < ROUTE_SLOTS; ++q) {
+            const bool v = ent[q] >= 0;
+            ce[q] = v ? cnt_e[ent[q]] : 0;
+            ci[q] = v ? cnt_i[ent[q]] : 0;
+            cp[q] = v ? prod_cnt[ent[q]] : 0;
+        }
+    }
+};
+
+// sum over the block's 4 waves, valid in every thread
+__device__ inline int block_sum_256(int v, int *sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    const int t = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(BASES_TILE) void k_pair_totals(int G, int nt, const int *inv_start, const int *inv_ent,
+                                                            const int *cnt_e, const int *cnt_i, const int *prod_cnt,
+                                                            int *total_e, int *alive_e, int *total_i, int *alive_i,
+                                                            int *prod_total, int *sums) {
+    __shared__ int sh[4];
+    const int d = blockIdx.x * BASES_TILE + threadIdx.x;
+    int te = 0, ti = 0, tp = 0;
+    if (d < G) {
+        DestEntries E;
+        E.load(d, inv_start, inv_ent, cnt_e, cnt_i, prod_cnt);
+        int ae = 0, ai = 0, pr = 0;
+#pragma unroll
+        for (int q = 0; q < ROUTE_SLOTS; ++q) { ae += E.ce[q]; ai += E.ci[q]; pr += E.cp[q]; }
+        total_e[d] = ae + pr; alive_e[d] = ae;
+        total_i[d] = ai + pr; alive_i[d] = ai;
+        prod_total[d] = pr;
+        te = ae + pr; ti = ai + pr; tp = pr;
+    }
+    te = block_sum_256(te, sh);
+    ti = block_sum_256(ti, sh);
+    tp = block_sum_256(tp, sh);
+    if (threadIdx.x == 0) { sums[blockIdx.x] = te; sums[nt + blockIdx.x] = ti; sums[2 * nt + blockIdx.x] = tp; }
+}
+
+__global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases(int G, int nt, const int *sums, const int *total_e,
+                                                                const int *total_i, const int *prod_total,
+                                                                const int *alive_e, const int *alive_i,
+                                                                const int *inv_start, const int *inv_ent,
+                                                                const int *cnt_e, const int *cnt_i,
+                                                                const int *prod_cnt, int *start_e, int *start_i,
+                                                                int *prod_start, int *base_e, int *base_i,
+                                                                int *pbase_e, int *pbase_i, int *pid_off) {
+    __shared__ int wtot[3][4];
+    __shared__ int sh_off[3], sh_tot[3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w == 0) {
+        // the tiles before this one, and all tiles, per array
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int before = 0, all = 0;
+            for (int t = lane; t < nt; t += 64) {
+                const int v = sums[a * nt + t];
+                all += v;
+                if (t < (int)blockIdx.x) before += v;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                before += __shfl_xor(before, off, 64);
+                all += __shfl_xor(all, off, 64);
+            }
+            if (lane == 0) { sh_off[a] = before; sh_tot[a] = all; }
+        }
+    }
+    const int d = blockIdx.x * BASES_TILE + threadIdx.x;
+    const bool in = d < G;
+    DestEntries E;
+    if (in) E.load(d, inv_start, inv_ent, cnt_e, cnt_i, prod_cnt);
+    const int v[3] = {in ? total_e[d] : 0, in ? total_i[d] : 0, in ? prod_total[d] : 0};
+    int incl[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        incl[a] = v[a];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(incl[a], off, 64);
+            if (lane >= off) incl[a] += u;
+        }
+        if (lane == 63) wtot[a][w] = incl[a];
+    }
+    __syncthreads();
+    int run[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int wbefore = 0;
+        for (int q = 0; q < w; ++q) wbefore += wtot[a][q];
+        run[a] = sh_off[a] + wbefore + incl[a] - v[a];
+    }
+    if (in) {
+        start_e[d] = run[0]; start_i[d] = run[1]; prod_start[d] = run[2];
+        const int pe0 = run[0] + alive_e[d], pi0 = run[1] + alive_i[d];
+        int se = 0, si = 0, sp = 0;
+#pragma unroll
+        for (int q = 0; q < ROUTE_SLOTS; ++q) {
+            if (q < E.n) {
+                const int e = E.ent[q];
+                base_e[e] = run[0] + se; base_i[e] = run[1] + si;
+                pbase_e[e] = pe0 + sp; pbase_i[e] = pi0 + sp; pid_off[e] = run[2] + sp;
+            }
+            se += E.ce[q]; si += E.ci[q]; sp += E.cp[q];
+        }
+    }
+    if (blockIdx.x == nt - 1 && threadIdx.x == 0) {
+        start_e[G] = sh_tot[0]; start_i[G] = sh_tot[1]; prod_start[G] = sh_tot[2];
+    }
 }
 
 // ============================================================ staged: Sort scatter
@@ -1271,23 +1350,21 @@ void launch_assign_bases(int Gsize, const int *inv_start, const int *inv_ent, co
     hipLaunchKernelGGL(k_assign_bases, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent, cnt,
                        prod_cnt, start, alive_total, prod_start, base, pbase, pid_off);
 }
-void launch_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
-                        const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i, int *prod_total,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_totals_pair, dim3(node_blocks(Gsize)), dim3(256), 0, s, Gsize, inv_start, inv_ent, cnt_e,
-                       cnt_i, prod_cnt, total_e, alive_e, total_i, alive_i, prod_total);
+void launch_bases_pair(int G, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                       const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i, int *prod_total,
+                       int *start_e, int *start_i, int *prod_start, int *base_e, int *base_i, int *pbase_e,
+                       int *pbase_i, int *pid_off, int *tmp, hipStream_t s) {
+    const int nt = std::max(1, (G + BASES_TILE - 1) / BASES_TILE);
+    hipLaunchKernelGGL(k_pair_totals, dim3(nt), dim3(BASES_TILE), 0, s, G, nt, inv_start, inv_ent, cnt_e, cnt_i, prod_cnt,
+                       total_e, alive_e, total_i, alive_i, prod_total, tmp);
+    hipLaunchKernelGGL(k_pair_scan_bases, dim3(nt), dim3(BASES_TILE), 0, s, G, nt, (const int *)tmp, (const int *)total_e,
+                       (const int *)total_i, (const int *)prod_total, (const int *)alive_e, (const int *)alive_i,
+                       inv_start, inv_ent, cnt_e, cnt_i, prod_cnt, start_e, start_i, prod_start, base_e, base_i,
+                       pbase_e, pbase_i, pid_off);
 }
-void launch_assign_bases_pair(int Gsize, const int *ent_dest, const int *ent_pos, const int *inv_start,
-                              const int *inv_ent, const int *cnt_e, const int *cnt_i, const int *prod_cnt,
-                              const int *start_e, const int *start_i, const int *alive_e, const int *alive_i,
-                              const int *prod_start, int *base_e, int *base_i, int *pbase_e, int *pbase_i,
-                              int *pid_off, hipStream_t s) {
-    const int nent = Gsize * ROUTE_SLOTS;
-    hipLaunchKernelGGL(k_assign_bases_pair, dim3((nent + 255) / 256), dim3(256), 0, s, nent, ent_dest, ent_pos,
-                       inv_start, inv_ent, cnt_e, cnt_i, prod_cnt, start_e, start_i, alive_e, alive_i, prod_start,
-                       base_e, base_i, pbase_e, pbase_i, pid_off);
-}
-int scan_tmp_ints(int n) { return 3 * ((n + SCAN_TILE - 1) / SCAN_TILE) + 3; }
+// tile sums of up to 3 arrays: for launch_scan_multi (SCAN_TILE tiles) and
+// launch_bases_pair (BASES_TILE tiles)
+int scan_tmp_ints(int n) { return 3 * ((n + BASES_TILE - 1) / BASES_TILE) + 3; }
 // exclusive scans of k <= 3 arrays of n ints: out[a][0..n], out[a][n] = total;
 // tmp holds scan_tmp_ints(n) ints
 void launch_scan_multi(int k, const int *const *in, int *const *out, int n, int *tmp, hipStream_t s) {

@@ -111,14 +111,13 @@ void launch_assign_bases(int, const int *, const int *, const int *, const int *
 void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s);
 int scan_tmp_ints(int n);
 void launch_scan_multi(int k, const int *const *in, int *const *out, int n, int *tmp, hipStream_t s);
-void launch_totals_pair(int Gsize, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
-                        const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i, int *prod_total,
-                        hipStream_t s);
-void launch_assign_bases_pair(int Gsize, const int *ent_dest, const int *ent_pos, const int *inv_start,
-                              const int *inv_ent, const int *cnt_e, const int *cnt_i, const int *prod_cnt,
-                              const int *start_e, const int *start_i, const int *alive_e, const int *alive_i,
-                              const int *prod_start, int *base_e, int *base_i, int *pbase_e, int *pbase_i,
-                              int *pid_off, hipStream_t s);
+// fused step, between the passes: per-destination totals of both species,
+// their scans (cell_start of both species, product order) and the base of
+// every (source cell, slot) entry; tmp holds scan_tmp_ints(G) ints
+void launch_bases_pair(int G, const int *inv_start, const int *inv_ent, const int *cnt_e, const int *cnt_i,
+                       const int *prod_cnt, int *total_e, int *alive_e, int *total_i, int *alive_i, int *prod_total,
+                       int *start_e, int *start_i, int *prod_start, int *base_e, int *base_i, int *pbase_e,
+                       int *pbase_i, int *pid_off, int *tmp, hipStream_t s);
 void launch_cell_counts(int Gsize, const int *cell_start, const int *pcnt, int *total, hipStream_t s);
 void launch_fill_int(int *p, int v, int n, hipStream_t s);
 void launch_bump_ids(uint32_t *next_id, const int *prod_total_end, int nsp, hipStream_t s);
