@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/picmcc.h"
+#include "mcc_o2.h"
 #include "pic_kernels.h"
 #include "pic_setup.h"
 
@@ -83,6 +84,7 @@ struct CaseBase {
     virtual int get_timing(double *, int, long long *) = 0;
     int rf_cycles = 0, dt_pic = 0;
     int solver_kind = 0;   // 0 Jacobi PCG, 1 separable fast solve
+    int gas = 0;           // 0 Ar, 1 O2 (selects the collision step)
 };
 
 template <typename R>
@@ -143,6 +145,11 @@ struct Case : CaseBase {
     int *condcharge = nullptr, *d_err = nullptr, *lost = nullptr;
     unsigned long long *sigma_hits = nullptr;   // nsp x G, this step's dielectric hits in fixed point
     bool diel_routes = false;                   // some cell border leads into a dielectric
+    // O2 chemistry (gas type 1): constants and buffers of its collision step
+    O2Const<R> oc{};
+    R *o2prob = nullptr, *o2fluid = nullptr, *o2bg = nullptr, *o2stash[O2_NSP] = {nullptr};
+    uint8_t *o2fate[O2_NSP] = {nullptr};
+    int *o2pcount = nullptr, *o2pstart = nullptr, *o2delreq = nullptr;
     double *sum_den = nullptr, *sum_phi = nullptr, *sum_ex = nullptr, *sum_ey = nullptr;
     long long n_accum = 0;
     long long tstep = 0;
@@ -186,9 +193,14 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey, total2, alive2, clk, eedf, lost, sigma_hits};
+                        sum_ey, total2, alive2, clk, eedf, lost, sigma_hits, o2prob, o2fluid, o2bg, o2pcount,
+                        o2pstart, o2delreq};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
+        for (int s = 0; s < O2_NSP; ++s) {
+            if (o2fate[s]) (void)hipFree(o2fate[s]);
+            if (o2stash[s]) (void)hipFree(o2stash[s]);
+        }
         for (int s = 0; s < MAXSP; ++s) {
             PView<R> *vs[2] = {&cur[s], &nxt[s]};
             for (PView<R> *v : vs) {
@@ -485,6 +497,28 @@ struct Case : CaseBase {
         kc.Ex = Ex; kc.Ey = Ey; kc.P1 = P1; kc.P2 = P2; kc.Pion = Pion; kc.BackVel1 = BackVel1;
         kc.vsave = vsave; kc.xe = xe; kc.cx = cx; kc.StructIdx = StructIdx; kc.route = route; kc.lost = lost;
         kc.sigma_hits = sigma_hits;
+        gas = d.gas;
+        if (gas == 1) {
+            if (nsp != O2_NSP || S.nsigv != O2_NSIGV || S.TnRct < O2_NROWS)
+                return fail(-19, "O2 setup does not match the collision tables");
+            HIPCHK(dalloc(&o2prob, (size_t)G * O2_NTARGETS));
+            HIPCHK(upload(&o2fluid, S.FluidN, st)); HIPCHK(upload(&o2bg, S.BackDen1, st));
+            HIPCHK(dalloc(&o2pcount, (size_t)O2_NSP * G)); HIPCHK(dalloc(&o2pstart, (size_t)O2_NSP * (G + 1)));
+            HIPCHK(dalloc(&o2delreq, G));
+            for (int s = 0; s < O2_NSP; ++s) {
+                HIPCHK(dalloc(&o2fate[s], cur[s].cap));
+                HIPCHK(dalloc(&o2stash[s], (size_t)6 * cur[s].cap));
+            }
+            oc.G = G; oc.ncy = S.ncy; oc.ngy = S.ngy; oc.Csize = S.Csize; oc.TnRct = S.TnRct; oc.nlogx = N_LOGX;
+            oc.nvel = S.nvel; oc.MCCn = S.MCCn;
+            oc.idLOGX = S.idLOGX; oc.dt = S.dt; oc.dtm = S.dt_mcc; oc.idx = S.idx; oc.idy = S.idy;
+            for (int s = 0; s < O2_NSP; ++s) { oc.np2c[s] = S.np2c[s]; oc.Escale[s] = S.Escale[s]; }
+            for (int j = 0; j < O2_NSIGV; ++j) oc.sigv[j] = S.sigv[j];
+            for (int r = 0; r < O2_NROWS; ++r) oc.Th_e[r] = S.Th_e[r];
+            oc.seed = (uint64_t)d.rng_seed;
+            oc.xe = xe; oc.cx = cx; oc.vsave = vsave; oc.BackDen1 = o2bg; oc.BackVel1 = BackVel1; oc.FluidN = o2fluid;
+            oc.den = den; oc.prob = o2prob;
+        }
         nc.Gsize = G; nc.ngx = S.ngx; nc.ngy = S.ngy; nc.nsp = nsp; nc.CondNUMR = S.CondNUMR;
         nc.hdx = S.hdx; nc.hdy = S.hdy; nc.idx = S.idx; nc.idy = S.idy; nc.idxy = S.idxy;
         nc.hxy = (R)(0.5 * (double)S.dx / (double)S.dy);
@@ -587,7 +621,32 @@ struct Case : CaseBase {
         HIPCHK(hipGetLastError());
         return 0;
     }
+    // O2 chemistry: collide every species, remove the O- taken by electron
+    // detachment, then rebuild all four buckets with the products
+    int mcc_o2() {
+        const uint32_t step = (uint32_t)tstep;
+        O2Set<R> set;
+        for (int s = 0; s < O2_NSP; ++s) { set.p[s] = cur[s]; set.fate[s] = o2fate[s]; set.stash[s] = o2stash[s]; }
+        launch_o2_probs<R>(oc, st);
+        launch_fill_int(o2pcount, 0, O2_NSP * G, st);
+        launch_fill_int(o2delreq, 0, G, st);
+        for (int s = 0; s < O2_NSP; ++s) launch_o2_collide<R>(oc, set, s, step, mccr, o2pcount, o2delreq, st);
+        launch_o2_victims<R>(oc, set, step, o2delreq, st);
+        for (int t = 0; t < O2_NSP; ++t) {
+            int *pc = o2pcount + (size_t)t * G, *ps = o2pstart + (size_t)t * (G + 1);
+            launch_fill_int(total, 0, G, st);
+            launch_o2_totals<R>(oc, set, t, pc, total, st);
+            launch_scan(total, nxt[t].cell_start, G, scan_tmp, st);
+            launch_scan(pc, ps, G, scan_tmp, st);
+            launch_o2_scatter<R>(oc, set, t, nxt[t], ps, next_id + t, nxt[t].cap, d_err, st);
+        }
+        for (int s = 0; s < O2_NSP; ++s) swap_species(s);
+        launch_o2_bump_ids(next_id, o2pstart, G, st);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     int mcc() override {
+        if (gas == 1) return mcc_o2();
         const uint32_t step = (uint32_t)tstep;
         // electrons first in the event order is irrelevant: each particle owns its stream
         launch_mcc_collide<R>(kc, cur[1], 1, step, mccr, cellp, prank, stage, st);   // ions (no products)
@@ -706,6 +765,17 @@ struct Case : CaseBase {
         return 0;
     }
     int step_fused(int n) override {
+        if (gas == 1) {
+            // the fused two-pass step is built for the argon chemistry; O2
+            // cases run the stages one by one
+            for (int i = 0; i < n; ++i) {
+                int e;
+                if ((e = advance_time()) || (e = field_solve()) || (e = efield()) || (e = move()) ||
+                    (e = sort_boundary()) || (e = mcc()) || (e = deposit()) || (e = diagnostic()))
+                    return e;
+            }
+            return 0;
+        }
         if (int e = ensure_clock()) return e;
         // graphs need a fixed launch sequence: the separable field solve, no timers
         const bool graph = use_graph && spectral && !timing;
@@ -900,7 +970,7 @@ struct Case : CaseBase {
         if (n == "xe") return H(S.xe.data(), S.xe.size(), rs, 1);
         if (n == "xee") return H(S.xee.data(), S.xee.size(), rs, 1);
         if (n == "cx") return H(S.cx.data(), S.cx.size(), rs, 1);
-        if (n == "sigv") return H(S.sigv, 3, rs, 1);
+        if (n == "sigv") return H(S.sigv, S.nsigv, rs, 1);
         if (n == "P1") return H(S.P1.data(), G, rs, 1);
         if (n == "P2") return H(S.P2.data(), G, rs, 1);
         if (n == "Pion") return H(S.Pion.data(), G, rs, 1);
@@ -1032,6 +1102,10 @@ static bool setup_scalar(Setup<R> &S, const std::string &n, double &v) {
     if (n == "nbg") { v = S.nbg; return true; }
     if (n == "bg_dens0") { v = (double)S.bg_dens[0]; return true; }
     if (n == "bg_dens1") { v = (double)S.bg_dens[1]; return true; }
+    if (n == "idLOGX") { v = (double)S.idLOGX; return true; }
+    if (n == "idx") { v = (double)S.idx; return true; }
+    if (n == "idy") { v = (double)S.idy; return true; }
+    if (n.compare(0, 4, "np2c") == 0 && n.size() == 5) { v = (double)S.np2c[n[4] - '0']; return true; }
     if (n.compare(0, 2, "np") == 0 && n.size() == 3) { v = S.sp[n[2] - '0'].n; return true; }
     if (n.compare(0, 6, "Ascale") == 0 && n.size() == 7) { v = (double)S.Ascale[n[6] - '0']; return true; }
     if (n.compare(0, 6, "Escale") == 0 && n.size() == 7) { v = (double)S.Escale[n[6] - '0']; return true; }
@@ -1075,9 +1149,8 @@ int pic_create(const char *deck_json, const pic_options *opt, pic_ctx **out) {
     Deck d;
     std::string err;
     if (int e = parse_deck(deck_json, d, err)) return fail(e, "%s", err.c_str());
-    if (d.gas != 0)
-        return fail(-19, "the O2 and Ar/O2 collision step is not built: gas type %d has host setup only "
-                         "(pic_setup_open)", d.gas);
+    if (d.gas == 2)
+        return fail(-19, "the Ar/O2 collision step is not built: gas type 2 has host setup only (pic_setup_open)");
     std::unique_ptr<pic_ctx> h(new pic_ctx());
     int e = 0;
     try {
@@ -1324,7 +1397,16 @@ void PCG_SOLVER(void) { shim_check(shim_case("PCG_SOLVER")->field_solve(), "PCG_
 void Efield_cuda_Basic(void) { shim_check(shim_case("Efield_cuda_Basic")->efield(), "Efield_cuda_Basic"); }
 void Move_cuda(void) { shim_check(shim_case("Move_cuda")->move(), "Move_cuda"); }
 void SortBounndary_cuda(void) { shim_check(shim_case("SortBounndary_cuda")->sort_boundary(), "SortBounndary_cuda"); }
-void MCC_Ar_cuda(void) { shim_check(shim_case("MCC_Ar_cuda")->mcc(), "MCC_Ar_cuda"); }
+void MCC_Ar_cuda(void) {
+    CaseBase *c = shim_case("MCC_Ar_cuda");
+    if (c->gas != 0) shim_check(fail(-19, "the case is not an argon case"), "MCC_Ar_cuda");
+    shim_check(c->mcc(), "MCC_Ar_cuda");
+}
+void MCC_O2_cuda(void) {
+    CaseBase *c = shim_case("MCC_O2_cuda");
+    if (c->gas != 1) shim_check(fail(-19, "the case is not an O2 case"), "MCC_O2_cuda");
+    shim_check(c->mcc(), "MCC_O2_cuda");
+}
 void Deposit_cuda(void) { shim_check(shim_case("Deposit_cuda")->deposit(), "Deposit_cuda"); }
 void Deposit_Basic(void) { shim_check(shim_case("Deposit_Basic")->deposit(), "Deposit_Basic"); }
 void main_cuda(void) {

@@ -1,0 +1,410 @@
+// Collision step of the O2 chemistry (gas type 1) for gfx950.
+//
+// Five launches per step, all keyed on one 64-lane wave per cell:
+//   probs    per node, the cumulative target probabilities of the four
+//            species (20 values; particle targets use this step's den)
+//   collide  per species: null-collision test, target, reaction and its
+//            outcome, written in place (survivors) or as a fate byte and up
+//            to two products (velocities in a stash indexed like the parent)
+//   victims  per cell with electron detachments: removes that many O- of
+//            the cell, smallest Philox keys first
+//   totals / scatter  per target species: survivors in old order, then
+//            products by (source species, parent, slot); new ids follow the
+//            global (cell, source species, parent, slot) order
+// Every count is an integer atomic or a ballot, so the result is the same
+// for any schedule.  Reference: src/cuda_mcc.cu:72-96 (MCC_O2_Basic),
+// src/cuda_mccO2.cu:2-977 (handlers and target test).
+#include <hip/hip_runtime.h>
+
+#include "mcc_o2.h"
+
+namespace pic {
+
+namespace {
+
+struct O2Row {
+    int r, sum, type, emode;
+    double gain;
+    int p0, p1;
+};
+struct O2Group {
+    int sp, flag, sig, first, n;
+};
+struct O2Target {
+    int sp, flag, sig, kind, idx;
+};
+
+#define PIC_O2_ROW(r, sm, ty, em, g, a, b) {r, sm, ty, em, g, a, b},
+#define PIC_O2_GRP(s, f, sg, fi, n) {s, f, sg, fi, n},
+#define PIC_O2_TGT(s, f, sg, kd, ix) {s, f, sg, kd, ix},
+__constant__ O2Row kO2Row[O2_NROWS] = {O2_REACTIONS(PIC_O2_ROW)};
+__constant__ O2Group kO2Grp[O2_NGROUPS] = {O2_GROUPS(PIC_O2_GRP)};
+__constant__ O2Target kO2Tgt[O2_NTARGETS] = {O2_TARGETS(PIC_O2_TGT)};
+__constant__ int kO2Group0[O2_NSP] = O2_GROUP0;
+__constant__ int kO2NGroup[O2_NSP] = O2_NGROUP;
+#undef PIC_O2_ROW
+#undef PIC_O2_GRP
+#undef PIC_O2_TGT
+
+// Target probabilities: P_j = P_{j-1} + 1 - exp(-dt_s sigv n), summed in
+// that order as the reference writes it (src/cuda_mccO2.cu:880-930).
+template <typename R>
+__global__ __launch_bounds__(256) void k_o2_probs(O2Const<R> k) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= k.G) return;
+    R acc = 0;
+    int sp = -1;
+    for (int t = 0; t < O2_NTARGETS; ++t) {
+        const O2Target T = kO2Tgt[t];
+        if (T.sp != sp) { sp = T.sp; acc = 0; }
+        R x = (T.sp <= 1 ? k.dtm : k.dt) * k.sigv[T.sig];
+        if (T.kind == 0) x = x * k.BackDen1[g];
+        else if (T.kind == 1) x = x * k.FluidN[(size_t)T.idx * k.G + g];
+        else x = x * k.den[(size_t)T.idx * k.G + g] * k.np2c[T.idx] * k.idx * k.idy;
+        acc = (acc + (R)1) - (R)dm_exp(-(double)x);
+        k.prob[(size_t)g * O2_NTARGETS + t] = acc;
+    }
+}
+
+template <typename R>
+__device__ inline void o2_maxwell(const O2Const<R> &k, int c, Stream &st, R &vx, R &vy, R &vz) {
+    const int n = (int)((R)(k.nvel - 1) * st.u<R>());
+    const R rphi = st.u<R>(), rthe = st.u<R>();
+    maxwell_draw<R>(vx, vy, vz, k.vsave[n], k.BackVel1[c], rphi, rthe);
+}
+
+// One collision of a particle of species isp with the target of group grp.
+// Returns the reaction (or -1 for a null collision); vx..vz are updated for
+// a survivor, and dead / products / detach describe the rest.
+template <typename R>
+__device__ inline int o2_handle(const O2Const<R> &k, int isp, int grp, int c, Stream &st, R &vx, R &vy, R &vz,
+                                int &dead, int ps[2], R pv[2][3], int &np, int &detach) {
+    const O2Group g = kO2Grp[grp];
+    R vnx = 0, vny = 0, vnz = 0;
+    const bool heavy = isp == 1 || isp == 2;   // positive ions collide in the neutral's frame
+    if (heavy) o2_maxwell<R>(k, c, st, vnx, vny, vnz);
+    R VX = vx - vnx, VY = vy - vny, VZ = vz - vnz;
+    const R dum = VX * VX + VY * VY + VZ * VZ;
+    R vel = rsqrt_ieee<R>(dum);
+    if (!heavy) { VX /= vel; VY /= vel; VZ /= vel; }
+    R engy = k.Escale[isp] * dum;
+    const R R1 = st.u<R>() * k.sigv[g.sig] / vel;
+    ArXs<R> xs;
+    xs.init(k, engy);
+    R sum = 0;
+    int hit = -1;
+    for (int q = g.first; q < g.first + g.n; ++q) {
+        const int r = kO2Row[q].r;
+        if (engy > k.Th_e[r]) {
+            const R sg = xs.sigma(k, r);
+            sum = kO2Row[q].sum ? sum + sg : sg;
+            if (R1 <= sum) { hit = q; break; }
+        }
+    }
+    if (hit < 0) return -1;
+    const O2Row row = kO2Row[hit];
+    switch (row.type) {
+    case O2T_LOSS:
+    case O2T_ION:
+    case O2T_DETACH: {
+        if (row.emode == O2E_LOSS) {
+            engy -= k.Th_e[row.r];
+            vel = rsqrt_ieee<R>(rabs<R>(engy) / k.Escale[isp]);
+        } else if (row.emode == O2E_GAIN) {
+            engy += (R)row.gain;
+            vel = rsqrt_ieee<R>(rabs<R>(engy) / k.Escale[isp]);
+        }
+        if (row.type == O2T_ION) {
+            for (int slot = 0; slot < 2; ++slot) {
+                const int sp = slot ? row.p1 : row.p0;
+                ps[np] = sp;
+                if (sp == 0) { pv[np][0] = VX; pv[np][1] = VY; pv[np][2] = VZ; }
+                else o2_maxwell<R>(k, c, st, pv[np][0], pv[np][1], pv[np][2]);
+                ++np;
+            }
+        } else if (row.type == O2T_DETACH) {
+            ps[np] = 0; pv[np][0] = VX; pv[np][1] = VY; pv[np][2] = VZ; ++np;
+            detach = 1;
+        }
+        const R a = st.u<R>(), b = st.u<R>();
+        anewvel<R>(engy, vel, VX, VY, VZ, a, b);
+        vx = VX; vy = VY; vz = VZ;
+        break;
+    }
+    case O2T_ATTACH:
+        dead = 1;
+        ps[np] = row.p0;
+        o2_maxwell<R>(k, c, st, pv[np][0], pv[np][1], pv[np][2]);
+        ++np;
+        break;
+    case O2T_DEL:
+        dead = 1;
+        break;
+    case O2T_SCAT: {
+        const R a = st.u<R>(), b = st.u<R>();
+        newvel_ionsc<R>(VX, VY, VZ, vel, a, b);
+        if (heavy) { vx = VX + vnx; vy = VY + vny; vz = VZ + vnz; }
+        else { vx = VX; vy = VY; vz = VZ; }
+        break;
+    }
+    case O2T_CXKEEP:
+        vx = vnx; vy = vny; vz = vnz;
+        break;
+    case O2T_CXTO:
+        dead = 1;
+        ps[np] = row.p0; pv[np][0] = vnx; pv[np][1] = vny; pv[np][2] = vnz; ++np;
+        break;
+    default:   // O2T_NEGDET
+        dead = 1;
+        ps[np] = 0; pv[np][0] = VX; pv[np][1] = VY; pv[np][2] = VZ; ++np;
+        break;
+    }
+    return row.r;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, int isp, uint32_t step,
+                                                    unsigned long long *mccr, int *pcount, int *delreq) {
+    int lane;
+    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    const PView<R> p = set.p[isp];
+    uint8_t *fate = set.fate[isp];
+    R *stash = set.stash[isp];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
+    const size_t cap = (size_t)p.cap;
+    const int g0 = kO2Group0[isp], ng = kO2NGroup[isp];
+    const R *P = k.prob + (size_t)c * O2_NTARGETS + g0;
+    const R Tprob = P[ng - 1];
+    const int draws = isp <= 1 ? k.MCCn : 1;
+    for (int kk = k0 + lane; kk < k1; kk += 64) {
+        Stream st;
+        st.init(k.seed, p.id[kk], step, mcc_tag(isp));
+        R R1 = 0;
+        for (int j = 0; j < draws; ++j) {
+            R1 = st.u<R>();
+            if (R1 < Tprob) break;
+        }
+        uint8_t f = 0;
+        if (R1 < Tprob) {
+            int flag = 0;
+            while (flag < ng - 1 && R1 > P[flag]) ++flag;
+            R vx = p.vx[kk], vy = p.vy[kk], vz = p.vz[kk];
+            int dead = 0, np = 0, detach = 0, ps[2] = {-1, -1};
+            R pv[2][3];
+            const int r = o2_handle<R>(k, isp, g0 + flag, c, st, vx, vy, vz, dead, ps, pv, np, detach);
+            if (r >= 0) {
+                atomicAdd(&mccr[(size_t)c * k.TnRct + r], 1ull);
+                if (dead) f |= O2F_DEAD;
+                else { p.vx[kk] = vx; p.vy[kk] = vy; p.vz[kk] = vz; }
+                for (int q = 0; q < np; ++q) {
+                    f |= (uint8_t)((ps[q] + 1) << (1 + 3 * q));
+                    stash[(3 * q) * cap + kk] = pv[q][0];
+                    stash[(3 * q + 1) * cap + kk] = pv[q][1];
+                    stash[(3 * q + 2) * cap + kk] = pv[q][2];
+                    atomicAdd(&pcount[(size_t)ps[q] * k.G + c], 1);
+                }
+                if (detach) atomicAdd(&delreq[c], 1);
+            }
+        }
+        fate[kk] = f;
+    }
+}
+
+// The reference removes a random O- of the cell for each detachment
+// (src/cuda_mccO2.cu:398-424, racing with the O- handler of that cell).
+// Here the candidates are the cell's O- that survived their own collision;
+// min(requests, candidates) of them go, in increasing order of a Philox key
+// of (seed, id, step), ties by bucket position.
+template <typename R>
+__global__ __launch_bounds__(256) void k_o2_victims(O2Const<R> k, O2Set<R> set, uint32_t step, const int *delreq) {
+    int lane;
+    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    const int req = delreq[c];
+    if (req == 0) return;
+    const PView<R> p = set.p[3];
+    uint8_t *fate = set.fate[3];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
+    int ncand = 0;
+    for (int b = k0; b < k1; b += 64) {
+        const int kk = b + lane;
+        ncand += __popcll(__ballot(kk < k1 && !(fate[kk] & O2F_DEAD)));
+    }
+    const int kdel = min(req, ncand);
+    if (kdel == 0) return;
+    const uint32_t s0 = (uint32_t)k.seed, s1 = (uint32_t)(k.seed >> 32);
+    for (int a0 = k0; a0 < k1; a0 += 64) {
+        const int i = a0 + lane;
+        const bool ai = i < k1 && !(fate[i] & O2F_DEAD);
+        uint32_t ki = 0;
+        if (ai) {
+            uint32_t o[4];
+            philox4x32_10(p.id[i], step, O2_VICTIM_TAG, 0u, s0, s1, o);
+            ki = o[0];
+        }
+        int rank = 0;
+        for (int b0 = k0; b0 < k1; b0 += 64) {
+            const int j = b0 + lane;
+            const bool aj = j < k1 && !(fate[j] & O2F_DEAD);
+            uint32_t kj = 0;
+            if (aj) {
+                uint32_t o[4];
+                philox4x32_10(p.id[j], step, O2_VICTIM_TAG, 0u, s0, s1, o);
+                kj = o[0];
+            }
+            for (int q = 0; q < 64; ++q) {
+                const uint32_t kq = (uint32_t)__shfl((int)kj, q, 64);
+                const int aq = __shfl((int)aj, q, 64);
+                if (aq && (kq < ki || (kq == ki && b0 + q < i))) ++rank;
+            }
+        }
+        // only bit O2F_DEAD decides candidacy, so marking here leaves the
+        // candidate set of the other lanes unchanged
+        if (ai && rank < kdel) fate[i] = (uint8_t)(fate[i] | O2F_VICTIM);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_o2_totals(O2Const<R> k, O2Set<R> set, int t, const int *pcount_t,
+                                                   int *total) {
+    int lane;
+    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    const PView<R> p = set.p[t];
+    const uint8_t *fate = set.fate[t];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
+    int n = 0;
+    for (int b = k0; b < k1; b += 64) {
+        const int kk = b + lane;
+        n += __popcll(__ballot(kk < k1 && !(fate[kk] & (O2F_DEAD | O2F_VICTIM))));
+    }
+    if (lane == 0) total[c] = n + pcount_t[c];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_o2_scatter(O2Const<R> k, O2Set<R> set, int t, PView<R> dst,
+                                                    const int *pstart_t, const uint32_t *next_id_t, int cap,
+                                                    int *err) {
+    int lane;
+    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    const int ns = dst.cell_start[c];
+    const unsigned long long lt = lanes_below(lane);
+    const PView<R> p = set.p[t];
+    const uint8_t *ft = set.fate[t];
+    int k0, k1;
+    cell_range(p, c, k0, k1);
+    int run = 0;
+    for (int b = k0; b < k1; b += 64) {
+        const int kk = b + lane;
+        const bool keep = kk < k1 && !(ft[kk] & (O2F_DEAD | O2F_VICTIM));
+        const unsigned long long m = __ballot(keep);
+        if (keep) {
+            const int pos = ns + run + __popcll(m & lt);
+            if (pos >= cap) atomicOr(err, 1);
+            else {
+                dst.x[pos] = p.x[kk]; dst.y[pos] = p.y[kk];
+                dst.vx[pos] = p.vx[kk]; dst.vy[pos] = p.vy[kk]; dst.vz[pos] = p.vz[kk]; dst.id[pos] = p.id[kk];
+            }
+        }
+        run += __popcll(m);
+    }
+    const uint32_t id0 = *next_id_t + (uint32_t)pstart_t[c];
+    int prun = 0;
+    for (int s2 = 0; s2 < O2_NSP; ++s2) {
+        const PView<R> q = set.p[s2];
+        const uint8_t *fq = set.fate[s2];
+        const R *sq = set.stash[s2];
+        const size_t qcap = (size_t)q.cap;
+        int q0, q1;
+        cell_range(q, c, q0, q1);
+        for (int b = q0; b < q1; b += 64) {
+            const int kk = b + lane;
+            int n0 = 0, n1 = 0;
+            if (kk < q1) {
+                const uint8_t f = fq[kk];
+                n0 = o2_fate_prod(f, 0) == t;
+                n1 = o2_fate_prod(f, 1) == t;
+            }
+            const int cnt = n0 + n1;
+            int incl = cnt;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += u;
+            }
+            const int tot = __shfl(incl, 63, 64);
+            int r = prun + incl - cnt;
+            for (int slot = 0; slot < 2; ++slot) {
+                if (!(slot ? n1 : n0)) continue;
+                const int pos = ns + run + r;
+                if (pos >= cap) atomicOr(err, 1);
+                else {
+                    dst.x[pos] = q.x[kk]; dst.y[pos] = q.y[kk];
+                    dst.vx[pos] = sq[(3 * slot) * qcap + kk];
+                    dst.vy[pos] = sq[(3 * slot + 1) * qcap + kk];
+                    dst.vz[pos] = sq[(3 * slot + 2) * qcap + kk];
+                    dst.id[pos] = id0 + (uint32_t)r;
+                }
+                ++r;
+            }
+            prun += tot;
+        }
+    }
+}
+
+__global__ void k_o2_bump(uint32_t *next_id, const int *pstart, int G) {
+    const int t = threadIdx.x;
+    if (t < O2_NSP) next_id[t] += (uint32_t)pstart[(size_t)t * (G + 1) + G];
+}
+
+inline int wave_blocks(int Csize) { return (Csize + 3) / 4; }
+
+}  // namespace
+
+template <typename R>
+void launch_o2_probs(const O2Const<R> &k, hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_probs<R>, dim3((k.G + 255) / 256), dim3(256), 0, s, k);
+}
+template <typename R>
+void launch_o2_collide(const O2Const<R> &k, const O2Set<R> &set, int isp, uint32_t step, unsigned long long *mccr,
+                       int *pcount, int *delreq, hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_collide<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, isp, step, mccr, pcount,
+                       delreq);
+}
+template <typename R>
+void launch_o2_victims(const O2Const<R> &k, const O2Set<R> &set, uint32_t step, const int *delreq, hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_victims<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, step, delreq);
+}
+template <typename R>
+void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, int t, const int *pcount_t, int *total,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_totals<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, t, pcount_t, total);
+}
+template <typename R>
+void launch_o2_scatter(const O2Const<R> &k, const O2Set<R> &set, int t, const PView<R> &dst, const int *pstart_t,
+                       const uint32_t *next_id_t, int cap, int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_scatter<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, t, dst, pstart_t,
+                       next_id_t, cap, err);
+}
+void launch_o2_bump_ids(uint32_t *next_id, const int *pstart, int G, hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_bump, dim3(1), dim3(64), 0, s, next_id, pstart, G);
+}
+
+#define INST(R)                                                                                                       \
+    template void launch_o2_probs<R>(const O2Const<R> &, hipStream_t);                                               \
+    template void launch_o2_collide<R>(const O2Const<R> &, const O2Set<R> &, int, uint32_t, unsigned long long *,     \
+                                       int *, int *, hipStream_t);                                                   \
+    template void launch_o2_victims<R>(const O2Const<R> &, const O2Set<R> &, uint32_t, const int *, hipStream_t);    \
+    template void launch_o2_totals<R>(const O2Const<R> &, const O2Set<R> &, int, const int *, int *, hipStream_t);   \
+    template void launch_o2_scatter<R>(const O2Const<R> &, const O2Set<R> &, int, const PView<R> &, const int *,     \
+                                       const uint32_t *, int, int *, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace pic

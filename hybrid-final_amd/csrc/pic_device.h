@@ -258,7 +258,8 @@ struct ArXs {
     int ee2;
     R a1, a2;
     double tail;    // dm_exp(-(lengy - xe[N-1])) above the table
-    __device__ void init(const DevConst<R> &k, R engy) {
+    template <class K>
+    __device__ void init(const K &k, R engy) {
         const int N = k.nlogx;
         if (engy == 0) { mode = 0; return; }
         const R lengy = (R)dm_log10((double)engy);
@@ -271,7 +272,8 @@ struct ArXs {
         else if (ee2 >= N - 1) mode = 3;
         else mode = 4;
     }
-    __device__ R sigma(const DevConst<R> &k, int r) const {
+    template <class K>
+    __device__ R sigma(const K &k, int r) const {
         const int N = k.nlogx, T = k.TnRct;
         switch (mode) {
         case 0: return 0;

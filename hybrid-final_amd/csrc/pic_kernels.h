@@ -26,6 +26,30 @@ struct NodeConst {
     unsigned long long *sigma_hits;   // null when no cell border leads into a dielectric
 };
 
+// One 64-lane wavefront per cell: the wave's cell (named by its lower-left
+// node), or -1 past the last cell.
+__device__ inline int wave_cell(int ncy, int ngy, int Csize, int &lane) {
+    lane = threadIdx.x & 63;
+    // wave-uniform: computed in scalar registers
+    const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (w >= Csize) return -1;
+    return (w / ncy) * ngy + (w % ncy);
+}
+
+// [k0, k1) of cell c, clamped to the array capacity: after a capacity
+// overflow (reported through the error flag) the offsets may point past the
+// arrays, and no kernel may follow them there
+template <typename R>
+__device__ inline void cell_range(const PView<R> &p, int c, int &k0, int &k1) {
+    k0 = min(p.cell_start[c], p.cap);
+    k1 = min(p.cell_start[c + 1], p.cap);
+    if (k1 < k0) k1 = k0;
+}
+
+__device__ inline unsigned long long lanes_below(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
 // particle views of all species of a case (kernel argument)
 template <typename R>
 struct PSet {

@@ -101,8 +101,9 @@ def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None,
     rest is Ar).  Charged-species densities default to a quasi-neutral mix
     (e 1e15, O- 2e14; positive ions make up the rest) with ppc macro-particles
     per cell for e and the molecular/atomic majority ions and ppc/5 for O+
-    and O-.  Every reaction switch is on.  Gas types 1 and 2 have host setup
-    only in this build (pic_setup_open); pic_create rejects them."""
+    and O-.  Every reaction switch is on.  Gas type 1 runs on the GPU (staged
+    steps); gas type 2 has host setup only (pic_setup_open) and pic_create
+    rejects it."""
     d = ccp_deck(**kw)
     ppc = kw.get("ppc", 50)
     dx = kw.get("dx", 5e-4)

@@ -242,3 +242,84 @@ class Oracle:
             fn = getattr(self.lib, name)
             return lambda *a: fn(self.h, *a)
         raise AttributeError(name)
+
+
+def mcc_o2(precision, k, buckets, next_id, mccr):
+    """O2 chemistry collision step (orc_mcc_o2) on host copies.
+
+    k: dict of the setup values (G, TnRct, nlogx, nvel, MCCn, idLOGX, dt,
+    dtm, idx, idy, np2c[4], Escale[4], sigv[20], seed, step) and arrays
+    (Th_e, xe, cx, vsave, BackDen1, BackVel1, FluidN, den).  buckets: four
+    dicts x, y, vx, vy, vz, id, cell_start, cap (e, O2+, O+, O-).  Returns
+    (new buckets, next_id, mccr, rc); the inputs are not modified.
+    """
+    path = os.path.join(HERE, f"liborc_{precision}.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library {path} is not built (run make -C oracle)")
+    L = C.CDLL(path)
+    real = np.float32 if precision == "f32" else np.float64
+    R = C.c_float if precision == "f32" else C.c_double
+    P = C.POINTER(R)
+
+    class In(C.Structure):
+        _fields_ = [("G", C.c_int), ("TnRct", C.c_int), ("nlogx", C.c_int), ("nvel", C.c_int), ("MCCn", C.c_int),
+                    ("idLOGX", R), ("dt", R), ("dtm", R), ("idx", R), ("idy", R),
+                    ("np2c", R * 4), ("Escale", R * 4), ("sigv", R * 20),
+                    ("Th_e", P), ("xe", P), ("cx", P), ("vsave", P), ("BackDen1", P), ("BackVel1", P),
+                    ("FluidN", P), ("den", P), ("seed", C.c_uint64), ("step", C.c_uint32)]
+
+    class Bucket(C.Structure):
+        _fields_ = [("x", P), ("y", P), ("vx", P), ("vy", P), ("vz", P), ("id", C.POINTER(C.c_uint32)),
+                    ("cell_start", C.POINTER(C.c_int)), ("n", C.c_int), ("cap", C.c_int)]
+
+    keep = []
+
+    def ptr(a, dt, ct):
+        a = np.ascontiguousarray(np.asarray(a, dtype=dt)).copy()
+        keep.append(a)
+        return a, a.ctypes.data_as(C.POINTER(ct))
+
+    inp = In()
+    for f in ("G", "TnRct", "nlogx", "nvel", "MCCn"):
+        setattr(inp, f, int(k[f]))
+    for f in ("idLOGX", "dt", "dtm", "idx", "idy"):
+        setattr(inp, f, float(real(k[f])))
+    for f, n in (("np2c", 4), ("Escale", 4), ("sigv", 20)):
+        arr = getattr(inp, f)
+        for i in range(n):
+            arr[i] = float(real(k[f][i]))
+    for f in ("Th_e", "xe", "cx", "vsave", "BackDen1", "BackVel1", "FluidN", "den"):
+        setattr(inp, f, ptr(k[f], real, R)[1])
+    inp.seed = int(k["seed"]) & (2 ** 64 - 1)
+    inp.step = int(k["step"]) & (2 ** 32 - 1)
+    bs = (Bucket * 4)()
+    arrays = []
+    for s in range(4):
+        b = buckets[s]
+        cap = int(b["cap"])
+        n = len(b["x"])
+        d = {}
+        for f in ("x", "y", "vx", "vy", "vz"):
+            a = np.zeros(cap, dtype=real)
+            a[:n] = b[f]
+            d[f], p = ptr(a, real, R)
+            setattr(bs[s], f, p)
+        a = np.zeros(cap, dtype=np.uint32)
+        a[:n] = b["id"]
+        d["id"], bs[s].id = ptr(a, np.uint32, C.c_uint32)
+        d["cell_start"], bs[s].cell_start = ptr(b["cell_start"], np.int32, C.c_int)
+        bs[s].n = n
+        bs[s].cap = cap
+        arrays.append(d)
+    nid, nid_p = ptr(next_id, np.uint32, C.c_uint32)
+    mc, mc_p = ptr(mccr, np.int64, C.c_int64)
+    L.orc_mcc_o2.restype = C.c_int
+    L.orc_mcc_o2.argtypes = [C.POINTER(In), C.POINTER(Bucket), C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]
+    rc = L.orc_mcc_o2(C.byref(inp), bs, nid_p, mc_p)
+    out = []
+    for s in range(4):
+        n = bs[s].n
+        d = {f: arrays[s][f][:n].copy() for f in ("x", "y", "vx", "vy", "vz", "id")}
+        d["cell_start"] = arrays[s]["cell_start"].copy()
+        out.append(d)
+    return out, nid.copy(), mc.copy(), rc

@@ -872,3 +872,250 @@ void orc_lcg_seq(long seed, int n, long *out) {
 
 /* argon cross section lookup, for the collision-statistics tests */
 double orc_ar_sigma(orc_case *c, int r, double engy) { return (double)ar_sigma(c, r, (real)engy); }
+
+/* ================= MCC (O2, gas type 1): src/cuda_mcc.cu:72-96 (MCC_O2_Basic);
+ *                   src/cuda_mccO2.cu:867-977 (target test), 2-866 (handlers) ============
+ * Each particle draws from its own stream (seed, id, step, (species << 8) | 1)
+ * in the reference's order: the target draws (up to MCCn for e and O2+, one
+ * for O+ and O-), the neutral's velocity (O2+, O+: index, phi, theta), the
+ * reaction draw, then each product's Maxwellian draws and the two angles of
+ * the scattered projectile.  Reactions are the rows of o2_chem.h. */
+#include "o2_chem.h"
+
+typedef struct { int r, sum, type, emode; double gain; int p0, p1; } o2_row;
+typedef struct { int sp, flag, sig, first, n; } o2_grp;
+typedef struct { int sp, flag, sig, kind, idx; } o2_tgt;
+#define ORC_O2_ROW(r, sm, ty, em, g, a, b) {r, sm, ty, em, g, a, b},
+#define ORC_O2_GRP(s, f, sg, fi, n) {s, f, sg, fi, n},
+#define ORC_O2_TGT(s, f, sg, kd, ix) {s, f, sg, kd, ix},
+static const o2_row o2_rows[O2_NROWS] = {O2_REACTIONS(ORC_O2_ROW)};
+static const o2_grp o2_grps[O2_NGROUPS] = {O2_GROUPS(ORC_O2_GRP)};
+static const o2_tgt o2_tgts[O2_NTARGETS] = {O2_TARGETS(ORC_O2_TGT)};
+static const int o2_group0[4] = O2_GROUP0, o2_ngroup[4] = O2_NGROUP;
+
+/* O2_CrossSection (src/cuda_mccO2.cu:978-...): the argon lookup's form on
+ * the O2 table */
+static real o2_sigma(const orc_o2_in *k, int r, real engy) {
+    if (engy == 0) return 0;
+    int N = k->nlogx, T = k->TnRct;
+    real lengy = (real)dm_log10((double)engy);
+    real ee1 = k->idLOGX * (lengy - k->xe[0]);
+    int ee2 = (int)ee1;
+    real a1 = ee1 - (real)ee2;
+    real a2 = 1 - a1;
+    if (lengy < k->xe[0]) return k->cx[r];
+    if (lengy > k->xe[N - 1])
+        return (real)((double)k->cx[(N - 1) * T + r] * 0.1 * dm_exp(-(double)(lengy - k->xe[N - 1])));
+    if (ee2 >= N - 1) return k->cx[(N - 1) * T + r];
+    return a2 * k->cx[ee2 * T + r] + a1 * k->cx[(ee2 + 1) * T + r];
+}
+
+static void o2_maxw(const orc_o2_in *k, int g, orc_stream *st, real *v) {
+    int n = (int)((real)(k->nvel - 1) * U(st));
+    real rphi = U(st), rthe = U(st);
+    dmaxwellv(&v[0], &v[1], &v[2], k->vsave[n], k->BackVel1[g], rphi, rthe);
+}
+
+typedef struct {
+    int r, dead, np, detach;
+    int ps[2];
+    real pv[2][3];
+} o2_out;
+
+/* one collision of species s with the target of group grp at node g */
+static void o2_collide(const orc_o2_in *k, int s, int grp, int g, orc_stream *st, real *v, o2_out *o) {
+    const o2_grp *G = &o2_grps[grp];
+    real vn[3] = {0, 0, 0};
+    int heavy = (s == 1 || s == 2), q;
+    o->r = -1; o->dead = 0; o->np = 0; o->detach = 0;
+    if (heavy) o2_maxw(k, g, st, vn);
+    real VX = v[0] - vn[0], VY = v[1] - vn[1], VZ = v[2] - vn[2];
+    real dum = VX * VX + VY * VY + VZ * VZ;
+    real vel = RSQRT(dum);
+    if (!heavy) { VX /= vel; VY /= vel; VZ /= vel; }
+    real engy = k->Escale[s] * dum;
+    real R1 = U(st) * k->sigv[G->sig] / vel;
+    real sum = 0;
+    const o2_row *row = 0;
+    for (q = G->first; q < G->first + G->n; ++q) {
+        if (!(engy > k->Th_e[o2_rows[q].r])) continue;
+        real sg = o2_sigma(k, o2_rows[q].r, engy);
+        sum = o2_rows[q].sum ? sum + sg : sg;
+        if (R1 <= sum) { row = &o2_rows[q]; break; }
+    }
+    if (!row) return;
+    o->r = row->r;
+    if (row->type == O2T_LOSS || row->type == O2T_ION || row->type == O2T_DETACH) {
+        if (row->emode == O2E_LOSS) { engy -= k->Th_e[row->r]; vel = RSQRT(RABS(engy) / k->Escale[s]); }
+        if (row->emode == O2E_GAIN) { engy += (real)row->gain; vel = RSQRT(RABS(engy) / k->Escale[s]); }
+        int prods[2] = {row->p0, row->type == O2T_ION ? row->p1 : -1}, j;
+        for (j = 0; j < 2; ++j) {
+            if (prods[j] < 0) continue;
+            o->ps[o->np] = prods[j];
+            if (prods[j] == 0) { o->pv[o->np][0] = VX; o->pv[o->np][1] = VY; o->pv[o->np][2] = VZ; }
+            else o2_maxw(k, g, st, o->pv[o->np]);
+            o->np++;
+        }
+        if (row->type == O2T_DETACH) o->detach = 1;
+        real a = U(st), b = U(st);
+        anewvel(engy, vel, &VX, &VY, &VZ, a, b);
+        v[0] = VX; v[1] = VY; v[2] = VZ;
+    } else if (row->type == O2T_ATTACH) {
+        o->dead = 1;
+        o->ps[0] = row->p0; o2_maxw(k, g, st, o->pv[0]); o->np = 1;
+    } else if (row->type == O2T_DEL) {
+        o->dead = 1;
+    } else if (row->type == O2T_SCAT) {
+        real a = U(st), b = U(st);
+        newvel_ionsc(&VX, &VY, &VZ, vel, a, b);
+        if (heavy) { v[0] = VX + vn[0]; v[1] = VY + vn[1]; v[2] = VZ + vn[2]; }
+        else { v[0] = VX; v[1] = VY; v[2] = VZ; }
+    } else if (row->type == O2T_CXKEEP) {
+        v[0] = vn[0]; v[1] = vn[1]; v[2] = vn[2];
+    } else if (row->type == O2T_CXTO) {
+        o->dead = 1;
+        o->ps[0] = row->p0; o->pv[0][0] = vn[0]; o->pv[0][1] = vn[1]; o->pv[0][2] = vn[2]; o->np = 1;
+    } else { /* O2T_NEGDET */
+        o->dead = 1;
+        o->ps[0] = 0; o->pv[0][0] = VX; o->pv[0][1] = VY; o->pv[0][2] = VZ; o->np = 1;
+    }
+}
+
+typedef struct { uint32_t key; int pos; } o2_key;
+static int o2_key_cmp(const void *a, const void *b) {
+    const o2_key *x = a, *y = b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->pos - y->pos;
+}
+
+int orc_mcc_o2(const orc_o2_in *k, orc_bucket *B, uint32_t *next_id, int64_t *mccr) {
+    int G = k->G, g, s, t, kk, j, rc = 0;
+    real *prob = zalloc((size_t)G * O2_NTARGETS, sizeof(real));
+    for (g = 0; g < G; ++g) {
+        real acc = 0;
+        int sp = -1;
+        for (t = 0; t < O2_NTARGETS; ++t) {
+            const o2_tgt *T = &o2_tgts[t];
+            if (T->sp != sp) { sp = T->sp; acc = 0; }
+            real x = (T->sp <= 1 ? k->dtm : k->dt) * k->sigv[T->sig];
+            if (T->kind == 0) x = x * k->BackDen1[g];
+            else if (T->kind == 1) x = x * k->FluidN[(size_t)T->idx * G + g];
+            else x = x * k->den[(size_t)T->idx * G + g] * k->np2c[T->idx] * k->idx * k->idy;
+            acc = (acc + (real)1) - (real)dm_exp(-(double)x);
+            prob[(size_t)g * O2_NTARGETS + t] = acc;
+        }
+    }
+    /* collisions: outcome per particle */
+    int *dead[4];
+    o2_out *out[4];
+    int *delreq = zalloc(G, sizeof(int));
+    for (s = 0; s < 4; ++s) {
+        dead[s] = zalloc(B[s].n + 1, sizeof(int));
+        out[s] = zalloc(B[s].n + 1, sizeof(o2_out));
+    }
+    for (s = 0; s < 4; ++s) {
+        int ng = o2_ngroup[s], g0 = o2_group0[s];
+        int draws = s <= 1 ? k->MCCn : 1;
+        for (g = 0; g < G; ++g) {
+            const real *P = prob + (size_t)g * O2_NTARGETS + g0;
+            real Tprob = P[ng - 1];
+            for (kk = B[s].cell_start[g]; kk < B[s].cell_start[g + 1]; ++kk) {
+                orc_stream st;
+                orc_stream_init(&st, k->seed, B[s].id[kk], k->step, (uint32_t)((s << 8) | 1));
+                real R1 = 0;
+                out[s][kk].r = -1;
+                out[s][kk].np = 0;
+                for (j = 0; j < draws; ++j) { R1 = U(&st); if (R1 < Tprob) break; }
+                if (!(R1 < Tprob)) continue;
+                int flag = 0;
+                while (flag < ng - 1 && R1 > P[flag]) ++flag;
+                real v[3] = {B[s].vx[kk], B[s].vy[kk], B[s].vz[kk]};
+                o2_collide(k, s, g0 + flag, g, &st, v, &out[s][kk]);
+                if (out[s][kk].r < 0) continue;
+                mccr[(size_t)g * k->TnRct + out[s][kk].r]++;
+                if (out[s][kk].dead) dead[s][kk] = 1;
+                else { B[s].vx[kk] = v[0]; B[s].vy[kk] = v[1]; B[s].vz[kk] = v[2]; }
+                if (out[s][kk].detach) delreq[g]++;
+            }
+        }
+    }
+    /* O- removed by electron detachment: per node, min(requests, O- that
+     * survived their own collision), smallest Philox keys first */
+    for (g = 0; g < G; ++g) {
+        if (!delreq[g]) continue;
+        int a = B[3].cell_start[g], b = B[3].cell_start[g + 1], m = 0;
+        o2_key *keys = zalloc(b - a + 1, sizeof(o2_key));
+        for (kk = a; kk < b; ++kk) {
+            if (dead[3][kk]) continue;
+            uint32_t ctr[4] = {B[3].id[kk], k->step, O2_VICTIM_TAG, 0};
+            uint32_t key[2] = {(uint32_t)k->seed, (uint32_t)(k->seed >> 32)}, o[4];
+            orc_philox4x32_10(ctr, key, o);
+            keys[m].key = o[0]; keys[m].pos = kk; ++m;
+        }
+        qsort(keys, m, sizeof(o2_key), o2_key_cmp);
+        int kdel = delreq[g] < m ? delreq[g] : m;
+        for (j = 0; j < kdel; ++j) dead[3][keys[j].pos] = 2;
+        free(keys);
+    }
+    /* rebuild: survivors in order, then products by (source species, parent,
+     * slot).  Every species' rebuild reads the old buckets of all four, so
+     * the new ones are built aside and copied back at the end. */
+    real *nx[4], *ny[4], *nvx[4], *nvy[4], *nvz[4];
+    uint32_t *nidv[4];
+    int *start[4], nnew[4];
+    for (t = 0; t < 4; ++t) {
+        orc_bucket *D = &B[t];
+        int cap = D->cap, pos = 0;
+        uint32_t nid = next_id[t];
+        nx[t] = zalloc(cap + 1, sizeof(real)); ny[t] = zalloc(cap + 1, sizeof(real));
+        nvx[t] = zalloc(cap + 1, sizeof(real)); nvy[t] = zalloc(cap + 1, sizeof(real));
+        nvz[t] = zalloc(cap + 1, sizeof(real));
+        nidv[t] = zalloc(cap + 1, sizeof(uint32_t));
+        start[t] = zalloc(G + 1, sizeof(int));
+        for (g = 0; g < G; ++g) {
+            start[t][g] = pos;
+            for (kk = D->cell_start[g]; kk < D->cell_start[g + 1]; ++kk) {
+                if (dead[t][kk]) continue;
+                if (pos < cap) {
+                    nx[t][pos] = D->x[kk]; ny[t][pos] = D->y[kk];
+                    nvx[t][pos] = D->vx[kk]; nvy[t][pos] = D->vy[kk]; nvz[t][pos] = D->vz[kk];
+                    nidv[t][pos] = D->id[kk];
+                }
+                ++pos;
+            }
+            int s2;
+            for (s2 = 0; s2 < 4; ++s2)
+                for (kk = B[s2].cell_start[g]; kk < B[s2].cell_start[g + 1]; ++kk) {
+                    const o2_out *o = &out[s2][kk];
+                    for (j = 0; j < o->np; ++j) {
+                        if (o->ps[j] != t) continue;
+                        if (pos < cap) {
+                            nx[t][pos] = B[s2].x[kk]; ny[t][pos] = B[s2].y[kk];
+                            nvx[t][pos] = o->pv[j][0]; nvy[t][pos] = o->pv[j][1]; nvz[t][pos] = o->pv[j][2];
+                            nidv[t][pos] = nid;
+                        }
+                        ++nid;
+                        ++pos;
+                    }
+                }
+        }
+        start[t][G] = pos;
+        if (pos > cap) rc = -30;
+        nnew[t] = pos < cap ? pos : cap;
+        next_id[t] = nid;
+    }
+    for (t = 0; t < 4; ++t) {
+        orc_bucket *D = &B[t];
+        int n = nnew[t];
+        memcpy(D->x, nx[t], n * sizeof(real)); memcpy(D->y, ny[t], n * sizeof(real));
+        memcpy(D->vx, nvx[t], n * sizeof(real)); memcpy(D->vy, nvy[t], n * sizeof(real));
+        memcpy(D->vz, nvz[t], n * sizeof(real));
+        memcpy(D->id, nidv[t], n * sizeof(uint32_t));
+        memcpy(D->cell_start, start[t], (G + 1) * sizeof(int));
+        D->n = n;
+        free(nx[t]); free(ny[t]); free(nvx[t]); free(nvy[t]); free(nvz[t]); free(nidv[t]); free(start[t]);
+    }
+    for (s = 0; s < 4; ++s) { free(dead[s]); free(out[s]); }
+    free(delreq); free(prob);
+    return rc;
+}

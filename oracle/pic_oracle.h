@@ -148,4 +148,24 @@ typedef struct {
     double *lu; int lu_bw;
 } orc_case;
 
+/* O2 chemistry (gas type 1) collision step, standalone: the caller passes
+ * the setup arrays and the four species' buckets (e, O2+, O+, O-), sorted by
+ * node with cell_start[G+1] and room for cap particles.  The buckets are
+ * rebuilt in place; next_id and mccr (G x TnRct) are advanced. */
+typedef struct {
+    int G, TnRct, nlogx, nvel, MCCn;
+    real idLOGX, dt, dtm, idx, idy;
+    real np2c[4], Escale[4], sigv[20];
+    const real *Th_e, *xe, *cx, *vsave, *BackDen1, *BackVel1, *FluidN, *den;
+    uint64_t seed;
+    uint32_t step;
+} orc_o2_in;
+typedef struct {
+    real *x, *y, *vx, *vy, *vz;
+    uint32_t *id;
+    int *cell_start;
+    int n, cap;
+} orc_bucket;
+int orc_mcc_o2(const orc_o2_in *k, orc_bucket *sp, uint32_t *next_id, int64_t *mccr);
+
 #endif

@@ -5,8 +5,9 @@ oracle/_ref, where the reference's own src/PhysicsData.c fit functions,
 CrossSection() data and FUNC_decay are compiled and driven with the table
 loops of Oxygen_CrossSectionSET / ArO2_CrossSectionSET.  The row, rate and
 pair specification is checked against the reference source text where that
-tree is present.  Gas types 1 and 2 have host setup only: creating a case
-for them is refused with -19.
+tree is present.  Gas type 2 has host setup only: creating a case for it is
+refused with -19 (the O2 collision step, gas type 1, is tested on the GPU in
+test_gpu_o2.py).
 """
 import base64
 import json
@@ -134,11 +135,10 @@ def test_reaction_names_follow_the_deck_format():
     assert not np.any(cx[:, 10]) and np.any(cx[:, 9])
 
 
-def test_case_creation_is_refused_for_oxygen():
-    for gas in (1, 2):
-        with pytest.raises(PicError) as e:
-            picmcc.Case(decks.oxygen_deck(gas=gas, ncx=8, ncy=6, ppc=5))
-        assert e.value.code == -19
+def test_case_creation_is_refused_for_ar_o2():
+    with pytest.raises(PicError) as e:
+        picmcc.Case(decks.oxygen_deck(gas=2, ncx=8, ncy=6, ppc=5))
+    assert e.value.code == -19
 
 
 @pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "src")), reason="reference tree not present")
