@@ -510,7 +510,7 @@ struct Case : CaseBase {
                 HIPCHK(dalloc(&o2stash[s], (size_t)6 * cur[s].cap));
             }
             oc.G = G; oc.ncy = S.ncy; oc.ngy = S.ngy; oc.Csize = S.Csize; oc.TnRct = S.TnRct; oc.nlogx = N_LOGX;
-            oc.nvel = S.nvel; oc.MCCn = S.MCCn;
+            oc.nvel = S.nvel; oc.MCCn = S.MCCn; oc.ions = d.o2_ion_coll != 0;
             oc.idLOGX = S.idLOGX; oc.dt = S.dt; oc.dtm = S.dt_mcc; oc.idx = S.idx; oc.idy = S.idy;
             for (int s = 0; s < O2_NSP; ++s) { oc.np2c[s] = S.np2c[s]; oc.Escale[s] = S.Escale[s]; }
             for (int j = 0; j < O2_NSIGV; ++j) oc.sigv[j] = S.sigv[j];

@@ -17,6 +17,7 @@ namespace pic {
 template <typename R>
 struct O2Const {
     int G, ncy, ngy, Csize, TnRct, nlogx, nvel, MCCn;
+    int ions;   // 0: only electrons collide, as the reference's MCC_O2_Basic (src/cuda_mcc.cu:76)
     R idLOGX, dt, dtm, idx, idy;
     R np2c[O2_NSP], Escale[O2_NSP];
     R sigv[O2_NSIGV];

@@ -173,6 +173,10 @@ __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, 
     R *stash = set.stash[isp];
     int k0, k1;
     cell_range(p, c, k0, k1);
+    if (isp != 0 && !k.ions) {
+        for (int kk = k0 + lane; kk < k1; kk += 64) fate[kk] = 0;
+        return;
+    }
     const size_t cap = (size_t)p.cap;
     const int g0 = kO2Group0[isp], ng = kO2NGroup[isp];
     const R *P = k.prob + (size_t)c * O2_NTARGETS + g0;

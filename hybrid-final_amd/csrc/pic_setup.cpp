@@ -117,6 +117,7 @@ int parse_deck(const std::string &text, Deck &d, std::string &err) {
     if (d.seed == 0) d.seed = 1;
     d.rng_seed = (int64_t)sm.number("PhiloxSeed");
     d.rf_cycles = (int)sm.number("RFCycles");
+    d.o2_ion_coll = (int)sm.number("O2IonCollisions");
     if (d.gas >= 0 && d.gas <= 2) {
         const JValue &ac = root.object("Collision_On_Off").object(kCollObj[d.gas]);
         for (int r = 0; r < kCollN[d.gas]; ++r) d.coll_flag[r] = ac.number(kCollKeys[d.gas][r]);

@@ -40,6 +40,7 @@ struct Deck {
     int n_ave = 0, n_smt = 0, np_limit = 0;
     double coll_flag[100] = {0};
     int64_t seed = 1, rng_seed = 0;
+    int o2_ion_coll = 0;   // O2: 1 = O2+, O+ and O- collide too (the reference's O2 step collides electrons only)
     int rf_cycles = 0;
 };
 

@@ -94,7 +94,7 @@ def reaction_keys(gas):
 OXYGEN_NEUTRALS = {"O2A": 1e19, "O2B": 5e18, "OP": 5e19, "OD": 1e18}
 
 
-def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None, **kw):
+def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None, ion_collisions=False, **kw):
     """O2 (gas=1) or Ar/O2 (gas=2) CCP deck on the ccp_deck geometry.
 
     ``o2_ratio`` is the O2 fraction of the background pressure (gas 2; the
@@ -142,6 +142,10 @@ def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None,
                                                  and "Max_np" in c)
     obj, keys = reaction_keys(gas)
     d["Collision_On_Off"] = {obj: {k: 1.0 for k in keys}}
+    if ion_collisions:
+        # O2 only: let O2+, O+ and O- collide too (the reference's O2 step
+        # collides electrons only, src/cuda_mcc.cu:76)
+        d["SimulationMethod"]["O2IonCollisions"] = 1
     return d
 
 

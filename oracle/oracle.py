@@ -249,7 +249,8 @@ def mcc_o2(precision, k, buckets, next_id, mccr):
 
     k: dict of the setup values (G, TnRct, nlogx, nvel, MCCn, idLOGX, dt,
     dtm, idx, idy, np2c[4], Escale[4], sigv[20], seed, step) and arrays
-    (Th_e, xe, cx, vsave, BackDen1, BackVel1, FluidN, den).  buckets: four
+    (Th_e, xe, cx, vsave, BackDen1, BackVel1, FluidN, den), and ions (1: the
+    positive and negative ions collide too; default 0).  buckets: four
     dicts x, y, vx, vy, vz, id, cell_start, cap (e, O2+, O+, O-).  Returns
     (new buckets, next_id, mccr, rc); the inputs are not modified.
     """
@@ -266,7 +267,7 @@ def mcc_o2(precision, k, buckets, next_id, mccr):
                     ("idLOGX", R), ("dt", R), ("dtm", R), ("idx", R), ("idy", R),
                     ("np2c", R * 4), ("Escale", R * 4), ("sigv", R * 20),
                     ("Th_e", P), ("xe", P), ("cx", P), ("vsave", P), ("BackDen1", P), ("BackVel1", P),
-                    ("FluidN", P), ("den", P), ("seed", C.c_uint64), ("step", C.c_uint32)]
+                    ("FluidN", P), ("den", P), ("seed", C.c_uint64), ("step", C.c_uint32), ("ions", C.c_int)]
 
     class Bucket(C.Structure):
         _fields_ = [("x", P), ("y", P), ("vx", P), ("vy", P), ("vz", P), ("id", C.POINTER(C.c_uint32)),
@@ -292,6 +293,7 @@ def mcc_o2(precision, k, buckets, next_id, mccr):
         setattr(inp, f, ptr(k[f], real, R)[1])
     inp.seed = int(k["seed"]) & (2 ** 64 - 1)
     inp.step = int(k["step"]) & (2 ** 32 - 1)
+    inp.ions = int(k.get("ions", 0))
     bs = (Bucket * 4)()
     arrays = []
     for s in range(4):

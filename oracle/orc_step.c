@@ -1014,6 +1014,7 @@ int orc_mcc_o2(const orc_o2_in *k, orc_bucket *B, uint32_t *next_id, int64_t *mc
         out[s] = zalloc(B[s].n + 1, sizeof(o2_out));
     }
     for (s = 0; s < 4; ++s) {
+        if (s != 0 && !k->ions) continue;   /* the reference's thread bound TID < Gsize */
         int ng = o2_ngroup[s], g0 = o2_group0[s];
         int draws = s <= 1 ? k->MCCn : 1;
         for (g = 0; g < G; ++g) {

@@ -159,6 +159,7 @@ typedef struct {
     const real *Th_e, *xe, *cx, *vsave, *BackDen1, *BackVel1, *FluidN, *den;
     uint64_t seed;
     uint32_t step;
+    int ions;   /* 0: only electrons collide, as the reference's MCC_O2_Basic (src/cuda_mcc.cu:76) */
 } orc_o2_in;
 typedef struct {
     real *x, *y, *vx, *vy, *vz;

@@ -25,9 +25,11 @@ EMIN = (0.01, 1e-3, 1e-3, 1e-3)
 
 def _deck(**kw):
     # fluid densities near the gas density, so that collisions with O2(a),
-    # O2(b), O(3P) and O(1D) are as frequent as with O2
+    # O2(b), O(3P) and O(1D) are as frequent as with O2; the ions collide too
+    # (the reference's O2 step collides electrons only, the default)
     dense = {"O2A": 2e21, "O2B": 2e21, "OP": 2e21, "OD": 2e21}
-    base = dict(gas=1, ncx=10, ncy=8, ppc=150, pres_torr=0.1, volts=100.0, neutral_densities=dense)
+    base = dict(gas=1, ncx=10, ncy=8, ppc=150, pres_torr=0.1, volts=100.0, neutral_densities=dense,
+                ion_collisions=True)
     base.update(kw)
     return decks.oxygen_deck(**base)
 
@@ -61,6 +63,7 @@ def _consts(hs, deck, c):
     for f in ("Th_e", "xe", "cx", "vsave", "BackDen1", "BackVel1", "FluidN"):
         k[f] = hs.array(f)
     k["seed"] = int(deck["SimulationMethod"]["PhiloxSeed"])
+    k["ions"] = int(deck["SimulationMethod"].get("O2IonCollisions", 0))
     return k
 
 
@@ -122,6 +125,31 @@ def test_o2_collision_step_matches_oracle(prec, name):
              "charge exchange O+ to O2+": [53, 56, 57], "recombination O-": [44, 45]}
     for kind, rs in kinds.items():
         assert fired[rs].sum() > 0, kind
+
+
+@pytest.mark.parametrize("prec,name", PRECS)
+def test_o2_default_collides_electrons_only(prec, name):
+    """Without O2IonCollisions the step does what the reference's
+    MCC_O2_Basic does (its thread bound TID < Gsize runs the electron
+    handler only, src/cuda_mcc.cu:76): ions keep their velocities and no ion
+    reaction is counted."""
+    deck = _deck(ion_collisions=False, ppc=60)
+    assert "O2IonCollisions" not in deck["SimulationMethod"]
+    hs = HostSetup(deck, prec, load=False)
+    c = Case(deck, prec)
+    try:
+        c.set_step(5)
+        _spread(c, hs, c.real, np.random.default_rng(3))
+        ions = [c.particles(s) for s in (1, 2, 3)]
+        n0, n1, d = _check_mcc(c, hs, deck, name, den_scale=6.0)
+        assert d[:, 41:].sum() == 0 and d[:, :41].sum() > 0
+        for s, before in zip((1, 2, 3), ions):
+            after = c.particles(s)
+            got = {int(i): v for i, v in zip(after["id"], after["vx"])}
+            assert all(got[int(i)] == v for i, v in zip(before["id"], before["vx"]) if int(i) in got)
+    finally:
+        c.close()
+        hs.close()
 
 
 @pytest.mark.parametrize("prec,name", PRECS)

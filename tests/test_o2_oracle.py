@@ -34,7 +34,7 @@ def _k(G=1, rows=(), sig=None, bg=0.0, fluid=(0, 0, 0, 0), den=(0, 0, 0, 0), th=
              Th_e=np.zeros(T) if th is None else th, xe=np.arange(81) * 0.1 - 4.0, cx=cx.ravel(),
              vsave=np.linspace(0.5, 2.0, 16), BackDen1=np.full(G, bg), BackVel1=np.full(G, 300.0),
              FluidN=np.repeat(np.asarray(fluid, dtype=float), G), den=np.repeat(np.asarray(den, dtype=float), G),
-             seed=12345, step=3)
+             seed=12345, step=3, ions=1)
     return k
 
 
