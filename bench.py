@@ -104,6 +104,29 @@ def run_pbatch(n, reps=3):
     return out
 
 
+def run_o2(steps, warmup=2):
+    """O2 chemistry (gas type 1), staged steps: 256x256 cells, dx=0.2 mm,
+    100 V, 50 mTorr, 50 particles/cell for e and O2+ and 10 for O+ and O-,
+    ions colliding too; FP32 particles with the FP64-refined field solve.
+    Particles are resident in HBM before the timed steps."""
+    from picmcc import Case, FP32_REFINED, decks
+    deck = decks.oxygen_deck(gas=1, ncx=256, ncy=256, dx=2e-4, volts=100.0, pres_torr=0.05, ppc=50,
+                             steps_per_cycle=800, ion_collisions=True)
+    c = Case(deck, FP32_REFINED)
+    c.step(warmup)
+    c.sync()
+    n = int(sum(c.info().np[s] for s in range(4)))
+    t0 = time.perf_counter()
+    c.step(steps)
+    c.sync()
+    dt = (time.perf_counter() - t0) / steps
+    c.close()
+    return {"workload": "O2 CCP 256x256 cells dx=0.2mm, 100 V, 50 mTorr, 50/50/10/10 particles per cell "
+                        "(e, O2+, O+, O-), ions colliding, dt=T_RF/800",
+            "particles": n, "steps": steps, "ms_per_step": dt * 1e3, "particle_steps_per_s": n / dt,
+            "path": "staged stages (the fused two-pass step is built for argon)"}
+
+
 def run_ensemble(n, rate_hz):
     """A served ensemble of n argon CCP cases (grids 32^2-96^2, 20-60
     particles/cell, 10-500 mTorr, 50-300 V, 1-3 RF cycles of 400 steps,
@@ -160,6 +183,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--pbatch", type=int, default=256, metavar="N",
                     help="independent Poisson problems (64^2-512^2) in the batched-solve leg (0 = skip)")
+    ap.add_argument("--o2-steps", type=int, default=20, metavar="K",
+                    help="staged O2-chemistry steps timed on a 256x256 O2 case (0 = skip)")
     ap.add_argument("--ensemble", type=int, default=16, metavar="N",
                     help="cases in the served argon ensemble (Poisson arrivals; concurrent server, first-come "
                          "first-served server and one by one; 0 = skip)")
@@ -254,6 +279,8 @@ def main():
     }
     if args.pbatch > 0:
         line["batched_poisson"] = run_pbatch(args.pbatch)
+    if args.o2_steps > 0:
+        line["o2_chemistry"] = run_o2(args.o2_steps)
     line["ensemble"] = ensemble if ensemble is not None else {"status": "not run (--ensemble 0)"}
     if args.cpu_steps > 0:
         cv, cdt = cpu_baseline(deck, args.cpu_steps)
