@@ -632,14 +632,16 @@ struct Case : CaseBase {
         launch_fill_int(o2delreq, 0, G, st);
         for (int s = 0; s < O2_NSP; ++s) launch_o2_collide<R>(oc, set, s, step, mccr, o2pcount, o2delreq, st);
         launch_o2_victims<R>(oc, set, step, o2delreq, st);
+        O2Set<R> dst = set;
         for (int t = 0; t < O2_NSP; ++t) {
             int *pc = o2pcount + (size_t)t * G, *ps = o2pstart + (size_t)t * (G + 1);
             launch_fill_int(total, 0, G, st);
             launch_o2_totals<R>(oc, set, t, pc, total, st);
             launch_scan(total, nxt[t].cell_start, G, scan_tmp, st);
             launch_scan(pc, ps, G, scan_tmp, st);
-            launch_o2_scatter<R>(oc, set, t, nxt[t], ps, next_id + t, nxt[t].cap, d_err, st);
+            dst.p[t] = nxt[t];
         }
+        launch_o2_scatter_all<R>(oc, set, dst, o2pstart, next_id, d_err, st);
         for (int s = 0; s < O2_NSP; ++s) swap_species(s);
         launch_o2_bump_ids(next_id, o2pstart, G, st);
         HIPCHK(hipGetLastError());

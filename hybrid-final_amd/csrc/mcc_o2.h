@@ -54,10 +54,12 @@ void launch_o2_victims(const O2Const<R> &k, const O2Set<R> &set, uint32_t step, 
 template <typename R>
 void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, int t, const int *pcount_t, int *total,
                       hipStream_t s);
-// new bucket = survivors in order, then products by (source species, parent, slot)
+// new buckets of all four species (dst.p[t], cell_start already set):
+// survivors in order, then products by (source species, parent, slot);
+// pstart is 4 x (G+1)
 template <typename R>
-void launch_o2_scatter(const O2Const<R> &k, const O2Set<R> &set, int t, const PView<R> &dst, const int *pstart_t,
-                       const uint32_t *next_id_t, int cap, int *err, hipStream_t s);
+void launch_o2_scatter_all(const O2Const<R> &k, const O2Set<R> &set, const O2Set<R> &dst, const int *pstart,
+                           const uint32_t *next_id, int *err, hipStream_t s);
 // next_id[t] += product total of species t (pstart is 4 x (G+1))
 void launch_o2_bump_ids(uint32_t *next_id, const int *pstart, int G, hipStream_t s);
 

@@ -8,7 +8,8 @@
 //            to two products (velocities in a stash indexed like the parent)
 //   victims  per cell with electron detachments: removes that many O- of
 //            the cell, smallest Philox keys first
-//   totals / scatter  per target species: survivors in old order, then
+//   totals   per species: survivors plus products per node (then scans)
+//   scatter  all species in one launch: survivors in old order, then
 //            products by (source species, parent, slot); new ids follow the
 //            global (cell, source species, parent, slot) order
 // Every count is an integer atomic or a ballot, so the result is the same
@@ -289,36 +290,45 @@ __global__ __launch_bounds__(256) void k_o2_totals(O2Const<R> k, O2Set<R> set, i
     if (lane == 0) total[c] = n + pcount_t[c];
 }
 
+// All four species in one pass per cell: survivors of each species, then
+// one walk over the parents of every source species, whose products are
+// placed into all target species with four wave scans: new bucket =
+// survivors in old order, then products by (source species, parent, slot).
 template <typename R>
-__global__ __launch_bounds__(256) void k_o2_scatter(O2Const<R> k, O2Set<R> set, int t, PView<R> dst,
-                                                    const int *pstart_t, const uint32_t *next_id_t, int cap,
-                                                    int *err) {
+__global__ __launch_bounds__(256) void k_o2_scatter_all(O2Const<R> k, O2Set<R> set, O2Set<R> dst, const int *pstart,
+                                                        const uint32_t *next_id, int *err) {
     int lane;
     const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    const int ns = dst.cell_start[c];
     const unsigned long long lt = lanes_below(lane);
-    const PView<R> p = set.p[t];
-    const uint8_t *ft = set.fate[t];
-    int k0, k1;
-    cell_range(p, c, k0, k1);
-    int run = 0;
-    for (int b = k0; b < k1; b += 64) {
-        const int kk = b + lane;
-        const bool keep = kk < k1 && !(ft[kk] & (O2F_DEAD | O2F_VICTIM));
-        const unsigned long long m = __ballot(keep);
-        if (keep) {
-            const int pos = ns + run + __popcll(m & lt);
-            if (pos >= cap) atomicOr(err, 1);
-            else {
-                dst.x[pos] = p.x[kk]; dst.y[pos] = p.y[kk];
-                dst.vx[pos] = p.vx[kk]; dst.vy[pos] = p.vy[kk]; dst.vz[pos] = p.vz[kk]; dst.id[pos] = p.id[kk];
+    int base[O2_NSP], id0[O2_NSP];
+#pragma unroll
+    for (int t = 0; t < O2_NSP; ++t) {
+        const PView<R> p = set.p[t];
+        const PView<R> d = dst.p[t];
+        const uint8_t *ft = set.fate[t];
+        int k0, k1;
+        cell_range(p, c, k0, k1);
+        const int ns = d.cell_start[c];
+        int run = 0;
+        for (int b = k0; b < k1; b += 64) {
+            const int kk = b + lane;
+            const bool keep = kk < k1 && !(ft[kk] & (O2F_DEAD | O2F_VICTIM));
+            const unsigned long long m = __ballot(keep);
+            if (keep) {
+                const int pos = ns + run + __popcll(m & lt);
+                if (pos >= d.cap) atomicOr(err, 1);
+                else {
+                    d.x[pos] = p.x[kk]; d.y[pos] = p.y[kk];
+                    d.vx[pos] = p.vx[kk]; d.vy[pos] = p.vy[kk]; d.vz[pos] = p.vz[kk]; d.id[pos] = p.id[kk];
+                }
             }
+            run += __popcll(m);
         }
-        run += __popcll(m);
+        base[t] = ns + run;
+        id0[t] = (int)(next_id[t] + (uint32_t)pstart[(size_t)t * (k.G + 1) + c]);
     }
-    const uint32_t id0 = *next_id_t + (uint32_t)pstart_t[c];
-    int prun = 0;
+    int prun[O2_NSP] = {0, 0, 0, 0};
     for (int s2 = 0; s2 < O2_NSP; ++s2) {
         const PView<R> q = set.p[s2];
         const uint8_t *fq = set.fate[s2];
@@ -328,35 +338,40 @@ __global__ __launch_bounds__(256) void k_o2_scatter(O2Const<R> k, O2Set<R> set, 
         cell_range(q, c, q0, q1);
         for (int b = q0; b < q1; b += 64) {
             const int kk = b + lane;
-            int n0 = 0, n1 = 0;
+            int pr0 = -1, pr1 = -1;
             if (kk < q1) {
                 const uint8_t f = fq[kk];
-                n0 = o2_fate_prod(f, 0) == t;
-                n1 = o2_fate_prod(f, 1) == t;
+                pr0 = o2_fate_prod(f, 0);
+                pr1 = o2_fate_prod(f, 1);
             }
-            const int cnt = n0 + n1;
-            int incl = cnt;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int u = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += u;
-            }
-            const int tot = __shfl(incl, 63, 64);
-            int r = prun + incl - cnt;
-            for (int slot = 0; slot < 2; ++slot) {
-                if (!(slot ? n1 : n0)) continue;
-                const int pos = ns + run + r;
-                if (pos >= cap) atomicOr(err, 1);
-                else {
-                    dst.x[pos] = q.x[kk]; dst.y[pos] = q.y[kk];
-                    dst.vx[pos] = sq[(3 * slot) * qcap + kk];
-                    dst.vy[pos] = sq[(3 * slot + 1) * qcap + kk];
-                    dst.vz[pos] = sq[(3 * slot + 2) * qcap + kk];
-                    dst.id[pos] = id0 + (uint32_t)r;
+            for (int t = 0; t < O2_NSP; ++t) {
+                const int n0 = pr0 == t, n1 = pr1 == t;
+                const int cnt = n0 + n1;
+                int incl = cnt;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int u = __shfl_up(incl, off, 64);
+                    if (lane >= off) incl += u;
                 }
-                ++r;
+                const int tot = __shfl(incl, 63, 64);
+                int r = prun[t] + incl - cnt;
+                const PView<R> d = dst.p[t];
+                for (int slot = 0; slot < 2; ++slot) {
+                    if (!(slot ? n1 : n0)) continue;
+                    const int pos = base[t] + r;
+                    if (pos >= d.cap) atomicOr(err, 1);
+                    else {
+                        d.x[pos] = q.x[kk]; d.y[pos] = q.y[kk];
+                        d.vx[pos] = sq[(3 * slot) * qcap + kk];
+                        d.vy[pos] = sq[(3 * slot + 1) * qcap + kk];
+                        d.vz[pos] = sq[(3 * slot + 2) * qcap + kk];
+                        d.id[pos] = (uint32_t)id0[t] + (uint32_t)r;
+                    }
+                    ++r;
+                }
+                prun[t] += tot;
             }
-            prun += tot;
         }
     }
 }
@@ -390,10 +405,10 @@ void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, int t, const int
     hipLaunchKernelGGL(k_o2_totals<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, t, pcount_t, total);
 }
 template <typename R>
-void launch_o2_scatter(const O2Const<R> &k, const O2Set<R> &set, int t, const PView<R> &dst, const int *pstart_t,
-                       const uint32_t *next_id_t, int cap, int *err, hipStream_t s) {
-    hipLaunchKernelGGL(k_o2_scatter<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, t, dst, pstart_t,
-                       next_id_t, cap, err);
+void launch_o2_scatter_all(const O2Const<R> &k, const O2Set<R> &set, const O2Set<R> &dst, const int *pstart,
+                           const uint32_t *next_id, int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_scatter_all<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, dst, pstart, next_id,
+                       err);
 }
 void launch_o2_bump_ids(uint32_t *next_id, const int *pstart, int G, hipStream_t s) {
     hipLaunchKernelGGL(k_o2_bump, dim3(1), dim3(64), 0, s, next_id, pstart, G);
@@ -405,8 +420,8 @@ void launch_o2_bump_ids(uint32_t *next_id, const int *pstart, int G, hipStream_t
                                        int *, int *, hipStream_t);                                                   \
     template void launch_o2_victims<R>(const O2Const<R> &, const O2Set<R> &, uint32_t, const int *, hipStream_t);    \
     template void launch_o2_totals<R>(const O2Const<R> &, const O2Set<R> &, int, const int *, int *, hipStream_t);   \
-    template void launch_o2_scatter<R>(const O2Const<R> &, const O2Set<R> &, int, const PView<R> &, const int *,     \
-                                       const uint32_t *, int, int *, hipStream_t);
+    template void launch_o2_scatter_all<R>(const O2Const<R> &, const O2Set<R> &, const O2Set<R> &, const int *,      \
+                                           const uint32_t *, int *, hipStream_t);
 INST(float)
 INST(double)
 #undef INST
