@@ -175,3 +175,55 @@ def test_o2_staged_steps_keep_the_bookkeeping(prec, name):
     finally:
         c.close()
         hs.close()
+
+
+# net change of (e, O2+, O+) per event of each reaction (O- is left out: its
+# count also drops by the detachment removals, which are not counted)
+def _stoich():
+    d = {}
+    for r in (10, 14, 22):
+        d[r] = (1, 1, 0)           # 2e + O2+
+    for r in (13, 21, 29, 37, 39):
+        d[r] = (1, 0, 1)           # 2e + O+
+    d[12] = (0, 0, 1)              # e + O+ + O-
+    for r in (6, 15, 23, 31):
+        d[r] = (-1, 0, 0)          # attachment, recombination
+    d[30] = (1, 0, 0)              # detachment by an electron
+    for r in (42, 43, 46):
+        d[r] = (1, 0, 0)           # O- detachment
+    for r in (47, 50):
+        d[r] = (0, -1, 1)          # O2+ -> O+
+    for r in (53, 56, 57):
+        d[r] = (0, 1, -1)          # O+ -> O2+
+    return d
+
+
+@pytest.mark.parametrize("prec", [FP32_REFINED, FP64_PLAIN])
+def test_o2_particle_balance_closes(prec):
+    """Whole O2 steps at 96x96 cells (about 1.1 M particles): the change of
+    the e, O2+ and O+ counts equals what the counted reactions make and
+    remove, less the particles taken by the walls and electrodes."""
+    deck = _deck(ncx=96, ncy=96, ppc=50, pres_torr=0.05)
+    c = Case(deck, prec)
+    try:
+        nsp = 4
+        ncond = c.info().CondNUMR
+
+        def state():
+            inf = c.info()
+            cc = c.array("CondCharge").astype(np.int64).reshape(nsp, ncond).sum(axis=1)
+            return (np.array([inf.np[s] for s in range(nsp)], dtype=np.int64),
+                    c.array("mccr").astype(np.int64).reshape(-1, 67).sum(axis=0),
+                    c.array("Lost").astype(np.int64) + cc)
+        n0, m0, w0 = state()
+        c.step(6)
+        c.sync()
+        n1, m1, w1 = state()
+        dm = m1 - m0
+        assert dm.sum() > 0
+        made = np.zeros(3, dtype=np.int64)
+        for r, v in _stoich().items():
+            made += dm[r] * np.array(v, dtype=np.int64)
+        assert np.array_equal(n1[:3] - n0[:3], made - (w1 - w0)[:3]), (n1 - n0, made, w1 - w0)
+    finally:
+        c.close()
