@@ -149,7 +149,7 @@ struct Case : CaseBase {
     O2Const<R> oc{};
     R *o2prob = nullptr, *o2fluid = nullptr, *o2bg = nullptr, *o2stash[O2_NSP] = {nullptr};
     uint8_t *o2fate[O2_NSP] = {nullptr};
-    int *o2pcount = nullptr, *o2pstart = nullptr, *o2delreq = nullptr;
+    int *o2pcount = nullptr, *o2pstart = nullptr, *o2delreq = nullptr, *o2total = nullptr;
     double *sum_den = nullptr, *sum_phi = nullptr, *sum_ex = nullptr, *sum_ey = nullptr;
     long long n_accum = 0;
     long long tstep = 0;
@@ -194,7 +194,7 @@ struct Case : CaseBase {
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
                         sum_ey, total2, alive2, clk, eedf, lost, sigma_hits, o2prob, o2fluid, o2bg, o2pcount,
-                        o2pstart, o2delreq};
+                        o2pstart, o2delreq, o2total};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (int s = 0; s < O2_NSP; ++s) {
@@ -505,6 +505,7 @@ struct Case : CaseBase {
             HIPCHK(upload(&o2fluid, S.FluidN, st)); HIPCHK(upload(&o2bg, S.BackDen1, st));
             HIPCHK(dalloc(&o2pcount, (size_t)O2_NSP * G)); HIPCHK(dalloc(&o2pstart, (size_t)O2_NSP * (G + 1)));
             HIPCHK(dalloc(&o2delreq, G));
+            HIPCHK(dalloc(&o2total, (size_t)O2_NSP * G));
             for (int s = 0; s < O2_NSP; ++s) {
                 HIPCHK(dalloc(&o2fate[s], cur[s].cap));
                 HIPCHK(dalloc(&o2stash[s], (size_t)6 * cur[s].cap));
@@ -633,14 +634,18 @@ struct Case : CaseBase {
         for (int s = 0; s < O2_NSP; ++s) launch_o2_collide<R>(oc, set, s, step, mccr, o2pcount, o2delreq, st);
         launch_o2_victims<R>(oc, set, step, o2delreq, st);
         O2Set<R> dst = set;
+        launch_fill_int(o2total, 0, O2_NSP * G, st);
+        launch_o2_totals<R>(oc, set, o2pcount, o2total, st);
+        // eight exclusive scans (new cell_start and product order of each
+        // species), three arrays per pass
+        const int *in[8];
+        int *out[8];
         for (int t = 0; t < O2_NSP; ++t) {
-            int *pc = o2pcount + (size_t)t * G, *ps = o2pstart + (size_t)t * (G + 1);
-            launch_fill_int(total, 0, G, st);
-            launch_o2_totals<R>(oc, set, t, pc, total, st);
-            launch_scan(total, nxt[t].cell_start, G, scan_tmp, st);
-            launch_scan(pc, ps, G, scan_tmp, st);
+            in[t] = o2total + (size_t)t * G; out[t] = nxt[t].cell_start;
+            in[4 + t] = o2pcount + (size_t)t * G; out[4 + t] = o2pstart + (size_t)t * (G + 1);
             dst.p[t] = nxt[t];
         }
+        for (int a = 0; a < 8; a += 3) launch_scan_multi(std::min(3, 8 - a), in + a, out + a, G, scan_tmp, st);
         launch_o2_scatter_all<R>(oc, set, dst, o2pstart, next_id, d_err, st);
         for (int s = 0; s < O2_NSP; ++s) swap_species(s);
         launch_o2_bump_ids(next_id, o2pstart, G, st);

@@ -49,11 +49,10 @@ void launch_o2_collide(const O2Const<R> &k, const O2Set<R> &set, int isp, uint32
 // marks the O- removed by electron detachment (smallest Philox keys first)
 template <typename R>
 void launch_o2_victims(const O2Const<R> &k, const O2Set<R> &set, uint32_t step, const int *delreq, hipStream_t s);
-// per node: survivors of species t plus its products (nodes that start no
-// cell keep the value already in total)
+// per node and species: survivors plus products, into total (4 x G; nodes
+// that start no cell keep the value already there)
 template <typename R>
-void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, int t, const int *pcount_t, int *total,
-                      hipStream_t s);
+void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, const int *pcount, int *total, hipStream_t s);
 // new buckets of all four species (dst.p[t], cell_start already set):
 // survivors in order, then products by (source species, parent, slot);
 // pstart is 4 x (G+1)

@@ -272,22 +272,26 @@ __global__ __launch_bounds__(256) void k_o2_victims(O2Const<R> k, O2Set<R> set, 
     }
 }
 
+// per node and species: survivors plus products (total is 4 x G; nodes
+// that start no cell keep the value already there)
 template <typename R>
-__global__ __launch_bounds__(256) void k_o2_totals(O2Const<R> k, O2Set<R> set, int t, const int *pcount_t,
-                                                   int *total) {
+__global__ __launch_bounds__(256) void k_o2_totals(O2Const<R> k, O2Set<R> set, const int *pcount, int *total) {
     int lane;
     const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
-    const PView<R> p = set.p[t];
-    const uint8_t *fate = set.fate[t];
-    int k0, k1;
-    cell_range(p, c, k0, k1);
-    int n = 0;
-    for (int b = k0; b < k1; b += 64) {
-        const int kk = b + lane;
-        n += __popcll(__ballot(kk < k1 && !(fate[kk] & (O2F_DEAD | O2F_VICTIM))));
+#pragma unroll
+    for (int t = 0; t < O2_NSP; ++t) {
+        const PView<R> p = set.p[t];
+        const uint8_t *fate = set.fate[t];
+        int k0, k1;
+        cell_range(p, c, k0, k1);
+        int n = 0;
+        for (int b = k0; b < k1; b += 64) {
+            const int kk = b + lane;
+            n += __popcll(__ballot(kk < k1 && !(fate[kk] & (O2F_DEAD | O2F_VICTIM))));
+        }
+        if (lane == 0) total[(size_t)t * k.G + c] = n + pcount[(size_t)t * k.G + c];
     }
-    if (lane == 0) total[c] = n + pcount_t[c];
 }
 
 // All four species in one pass per cell: survivors of each species, then
@@ -400,9 +404,8 @@ void launch_o2_victims(const O2Const<R> &k, const O2Set<R> &set, uint32_t step, 
     hipLaunchKernelGGL(k_o2_victims<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, step, delreq);
 }
 template <typename R>
-void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, int t, const int *pcount_t, int *total,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_o2_totals<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, t, pcount_t, total);
+void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, const int *pcount, int *total, hipStream_t s) {
+    hipLaunchKernelGGL(k_o2_totals<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, pcount, total);
 }
 template <typename R>
 void launch_o2_scatter_all(const O2Const<R> &k, const O2Set<R> &set, const O2Set<R> &dst, const int *pstart,
@@ -419,7 +422,7 @@ void launch_o2_bump_ids(uint32_t *next_id, const int *pstart, int G, hipStream_t
     template void launch_o2_collide<R>(const O2Const<R> &, const O2Set<R> &, int, uint32_t, unsigned long long *,     \
                                        int *, int *, hipStream_t);                                                   \
     template void launch_o2_victims<R>(const O2Const<R> &, const O2Set<R> &, uint32_t, const int *, hipStream_t);    \
-    template void launch_o2_totals<R>(const O2Const<R> &, const O2Set<R> &, int, const int *, int *, hipStream_t);   \
+    template void launch_o2_totals<R>(const O2Const<R> &, const O2Set<R> &, const int *, int *, hipStream_t);   \
     template void launch_o2_scatter_all<R>(const O2Const<R> &, const O2Set<R> &, const O2Set<R> &, const int *,      \
                                            const uint32_t *, int *, hipStream_t);
 INST(float)
