@@ -631,7 +631,7 @@ struct Case : CaseBase {
         launch_o2_probs<R>(oc, st);
         launch_fill_int(o2pcount, 0, O2_NSP * G, st);
         launch_fill_int(o2delreq, 0, G, st);
-        for (int s = 0; s < O2_NSP; ++s) launch_o2_collide<R>(oc, set, s, step, mccr, o2pcount, o2delreq, st);
+        launch_o2_collide<R>(oc, set, -1, step, mccr, o2pcount, o2delreq, st);   // all species
         launch_o2_victims<R>(oc, set, step, o2delreq, st);
         O2Set<R> dst = set;
         launch_fill_int(o2total, 0, O2_NSP * G, st);

@@ -41,8 +41,9 @@ struct O2Set {
 };
 
 template <typename R> void launch_o2_probs(const O2Const<R> &k, hipStream_t s);
-// collisions of species isp, in place; counts products per (species, node)
-// into pcount (4 x G) and O- removal requests per node into delreq
+// collisions of species isp (all four species when isp < 0), in place;
+// counts products per (species, node) into pcount (4 x G) and O- removal
+// requests per node into delreq
 template <typename R>
 void launch_o2_collide(const O2Const<R> &k, const O2Set<R> &set, int isp, uint32_t step, unsigned long long *mccr,
                        int *pcount, int *delreq, hipStream_t s);

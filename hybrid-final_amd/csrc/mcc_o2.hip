@@ -169,6 +169,7 @@ __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, 
     int lane;
     const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
+    if (isp < 0) isp = blockIdx.y;   // all species in one launch, one grid row each
     const PView<R> p = set.p[isp];
     uint8_t *fate = set.fate[isp];
     R *stash = set.stash[isp];
@@ -396,8 +397,8 @@ void launch_o2_probs(const O2Const<R> &k, hipStream_t s) {
 template <typename R>
 void launch_o2_collide(const O2Const<R> &k, const O2Set<R> &set, int isp, uint32_t step, unsigned long long *mccr,
                        int *pcount, int *delreq, hipStream_t s) {
-    hipLaunchKernelGGL(k_o2_collide<R>, dim3(wave_blocks(k.Csize)), dim3(256), 0, s, k, set, isp, step, mccr, pcount,
-                       delreq);
+    hipLaunchKernelGGL(k_o2_collide<R>, dim3(wave_blocks(k.Csize), isp < 0 ? O2_NSP : 1), dim3(256), 0, s, k, set, isp,
+                       step, mccr, pcount, delreq);
 }
 template <typename R>
 void launch_o2_victims(const O2Const<R> &k, const O2Set<R> &set, uint32_t step, const int *delreq, hipStream_t s) {
