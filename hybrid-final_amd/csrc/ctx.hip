@@ -84,7 +84,7 @@ struct CaseBase {
     virtual int get_timing(double *, int, long long *) = 0;
     int rf_cycles = 0, dt_pic = 0;
     int solver_kind = 0;   // 0 Jacobi PCG, 1 separable fast solve
-    int gas = 0;           // 0 Ar, 1 O2 (selects the collision step)
+    int gas = 0;           // 0 Ar, 1 O2, 2 Ar/O2 (selects the collision step)
 };
 
 template <typename R>
@@ -145,11 +145,14 @@ struct Case : CaseBase {
     int *condcharge = nullptr, *d_err = nullptr, *lost = nullptr;
     unsigned long long *sigma_hits = nullptr;   // nsp x G, this step's dielectric hits in fixed point
     bool diel_routes = false;                   // some cell border leads into a dielectric
-    // O2 chemistry (gas type 1): constants and buffers of its collision step
+    // oxygen chemistries (gas types 1 and 2): constants and buffers of their
+    // collision step
     O2Const<R> oc{};
-    R *o2prob = nullptr, *o2fluid = nullptr, *o2bg = nullptr, *o2stash[O2_NSP] = {nullptr};
-    uint8_t *o2fate[O2_NSP] = {nullptr};
+    R *o2prob = nullptr, *o2fluid = nullptr, *o2bg = nullptr, *o2bg2 = nullptr, *o2bv2 = nullptr;
+    R *o2stash[CHEM_MAXSP] = {nullptr};
+    uint8_t *o2fate[CHEM_MAXSP] = {nullptr};
     int *o2pcount = nullptr, *o2pstart = nullptr, *o2delreq = nullptr, *o2total = nullptr;
+    unsigned long long *o2delcnt = nullptr;   // O- removed by electron detachment, per node, cumulative
     double *sum_den = nullptr, *sum_phi = nullptr, *sum_ex = nullptr, *sum_ey = nullptr;
     long long n_accum = 0;
     long long tstep = 0;
@@ -193,11 +196,11 @@ struct Case : CaseBase {
                         V, part, brhs, Area, eps, stage, StructIdx, Boundary, Face, CondID, A_idx, route, inv_start,
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
-                        sum_ey, total2, alive2, clk, eedf, lost, sigma_hits, o2prob, o2fluid, o2bg, o2pcount,
-                        o2pstart, o2delreq, o2total};
+                        sum_ey, total2, alive2, clk, eedf, lost, sigma_hits, o2prob, o2fluid, o2bg, o2bg2, o2bv2,
+                        o2pcount, o2pstart, o2delreq, o2total, o2delcnt};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
-        for (int s = 0; s < O2_NSP; ++s) {
+        for (int s = 0; s < CHEM_MAXSP; ++s) {
             if (o2fate[s]) (void)hipFree(o2fate[s]);
             if (o2stash[s]) (void)hipFree(o2stash[s]);
         }
@@ -498,26 +501,42 @@ struct Case : CaseBase {
         kc.vsave = vsave; kc.xe = xe; kc.cx = cx; kc.StructIdx = StructIdx; kc.route = route; kc.lost = lost;
         kc.sigma_hits = sigma_hits;
         gas = d.gas;
-        if (gas == 1) {
-            if (nsp != O2_NSP || S.nsigv != O2_NSIGV || S.TnRct < O2_NROWS)
-                return fail(-19, "O2 setup does not match the collision tables");
-            HIPCHK(dalloc(&o2prob, (size_t)G * O2_NTARGETS));
+        if (gas == 1 || gas == 2) {
+            const bool aro2 = gas == 2;
+            const int want_nsp = aro2 ? ARO2_NSP : O2_NSP, want_sigv = aro2 ? ARO2_NSIGV : O2_NSIGV;
+            const int want_rows = aro2 ? ARO2_NROWS : O2_NROWS;
+            if (nsp != want_nsp || S.nsigv != want_sigv || S.TnRct < want_rows || (aro2 && S.BackDen2.empty()))
+                return fail(-19, "oxygen setup does not match the collision tables");
+            oc.chem = aro2 ? CHEM_ARO2 : CHEM_O2;
+            oc.nsp = nsp;
+            oc.ntgt = aro2 ? ARO2_NTARGETS : O2_NTARGETS;
+            oc.neg = aro2 ? ARO2_NEG : O2_NEG;
+            oc.sub_mask = aro2 ? ARO2_SUBCYCLED : O2_SUBCYCLED;
+            oc.victim_tag = aro2 ? ARO2_VICTIM_TAG : O2_VICTIM_TAG;
+            HIPCHK(dalloc(&o2prob, (size_t)G * oc.ntgt));
             HIPCHK(upload(&o2fluid, S.FluidN, st)); HIPCHK(upload(&o2bg, S.BackDen1, st));
-            HIPCHK(dalloc(&o2pcount, (size_t)O2_NSP * G)); HIPCHK(dalloc(&o2pstart, (size_t)O2_NSP * (G + 1)));
+            if (aro2) { HIPCHK(upload(&o2bg2, S.BackDen2, st)); HIPCHK(upload(&o2bv2, S.BackVel2, st)); }
+            HIPCHK(dalloc(&o2pcount, (size_t)nsp * G)); HIPCHK(dalloc(&o2pstart, (size_t)nsp * (G + 1)));
             HIPCHK(dalloc(&o2delreq, G));
-            HIPCHK(dalloc(&o2total, (size_t)O2_NSP * G));
-            for (int s = 0; s < O2_NSP; ++s) {
+            HIPCHK(dalloc(&o2total, (size_t)nsp * G));
+            HIPCHK(dalloc(&o2delcnt, G));
+            HIPCHK(hipMemsetAsync(o2delcnt, 0, (size_t)G * sizeof(unsigned long long), st));
+            for (int s = 0; s < nsp; ++s) {
                 HIPCHK(dalloc(&o2fate[s], cur[s].cap));
                 HIPCHK(dalloc(&o2stash[s], (size_t)6 * cur[s].cap));
             }
             oc.G = G; oc.ncy = S.ncy; oc.ngy = S.ngy; oc.Csize = S.Csize; oc.TnRct = S.TnRct; oc.nlogx = N_LOGX;
-            oc.nvel = S.nvel; oc.MCCn = S.MCCn; oc.ions = d.o2_ion_coll != 0;
+            oc.nvel = S.nvel; oc.MCCn = S.MCCn;
+            // the Ar/O2 dispatch runs every species' handler (src/cuda_mcc.cu:25); the O2 one only
+            // the electrons' unless the deck asks for the ions too
+            oc.ions = aro2 || d.o2_ion_coll != 0;
             oc.idLOGX = S.idLOGX; oc.dt = S.dt; oc.dtm = S.dt_mcc; oc.idx = S.idx; oc.idy = S.idy;
-            for (int s = 0; s < O2_NSP; ++s) { oc.np2c[s] = S.np2c[s]; oc.Escale[s] = S.Escale[s]; }
-            for (int j = 0; j < O2_NSIGV; ++j) oc.sigv[j] = S.sigv[j];
-            for (int r = 0; r < O2_NROWS; ++r) oc.Th_e[r] = S.Th_e[r];
+            for (int s = 0; s < nsp; ++s) { oc.np2c[s] = S.np2c[s]; oc.Escale[s] = S.Escale[s]; }
+            for (int j = 0; j < want_sigv; ++j) oc.sigv[j] = S.sigv[j];
+            for (int r = 0; r < want_rows; ++r) oc.Th_e[r] = S.Th_e[r];
             oc.seed = (uint64_t)d.rng_seed;
             oc.xe = xe; oc.cx = cx; oc.vsave = vsave; oc.BackDen1 = o2bg; oc.BackVel1 = BackVel1; oc.FluidN = o2fluid;
+            oc.BackDen2 = aro2 ? o2bg2 : o2bg; oc.BackVel2 = aro2 ? o2bv2 : BackVel1;
             oc.den = den; oc.prob = o2prob;
         }
         nc.Gsize = G; nc.ngx = S.ngx; nc.ngy = S.ngy; nc.nsp = nsp; nc.CondNUMR = S.CondNUMR;
@@ -622,38 +641,39 @@ struct Case : CaseBase {
         HIPCHK(hipGetLastError());
         return 0;
     }
-    // O2 chemistry: collide every species, remove the O- taken by electron
-    // detachment, then rebuild all four buckets with the products
+    // oxygen chemistries: collide every species, remove the O- taken by
+    // electron detachment, then rebuild every bucket with the products
     int mcc_o2() {
         const uint32_t step = (uint32_t)tstep;
         O2Set<R> set;
-        for (int s = 0; s < O2_NSP; ++s) { set.p[s] = cur[s]; set.fate[s] = o2fate[s]; set.stash[s] = o2stash[s]; }
+        for (int s = 0; s < nsp; ++s) { set.p[s] = cur[s]; set.fate[s] = o2fate[s]; set.stash[s] = o2stash[s]; }
         launch_o2_probs<R>(oc, st);
-        launch_fill_int(o2pcount, 0, O2_NSP * G, st);
+        launch_fill_int(o2pcount, 0, nsp * G, st);
         launch_fill_int(o2delreq, 0, G, st);
         launch_o2_collide<R>(oc, set, -1, step, mccr, o2pcount, o2delreq, st);   // all species
-        launch_o2_victims<R>(oc, set, step, o2delreq, st);
+        launch_o2_victims<R>(oc, set, step, o2delreq, o2delcnt, st);
         O2Set<R> dst = set;
-        launch_fill_int(o2total, 0, O2_NSP * G, st);
+        launch_fill_int(o2total, 0, nsp * G, st);
         launch_o2_totals<R>(oc, set, o2pcount, o2total, st);
-        // eight exclusive scans (new cell_start and product order of each
+        // 2 nsp exclusive scans (new cell_start and product order of each
         // species), three arrays per pass
-        const int *in[8];
-        int *out[8];
-        for (int t = 0; t < O2_NSP; ++t) {
+        const int nscan = 2 * nsp;
+        const int *in[2 * CHEM_MAXSP];
+        int *out[2 * CHEM_MAXSP];
+        for (int t = 0; t < nsp; ++t) {
             in[t] = o2total + (size_t)t * G; out[t] = nxt[t].cell_start;
-            in[4 + t] = o2pcount + (size_t)t * G; out[4 + t] = o2pstart + (size_t)t * (G + 1);
+            in[nsp + t] = o2pcount + (size_t)t * G; out[nsp + t] = o2pstart + (size_t)t * (G + 1);
             dst.p[t] = nxt[t];
         }
-        for (int a = 0; a < 8; a += 3) launch_scan_multi(std::min(3, 8 - a), in + a, out + a, G, scan_tmp, st);
+        for (int a = 0; a < nscan; a += 3) launch_scan_multi(std::min(3, nscan - a), in + a, out + a, G, scan_tmp, st);
         launch_o2_scatter_all<R>(oc, set, dst, o2pstart, next_id, d_err, st);
-        for (int s = 0; s < O2_NSP; ++s) swap_species(s);
-        launch_o2_bump_ids(next_id, o2pstart, G, st);
+        for (int s = 0; s < nsp; ++s) swap_species(s);
+        launch_o2_bump_ids(next_id, o2pstart, G, nsp, st);
         HIPCHK(hipGetLastError());
         return 0;
     }
     int mcc() override {
-        if (gas == 1) return mcc_o2();
+        if (gas == 1 || gas == 2) return mcc_o2();
         const uint32_t step = (uint32_t)tstep;
         // electrons first in the event order is irrelevant: each particle owns its stream
         launch_mcc_collide<R>(kc, cur[1], 1, step, mccr, cellp, prank, stage, st);   // ions (no products)
@@ -772,9 +792,9 @@ struct Case : CaseBase {
         return 0;
     }
     int step_fused(int n) override {
-        if (gas == 1) {
+        if (gas == 1 || gas == 2) {
             // the fused two-pass step is built for the argon chemistry; O2
-            // cases run the stages one by one
+            // and Ar/O2 cases run the stages one by one
             for (int i = 0; i < n; ++i) {
                 int e;
                 if ((e = advance_time()) || (e = field_solve()) || (e = efield()) || (e = move()) ||
@@ -948,6 +968,10 @@ struct Case : CaseBase {
         if (n == "sum_ex") return D(sum_ex, G, 8, 1);
         if (n == "sum_ey") return D(sum_ey, G, 8, 1);
         if (n == "next_id") return D(next_id, nsp, 4, 0);
+        if (n == "detach_removed") {
+            if (!o2delcnt) return false;
+            return D(o2delcnt, G, 8, 0);
+        }
         if (n == "eedf") {
             if (!eedf) return false;
             return D(eedf, EEDF_BINS, 8, 0, false);
@@ -973,6 +997,8 @@ struct Case : CaseBase {
         if (n == "lap_phi") return H(S.lap_phi.data(), S.lap_phi.size(), rs, 1);
         if (n == "BackDen1") return H(S.BackDen1.data(), G, rs, 1);
         if (n == "BackVel1") return H(S.BackVel1.data(), G, rs, 1);
+        if (n == "BackDen2" && !S.BackDen2.empty()) return H(S.BackDen2.data(), G, rs, 1);
+        if (n == "BackVel2" && !S.BackVel2.empty()) return H(S.BackVel2.data(), G, rs, 1);
         if (n == "FluidN") return H(S.FluidN.data(), S.FluidN.size(), rs, 1);
         if (n == "xe") return H(S.xe.data(), S.xe.size(), rs, 1);
         if (n == "xee") return H(S.xee.data(), S.xee.size(), rs, 1);
@@ -1064,6 +1090,8 @@ static bool setup_lookup(Setup<R> &S, const std::string &n, const void *&p, size
     if (n == "lap_phi") return H(S.lap_phi.data(), S.lap_phi.size(), rs, 1);
     if (n == "BackDen1") return H(S.BackDen1.data(), S.BackDen1.size(), rs, 1);
     if (n == "BackVel1") return H(S.BackVel1.data(), S.BackVel1.size(), rs, 1);
+    if (n == "BackDen2") return H(S.BackDen2.data(), S.BackDen2.size(), rs, 1);
+    if (n == "BackVel2") return H(S.BackVel2.data(), S.BackVel2.size(), rs, 1);
     if (n == "FluidN") return H(S.FluidN.data(), S.FluidN.size(), rs, 1);
     if (n == "xe") return H(S.xe.data(), S.xe.size(), rs, 1);
     if (n == "xee") return H(S.xee.data(), S.xee.size(), rs, 1);
@@ -1156,8 +1184,6 @@ int pic_create(const char *deck_json, const pic_options *opt, pic_ctx **out) {
     Deck d;
     std::string err;
     if (int e = parse_deck(deck_json, d, err)) return fail(e, "%s", err.c_str());
-    if (d.gas == 2)
-        return fail(-19, "the Ar/O2 collision step is not built: gas type 2 has host setup only (pic_setup_open)");
     std::unique_ptr<pic_ctx> h(new pic_ctx());
     int e = 0;
     try {
@@ -1279,8 +1305,9 @@ int pic_set_array(pic_ctx *ctx, const char *name, const void *host, size_t bytes
 }
 const char *pic_array_names(void) {
     return "Ex Ey phi TotPot Sigma Source den sigma_sp sigma_hits b xsol V_cond mccr CondCharge sum_den sum_phi sum_ex sum_ey "
-           "eedf Lost next_id Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai Aj Aval "
-           "Atemp Mdiag cond_b temp_b lap_phi BackDen1 BackVel1 FluidN xe xee cx sigv P1 P2 Pion vsave";
+           "eedf Lost next_id detach_removed Boundary CondID Face DensRegion Area Temp PlasmaRegion eps_r eps StructIdx A_idx Ai "
+           "Aj Aval Atemp Mdiag cond_b temp_b lap_phi BackDen1 BackVel1 BackDen2 BackVel2 FluidN xe xee cx sigv P1 P2 Pion "
+           "vsave";
 }
 int pic_set_step(pic_ctx *ctx, long long tstep) { CTX_OR_FAIL(ctx); return ctx->c->set_step(tstep); }
 int pic_last_field_solve(pic_ctx *ctx, int *iters, int *outer, double *rel_res) {
@@ -1413,6 +1440,11 @@ void MCC_O2_cuda(void) {
     CaseBase *c = shim_case("MCC_O2_cuda");
     if (c->gas != 1) shim_check(fail(-19, "the case is not an O2 case"), "MCC_O2_cuda");
     shim_check(c->mcc(), "MCC_O2_cuda");
+}
+void MCC_ArO2_cuda(void) {
+    CaseBase *c = shim_case("MCC_ArO2_cuda");
+    if (c->gas != 2) shim_check(fail(-19, "the case is not an Ar/O2 case"), "MCC_ArO2_cuda");
+    shim_check(c->mcc(), "MCC_ArO2_cuda");
 }
 void Deposit_cuda(void) { shim_check(shim_case("Deposit_cuda")->deposit(), "Deposit_cuda"); }
 void Deposit_Basic(void) { shim_check(shim_case("Deposit_Basic")->deposit(), "Deposit_Basic"); }

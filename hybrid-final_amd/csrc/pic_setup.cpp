@@ -528,13 +528,20 @@ int presolves(Setup<R> &s, std::string &err) {
     lut.solve(b.data(), x.data());
     for (int g = 0; g < s.Gsize; ++g)
         if (s.A_idx[g]) s.Temp[g] = (R)x[s.A_idx[g] - 1];
+    // Calculate_1GasPara / Calculate_2GasPara (src/cuda_Field.cu:661-678)
     s.BackDen1.assign(s.Gsize, 0);
     s.BackVel1.assign(s.Gsize, 0);
+    s.BackDen2.assign(s.nbg > 1 ? s.Gsize : 0, 0);
+    s.BackVel2.assign(s.nbg > 1 ? s.Gsize : 0, 0);
     for (int g = 0; g < s.Gsize; ++g) {
         if (!s.DensRegion[g]) continue;
         double T = (double)s.Temp[g];
         s.BackDen1[g] = (R)(kNperTorr * (double)s.bg_pres[0] / (T * 8.6142e-5 + kDblMin));
         s.BackVel1[g] = (R)std::sqrt(T * 1.38e-23 / (double)s.bg_mass[0]);
+        if (s.nbg > 1) {
+            s.BackDen2[g] = (R)(kNperTorr * (double)s.bg_pres[1] / (T * 8.6142e-5 + kDblMin));
+            s.BackVel2[g] = (R)std::sqrt(T * 1.38e-23 / (double)s.bg_mass[1]);
+        }
     }
     // metastable density at nodes (src/Fluid.c:3-68, src/cuda_Fluid.cu:321-339)
     s.FluidN.assign((size_t)s.nfsp * s.Gsize, 0);

@@ -73,8 +73,9 @@ struct Setup {
     int A_size = 0;
     std::vector<int> Ai, Aj;
     std::vector<R> Aval, Atemp, Mdiag, cond_b, temp_b;
-    // Laplace solutions (CondNUMR x Gsize) and gas fields
-    std::vector<R> lap_phi, BackDen1, BackVel1, FluidN;
+    // Laplace solutions (CondNUMR x Gsize) and gas fields; BackDen2 and
+    // BackVel2 are the second background gas (O2 of an Ar/O2 mixture)
+    std::vector<R> lap_phi, BackDen1, BackVel1, BackDen2, BackVel2, FluidN;
     std::vector<int> Efield_Flag;
     // collision data
     int TnRct = 0, MCCn = 1;

@@ -101,9 +101,8 @@ def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None,
     rest is Ar).  Charged-species densities default to a quasi-neutral mix
     (e 1e15, O- 2e14; positive ions make up the rest) with ppc macro-particles
     per cell for e and the molecular/atomic majority ions and ppc/5 for O+
-    and O-.  Every reaction switch is on.  Gas type 1 runs on the GPU (staged
-    steps); gas type 2 has host setup only (pic_setup_open) and pic_create
-    rejects it."""
+    and O-.  Every reaction switch is on.  Both gas types run on the GPU as
+    staged steps."""
     d = ccp_deck(**kw)
     ppc = kw.get("ppc", 50)
     dx = kw.get("dx", 5e-4)
@@ -151,8 +150,7 @@ def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None,
 
 def config3():
     """512x256 Ar/O2 CCP (O2 20 %), dx=0.2 mm, 150 V, 30 mTorr, 100 /cell for e,
-    Ar+ and O2+ and 20 /cell for O+ and O-, dt=T/800, 5 cycles.  Host setup
-    only in this build."""
+    Ar+ and O2+ and 20 /cell for O+ and O-, dt=T/800, 5 cycles."""
     return oxygen_deck(gas=2, o2_ratio=0.2, ncx=512, ncy=256, dx=2e-4, volts=150.0, pres_torr=0.03,
                        ppc=100, steps_per_cycle=800, cycles=5)
 

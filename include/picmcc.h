@@ -72,7 +72,8 @@ int pic_efield(pic_ctx *ctx);        /* Efield_cuda_Basic, src/cuda_Field.cu:2-4
 int pic_move(pic_ctx *ctx);          /* Move_cuda, src/cuda_Move.cu:3-7 */
 int pic_sort_boundary(pic_ctx *ctx); /* SortBounndary_cuda, src/cuda_Sortboundary.cu:4-13 */
 int pic_mcc(pic_ctx *ctx);           /* MCC_Ar_cuda, src/cuda_mcc.cu:97-106; for gas type 1
-                                       MCC_O2_cuda, src/cuda_mcc.cu:72-96 */
+                                       MCC_O2_cuda, src/cuda_mcc.cu:72-96; for gas type 2
+                                       MCC_ArO2_cuda, src/cuda_mcc.cu:21-71 */
 int pic_deposit(pic_ctx *ctx);       /* Deposit_cuda / Deposit_Basic, src/cuda_Deposit.cu:3-29 */
 int pic_diagnostic(pic_ctx *ctx);    /* profile accumulators, src/cuda_Diagnostic.cu:505-547 */
 /* electron energy distribution: adds a 256-bin histogram of the electrons'
@@ -143,9 +144,8 @@ void pic_setup_close(pic_setup_h *h);
 /* Reaction switches of gas type 0 (Ar), 1 (O2) or 2 (Ar/O2): the name of
  * the deck's object under "Collision_On_Off", then every reaction key in
  * reaction order, newline-separated (src/start.c:1003-1420); NULL for an
- * unknown type.  Gas type 1 runs (pic_mcc applies the O2 collision step;
- * pic_step runs the stages one by one for it).  Gas type 2 has host setup
- * only: pic_create rejects it (-19) because its collision step is not built. */
+ * unknown type.  For gas types 1 and 2 pic_mcc applies the O2 or Ar/O2
+ * collision step, and pic_step runs the stages one by one. */
 const char *pic_reaction_names(int gas);
 
 /* ---------------- reference-named zero-argument shims ---------------- */
@@ -166,6 +166,7 @@ void Move_cuda(void);           /* src/cuda_Move.cu:3 */
 void SortBounndary_cuda(void);  /* src/cuda_Sortboundary.cu:4 (sic) */
 void MCC_Ar_cuda(void);         /* src/cuda_mcc.cu:97 dispatch */
 void MCC_O2_cuda(void);         /* src/cuda_mcc.cu:7-10 dispatch (gas type 1) */
+void MCC_ArO2_cuda(void);       /* src/cuda_mcc.cu:2-6 dispatch (gas type 2) */
 void Deposit_cuda(void);        /* src/cuda_Deposit.cu:3 */
 void Deposit_Basic(void);       /* src/cuda_Deposit.cu:21 */
 

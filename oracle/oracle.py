@@ -244,15 +244,19 @@ class Oracle:
         raise AttributeError(name)
 
 
-def mcc_o2(precision, k, buckets, next_id, mccr):
-    """O2 chemistry collision step (orc_mcc_o2) on host copies.
+def mcc_o2(precision, k, buckets, next_id, mccr, delcnt=None):
+    """Collision step of the oxygen chemistries (orc_mcc_o2) on host copies.
 
     k: dict of the setup values (G, TnRct, nlogx, nvel, MCCn, idLOGX, dt,
-    dtm, idx, idy, np2c[4], Escale[4], sigv[20], seed, step) and arrays
-    (Th_e, xe, cx, vsave, BackDen1, BackVel1, FluidN, den), and ions (1: the
-    positive and negative ions collide too; default 0).  buckets: four
-    dicts x, y, vx, vy, vz, id, cell_start, cap (e, O2+, O+, O-).  Returns
-    (new buckets, next_id, mccr, rc); the inputs are not modified.
+    dtm, idx, idy, np2c, Escale, sigv, seed, step) and arrays (Th_e, xe, cx,
+    vsave, BackDen1, BackVel1, FluidN, den; BackDen2 and BackVel2 for Ar/O2),
+    gas (1 O2, the default; 2 Ar/O2) and ions (O2 only: 1 = the positive and
+    negative ions collide too; default 0).  buckets: one dict per species
+    (x, y, vx, vy, vz, id, cell_start, cap): e, O2+, O+, O- for O2; e, Ar+,
+    O2+, O+, O- for Ar/O2.  delcnt: per-node count of the O- removed by
+    electron detachment, advanced when given.  Returns (new buckets,
+    next_id, mccr, rc), or with delcnt (..., rc, delcnt); the inputs are not
+    modified.
     """
     path = os.path.join(HERE, f"liborc_{precision}.so")
     if not os.path.exists(path):
@@ -261,13 +265,19 @@ def mcc_o2(precision, k, buckets, next_id, mccr):
     real = np.float32 if precision == "f32" else np.float64
     R = C.c_float if precision == "f32" else C.c_double
     P = C.POINTER(R)
+    gas = int(k.get("gas", 1))
+    if gas not in (1, 2):
+        raise ValueError("gas must be 1 (O2) or 2 (Ar/O2)")
+    nsp = 4 if gas == 1 else 5
+    nsig = 20 if gas == 1 else 25
 
     class In(C.Structure):
         _fields_ = [("G", C.c_int), ("TnRct", C.c_int), ("nlogx", C.c_int), ("nvel", C.c_int), ("MCCn", C.c_int),
                     ("idLOGX", R), ("dt", R), ("dtm", R), ("idx", R), ("idy", R),
-                    ("np2c", R * 4), ("Escale", R * 4), ("sigv", R * 20),
+                    ("np2c", R * 5), ("Escale", R * 5), ("sigv", R * 25),
                     ("Th_e", P), ("xe", P), ("cx", P), ("vsave", P), ("BackDen1", P), ("BackVel1", P),
-                    ("FluidN", P), ("den", P), ("seed", C.c_uint64), ("step", C.c_uint32), ("ions", C.c_int)]
+                    ("FluidN", P), ("den", P), ("seed", C.c_uint64), ("step", C.c_uint32), ("ions", C.c_int),
+                    ("chem", C.c_int), ("BackDen2", P), ("BackVel2", P), ("delcnt", C.POINTER(C.c_int64))]
 
     class Bucket(C.Structure):
         _fields_ = [("x", P), ("y", P), ("vx", P), ("vy", P), ("vz", P), ("id", C.POINTER(C.c_uint32)),
@@ -285,18 +295,24 @@ def mcc_o2(precision, k, buckets, next_id, mccr):
         setattr(inp, f, int(k[f]))
     for f in ("idLOGX", "dt", "dtm", "idx", "idy"):
         setattr(inp, f, float(real(k[f])))
-    for f, n in (("np2c", 4), ("Escale", 4), ("sigv", 20)):
+    for f, n in (("np2c", nsp), ("Escale", nsp), ("sigv", nsig)):
         arr = getattr(inp, f)
         for i in range(n):
             arr[i] = float(real(k[f][i]))
     for f in ("Th_e", "xe", "cx", "vsave", "BackDen1", "BackVel1", "FluidN", "den"):
         setattr(inp, f, ptr(k[f], real, R)[1])
+    for f in ("BackDen2", "BackVel2"):
+        setattr(inp, f, ptr(k[f] if gas == 2 else k[f.replace("2", "1")], real, R)[1])
     inp.seed = int(k["seed"]) & (2 ** 64 - 1)
     inp.step = int(k["step"]) & (2 ** 32 - 1)
     inp.ions = int(k.get("ions", 0))
-    bs = (Bucket * 4)()
+    inp.chem = gas - 1
+    dc = None
+    if delcnt is not None:
+        dc, inp.delcnt = ptr(delcnt, np.int64, C.c_int64)
+    bs = (Bucket * nsp)()
     arrays = []
-    for s in range(4):
+    for s in range(nsp):
         b = buckets[s]
         cap = int(b["cap"])
         n = len(b["x"])
@@ -319,9 +335,11 @@ def mcc_o2(precision, k, buckets, next_id, mccr):
     L.orc_mcc_o2.argtypes = [C.POINTER(In), C.POINTER(Bucket), C.POINTER(C.c_uint32), C.POINTER(C.c_int64)]
     rc = L.orc_mcc_o2(C.byref(inp), bs, nid_p, mc_p)
     out = []
-    for s in range(4):
+    for s in range(nsp):
         n = bs[s].n
         d = {f: arrays[s][f][:n].copy() for f in ("x", "y", "vx", "vy", "vz", "id")}
         d["cell_start"] = arrays[s]["cell_start"].copy()
         out.append(d)
+    if delcnt is not None:
+        return out, nid.copy(), mc.copy(), rc, dc.copy()
     return out, nid.copy(), mc.copy(), rc

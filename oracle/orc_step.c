@@ -873,28 +873,40 @@ void orc_lcg_seq(long seed, int n, long *out) {
 /* argon cross section lookup, for the collision-statistics tests */
 double orc_ar_sigma(orc_case *c, int r, double engy) { return (double)ar_sigma(c, r, (real)engy); }
 
-/* ================= MCC (O2, gas type 1): src/cuda_mcc.cu:72-96 (MCC_O2_Basic);
- *                   src/cuda_mccO2.cu:867-977 (target test), 2-866 (handlers) ============
+/* ================= MCC (O2, gas type 1; Ar/O2, gas type 2):
+ *   src/cuda_mcc.cu:21-96 (MCC_ArO2_Basic, MCC_O2_Basic);
+ *   src/cuda_mccO2.cu:867-977 and src/cuda_mccArO2.cu:1130-1265 (target tests);
+ *   src/cuda_mccO2.cu:2-866 and src/cuda_mccArO2.cu:51-1129 (handlers) ============
  * Each particle draws from its own stream (seed, id, step, (species << 8) | 1)
- * in the reference's order: the target draws (up to MCCn for e and O2+, one
- * for O+ and O-), the neutral's velocity (O2+, O+: index, phi, theta), the
- * reaction draw, then each product's Maxwellian draws and the two angles of
- * the scattered projectile.  Reactions are the rows of o2_chem.h. */
+ * in the reference's order: the target draws (up to MCCn for the sub-cycled
+ * species, one for the others), the neutral's velocity (ions that collide in
+ * the neutral's frame: index, phi, theta), the reaction draw, then for an
+ * ionisation with energy sharing the sharing draw, each product's draws (an
+ * electron's two scattering angles, a heavy product's Maxwellian), and the two
+ * angles of the scattered projectile.  Reactions are the rows of o2_chem.h and
+ * aro2_chem.h. */
+#include "aro2_chem.h"
 #include "o2_chem.h"
 
-typedef struct { int r, sum, type, emode; double gain; int p0, p1; } o2_row;
-typedef struct { int sp, flag, sig, first, n; } o2_grp;
+typedef struct { int r, sum, type, emode; double gain; int p0, p1, th; } o2_row;
+typedef struct { int sp, flag, sig, first, n, vsrc; } o2_grp;
 typedef struct { int sp, flag, sig, kind, idx; } o2_tgt;
-#define ORC_O2_ROW(r, sm, ty, em, g, a, b) {r, sm, ty, em, g, a, b},
-#define ORC_O2_GRP(s, f, sg, fi, n) {s, f, sg, fi, n},
-#define ORC_O2_TGT(s, f, sg, kd, ix) {s, f, sg, kd, ix},
-static const o2_row o2_rows[O2_NROWS] = {O2_REACTIONS(ORC_O2_ROW)};
-static const o2_grp o2_grps[O2_NGROUPS] = {O2_GROUPS(ORC_O2_GRP)};
-static const o2_tgt o2_tgts[O2_NTARGETS] = {O2_TARGETS(ORC_O2_TGT)};
-static const int o2_group0[4] = O2_GROUP0, o2_ngroup[4] = O2_NGROUP;
+#define ORC_O2_ROW(r, sm, ty, em, g, a, b) {r, sm, ty, em, g, a, b, r},
+#define ORC_ARO2_ROW(r, sm, ty, em, g, a, b, th) {r, sm, ty, em, g, a, b, th},
+#define ORC_O2_GRP(s, f, sg, fi, n) {s, f, sg, fi, n, (s) == 1 || (s) == 2 ? 1 : 0},
+#define ORC_ARO2_GRP(s, f, sg, fi, n, v) {s, f, sg, fi, n, v},
+#define ORC_TGT(s, f, sg, kd, ix) {s, f, sg, kd, ix},
+static const o2_row o2_rows[2][CHEM_MAXROWS] = {{O2_REACTIONS(ORC_O2_ROW)}, {ARO2_REACTIONS(ORC_ARO2_ROW)}};
+static const o2_grp o2_grps[2][CHEM_MAXGROUPS] = {{O2_GROUPS(ORC_O2_GRP)}, {ARO2_GROUPS(ORC_ARO2_GRP)}};
+static const o2_tgt o2_tgts[2][CHEM_MAXTARGETS] = {{O2_TARGETS(ORC_TGT)}, {ARO2_TARGETS(ORC_TGT)}};
+static const int o2_group0[2][CHEM_MAXSP] = {O2_GROUP0, ARO2_GROUP0};
+static const int o2_ngroup[2][CHEM_MAXSP] = {O2_NGROUP, ARO2_NGROUP};
+static const int o2_nsp[2] = {O2_NSP, ARO2_NSP}, o2_ntgt[2] = {O2_NTARGETS, ARO2_NTARGETS};
+static const int o2_neg[2] = {O2_NEG, ARO2_NEG}, o2_sub[2] = {O2_SUBCYCLED, ARO2_SUBCYCLED};
+static const uint32_t o2_vtag[2] = {O2_VICTIM_TAG, ARO2_VICTIM_TAG};
 
-/* O2_CrossSection (src/cuda_mccO2.cu:978-...): the argon lookup's form on
- * the O2 table */
+/* O2_CrossSection / ArO2_CrossSection (src/cuda_mccO2.cu:978-...,
+ * src/cuda_mccArO2.cu:2461-...): the argon lookup's form on the table */
 static real o2_sigma(const orc_o2_in *k, int r, real engy) {
     if (engy == 0) return 0;
     int N = k->nlogx, T = k->TnRct;
@@ -910,10 +922,10 @@ static real o2_sigma(const orc_o2_in *k, int r, real engy) {
     return a2 * k->cx[ee2 * T + r] + a1 * k->cx[(ee2 + 1) * T + r];
 }
 
-static void o2_maxw(const orc_o2_in *k, int g, orc_stream *st, real *v) {
+static void o2_maxw(const orc_o2_in *k, real vti, orc_stream *st, real *v) {
     int n = (int)((real)(k->nvel - 1) * U(st));
     real rphi = U(st), rthe = U(st);
-    dmaxwellv(&v[0], &v[1], &v[2], k->vsave[n], k->BackVel1[g], rphi, rthe);
+    dmaxwellv(&v[0], &v[1], &v[2], k->vsave[n], vti, rphi, rthe);
 }
 
 typedef struct {
@@ -924,11 +936,12 @@ typedef struct {
 
 /* one collision of species s with the target of group grp at node g */
 static void o2_collide(const orc_o2_in *k, int s, int grp, int g, orc_stream *st, real *v, o2_out *o) {
-    const o2_grp *G = &o2_grps[grp];
+    const o2_grp *G = &o2_grps[k->chem][grp];
+    const o2_row *rows = o2_rows[k->chem];
     real vn[3] = {0, 0, 0};
-    int heavy = (s == 1 || s == 2), q;
+    int heavy = G->vsrc != 0, q;
     o->r = -1; o->dead = 0; o->np = 0; o->detach = 0;
-    if (heavy) o2_maxw(k, g, st, vn);
+    if (heavy) o2_maxw(k, G->vsrc == 2 ? k->BackVel2[g] : k->BackVel1[g], st, vn);
     real VX = v[0] - vn[0], VY = v[1] - vn[1], VZ = v[2] - vn[2];
     real dum = VX * VX + VY * VY + VZ * VZ;
     real vel = RSQRT(dum);
@@ -938,31 +951,56 @@ static void o2_collide(const orc_o2_in *k, int s, int grp, int g, orc_stream *st
     real sum = 0;
     const o2_row *row = 0;
     for (q = G->first; q < G->first + G->n; ++q) {
-        if (!(engy > k->Th_e[o2_rows[q].r])) continue;
-        real sg = o2_sigma(k, o2_rows[q].r, engy);
-        sum = o2_rows[q].sum ? sum + sg : sg;
-        if (R1 <= sum) { row = &o2_rows[q]; break; }
+        if (!(engy > k->Th_e[rows[q].r])) continue;
+        real sg = o2_sigma(k, rows[q].r, engy);
+        sum = rows[q].sum ? sum + sg : sg;
+        if (R1 <= sum) { row = &rows[q]; break; }
     }
     if (!row) return;
     o->r = row->r;
     if (row->type == O2T_LOSS || row->type == O2T_ION || row->type == O2T_DETACH) {
-        if (row->emode == O2E_LOSS) { engy -= k->Th_e[row->r]; vel = RSQRT(RABS(engy) / k->Escale[s]); }
+        if (row->emode == O2E_LOSS) { engy -= k->Th_e[row->th]; vel = RSQRT(RABS(engy) / k->Escale[s]); }
         if (row->emode == O2E_GAIN) { engy += (real)row->gain; vel = RSQRT(RABS(engy) / k->Escale[s]); }
         int prods[2] = {row->p0, row->type == O2T_ION ? row->p1 : -1}, j;
         for (j = 0; j < 2; ++j) {
             if (prods[j] < 0) continue;
             o->ps[o->np] = prods[j];
             if (prods[j] == 0) { o->pv[o->np][0] = VX; o->pv[o->np][1] = VY; o->pv[o->np][2] = VZ; }
-            else o2_maxw(k, g, st, o->pv[o->np]);
+            else o2_maxw(k, k->BackVel1[g], st, o->pv[o->np]);
             o->np++;
         }
         if (row->type == O2T_DETACH) o->detach = 1;
         real a = U(st), b = U(st);
         anewvel(engy, vel, &VX, &VY, &VZ, a, b);
         v[0] = VX; v[1] = VY; v[2] = VZ;
+    } else if (row->type == O2T_IONSHARE) {
+        /* src/cuda_mccArO2.cu:108-114 (energy sharing), 470-510 (products) */
+        int j;
+        engy -= k->Th_e[row->th];
+        real u = U(st);
+        real rengy = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+        engy -= rengy;
+        real velr = RSQRT(RABS(rengy) / k->Escale[s]);
+        real vel2 = RSQRT(RABS(engy) / k->Escale[s]);
+        int prods[2] = {row->p0, row->p1};
+        for (j = 0; j < 2; ++j) {
+            o->ps[o->np] = prods[j];
+            if (prods[j] == 0) {
+                real ex = VX, ey = VY, ez = VZ;
+                real a = U(st), b = U(st);
+                anewvel(rengy, velr, &ex, &ey, &ez, a, b);
+                o->pv[o->np][0] = ex; o->pv[o->np][1] = ey; o->pv[o->np][2] = ez;
+            } else {
+                o2_maxw(k, k->BackVel1[g], st, o->pv[o->np]);
+            }
+            o->np++;
+        }
+        real a = U(st), b = U(st);
+        anewvel(engy, vel2, &VX, &VY, &VZ, a, b);
+        v[0] = VX; v[1] = VY; v[2] = VZ;
     } else if (row->type == O2T_ATTACH) {
         o->dead = 1;
-        o->ps[0] = row->p0; o2_maxw(k, g, st, o->pv[0]); o->np = 1;
+        o->ps[0] = row->p0; o2_maxw(k, k->BackVel1[g], st, o->pv[0]); o->np = 1;
     } else if (row->type == O2T_DEL) {
         o->dead = 1;
     } else if (row->type == O2T_SCAT) {
@@ -989,36 +1027,37 @@ static int o2_key_cmp(const void *a, const void *b) {
 }
 
 int orc_mcc_o2(const orc_o2_in *k, orc_bucket *B, uint32_t *next_id, int64_t *mccr) {
+    const int ch = k->chem, nsp = o2_nsp[ch], ntgt = o2_ntgt[ch], neg = o2_neg[ch];
     int G = k->G, g, s, t, kk, j, rc = 0;
-    real *prob = zalloc((size_t)G * O2_NTARGETS, sizeof(real));
+    real *prob = zalloc((size_t)G * ntgt, sizeof(real));
     for (g = 0; g < G; ++g) {
         real acc = 0;
         int sp = -1;
-        for (t = 0; t < O2_NTARGETS; ++t) {
-            const o2_tgt *T = &o2_tgts[t];
+        for (t = 0; t < ntgt; ++t) {
+            const o2_tgt *T = &o2_tgts[ch][t];
             if (T->sp != sp) { sp = T->sp; acc = 0; }
-            real x = (T->sp <= 1 ? k->dtm : k->dt) * k->sigv[T->sig];
-            if (T->kind == 0) x = x * k->BackDen1[g];
+            real x = (((o2_sub[ch] >> T->sp) & 1) ? k->dtm : k->dt) * k->sigv[T->sig];
+            if (T->kind == 0) x = x * (T->idx ? k->BackDen2[g] : k->BackDen1[g]);
             else if (T->kind == 1) x = x * k->FluidN[(size_t)T->idx * G + g];
             else x = x * k->den[(size_t)T->idx * G + g] * k->np2c[T->idx] * k->idx * k->idy;
             acc = (acc + (real)1) - (real)dm_exp(-(double)x);
-            prob[(size_t)g * O2_NTARGETS + t] = acc;
+            prob[(size_t)g * ntgt + t] = acc;
         }
     }
     /* collisions: outcome per particle */
-    int *dead[4];
-    o2_out *out[4];
+    int *dead[CHEM_MAXSP];
+    o2_out *out[CHEM_MAXSP];
     int *delreq = zalloc(G, sizeof(int));
-    for (s = 0; s < 4; ++s) {
+    for (s = 0; s < nsp; ++s) {
         dead[s] = zalloc(B[s].n + 1, sizeof(int));
         out[s] = zalloc(B[s].n + 1, sizeof(o2_out));
     }
-    for (s = 0; s < 4; ++s) {
-        if (s != 0 && !k->ions) continue;   /* the reference's thread bound TID < Gsize */
-        int ng = o2_ngroup[s], g0 = o2_group0[s];
-        int draws = s <= 1 ? k->MCCn : 1;
+    for (s = 0; s < nsp; ++s) {
+        if (s != 0 && !k->ions && ch == 0) continue;   /* O2: the reference's thread bound TID < Gsize */
+        int ng = o2_ngroup[ch][s], g0 = o2_group0[ch][s];
+        int draws = ((o2_sub[ch] >> s) & 1) ? k->MCCn : 1;
         for (g = 0; g < G; ++g) {
-            const real *P = prob + (size_t)g * O2_NTARGETS + g0;
+            const real *P = prob + (size_t)g * ntgt + g0;
             real Tprob = P[ng - 1];
             for (kk = B[s].cell_start[g]; kk < B[s].cell_start[g + 1]; ++kk) {
                 orc_stream st;
@@ -1044,27 +1083,28 @@ int orc_mcc_o2(const orc_o2_in *k, orc_bucket *B, uint32_t *next_id, int64_t *mc
      * survived their own collision), smallest Philox keys first */
     for (g = 0; g < G; ++g) {
         if (!delreq[g]) continue;
-        int a = B[3].cell_start[g], b = B[3].cell_start[g + 1], m = 0;
+        int a = B[neg].cell_start[g], b = B[neg].cell_start[g + 1], m = 0;
         o2_key *keys = zalloc(b - a + 1, sizeof(o2_key));
         for (kk = a; kk < b; ++kk) {
-            if (dead[3][kk]) continue;
-            uint32_t ctr[4] = {B[3].id[kk], k->step, O2_VICTIM_TAG, 0};
+            if (dead[neg][kk]) continue;
+            uint32_t ctr[4] = {B[neg].id[kk], k->step, o2_vtag[ch], 0};
             uint32_t key[2] = {(uint32_t)k->seed, (uint32_t)(k->seed >> 32)}, o[4];
             orc_philox4x32_10(ctr, key, o);
             keys[m].key = o[0]; keys[m].pos = kk; ++m;
         }
         qsort(keys, m, sizeof(o2_key), o2_key_cmp);
         int kdel = delreq[g] < m ? delreq[g] : m;
-        for (j = 0; j < kdel; ++j) dead[3][keys[j].pos] = 2;
+        for (j = 0; j < kdel; ++j) dead[neg][keys[j].pos] = 2;
+        if (k->delcnt) k->delcnt[g] += kdel;
         free(keys);
     }
     /* rebuild: survivors in order, then products by (source species, parent,
-     * slot).  Every species' rebuild reads the old buckets of all four, so
+     * slot).  Every species' rebuild reads the old buckets of all species, so
      * the new ones are built aside and copied back at the end. */
-    real *nx[4], *ny[4], *nvx[4], *nvy[4], *nvz[4];
-    uint32_t *nidv[4];
-    int *start[4], nnew[4];
-    for (t = 0; t < 4; ++t) {
+    real *nx[CHEM_MAXSP], *ny[CHEM_MAXSP], *nvx[CHEM_MAXSP], *nvy[CHEM_MAXSP], *nvz[CHEM_MAXSP];
+    uint32_t *nidv[CHEM_MAXSP];
+    int *start[CHEM_MAXSP], nnew[CHEM_MAXSP];
+    for (t = 0; t < nsp; ++t) {
         orc_bucket *D = &B[t];
         int cap = D->cap, pos = 0;
         uint32_t nid = next_id[t];
@@ -1085,7 +1125,7 @@ int orc_mcc_o2(const orc_o2_in *k, orc_bucket *B, uint32_t *next_id, int64_t *mc
                 ++pos;
             }
             int s2;
-            for (s2 = 0; s2 < 4; ++s2)
+            for (s2 = 0; s2 < nsp; ++s2)
                 for (kk = B[s2].cell_start[g]; kk < B[s2].cell_start[g + 1]; ++kk) {
                     const o2_out *o = &out[s2][kk];
                     for (j = 0; j < o->np; ++j) {
@@ -1105,7 +1145,7 @@ int orc_mcc_o2(const orc_o2_in *k, orc_bucket *B, uint32_t *next_id, int64_t *mc
         nnew[t] = pos < cap ? pos : cap;
         next_id[t] = nid;
     }
-    for (t = 0; t < 4; ++t) {
+    for (t = 0; t < nsp; ++t) {
         orc_bucket *D = &B[t];
         int n = nnew[t];
         memcpy(D->x, nx[t], n * sizeof(real)); memcpy(D->y, ny[t], n * sizeof(real));
@@ -1116,7 +1156,7 @@ int orc_mcc_o2(const orc_o2_in *k, orc_bucket *B, uint32_t *next_id, int64_t *mc
         D->n = n;
         free(nx[t]); free(ny[t]); free(nvx[t]); free(nvy[t]); free(nvz[t]); free(nidv[t]); free(start[t]);
     }
-    for (s = 0; s < 4; ++s) { free(dead[s]); free(out[s]); }
+    for (s = 0; s < nsp; ++s) { free(dead[s]); free(out[s]); }
     free(delreq); free(prob);
     return rc;
 }

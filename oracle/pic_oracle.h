@@ -148,18 +148,23 @@ typedef struct {
     double *lu; int lu_bw;
 } orc_case;
 
-/* O2 chemistry (gas type 1) collision step, standalone: the caller passes
- * the setup arrays and the four species' buckets (e, O2+, O+, O-), sorted by
- * node with cell_start[G+1] and room for cap particles.  The buckets are
- * rebuilt in place; next_id and mccr (G x TnRct) are advanced. */
+/* Collision step of the oxygen chemistries, standalone: O2 (gas type 1,
+ * chem 0; species e, O2+, O+, O-) or Ar/O2 (gas type 2, chem 1; species e,
+ * Ar+, O2+, O+, O-).  The caller passes the setup arrays and the species'
+ * buckets, sorted by node with cell_start[G+1] and room for cap particles.
+ * The buckets are rebuilt in place; next_id, mccr (G x TnRct) and, when
+ * not NULL, delcnt (G: O- removed by electron detachment) are advanced. */
 typedef struct {
     int G, TnRct, nlogx, nvel, MCCn;
     real idLOGX, dt, dtm, idx, idy;
-    real np2c[4], Escale[4], sigv[20];
+    real np2c[5], Escale[5], sigv[25];
     const real *Th_e, *xe, *cx, *vsave, *BackDen1, *BackVel1, *FluidN, *den;
     uint64_t seed;
     uint32_t step;
-    int ions;   /* 0: only electrons collide, as the reference's MCC_O2_Basic (src/cuda_mcc.cu:76) */
+    int ions;   /* O2: 0 = only electrons collide, as the reference's MCC_O2_Basic (src/cuda_mcc.cu:76) */
+    int chem;   /* 0 O2, 1 Ar/O2 */
+    const real *BackDen2, *BackVel2;   /* Ar/O2: the O2 background */
+    int64_t *delcnt;
 } orc_o2_in;
 typedef struct {
     real *x, *y, *vx, *vy, *vz;

@@ -177,32 +177,36 @@ def test_o2_staged_steps_keep_the_bookkeeping(prec, name):
         hs.close()
 
 
-# net change of (e, O2+, O+) per event of each reaction (O- is left out: its
-# count also drops by the detachment removals, which are not counted)
+# net change of (e, O2+, O+, O-) per event of each reaction; the O- also
+# lose the electron-detachment removals, which are counted apart
 def _stoich():
     d = {}
     for r in (10, 14, 22):
-        d[r] = (1, 1, 0)           # 2e + O2+
+        d[r] = (1, 1, 0, 0)        # 2e + O2+
     for r in (13, 21, 29, 37, 39):
-        d[r] = (1, 0, 1)           # 2e + O+
-    d[12] = (0, 0, 1)              # e + O+ + O-
-    for r in (6, 15, 23, 31):
-        d[r] = (-1, 0, 0)          # attachment, recombination
-    d[30] = (1, 0, 0)              # detachment by an electron
+        d[r] = (1, 0, 1, 0)        # 2e + O+
+    d[12] = (0, 0, 1, 1)           # e + O+ + O-
+    for r in (6, 15, 23):
+        d[r] = (-1, 0, 0, 1)       # attachment
+    d[31] = (-1, 0, 0, 0)          # recombination
+    d[30] = (1, 0, 0, 0)           # detachment by an electron
     for r in (42, 43, 46):
-        d[r] = (1, 0, 0)           # O- detachment
+        d[r] = (1, 0, 0, -1)       # O- detachment
+    for r in (44, 45):
+        d[r] = (0, 0, 0, -1)       # O- recombination
     for r in (47, 50):
-        d[r] = (0, -1, 1)          # O2+ -> O+
+        d[r] = (0, -1, 1, 0)       # O2+ -> O+
     for r in (53, 56, 57):
-        d[r] = (0, 1, -1)          # O+ -> O2+
+        d[r] = (0, 1, -1, 0)       # O+ -> O2+
     return d
 
 
 @pytest.mark.parametrize("prec", [FP32_REFINED, FP64_PLAIN])
 def test_o2_particle_balance_closes(prec):
     """Whole O2 steps at 96x96 cells (about 1.1 M particles): the change of
-    the e, O2+ and O+ counts equals what the counted reactions make and
-    remove, less the particles taken by the walls and electrodes."""
+    every species' count equals what the counted reactions make and remove,
+    less the O- removed by electron detachment and the particles taken by
+    the walls and electrodes."""
     deck = _deck(ncx=96, ncy=96, ppc=50, pres_torr=0.05)
     c = Case(deck, prec)
     try:
@@ -214,16 +218,18 @@ def test_o2_particle_balance_closes(prec):
             cc = c.array("CondCharge").astype(np.int64).reshape(nsp, ncond).sum(axis=1)
             return (np.array([inf.np[s] for s in range(nsp)], dtype=np.int64),
                     c.array("mccr").astype(np.int64).reshape(-1, 67).sum(axis=0),
-                    c.array("Lost").astype(np.int64) + cc)
-        n0, m0, w0 = state()
+                    c.array("Lost").astype(np.int64) + cc,
+                    int(c.array("detach_removed").astype(np.int64).sum()))
+        n0, m0, w0, r0 = state()
         c.step(6)
         c.sync()
-        n1, m1, w1 = state()
+        n1, m1, w1, r1 = state()
         dm = m1 - m0
         assert dm.sum() > 0
-        made = np.zeros(3, dtype=np.int64)
+        made = np.zeros(4, dtype=np.int64)
         for r, v in _stoich().items():
             made += dm[r] * np.array(v, dtype=np.int64)
-        assert np.array_equal(n1[:3] - n0[:3], made - (w1 - w0)[:3]), (n1 - n0, made, w1 - w0)
+        made[3] -= r1 - r0
+        assert np.array_equal(n1 - n0, made - (w1 - w0)), (n1 - n0, made, w1 - w0, r1 - r0)
     finally:
         c.close()

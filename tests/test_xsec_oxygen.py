@@ -5,9 +5,8 @@ oracle/_ref, where the reference's own src/PhysicsData.c fit functions,
 CrossSection() data and FUNC_decay are compiled and driven with the table
 loops of Oxygen_CrossSectionSET / ArO2_CrossSectionSET.  The row, rate and
 pair specification is checked against the reference source text where that
-tree is present.  Gas type 2 has host setup only: creating a case for it is
-refused with -19 (the O2 collision step, gas type 1, is tested on the GPU in
-test_gpu_o2.py).
+tree is present.  The collision steps that use them are tested on the GPU
+in test_gpu_o2.py and test_gpu_aro2.py.
 """
 import base64
 import json
@@ -135,10 +134,22 @@ def test_reaction_names_follow_the_deck_format():
     assert not np.any(cx[:, 10]) and np.any(cx[:, 9])
 
 
-def test_case_creation_is_refused_for_ar_o2():
-    with pytest.raises(PicError) as e:
-        picmcc.Case(decks.oxygen_deck(gas=2, ncx=8, ncy=6, ppc=5))
-    assert e.value.code == -19
+@pytest.mark.parametrize("prec", [FP32_REFINED, FP64_PLAIN])
+def test_ar_o2_second_background_gas(prec):
+    """Calculate_2GasPara (src/cuda_Field.cu:670-678): the O2 background of an
+    Ar/O2 deck has the O2 share of the pressure at the same temperature, so
+    BackDen2 / BackDen1 = ratio / (1 - ratio) and BackVel2 / BackVel1 =
+    sqrt(m_Ar / m_O2) at every gas node, and both are 0 elsewhere."""
+    hs = HostSetup(decks.oxygen_deck(gas=2, o2_ratio=0.25, ncx=8, ncy=6, ppc=5), prec, load=False)
+    d1, d2 = hs.array("BackDen1"), hs.array("BackDen2")
+    v1, v2 = hs.array("BackVel1"), hs.array("BackVel2")
+    gas = hs.array("DensRegion") != 0
+    assert gas.any() and not np.any(d2[~gas]) and not np.any(v2[~gas])
+    tol = 1e-6 if prec == FP32_REFINED else 1e-13
+    np.testing.assert_allclose(d2[gas] / d1[gas], 0.25 / 0.75, rtol=tol)
+    np.testing.assert_allclose(v2[gas] / v1[gas], np.sqrt(39.950 / 32.0), rtol=tol)
+    # the argon-only and O2-only decks have no second gas
+    assert HostSetup(decks.oxygen_deck(gas=1, ncx=8, ncy=6, ppc=5), prec, load=False).array("BackDen2").size == 0
 
 
 @pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "src")), reason="reference tree not present")
