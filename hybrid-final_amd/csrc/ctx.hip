@@ -1412,6 +1412,7 @@ void pic_shim_close(void) {
     if (g_default) pic_destroy(g_default);
     g_default = nullptr;
 }
+pic_ctx *pic_shim_ctx(void) { return g_default; }
 void info_Device(void) {
     int dev = 0;
     hipDeviceProp_t p;
@@ -1448,6 +1449,7 @@ void MCC_ArO2_cuda(void) {
 }
 void Deposit_cuda(void) { shim_check(shim_case("Deposit_cuda")->deposit(), "Deposit_cuda"); }
 void Deposit_Basic(void) { shim_check(shim_case("Deposit_Basic")->deposit(), "Deposit_Basic"); }
+void Diagnostic(void) { shim_check(shim_case("Diagnostic")->diagnostic(), "Diagnostic"); }
 void main_cuda(void) {
     CaseBase *c = shim_case("main_cuda");
     long long nsteps = (long long)c->rf_cycles * c->dt_pic;

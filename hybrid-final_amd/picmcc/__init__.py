@@ -91,11 +91,12 @@ EXPORTS = {
     "pic_reaction_names": (C.c_char_p, [C.c_int]),
     "pic_shim_open": (C.c_int, [C.c_char_p, C.c_int]),
     "pic_shim_close": (None, []),
+    "pic_shim_ctx": (C.c_void_p, []),
 }
 SHIMS = ["main_cuda", "start_cuda", "info_Device", "Set_Device_Parameter", "Set_Particle_cuda",
          "Set_NullCollisionTime_cuda", "Set_SortBoundary_cuda", "Set_MatrixPCG_cuda", "PCG_SOLVER_Laplace",
-         "PCG_SOLVER", "Efield_cuda_Basic", "Move_cuda", "SortBounndary_cuda", "MCC_Ar_cuda", "Deposit_cuda",
-         "Deposit_Basic"]
+         "PCG_SOLVER", "Efield_cuda_Basic", "Move_cuda", "SortBounndary_cuda", "MCC_Ar_cuda", "MCC_O2_cuda",
+         "MCC_ArO2_cuda", "Deposit_cuda", "Deposit_Basic", "Diagnostic"]
 
 
 def lib():

@@ -151,6 +151,11 @@ const char *pic_reaction_names(int gas);
 /* ---------------- reference-named zero-argument shims ---------------- */
 int pic_shim_open(const char *deck_path, int precision);
 void pic_shim_close(void);
+/* the context the shims act on (NULL before pic_shim_open), so that a C
+ * driver that calls the stage shims one by one can advance the clock with
+ * pic_advance_time, as the reference's loop does with t += dt; tstep++
+ * (src/cuda_main.cu:114-116), and read results with the pic_* calls */
+pic_ctx *pic_shim_ctx(void);
 void main_cuda(void);           /* src/cuda_main.cu:11 (runs RFCycles cycles) */
 void start_cuda(void);          /* src/cuda_start.cu:2 */
 void info_Device(void);         /* src/cuda_start.cu:36 */
@@ -169,6 +174,8 @@ void MCC_O2_cuda(void);         /* src/cuda_mcc.cu:7-10 dispatch (gas type 1) */
 void MCC_ArO2_cuda(void);       /* src/cuda_mcc.cu:2-6 dispatch (gas type 2) */
 void Deposit_cuda(void);        /* src/cuda_Deposit.cu:3 */
 void Deposit_Basic(void);       /* src/cuda_Deposit.cu:21 */
+void Diagnostic(void);          /* DIAG table entry (src/cuda_start.cu:33): the profile
+                                   accumulators of src/cuda_Diagnostic.cu:505-547 */
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,128 @@
+"""The reference-named zero-argument shims, driven from C.
+
+tests/shim_driver.c is compiled with gcc against include/picmcc.h and linked
+with libpicmcc.so, as a reference-side C driver would be.  It binds the
+stage table (inc/main.h:34-41) to the shims and steps through it, or calls
+main_cuda (src/cuda_main.cu:11); the state it writes must equal, bit for
+bit, the same deck run through the explicit-context pic_* path
+(pic_run_staged / pic_step) from Python.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from picmcc import FP32_REFINED, FP64_PLAIN, HERE, Case, decks
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _build(tmp_path):
+    exe = str(tmp_path / "shim_driver")
+    subprocess.check_call(["gcc", "-std=c99", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "shim_driver.c"), "-L", HERE, "-lpicmcc",
+                           "-Wl,-rpath," + HERE, "-o", exe])
+    return exe
+
+
+def test_c_driver_compiles_and_links(tmp_path):
+    """Every shim and pic_* call the driver makes resolves in libpicmcc.so."""
+    exe = _build(tmp_path)
+    assert os.path.exists(exe)
+    out = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True, check=True).stdout
+    for name in ("main_cuda", "PCG_SOLVER", "Efield_cuda_Basic", "Move_cuda", "SortBounndary_cuda", "MCC_Ar_cuda",
+                 "MCC_O2_cuda", "MCC_ArO2_cuda", "Deposit_cuda", "Diagnostic", "pic_shim_ctx"):
+        assert name in out
+
+
+def _read(path, nsp, G, real):
+    with open(path, "rb") as f:
+        buf = f.read()
+    off = 0
+    sp = []
+    rs = np.dtype(real).itemsize
+    for _ in range(nsp):
+        n = int(np.frombuffer(buf, np.int64, 1, off)[0])
+        off += 8
+        d = {}
+        for k in ("x", "y", "vx", "vy", "vz"):
+            d[k] = np.frombuffer(buf, real, n, off)
+            off += n * rs
+        d["id"] = np.frombuffer(buf, np.uint32, n, off)
+        off += 4 * n
+        d["cell_start"] = np.frombuffer(buf, np.int32, G + 1, off)
+        off += 4 * (G + 1)
+        sp.append(d)
+    arrs = {}
+    for name, dt in (("phi", real), ("mccr", np.uint64)):
+        nb = int(np.frombuffer(buf, np.int64, 1, off)[0])
+        off += 8
+        arrs[name] = np.frombuffer(buf, dt, nb // np.dtype(dt).itemsize, off)
+        off += nb
+    assert off == len(buf)
+    return sp, arrs
+
+
+def _compare(case, sp, arrs):
+    for s, d in enumerate(sp):
+        p = case.particles(s)
+        for k in ("x", "y", "vx", "vy", "vz", "id", "cell_start"):
+            assert np.array_equal(p[k].view(np.uint8), np.ascontiguousarray(d[k]).view(np.uint8)), (s, k)
+    assert np.array_equal(case.array("phi").view(np.uint8), np.ascontiguousarray(arrs["phi"]).view(np.uint8))
+    assert np.array_equal(case.array("mccr"), arrs["mccr"])
+
+
+DECKS = {
+    "ar": lambda: decks.small(ncx=16, ncy=12, ppc=30, volts=150.0),
+    "o2": lambda: decks.oxygen_deck(gas=1, ncx=10, ncy=8, ppc=40, pres_torr=0.1, volts=100.0),
+    "aro2": lambda: decks.oxygen_deck(gas=2, ncx=10, ncy=8, ppc=40, pres_torr=0.1, volts=100.0),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gas", ["ar", "o2", "aro2"])
+@pytest.mark.parametrize("prec", [FP32_REFINED, FP64_PLAIN])
+def test_stage_table_from_c_equals_run_staged(tmp_path, gas, prec):
+    exe = _build(tmp_path)
+    deck = DECKS[gas]()
+    path = tmp_path / "deck.json"
+    path.write_text(json.dumps(deck))
+    out = tmp_path / "state.bin"
+    nsteps = 5
+    print("C driver, staged:", gas, prec, flush=True)
+    subprocess.run([exe, str(path), str(prec), "staged", str(nsteps), str(out)], check=True, timeout=120)
+    c = Case(deck, prec)
+    try:
+        c.run_staged(nsteps)
+        c.sync()
+        inf = c.info()
+        sp, arrs = _read(out, inf.nsp, inf.ngx * inf.ngy, c.real)
+        _compare(c, sp, arrs)
+    finally:
+        c.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [FP32_REFINED, FP64_PLAIN])
+def test_main_cuda_from_c_equals_pic_step(tmp_path, prec):
+    """main_cuda runs RFCycles cycles of TimeStep_PIC fused steps."""
+    exe = _build(tmp_path)
+    deck = decks.small(ncx=16, ncy=12, ppc=30, volts=150.0, steps_per_cycle=100, cycles=2)
+    path = tmp_path / "deck.json"
+    path.write_text(json.dumps(deck))
+    out = tmp_path / "state.bin"
+    print("C driver, main_cuda:", prec, flush=True)
+    r = subprocess.run([exe, str(path), str(prec), "main", "0", str(out)], check=True, timeout=120,
+                       capture_output=True, text=True)
+    assert "tstep 200" in r.stdout
+    c = Case(deck, prec)
+    try:
+        c.step(200)
+        c.sync()
+        inf = c.info()
+        sp, arrs = _read(out, inf.nsp, inf.ngx * inf.ngy, c.real)
+        _compare(c, sp, arrs)
+    finally:
+        c.close()
