@@ -24,13 +24,14 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s, FP32-refined vs FP64 plain; p50/p99 ensemble case latency"
 
 
-def run_gpu(deck, precision, steps, warmup, timing_steps=0):
+def run_gpu(deck, precision, steps, warmup, timing_steps=0, field_solver=0):
     """Warm up, time `steps` steps with no instrumentation, then (optionally)
     `timing_steps` more steps with HIP events around every stage kernel on the
     case's stream (the per-kernel times; the events add launch gaps, so they
-    are kept out of the throughput run)."""
+    are kept out of the throughput run).  field_solver: 0 the default of the
+    precision, 1 Jacobi PCG, 2 the separable FP64 solve in either precision."""
     from picmcc import Case
-    c = Case(deck, precision)
+    c = Case(deck, precision, field_solver=field_solver)
     c.step(warmup)
     if os.environ.get("PIC_BENCH_DEBUG"):
         print("debug: last field solve", c.last_field_solve(), file=sys.stderr, flush=True)
@@ -59,6 +60,55 @@ def run_gpu(deck, precision, steps, warmup, timing_steps=0):
     }
     c.close()
     return out
+
+
+SOLVER_LABEL = {0: "FP64 Jacobi PCG (the reference algorithm in FP64), tol 1e-10 relative",
+                1: "FP64 separable fast-Poisson inverse (closed-form DCT/DST eigenbases, FP64 matrix-core GEMMs) "
+                   "in FP64 refinement, tol 1e-10 relative"}
+REFINED_PCG_LABEL = "FP32 Jacobi PCG in FP64 refinement, tol 1e-10 relative"
+
+
+def solver_label(r, precision):
+    from picmcc import FP32_REFINED
+    if r["solver_kind"] == 1:
+        return SOLVER_LABEL[1]
+    return REFINED_PCG_LABEL if precision == FP32_REFINED else SOLVER_LABEL[0]
+
+
+def profile_check(deck, cycles, sample_every=40, emax=40.0):
+    """Time-averaged profiles over `cycles` RF cycles on the FP32-refined path
+    and on the FP64 path with the same (separable, FP64) field solve, from the
+    same loaded particles and Philox stream: the x-averaged electron and ion
+    density and potential profiles and the EEDF (sampled every sample_every
+    steps), as the largest difference relative to the FP64 profile's peak."""
+    import numpy as np
+    from picmcc import Case, FP32_REFINED, FP64_PLAIN
+    steps = cycles * int(deck["SimulationMethod"]["TimeStep_PIC"])
+    out = {}
+    for name, prec in (("fp32", FP32_REFINED), ("fp64", FP64_PLAIN)):
+        c = Case(deck, prec, field_solver=2)
+        for _ in range(steps // sample_every):
+            c.step(sample_every)
+            c.eedf_sample(emax)
+        c.sync()
+        inf = c.info()
+        ngx, ngy, nsp = inf.ngx, inf.ngy, inf.nsp
+        den = c.array("sum_den").reshape(nsp, ngx, ngy) / steps
+        phi = c.array("sum_phi").reshape(ngx, ngy) / steps
+        eedf = c.array("eedf").astype(np.float64)
+        out[name] = (den, phi, eedf / eedf.sum())
+        c.close()
+    (d32, p32, f32), (d64, p64, f64) = out["fp32"], out["fp64"]
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    return {"cycles": cycles, "steps": steps,
+            "electron_density": rel(d32[0].mean(axis=0), d64[0].mean(axis=0)),
+            "ion_density": rel(d32[1].mean(axis=0), d64[1].mean(axis=0)),
+            "potential": rel(p32.mean(axis=0), p64.mean(axis=0)),
+            "eedf": rel(f32, f64),
+            "measure": "max |FP32 - FP64| / peak of the FP64 profile; density and potential averaged over x "
+                       "(the Neumann direction), EEDF sampled every %d steps over 0-%g eV" % (sample_every, emax)}
 
 
 def particles_avg(r):
@@ -127,6 +177,32 @@ def run_o2(steps, warmup=2):
             "path": "staged stages (the fused two-pass step is built for argon)"}
 
 
+def run_aro2(steps, warmup=2):
+    """Ar/O2 chemistry (gas type 2), staged steps on the 512x256 workload:
+    O2 20 %, 30 mTorr, 150 V, dx=0.2 mm, 100 particles/cell for e, Ar+ and
+    O2+ and 20 for O+ and O- (44.6 M particles), dt=T_RF/800; FP32 particles
+    with the FP64-refined field solve.  Particles are resident in HBM before
+    the timed steps."""
+    from picmcc import Case, FP32_REFINED, decks
+    c = Case(decks.config3(), FP32_REFINED)
+    c.step(warmup)
+    c.sync()
+    inf0 = c.info()
+    n0 = int(sum(inf0.np[s] for s in range(inf0.nsp)))
+    t0 = time.perf_counter()
+    c.step(steps)
+    c.sync()
+    dt = (time.perf_counter() - t0) / steps
+    inf1 = c.info()
+    n1 = int(sum(inf1.np[s] for s in range(inf1.nsp)))
+    c.close()
+    n = 0.5 * (n0 + n1)
+    return {"workload": "Ar/O2 CCP 512x256 cells dx=0.2mm, O2 20 %, 30 mTorr, 150 V, 100/100/100/20/20 particles "
+                        "per cell (e, Ar+, O2+, O+, O-), dt=T_RF/800",
+            "particles": n1, "steps": steps, "ms_per_step": dt * 1e3, "particle_steps_per_s": n / dt,
+            "path": "staged stages with the oxygen collision step (the fused two-pass step is built for argon)"}
+
+
 def run_ensemble(n, rate_hz):
     """A served ensemble of n argon CCP cases (grids 32^2-96^2, 20-60
     particles/cell, 10-500 mTorr, 50-300 V, 1-3 RF cycles of 400 steps,
@@ -179,12 +255,20 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=20, help="steps timed on the FP64 plain path")
+    ap.add_argument("--same-solver-steps", type=int, default=200,
+                    help="steps timed on the FP64 path with the separable FP64 field solve (0 = skip)")
+    ap.add_argument("--refined-pcg-steps", type=int, default=20,
+                    help="steps timed on the FP32 path with the FP32 PCG in FP64 refinement (0 = skip)")
+    ap.add_argument("--profile-cycles", type=int, default=10,
+                    help="RF cycles of the FP32 vs FP64 time-averaged profile comparison (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--pbatch", type=int, default=256, metavar="N",
                     help="independent Poisson problems (64^2-512^2) in the batched-solve leg (0 = skip)")
     ap.add_argument("--o2-steps", type=int, default=20, metavar="K",
                     help="staged O2-chemistry steps timed on a 256x256 O2 case (0 = skip)")
+    ap.add_argument("--aro2-steps", type=int, default=10, metavar="K",
+                    help="staged Ar/O2 steps timed on the 512x256 Ar/O2 workload (0 = skip)")
     ap.add_argument("--ensemble", type=int, default=16, metavar="N",
                     help="cases in the served argon ensemble (Poisson arrivals; concurrent server, first-come "
                          "first-served server and one by one; 0 = skip)")
@@ -214,6 +298,18 @@ def main():
     if args.plain_steps > 0:
         plain = run_gpu(deck, FP64_PLAIN, args.plain_steps, 2)
         plain_value = particles_avg(plain) * args.plain_steps / plain["seconds"]
+    # the same precision comparison with the same field solver on both sides
+    same, same_value = None, 0.0
+    if args.same_solver_steps > 0:
+        same = run_gpu(deck, FP64_PLAIN, args.same_solver_steps, 5, timing_steps=min(args.same_solver_steps, 50),
+                       field_solver=2)
+        same_value = particles_avg(same) * args.same_solver_steps / same["seconds"]
+    # the FP32 Jacobi PCG inside FP64 refinement (the solver for operators
+    # that are not separable) on the same workload
+    rpcg, rpcg_value = None, 0.0
+    if args.refined_pcg_steps > 0:
+        rpcg = run_gpu(deck, FP32_REFINED, args.refined_pcg_steps, 2, field_solver=1)
+        rpcg_value = particles_avg(rpcg) * args.refined_pcg_steps / rpcg["seconds"]
 
     # roofline of the dominant particle kernel (HIP events on the case stream)
     st = ref["stages_ms"]
@@ -257,14 +353,28 @@ def main():
                          "Ar CCP 64x64 cells dx=0.5mm, 100 V 13.56 MHz, 100 mTorr, 50 particles/cell/species, "
                          "dt=T_RF/400"),
             "particles": int(sum(ref["np1"])),
-            "field_solve": "FP32 separable inverse in FP64 refinement, tol 1e-10 relative"
-            if ref["solver_kind"] == 1 else "FP32 Jacobi PCG in FP64 refinement, tol 1e-10 relative",
+            "field_solve": solver_label(ref, FP32_REFINED),
         },
         "fp64_plain": {"value": plain_value, "unit": "particle-steps/s",
                        "ms_per_step": plain["seconds"] / args.plain_steps * 1e3, "steps": args.plain_steps,
-                       "field_solve": "FP64 Jacobi PCG (reference algorithm), tol 1e-10 relative",
+                       "field_solve": solver_label(plain, FP64_PLAIN),
                        "field_iters_last": plain["field_iters"]} if plain else None,
+        # headline ratio: the FP32-refined path against the reference
+        # algorithm in FP64 (its Jacobi PCG dominates the FP64 step)
         "refined_over_plain": value / plain_value if plain_value > 0 else None,
+        "fp64_same_solver": {"value": same_value, "unit": "particle-steps/s",
+                             "ms_per_step": same["seconds"] / args.same_solver_steps * 1e3,
+                             "steps": args.same_solver_steps, "field_solve": solver_label(same, FP64_PLAIN),
+                             "stage_ms_per_step": {k: round(v, 5) for k, v in same["stages_ms"].items()}}
+        if same else None,
+        # precision alone: FP32 against FP64 particles and fields, both with
+        # the separable FP64 field solve
+        "fp32_over_fp64_same_solver": value / same_value if same_value > 0 else None,
+        "fp32_refined_pcg": {"value": rpcg_value, "unit": "particle-steps/s",
+                             "ms_per_step": rpcg["seconds"] / args.refined_pcg_steps * 1e3,
+                             "steps": args.refined_pcg_steps, "field_solve": solver_label(rpcg, FP32_REFINED),
+                             "field_iters_last": rpcg["field_iters"], "field_cycles_last": rpcg["field_cycles"],
+                             "field_rel_res_last": rpcg["field_rel_res"]} if rpcg else None,
         "field_solve_last": {"cycles": ref["field_cycles"], "rel_residual": ref["field_rel_res"]},
         "stage_ms_per_step": {k: round(v, 5) for k, v in st.items()},
         # algorithmic HBM bytes of each particle stage over its time (particle
@@ -277,10 +387,14 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_, "ms_per_launch": ms},
     }
+    if args.profile_cycles > 0:
+        line["profile_error_vs_fp64"] = profile_check(deck, args.profile_cycles)
     if args.pbatch > 0:
         line["batched_poisson"] = run_pbatch(args.pbatch)
     if args.o2_steps > 0:
         line["o2_chemistry"] = run_o2(args.o2_steps)
+    if args.aro2_steps > 0:
+        line["aro2_chemistry"] = run_aro2(args.aro2_steps)
     line["ensemble"] = ensemble if ensemble is not None else {"status": "not run (--ensemble 0)"}
     if args.cpu_steps > 0:
         cv, cdt = cpu_baseline(deck, args.cpu_steps)

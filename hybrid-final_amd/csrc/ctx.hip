@@ -391,7 +391,7 @@ struct Case : CaseBase {
             if (!pw) return fail(-40, "field solver workspace allocation failed");
             HIPCHK(dalloc(&d_res, 1));
             HIPCHK(hipMemsetAsync(d_res, 0, sizeof(PcgResult), st));
-            if (precision == PIC_FP32_REFINED && field_solver == 0) {
+            if ((precision == PIC_FP32_REFINED && field_solver == 0) || field_solver == 2) {
                 Spectral sp;
                 std::string why;
                 if (build_spectral<R>(S, sp, why)) {
@@ -590,9 +590,8 @@ struct Case : CaseBase {
         const int N = S.A_size;
         const double tol = S.deck.pcg_tol > 0 ? S.deck.pcg_tol : 1e-10;
         if (spectral) {
-            // the first residual converts the FP32 rhs (spectral implies R = float)
-            spectral_solve_refined(op, spec, (const float *)(const void *)brhs, b64, x64, tol, spec_cycles,
-                                   spec_state, spec_part, d_res, st);
+            // the first residual converts the rhs into b64
+            spectral_solve_refined<R>(op, spec, brhs, b64, x64, tol, spec_cycles, spec_state, spec_part, d_res, st);
         } else {
             launch_rhs_to_fp64<R>(N, brhs, b64, st);
             if (precision == PIC_FP64_PLAIN) HIPCHK(pcg_solve_plain_fp64(op, b64, x64, tol, pcg_maxit, pw, d_res, st));
@@ -859,6 +858,14 @@ struct Case : CaseBase {
         HIPCHK(hipMemcpyAsync(&r, d_res, sizeof r, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (r.status == 1) return fail(-31, "field solve hit the iteration limit (rel. residual %g)", r.rel_res);
+        if (r.status == 3) {
+            // the abandoned solve has finished: clear its barrier and status for the next solve
+            HIPCHK(pcg_work_reset(pw, st));
+            const PcgResult z{};
+            HIPCHK(hipMemcpyAsync(d_res, &z, sizeof z, hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+            return fail(-34, "field solve abandoned: a grid barrier wait of the persistent solver timed out");
+        }
         if (std::isnan(r.rel_res)) return fail(-32, "Field solver Error! (NaN residual)");
         return 0;
     }

@@ -1,5 +1,5 @@
 // Field solve: Jacobi-preconditioned conjugate gradients as one persistent
-// cooperative kernel per solve (the reference's PCG_float,
+// kernel per solve (the reference's PCG_float,
 // src/cuda_Field.cu:942-1072, uses the same single-launch structure).
 //
 //  * plain FP64 path: vectors, operator and dot products in FP64;
@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 
 #include "pic_kernels.h"
 
@@ -26,6 +27,8 @@ constexpr int PCG_NRED = 4;   // values reduced per barrier
 struct GridSync {
     unsigned int count;
     unsigned int gen;
+    unsigned int abort;   // set when a barrier wait ran out of time; every block then leaves
+    unsigned int pad;
 };
 
 struct PcgWork {
@@ -34,29 +37,55 @@ struct PcgWork {
     float *rf = nullptr, *zf = nullptr, *pf = nullptr, *qf = nullptr, *df = nullptr;
     double *partials = nullptr;   // 2 sets x nblk x PCG_NRED
     GridSync *gs = nullptr;
+    unsigned long long spin_limit = 0;   // barrier wait limit, in ticks of the 100 MHz constant clock
 };
 
 namespace {
 
-__device__ inline void grid_sync(GridSync *gs, unsigned nblk) {
+// Grid barrier of the persistent solver.  The solver is launched as an
+// ordinary kernel with no more blocks than the device holds at once, so every
+// block becomes resident once the work ahead of it drains; a wait is still
+// bounded: after spin_limit ticks of the 100 MHz constant clock the waiting
+// block raises gs->abort, and every block that sees it leaves the solve
+// (status 3).  Returns false when the solve is being abandoned.
+__device__ inline bool grid_sync(GridSync *gs, unsigned nblk, unsigned long long spin_limit) {
+    __shared__ int ok;
     // the block's stores happen-before thread 0's agent-scope release via the
     // workgroup barrier; thread 0's acquire makes every block's stores
     // visible to the whole block after the second barrier
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        unsigned g = __hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned arrived = __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblk - 1) {
-            __hip_atomic_store(&gs->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
-                __builtin_amdgcn_s_sleep(1);
+        int good = __hip_atomic_load(&gs->abort, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        if (good) {
+            unsigned g = __hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned arrived = __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (arrived == nblk - 1) {
+                __hip_atomic_store(&gs->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+                while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                    if (__hip_atomic_load(&gs->abort, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                        good = 0;
+                        break;
+                    }
+                    if (__builtin_amdgcn_s_memrealtime() - t0 > spin_limit) {
+                        __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        good = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
         }
         __threadfence();
+        ok = good;
     }
     __syncthreads();
+    const bool r = ok != 0;
+    __syncthreads();   // ok is rewritten by the next barrier
+    return r;
 }
 
 // block-level sum of NV doubles with a fixed tree; result valid in all threads
@@ -80,14 +109,15 @@ __device__ inline void block_sum(double (&v)[NV], double *sh) {
 
 // publish block partials, barrier, and sum all partials in block order
 template <int NV>
-__device__ inline void grid_sum(double (&v)[NV], double *sh, double *partials, int &phase, GridSync *gs, unsigned nblk) {
+__device__ inline bool grid_sum(double (&v)[NV], double *sh, double *partials, int &phase, GridSync *gs, unsigned nblk,
+                                unsigned long long spin_limit) {
     block_sum<NV>(v, sh);
     double *set = partials + (size_t)(phase & 1) * nblk * PCG_NRED;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < NV; ++j) set[blockIdx.x * PCG_NRED + j] = v[j];
     }
-    grid_sync(gs, nblk);
+    if (!grid_sync(gs, nblk, spin_limit)) return false;
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = 0.0;
@@ -100,6 +130,7 @@ __device__ inline void grid_sum(double (&v)[NV], double *sh, double *partials, i
 #pragma unroll
     for (int j = 0; j < NV; ++j) v[j] = acc[j];
     ++phase;
+    return true;
 }
 
 template <typename V>
@@ -113,10 +144,23 @@ __device__ inline V spmv_row(const int *col, const V *val, const V *x, int i) {
     return s;
 }
 
+// a barrier wait ran out of time: the solve is abandoned (status 3)
+__device__ inline void pcg_abandon(PcgResult *res) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) res->status = 3;
+}
+#define PCG_SUM(NV, v)                                                                                               \
+    do {                                                                                                             \
+        if (!grid_sum<NV>(v, sh, partials, phase, gs, nblk, spin_limit)) {                                           \
+            pcg_abandon(res);                                                                                        \
+            return;                                                                                                  \
+        }                                                                                                            \
+    } while (0)
+
 // ------------------------------------------------------------ plain FP64
 __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_plain(PcgOperator A, const double *b, double *x, double *r,
                                                         double *z, double *p, double *q, double *partials,
-                                                        GridSync *gs, double tol, int maxit, PcgResult *res) {
+                                                        GridSync *gs, double tol, int maxit, PcgResult *res,
+                                                        unsigned long long spin_limit) {
     __shared__ double sh[PCG_NRED * PCG_BLOCK];
     const unsigned nblk = gridDim.x;
     const int N = A.N;
@@ -130,7 +174,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_plain(PcgOperator A, const do
         r[i] = ri; z[i] = zi; p[i] = zi;
         v3[0] += ri * zi; v3[1] += ri * ri; v3[2] += b[i] * b[i];
     }
-    grid_sum<3>(v3, sh, partials, phase, gs, nblk);
+    PCG_SUM(3, v3);
     double rz = v3[0], rr = v3[1], bb = v3[2];
     const double tol2 = tol * tol * bb;
     int it = 0;
@@ -145,7 +189,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_plain(PcgOperator A, const do
             q[i] = qi;
             v1[0] += p[i] * qi;
         }
-        grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+        PCG_SUM(1, v1);
         double pq = v1[0];
         for (;;) {
             double alpha = pq != 0.0 ? rz / pq : 0.0;
@@ -157,7 +201,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_plain(PcgOperator A, const do
                 r[i] = ri; z[i] = zi;
                 v2[0] += ri * zi; v2[1] += ri * ri;
             }
-            grid_sum<2>(v2, sh, partials, phase, gs, nblk);
+            PCG_SUM(2, v2);
             ++it;
             double rz_new = v2[0];
             rr = v2[1];
@@ -172,7 +216,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_plain(PcgOperator A, const do
                 p[i] = pi; q[i] = qi;
                 v1[0] += pi * qi;
             }
-            grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+            PCG_SUM(1, v1);
             pq = v1[0];
         }
     }
@@ -188,7 +232,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_plain(PcgOperator A, const do
 __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const double *b, double *x, double *r64,
                                                           float *r, float *z, float *p, float *q, float *d,
                                                           double *partials, GridSync *gs, double tol, double inner_tol,
-                                                          int maxit, PcgResult *res) {
+                                                          int maxit, PcgResult *res, unsigned long long spin_limit) {
     __shared__ double sh[PCG_NRED * PCG_BLOCK];
     const unsigned nblk = gridDim.x;
     const int N = A.N;
@@ -206,7 +250,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
             v2[0] += ri * ri;
             if (bb < 0.0) v2[1] += b[i] * b[i];
         }
-        grid_sum<2>(v2, sh, partials, phase, gs, nblk);
+        PCG_SUM(2, v2);
         rr64 = v2[0];
         if (bb < 0.0) bb = v2[1];
         if (bb == 0.0) {
@@ -226,7 +270,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
             v3[0] += (double)ri * (double)zi;
             v3[1] += (double)ri * (double)ri;
         }
-        grid_sum<2>(v3, sh, partials, phase, gs, nblk);
+        PCG_SUM(2, v3);
         double rz = v3[0];
         const double stop2 = inner_tol * inner_tol * v3[1];
         double v1[1] = {0};
@@ -235,7 +279,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
             q[i] = qi;
             v1[0] += (double)p[i] * (double)qi;
         }
-        grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+        PCG_SUM(1, v1);
         double pq = v1[0];
         for (;;) {
             float alpha = (float)(pq != 0.0 ? rz / pq : 0.0);
@@ -248,7 +292,7 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
                 w2[0] += (double)ri * (double)zi;
                 w2[1] += (double)ri * (double)ri;
             }
-            grid_sum<2>(w2, sh, partials, phase, gs, nblk);
+            PCG_SUM(2, w2);
             ++it;
             double rz_new = w2[0];
             if (w2[1] <= stop2 || it >= maxit) break;
@@ -261,12 +305,15 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
                 p[i] = pi; q[i] = qi;
                 v1[0] += (double)pi * (double)qi;
             }
-            grid_sum<1>(v1, sh, partials, phase, gs, nblk);
+            PCG_SUM(1, v1);
             pq = v1[0];
         }
         for (int i = t0; i < N; i += stride) x[i] += (double)d[i];
         // x is read by other blocks in the residual sweep
-        grid_sync(gs, nblk);
+        if (!grid_sync(gs, nblk, spin_limit)) {
+            pcg_abandon(res);
+            return;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         res->iters = it;
@@ -274,6 +321,12 @@ __global__ __launch_bounds__(PCG_BLOCK) void k_pcg_refined(PcgOperator A, const 
         res->rel_res = bb > 0.0 ? sqrt(rr64 / bb) : 0.0;
         res->status = status;
     }
+}
+
+#undef PCG_SUM
+
+__global__ void k_pcg_reset(GridSync *gs) {
+    if (threadIdx.x == 0) { gs->count = 0; gs->gen = 0; gs->abort = 0; gs->pad = 0; }
 }
 
 }  // namespace
@@ -306,6 +359,11 @@ PcgWork *pcg_work_create(int N, int *err, int want_blocks, hipStream_t stream) {
     if (nblk > PCG_BLOCK) nblk = PCG_BLOCK;      // grid_sum keeps one partial per thread
     if (nblk < 1) nblk = 1;
     w->nblk = nblk;
+    // a barrier wait normally takes microseconds; 2 s means the grid never
+    // became resident together (PIC_PCG_SPIN_US overrides, for tests of the
+    // abandon path)
+    w->spin_limit = 200000000ull;
+    if (const char *sl = getenv("PIC_PCG_SPIN_US")) w->spin_limit = 100ull * strtoull(sl, nullptr, 10);
     size_t n = (size_t)N;
     hipError_t e = hipSuccess;
     e = (hipError_t)(e | hipMalloc(&w->r, n * sizeof(double)));
@@ -343,21 +401,23 @@ void pcg_work_destroy(PcgWork *w) {
 
 int pcg_grid_blocks(PcgWork *w) { return w ? w->nblk : 0; }
 
+hipError_t pcg_work_reset(PcgWork *w, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, w->gs);
+    return hipGetLastError();
+}
+
 hipError_t pcg_solve_plain_fp64(const PcgOperator &A, const double *b, double *x, double tol, int maxit, PcgWork *w,
                                 PcgResult *res, hipStream_t s) {
-    void *args[] = {(void *)&A, (void *)&b, (void *)&x, (void *)&w->r, (void *)&w->z, (void *)&w->p, (void *)&w->q,
-                    (void *)&w->partials, (void *)&w->gs, (void *)&tol, (void *)&maxit, (void *)&res};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_plain), dim3(w->nblk), dim3(PCG_BLOCK), args,
-                                      0, s);
+    hipLaunchKernelGGL(k_pcg_plain, dim3(w->nblk), dim3(PCG_BLOCK), 0, s, A, b, x, w->r, w->z, w->p, w->q, w->partials,
+                       w->gs, tol, maxit, res, w->spin_limit);
+    return hipGetLastError();
 }
 
 hipError_t pcg_solve_refined(const PcgOperator &A, const double *b, double *x, double tol, double inner_tol,
                              int maxit, PcgWork *w, PcgResult *res, hipStream_t s) {
-    void *args[] = {(void *)&A, (void *)&b, (void *)&x, (void *)&w->r, (void *)&w->rf, (void *)&w->zf,
-                    (void *)&w->pf, (void *)&w->qf, (void *)&w->df, (void *)&w->partials, (void *)&w->gs,
-                    (void *)&tol, (void *)&inner_tol, (void *)&maxit, (void *)&res};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_pcg_refined), dim3(w->nblk), dim3(PCG_BLOCK),
-                                      args, 0, s);
+    hipLaunchKernelGGL(k_pcg_refined, dim3(w->nblk), dim3(PCG_BLOCK), 0, s, A, b, x, w->r, w->rf, w->zf, w->pf, w->qf,
+                       w->df, w->partials, w->gs, tol, inner_tol, maxit, res, w->spin_limit);
+    return hipGetLastError();
 }
 
 }  // namespace pic

@@ -153,7 +153,7 @@ struct PcgResult {
     int iters;          // total inner iterations
     int outer;          // refinement cycles (1 for the plain path)
     double rel_res;     // final ||b - A x|| / ||b|| in FP64
-    int status;         // 0 converged, 1 hit the iteration limit
+    int status;         // 0 converged, 1 hit the iteration limit, 3 abandoned (a grid barrier wait timed out)
 };
 
 // Operator in padded 5-entry row form (missing entries have col = row, val = 0).
@@ -170,6 +170,8 @@ struct PcgWork;   // device scratch owned by the context
 PcgWork *pcg_work_create(int N, int *err, int want_blocks, hipStream_t stream);
 void pcg_work_destroy(PcgWork *);
 int pcg_grid_blocks(PcgWork *);
+// clears the grid barrier after an abandoned solve (stream-ordered after it)
+hipError_t pcg_work_reset(PcgWork *, hipStream_t);
 
 // x (FP64, warm start, in/out), b (FP64).  Writes the result to *res (device
 // memory) so the solve can run inside a captured graph.
@@ -188,8 +190,9 @@ struct SpecState {
     int done, outer, status, pad;
     double rel;
 };
-// b32 = FP32 right-hand side, converted into b (FP64) by the first residual
-void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const float *b32, double *b, double *x,
+// b32 = right-hand side (FP32 or FP64), converted into b (FP64) by the first residual
+template <typename RB>
+void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const RB *b32, double *b, double *x,
                             double tol, int max_cycles, SpecState *state, double *partials, PcgResult *res,
                             hipStream_t st);
 

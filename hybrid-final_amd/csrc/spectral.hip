@@ -33,8 +33,9 @@ constexpr int GT = 16;   // GEMM output tile edge
 
 // r = b - A x in FP64; per-block partial sums of r.r and b.b.  The first
 // residual of a solve (b32 != null) also starts the solve: it converts the
-// FP32 right-hand side into b and resets the solve state.
-__global__ __launch_bounds__(RES_BLOCK) void k_residual64(PcgOperator A, const float *b32, double *b, const double *x,
+// right-hand side (FP32 or FP64) into b and resets the solve state.
+template <typename RB>
+__global__ __launch_bounds__(RES_BLOCK) void k_residual64(PcgOperator A, const RB *b32, double *b, const double *x,
                                                           double *r, double *partials, SpecState *s) {
     if (b32) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { s->done = 0; s->outer = 0; s->rel = 0.0; s->status = 0; }
@@ -143,7 +144,8 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
 
 }  // namespace
 
-void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const float *b32, double *b, double *x,
+template <typename RB>
+void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const RB *b32, double *b, double *x,
                             double tol, int max_cycles, SpecState *state, double *partials, PcgResult *res,
                             hipStream_t st) {
     const int N = A.N;
@@ -152,7 +154,7 @@ void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const float
     const dim3 blk(GT * GT);
     const SpecState *cs = state;
     for (int k = 0; k <= max_cycles; ++k) {
-        hipLaunchKernelGGL(k_residual64, dim3(nblk), dim3(RES_BLOCK), 0, st, A, k == 0 ? b32 : (const float *)nullptr,
+        hipLaunchKernelGGL(k_residual64<RB>, dim3(nblk), dim3(RES_BLOCK), 0, st, A, k == 0 ? b32 : (const RB *)nullptr,
                            b, (const double *)x, sp.r, partials, state);
         hipLaunchKernelGGL(k_spec_finalize, dim3(1), dim3(RES_BLOCK), 0, st, (const double *)partials, nblk, tol,
                            k == max_cycles ? 1 : 0, state, res);
@@ -168,5 +170,9 @@ void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const float
                            (double *)nullptr, (const double *)nullptr, x, cs);
     }
 }
+template void spectral_solve_refined<float>(const PcgOperator &, const SpecDev &, const float *, double *, double *,
+                                            double, int, SpecState *, double *, PcgResult *, hipStream_t);
+template void spectral_solve_refined<double>(const PcgOperator &, const SpecDev &, const double *, double *, double *,
+                                             double, int, SpecState *, double *, PcgResult *, hipStream_t);
 
 }  // namespace pic

@@ -33,7 +33,8 @@ typedef struct {
     double inner_tol;     /* refined path: relative residual drop per FP32 inner solve (0 = 1e-4) */
     int pcg_maxit;        /* iteration cap per field solve (0 = 100000, the reference's cap) */
     int load;             /* 1 = load the initial particles from the deck (default when opt is NULL) */
-    int field_solver;     /* 0 = fast separable solve when the operator allows it (refined path), 1 = PCG */
+    int field_solver;     /* 0 = fast separable solve when the operator allows it (refined path), 1 = PCG,
+                             2 = fast separable solve when the operator allows it, either precision */
 } pic_options;
 
 typedef struct {
@@ -110,7 +111,7 @@ int pic_last_field_solve(pic_ctx *ctx, int *iters, int *outer, double *rel_res);
  * [7] deposit, [8] accumulators; summed over *steps steps since enabling */
 int pic_stage_timing(pic_ctx *ctx, int enable);
 int pic_stage_times(pic_ctx *ctx, double *ms, int n, long long *steps);
-/* 0 = Jacobi PCG, 1 = separable fast solve (FP32 spectral inverse in FP64 refinement) */
+/* 0 = Jacobi PCG, 1 = separable fast solve (FP64 fast-Poisson inverse in FP64 refinement) */
 int pic_field_solver_kind(pic_ctx *ctx);
 
 /* ---------------- batched Poisson solves ---------------- */

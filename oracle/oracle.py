@@ -188,6 +188,8 @@ class Oracle:
         L.orc_run.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_np.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_set_tstep.argtypes = [C.c_void_p, C.c_long]
+        L.orc_destroy.restype = None
+        L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_voltages.argtypes = [C.c_void_p, C.c_double]
         assert L.orc_sizeof_deck() == C.sizeof(OrcDeck), "orc_deck layout mismatch"
         self.real = np.float32 if precision == "f32" else np.float64
@@ -220,6 +222,18 @@ class Oracle:
 
     def scalar(self, name):
         return self.lib.orc_scalar(self.h, name.encode())
+
+    def close(self):
+        """Frees the C case (the arrays returned by arr() die with it)."""
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def species(self, s):
         n = int(self.scalar(f"np{s}"))

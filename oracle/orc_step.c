@@ -61,6 +61,25 @@ orc_case *orc_create(const orc_deck *d, int *err) {
     return c;
 }
 
+/* Frees a case and everything it owns (free(NULL) is a no-op, so a case
+ * whose setup failed part-way is released too). */
+void orc_destroy(orc_case *c) {
+    int s;
+    if (!c) return;
+    void *own[] = {c->Boundary, c->CondID, c->Face, c->DensRegion, c->Area, c->Temp, c->PlasmaRegion, c->eps_r,
+                   c->eps, c->StructIdx, c->A_idx, c->Ai, c->Aj, c->Aval, c->Atemp_val, c->Mdiag, c->cond_b,
+                   c->temp_b, c->lap_phi, c->BackDen1, c->BackVel1, c->BackDen2, c->BackVel2, c->FluidN,
+                   c->Efield_Flag, c->xe, c->xee, c->cx, c->P1, c->P2, c->Pion, c->vsave, c->tag, c->phi, c->TotPot,
+                   c->Sigma, c->Source, c->Ex, c->Ey, c->den, c->sigma_sp, c->b, c->xsol, c->V_cond, c->CondCharge,
+                   c->Lost, c->mccr, c->sum_den, c->sum_phi, c->sum_ex, c->sum_ey, c->lu};
+    for (s = 0; s < (int)(sizeof own / sizeof own[0]); ++s) free(own[s]);
+    for (s = 0; s < ORC_MAXSP; ++s) {
+        orc_species *p = &c->sp[s];
+        free(p->x); free(p->y); free(p->vx); free(p->vy); free(p->vz); free(p->id); free(p->cell_start);
+    }
+    free(c);
+}
+
 /* ================= field solve: src/cuda_Field.cu:133-181, 639-645 =========
  * The oracle solves A x = b exactly (FP64 banded LU); b comes from the last
  * deposit.  x is scattered to node phi with 0 at eliminated nodes. */

@@ -33,3 +33,10 @@ def test_library_exports_every_declared_function():
 def test_python_mirror_binds():
     L = lib()
     assert L.pic_array_names().decode().split()[0] == "Ex"
+    import picmcc
+    for m in ("advance_time", "field_solve", "efield", "move", "sort_boundary", "mcc", "deposit", "diagnostic",
+              "step", "run_staged", "set_step", "info", "particles", "set_particles", "array", "set_array", "sync",
+              "close", "eedf_sample", "stage_timing", "stage_times", "last_field_solve", "field_solver_kind"):
+        assert callable(getattr(picmcc.Case, m, None)), f"Case.{m} is missing"
+    for m in ("set_rhs", "solve", "sync", "result", "close"):
+        assert callable(getattr(picmcc.PoissonBatch, m, None)), f"PoissonBatch.{m} is missing"

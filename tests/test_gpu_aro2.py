@@ -122,11 +122,63 @@ def test_aro2_collision_step_matches_oracle(prec, name):
              "superelastic": [21, 22, 29, 30, 45], "ionisation Ar": [3, 4], "ionisation O2+": [15, 19, 27],
              "ionisation O+": [18, 26, 34, 42, 44], "dissociative ionisation O+ + O-": [17],
              "attachment": [11, 20, 28], "detachment e+O-": [35], "recombination e": [36],
-             "Ar+ charge exchange": [65], "Ar+ scattering": [66, 67], "O2+ to O+": [52, 55], "O2+ to Ar+": [58],
+             "Ar+ charge exchange": [65], "Ar+ scattering": [66, 67], "O2+ to O+": [52, 55],
              "O2+ kept": [53, 56, 57, 59], "O+ to O2+": [60, 63, 64], "O+ kept": [62],
              "O- detachment": [47, 48, 51], "O- recombination": [49, 50]}
     for kind, rs in kinds.items():
         assert fired[rs].sum() > 0, kind
+
+
+def _keV_o2_plus(c, escale, rng, per_cell):
+    """A synthetic O2+ bucket: per_cell ions in every cell, energies
+    log-uniform over 1-10 keV, isotropic, ids 0..n-1."""
+    inf = c.info()
+    ngy = inf.ngy
+    counts = np.zeros(inf.ngx * ngy, dtype=np.int64)
+    for i in range(inf.ngx - 1):
+        counts[i * ngy: i * ngy + ngy - 1] = per_cell
+    cs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    n = int(cs[-1])
+    e = np.exp(rng.uniform(np.log(1e3), np.log(1e4), n))
+    sp = np.sqrt(e / escale)
+    mu = rng.uniform(-1, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    st = np.sqrt(1 - mu * mu)
+    real = c.real
+    p = dict(x=rng.random(n).astype(real), y=rng.random(n).astype(real), vx=(sp * st * np.cos(ph)).astype(real),
+             vy=(sp * st * np.sin(ph)).astype(real), vz=(sp * mu).astype(real), id=np.arange(n, dtype=np.uint32),
+             cell_start=cs)
+    c.set_particles(2, p, n)
+
+
+@pytest.mark.parametrize("prec,name", PRECS)
+def test_aro2_o2_ion_charge_transfer_to_argon(prec, name):
+    """O2+ + Ar -> Ar+ + O2 (reaction 58) has a thousandth of the cross
+    section of its partner 59 at every energy, so the mixed test above sees
+    it about once in ten runs.  Here 19 200 O2+ of 1-10 keV (where the
+    collision rate is near its null-collision maximum) cross 1 Torr of 95 %
+    argon with dt = T_RF/40 for three steps: reaction 58 fires (14 times in
+    FP32 for this seed) and every bucket, id and count still equals the
+    oracle's."""
+    deck = _deck(o2_ratio=0.05, pres_torr=1.0, steps_per_cycle=40, ncx=16, ncy=12, ppc=100)
+    for sp in ("O+", "Ar+"):   # room for the charge-exchange products of three steps
+        deck["GasSpecies"]["ChargeSpecies"][sp]["Max_np"] = 60000
+    deck["SimulationMethod"]["ParticleLimit"] = 60000
+    hs = HostSetup(deck, prec, load=False)
+    c = Case(deck, prec)
+    rng = np.random.default_rng(58)
+    fired = np.zeros(T, dtype=np.int64)
+    try:
+        for it in range(3):
+            c.set_step(100 + it)
+            _keV_o2_plus(c, hs.scalar("Escale2"), rng, 100)
+            _, _, d = _check_mcc(c, hs, deck, name, den_scale=1.0)
+            fired += d.sum(axis=0)
+    finally:
+        c.close()
+        hs.close()
+    print("O2+ events 52..59", fired[52:60].tolist(), flush=True)
+    assert fired[58] > 0 and fired[59] > 1000
 
 
 @pytest.mark.parametrize("prec,name", PRECS)

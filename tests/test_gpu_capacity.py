@@ -4,11 +4,12 @@ dt = T_RF/60 at 0.36 Torr and 170 V makes this small argon discharge run
 away (the CPU oracle overflows its capacity at step 19).  The library must
 report the overflow (-30) at the next synchronisation, keep every kernel
 inside the particle arrays (cell offsets past the capacity are clamped),
-and still return at most `capacity` particles.
+and still return at most `capacity` particles.  A field solve abandoned at a
+timed-out grid barrier is reported the same way (-34).
 """
 import pytest
 
-from picmcc import Case, FP32_REFINED, PicError, decks
+from picmcc import Case, FP32_REFINED, FP64_PLAIN, PicError, decks
 
 pytestmark = pytest.mark.gpu
 
@@ -27,3 +28,34 @@ def test_capacity_overflow_is_reported_and_contained():
         p = c.particles(s)
         assert len(p["x"]) == inf.np[s]
     c.close()
+
+
+def test_abandoned_field_solve_is_reported_and_cleared(monkeypatch):
+    """The persistent solver's grid barrier waits are bounded.  With the
+    limit forced to zero from the host (PIC_PCG_SPIN_US, read when the case
+    is created), a multi-block FP64 PCG solve abandons at its first wait:
+    every block leaves the kernel, pic_sync reports -34 and clears the
+    barrier, and the next solve on the same case ends the same way instead of
+    hanging.  A case created without the override solves normally."""
+    d = decks.small(ncx=24, ncy=24, ppc=10, volts=150.0)
+    monkeypatch.setenv("PIC_PCG_SPIN_US", "0")
+    c = Case(d, FP64_PLAIN)
+    monkeypatch.delenv("PIC_PCG_SPIN_US")
+    try:
+        for _ in range(2):
+            c.advance_time()
+            c.field_solve()
+            with pytest.raises(PicError) as e:
+                c.sync()
+            assert e.value.code == -34
+    finally:
+        c.close()
+    ok = Case(d, FP64_PLAIN)
+    try:
+        ok.advance_time()
+        ok.field_solve()
+        ok.sync()
+        it, outer, rel = ok.last_field_solve()
+        assert it > 0 and rel <= 1e-10
+    finally:
+        ok.close()

@@ -1,22 +1,22 @@
 """Size-independent checks of the fused step at the full 256x256 argon
-workload (26 M particles), where the CPU oracle is too slow to follow.
+workload (26 M particles), where the CPU oracle is too slow to follow, on
+both paths (FP32 with the FP64-refined field solve; the FP64 plain path
+with the reference's Jacobi PCG in FP64).
 
-After 20 fused steps (graph replays):
+After 20 (FP32) or 8 (FP64) fused steps (graph replays):
 - the particle balance closes exactly for both species:
   N = N0 + ionisations - absorbed - deleted at walls;
 - the buckets are well formed: cell offsets non-decreasing, the last equal to
   the particle count, every in-cell offset in [0, 1);
 - particle ids are unique per species and below the species' id counter;
-- the same 20 steps run kernel by kernel (no graph) give the same bits.
+- the same steps run kernel by kernel (no graph) give the same bits.
 """
 import numpy as np
 import pytest
 
-from picmcc import Case, FP32_REFINED, decks
+from picmcc import Case, FP32_REFINED, FP64_PLAIN, decks
 
 pytestmark = pytest.mark.gpu
-
-STEPS = 20
 
 
 def counts(c):
@@ -28,9 +28,10 @@ def counts(c):
     return [int(inf.np[s]) for s in range(inf.nsp)], pairs, absorbed, lost
 
 
-def test_full_size_invariants_and_graph_replay():
+@pytest.mark.parametrize("prec,STEPS", [(FP32_REFINED, 20), (FP64_PLAIN, 8)], ids=["fp32_refined", "fp64_plain"])
+def test_full_size_invariants_and_graph_replay(prec, STEPS):
     d = decks.config2()
-    g = Case(d, FP32_REFINED)
+    g = Case(d, prec)
     n0, _, _, _ = counts(g)
     g.step(STEPS)
     g.sync()
@@ -51,7 +52,7 @@ def test_full_size_invariants_and_graph_replay():
         assert int(ids.max()) < int(nid[s])
     g.close()
     # the same steps launched kernel by kernel (stage timers force the eager path)
-    e = Case(d, FP32_REFINED)
+    e = Case(d, prec)
     e.stage_timing(True)
     e.step(STEPS)
     e.sync()

@@ -90,15 +90,20 @@ DECK_IDS = ["small", "dense", "dielectric", "inner_conductor", "tall", "anisotro
 @pytest.mark.parametrize("deck", DECKS, ids=DECK_IDS)
 @pytest.mark.parametrize("prec,pn", PRECS)
 def test_stage_by_stage(prec, pn, deck):
-    d = deck()
-    case = Case(d, prec)
+    stage_parity(deck(), prec, pn, 3)
+
+
+def stage_parity(d, prec, pn, steps, field_solver=0):
+    """`steps` timesteps of deck d, each stage run on the GPU and in the
+    oracle from the same state and compared."""
+    case = Case(d, prec, field_solver=field_solver)
     orc = Oracle(d, pn)
     tol = 1e-5 if pn == "f32" else 1e-10
     for s in range(2):
         same_particles(case, orc, s, "load")
     orc.orc_deposit()
     np.testing.assert_allclose(case.array("den"), orc.arr("den"), rtol=1e-5, atol=1e-4)
-    for step in range(1, 4):
+    for step in range(1, steps + 1):
         case.advance_time()
         orc.orc_set_tstep(step)
         # field solve: GPU PCG vs FP64 direct solve of the same rhs
@@ -152,6 +157,8 @@ def test_stage_by_stage(prec, pn, deck):
         close(case.array("sigma_sp"), orc.arr("sigma_sp"), tol, f"sigma_sp after deposit, step {step}")
         close(case.array("Sigma"), orc.arr("Sigma"), tol, f"Sigma after deposit, step {step}")
     case.sync()
+    case.close()
+    orc.close()
 
 
 @pytest.mark.parametrize("deck", DECKS, ids=DECK_IDS)
@@ -210,3 +217,29 @@ def test_graph_steps_equal_eager_steps():
         a, b = g.particles(s), e.particles(s)
         for k in FIELDS:
             assert np.array_equal(a[k], b[k]), f"after staged step: species {s} field {k} differs"
+
+
+def test_fp64_with_the_separable_field_solve():
+    """The FP64 path with the separable FP64 field solve (the solver of the
+    FP32 path, chosen with field_solver = 2): stage by stage against the
+    oracle, and its graph-replayed fused steps equal the same steps launched
+    kernel by kernel."""
+    d = deck_small()
+    stage_parity(d, FP64_PLAIN, "f64", 3, field_solver=2)
+    g = Case(d, FP64_PLAIN, field_solver=2)
+    e = Case(d, FP64_PLAIN, field_solver=2)
+    assert g.field_solver_kind() == 1
+    e.stage_timing(True)
+    for rep in range(4):
+        g.step(1)
+        e.step(1)
+        g.sync()
+        e.sync()
+        for s in range(2):
+            a, b = g.particles(s), e.particles(s)
+            for k in FIELDS:
+                assert np.array_equal(a[k], b[k]), f"replay {rep}: species {s} field {k} differs"
+        for nm in ("phi", "Ex", "Ey", "den", "sum_den", "mccr"):
+            assert np.array_equal(g.array(nm), e.array(nm)), f"replay {rep}: {nm} differs"
+    g.close()
+    e.close()

@@ -37,6 +37,13 @@ def test_c_driver_compiles_and_links(tmp_path):
         assert name in out
 
 
+def _drive(args):
+    """Runs the C driver; its own output is shown when it fails."""
+    r = subprocess.run(args, timeout=120, capture_output=True, text=True)
+    assert r.returncode == 0, f"driver exit {r.returncode}\nstdout:\n{r.stdout}\nstderr:\n{r.stderr}"
+    return r
+
+
 def _read(path, nsp, G, real):
     with open(path, "rb") as f:
         buf = f.read()
@@ -92,7 +99,7 @@ def test_stage_table_from_c_equals_run_staged(tmp_path, gas, prec):
     out = tmp_path / "state.bin"
     nsteps = 5
     print("C driver, staged:", gas, prec, flush=True)
-    subprocess.run([exe, str(path), str(prec), "staged", str(nsteps), str(out)], check=True, timeout=120)
+    _drive([exe, str(path), str(prec), "staged", str(nsteps), str(out)])
     c = Case(deck, prec)
     try:
         c.run_staged(nsteps)
@@ -109,13 +116,14 @@ def test_stage_table_from_c_equals_run_staged(tmp_path, gas, prec):
 def test_main_cuda_from_c_equals_pic_step(tmp_path, prec):
     """main_cuda runs RFCycles cycles of TimeStep_PIC fused steps."""
     exe = _build(tmp_path)
-    deck = decks.small(ncx=16, ncy=12, ppc=30, volts=150.0, steps_per_cycle=100, cycles=2)
+    # dt = T_RF/200 for one cycle: the case stays within its particle capacity
+    # (at T_RF/100 this small discharge runs away and overflows it, -30)
+    deck = decks.small(ncx=16, ncy=12, ppc=30, volts=150.0, steps_per_cycle=200, cycles=1)
     path = tmp_path / "deck.json"
     path.write_text(json.dumps(deck))
     out = tmp_path / "state.bin"
     print("C driver, main_cuda:", prec, flush=True)
-    r = subprocess.run([exe, str(path), str(prec), "main", "0", str(out)], check=True, timeout=120,
-                       capture_output=True, text=True)
+    r = _drive([exe, str(path), str(prec), "main", "0", str(out)])
     assert "tstep 200" in r.stdout
     c = Case(deck, prec)
     try:
