@@ -16,8 +16,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <mutex>
 #include <cstdio>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <string>
 #include <tuple>
@@ -247,13 +250,32 @@ static int pb_destroy(pic_pbatch *B) {
     return 0;
 }
 
-extern "C" {
+// Host-side tables of a batch: the problems, their GEMM tiles and residual
+// blocks, and the matrices each problem reads (eigenbases shared between
+// problems with the same size and ends, and each problem's 1/(lx + ly)).
+// ProbDev's matrix pointers are filled by bind() once the matrices are on
+// the device.
+struct PbPlan {
+    std::vector<ProbDev> probs;
+    std::vector<std::vector<double>> mats;
+    std::vector<std::array<int, 5>> refs;   // per problem: Vx, Vxi, VyT, VyiT, dinv (indices into mats)
+    std::vector<GemmTask> tasks;
+    std::vector<int2> blocks;
+    std::vector<int> bstart;
+    long long N = 0;
+    void bind(const std::vector<const double *> &dev) {
+        for (size_t p = 0; p < probs.size(); ++p) {
+            probs[p].Vx = dev[refs[p][0]];
+            probs[p].Vxi = dev[refs[p][1]];
+            probs[p].VyT = dev[refs[p][2]];
+            probs[p].VyiT = dev[refs[p][3]];
+            probs[p].dinv = dev[refs[p][4]];
+        }
+    }
+};
 
-const char *pic_pbatch_last_error(void) { return g_pb_err.c_str(); }
-
-int pic_pbatch_create(int nprob, const int *nx, const int *ny, const int *bc, int device, pic_pbatch **out) {
-    if (!out || nprob <= 0 || !nx || !ny || !bc) { g_pb_err = "invalid arguments"; return -2; }
-    *out = nullptr;
+static int pb_plan(int nprob, const int *nx, const int *ny, const int *bc, PbPlan &pl) {
+    if (nprob <= 0 || !nx || !ny || !bc) { g_pb_err = "invalid arguments"; return -2; }
     for (int p = 0; p < nprob; ++p) {
         if (nx[p] < 1 || ny[p] < 1) { g_pb_err = "problem sizes must be positive"; return -2; }
         const int *c = bc + 4 * p;
@@ -261,6 +283,119 @@ int pic_pbatch_create(int nprob, const int *nx, const int *ny, const int *bc, in
             if (c[s] != 0 && c[s] != 1) { g_pb_err = "boundary codes are 0 (Neumann) or 1 (Dirichlet)"; return -2; }
         if (!(c[0] || c[1] || c[2] || c[3])) { g_pb_err = "a pure Neumann problem is singular"; return -2; }
     }
+    std::map<std::tuple<int, int, int, int>, std::pair<int, int>> cache;   // (axis, n, ends) -> (V, Vi)
+    auto basis = [&](int axis, int n, int l, int r, int &iv, int &ivi, std::vector<double> &lam) {
+        std::vector<double> v, vi;
+        eigen_1d(n, l, r, v, vi, lam);
+        const auto key = std::make_tuple(axis, n, l, r);
+        const auto it = cache.find(key);
+        if (it != cache.end()) { iv = it->second.first; ivi = it->second.second; return; }
+        if (axis == 1) {   // the y bases are used as V^T and (V^-1)^T
+            std::vector<double> vt((size_t)n * n), vit((size_t)n * n);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) { vt[(size_t)i * n + j] = v[(size_t)j * n + i]; vit[(size_t)i * n + j] = vi[(size_t)j * n + i]; }
+            v.swap(vt);
+            vi.swap(vit);
+        }
+        iv = (int)pl.mats.size();
+        pl.mats.push_back(std::move(v));
+        ivi = (int)pl.mats.size();
+        pl.mats.push_back(std::move(vi));
+        cache[key] = {iv, ivi};
+    };
+    long long off = 0;
+    pl.bstart.assign(nprob + 1, 0);
+    for (int p = 0; p < nprob; ++p) {
+        ProbDev q{};
+        q.nx = nx[p]; q.ny = ny[p]; q.off = off;
+        for (int s = 0; s < 4; ++s) q.bc[s] = bc[4 * p + s];
+        std::vector<double> lx, ly;
+        std::array<int, 5> ref{};
+        basis(0, q.nx, q.bc[0], q.bc[1], ref[0], ref[1], lx);
+        basis(1, q.ny, q.bc[2], q.bc[3], ref[2], ref[3], ly);
+        std::vector<double> dinv((size_t)q.nx * q.ny);
+        for (int a = 0; a < q.nx; ++a)
+            for (int c = 0; c < q.ny; ++c) {
+                const double d = lx[a] + ly[c];
+                dinv[(size_t)a * q.ny + c] = d > 1e-300 ? 1.0 / d : 0.0;
+            }
+        ref[4] = (int)pl.mats.size();
+        pl.mats.push_back(std::move(dinv));
+        for (int r0 = 0; r0 < q.nx; r0 += GT)
+            for (int c0 = 0; c0 < q.ny; c0 += GT) pl.tasks.push_back(GemmTask{p, r0, c0, 0});
+        pl.bstart[p] = (int)pl.blocks.size();
+        for (int i0 = 0; i0 < q.nx * q.ny; i0 += RB) pl.blocks.push_back(make_int2(p, i0));
+        pl.probs.push_back(q);
+        pl.refs.push_back(ref);
+        off += (long long)q.nx * q.ny;
+    }
+    pl.bstart[nprob] = (int)pl.blocks.size();
+    pl.N = off;
+    return 0;
+}
+
+// Work space of pic_pbatch_solve_noalloc: one array per device, reached
+// from host code only through the address hipGetSymbolAddress returns for
+// the current device (the symbol's name is not a device address on the
+// host).  Every call on a device shares it.
+constexpr size_t PB_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[PB_SCRATCH_BYTES];
+constexpr int PB_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[PB_MAX_DEVICES];
+
+// address of this device's scratch array (looked up once per device)
+static int pb_scratch(int device, unsigned char **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            g_pb_err = std::string("cannot locate the solver work space: ") + hipGetErrorString(e);
+            return -1;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = (unsigned char *)g_scratch_ptr[device];
+    return 0;
+}
+
+// The launch sequence of one batched solve from x = 0: residual and
+// convergence test per cycle, then the four GEMMs of the fast inverse on the
+// problems not yet converged.  B supplies the problem tables and work
+// vectors; b and x are the right-hand sides and the solution.
+static void pb_enqueue(const pic_pbatch &B, const double *b, double *x, double tol, int max_cycles, hipStream_t st) {
+    const int tb = 256;
+    hipLaunchKernelGGL(k_pb_begin, dim3((B.nprob + tb - 1) / tb), dim3(tb), 0, st, B.nprob, B.done, B.status);
+    hipLaunchKernelGGL(k_pb_zero, dim3((unsigned)((B.N + tb - 1) / tb)), dim3(tb), 0, st, B.N, x);
+    for (int k = 0; k <= max_cycles; ++k) {
+        hipLaunchKernelGGL(k_pb_residual, dim3(B.nblk), dim3(RB), 0, st, (const ProbDev *)B.P,
+                           (const int2 *)B.blocks, b, (const double *)x, B.r, B.part, (const int *)B.done);
+        hipLaunchKernelGGL(k_pb_finalize, dim3((B.nprob + tb - 1) / tb), dim3(tb), 0, st, B.nprob,
+                           (const int *)B.bstart, (const double *)B.part, tol, k == max_cycles ? 1 : 0, B.done,
+                           B.rel, B.status);
+        if (k == max_cycles) break;
+        const dim3 g(B.ntask), blk(GT * GT);
+        hipLaunchKernelGGL(k_pb_gemm<0>, g, blk, 0, st, (const ProbDev *)B.P, (const GemmTask *)B.tasks,
+                           (const double *)B.r, B.T1, B.T2, x, (const int *)B.done);
+        hipLaunchKernelGGL(k_pb_gemm<1>, g, blk, 0, st, (const ProbDev *)B.P, (const GemmTask *)B.tasks,
+                           (const double *)B.r, B.T1, B.T2, x, (const int *)B.done);
+        hipLaunchKernelGGL(k_pb_gemm<2>, g, blk, 0, st, (const ProbDev *)B.P, (const GemmTask *)B.tasks,
+                           (const double *)B.r, B.T1, B.T2, x, (const int *)B.done);
+        hipLaunchKernelGGL(k_pb_gemm<3>, g, blk, 0, st, (const ProbDev *)B.P, (const GemmTask *)B.tasks,
+                           (const double *)B.r, B.T1, B.T2, x, (const int *)B.done);
+    }
+}
+
+extern "C" {
+
+const char *pic_pbatch_last_error(void) { return g_pb_err.c_str(); }
+
+int pic_pbatch_create(int nprob, const int *nx, const int *ny, const int *bc, int device, pic_pbatch **out) {
+    if (!out) { g_pb_err = "invalid arguments"; return -2; }
+    *out = nullptr;
+    PbPlan pl;
+    if (const int rc = pb_plan(nprob, nx, ny, bc, pl)) return rc;
     pic_pbatch *B = new pic_pbatch();
     B->nprob = nprob;
     B->dev = device;
@@ -269,75 +404,22 @@ int pic_pbatch_create(int nprob, const int *nx, const int *ny, const int *bc, in
         pb_destroy(B);
         return -1;
     }
-    // eigenbases, shared between problems with the same (n, ends)
-    std::map<std::tuple<int, int, int>, std::pair<const double *, const double *>> cache_x, cache_y;
-    std::map<std::tuple<int, int, int>, std::vector<double>> lam_cache;
-    auto basis = [&](int n, int l, int r, bool transposed, const double *&V, const double *&Vi,
-                     std::vector<double> &lam) -> int {
-        auto key = std::make_tuple(n, l, r);
-        auto &cache = transposed ? cache_y : cache_x;
-        auto it = cache.find(key);
-        std::vector<double> v, vi, lm;
-        eigen_1d(n, l, r, v, vi, lm);
-        lam = lm;
-        if (it != cache.end()) { V = it->second.first; Vi = it->second.second; return 0; }
-        if (transposed) {   // the y bases are used as V^T and (V^-1)^T
-            std::vector<double> vt((size_t)n * n), vit((size_t)n * n);
-            for (int i = 0; i < n; ++i)
-                for (int j = 0; j < n; ++j) { vt[(size_t)i * n + j] = v[(size_t)j * n + i]; vit[(size_t)i * n + j] = vi[(size_t)j * n + i]; }
-            v.swap(vt);
-            vi.swap(vit);
-        }
-        double *dV, *dVi;
-        if (pb_upload(B, &dV, v) != hipSuccess || pb_upload(B, &dVi, vi) != hipSuccess) return -1;
-        cache[key] = {dV, dVi};
-        V = dV;
-        Vi = dVi;
-        return 0;
-    };
-    long long off = 0;
-    std::vector<GemmTask> tasks;
-    std::vector<int2> blocks;
-    std::vector<int> bstart(nprob + 1, 0);
-    for (int p = 0; p < nprob; ++p) {
-        ProbDev q{};
-        q.nx = nx[p]; q.ny = ny[p]; q.off = off;
-        for (int s = 0; s < 4; ++s) q.bc[s] = bc[4 * p + s];
-        std::vector<double> lx, ly;
-        const double *Vxi = nullptr, *VyiT = nullptr;
-        if (basis(q.nx, q.bc[0], q.bc[1], false, q.Vx, Vxi, lx) || basis(q.ny, q.bc[2], q.bc[3], true, q.VyT, VyiT, ly)) {
-            g_pb_err = "device allocation failed";
-            pb_destroy(B);
-            return -1;
-        }
-        q.Vxi = Vxi;
-        q.VyiT = VyiT;
-        std::vector<double> dinv((size_t)q.nx * q.ny);
-        for (int a = 0; a < q.nx; ++a)
-            for (int c = 0; c < q.ny; ++c) {
-                const double d = lx[a] + ly[c];
-                dinv[(size_t)a * q.ny + c] = d > 1e-300 ? 1.0 / d : 0.0;
-            }
-        double *dd;
-        if (pb_upload(B, &dd, dinv) != hipSuccess) { g_pb_err = "device allocation failed"; pb_destroy(B); return -1; }
-        q.dinv = dd;
-        for (int r0 = 0; r0 < q.nx; r0 += GT)
-            for (int c0 = 0; c0 < q.ny; c0 += GT) tasks.push_back(GemmTask{p, r0, c0, 0});
-        bstart[p] = (int)blocks.size();
-        for (int i0 = 0; i0 < q.nx * q.ny; i0 += RB) blocks.push_back(make_int2(p, i0));
-        B->host.push_back(q);
-        B->offs.push_back(off);
-        off += (long long)q.nx * q.ny;
+    std::vector<const double *> mdev(pl.mats.size());
+    for (size_t m = 0; m < pl.mats.size(); ++m) {
+        double *d;
+        if (pb_upload(B, &d, pl.mats[m]) != hipSuccess) { g_pb_err = "device allocation failed"; pb_destroy(B); return -1; }
+        mdev[m] = d;
     }
-    bstart[nprob] = (int)blocks.size();
-    B->N = off;
-    B->ntask = (int)tasks.size();
-    B->nblk = (int)blocks.size();
-    if (pb_upload(B, &B->P, B->host) != hipSuccess || pb_upload(B, &B->tasks, tasks) != hipSuccess ||
-        pb_upload(B, &B->blocks, blocks) != hipSuccess || pb_upload(B, &B->bstart, bstart) != hipSuccess ||
-        pb_alloc(B, &B->b, off) != hipSuccess || pb_alloc(B, &B->x, off) != hipSuccess ||
-        pb_alloc(B, &B->r, off) != hipSuccess || pb_alloc(B, &B->T1, off) != hipSuccess ||
-        pb_alloc(B, &B->T2, off) != hipSuccess || pb_alloc(B, &B->part, 2 * (size_t)B->nblk) != hipSuccess ||
+    pl.bind(mdev);
+    B->host = pl.probs;
+    B->N = pl.N;
+    B->ntask = (int)pl.tasks.size();
+    B->nblk = (int)pl.blocks.size();
+    if (pb_upload(B, &B->P, pl.probs) != hipSuccess || pb_upload(B, &B->tasks, pl.tasks) != hipSuccess ||
+        pb_upload(B, &B->blocks, pl.blocks) != hipSuccess || pb_upload(B, &B->bstart, pl.bstart) != hipSuccess ||
+        pb_alloc(B, &B->b, pl.N) != hipSuccess || pb_alloc(B, &B->x, pl.N) != hipSuccess ||
+        pb_alloc(B, &B->r, pl.N) != hipSuccess || pb_alloc(B, &B->T1, pl.N) != hipSuccess ||
+        pb_alloc(B, &B->T2, pl.N) != hipSuccess || pb_alloc(B, &B->part, 2 * (size_t)B->nblk) != hipSuccess ||
         pb_alloc(B, &B->rel, nprob) != hipSuccess || pb_alloc(B, &B->done, nprob) != hipSuccess ||
         pb_alloc(B, &B->status, nprob) != hipSuccess) {
         g_pb_err = "device allocation failed";
@@ -363,28 +445,7 @@ int pic_pbatch_set_rhs(pic_pbatch *B, const double *b) {
 // enqueued on the batch's stream (no host synchronisation).
 int pic_pbatch_solve(pic_pbatch *B, double tol, int max_cycles) {
     if (!B || tol <= 0 || max_cycles < 1) { g_pb_err = "invalid arguments"; return -2; }
-    hipStream_t st = B->st;
-    const int tb = 256;
-    hipLaunchKernelGGL(k_pb_begin, dim3((B->nprob + tb - 1) / tb), dim3(tb), 0, st, B->nprob, B->done, B->status);
-    hipLaunchKernelGGL(k_pb_zero, dim3((unsigned)((B->N + tb - 1) / tb)), dim3(tb), 0, st, B->N, B->x);
-    for (int k = 0; k <= max_cycles; ++k) {
-        hipLaunchKernelGGL(k_pb_residual, dim3(B->nblk), dim3(RB), 0, st, (const ProbDev *)B->P,
-                           (const int2 *)B->blocks, (const double *)B->b, (const double *)B->x, B->r, B->part,
-                           (const int *)B->done);
-        hipLaunchKernelGGL(k_pb_finalize, dim3((B->nprob + tb - 1) / tb), dim3(tb), 0, st, B->nprob,
-                           (const int *)B->bstart, (const double *)B->part, tol, k == max_cycles ? 1 : 0, B->done,
-                           B->rel, B->status);
-        if (k == max_cycles) break;
-        const dim3 g(B->ntask), blk(GT * GT);
-        hipLaunchKernelGGL(k_pb_gemm<0>, g, blk, 0, st, (const ProbDev *)B->P, (const GemmTask *)B->tasks,
-                           (const double *)B->r, B->T1, B->T2, B->x, (const int *)B->done);
-        hipLaunchKernelGGL(k_pb_gemm<1>, g, blk, 0, st, (const ProbDev *)B->P, (const GemmTask *)B->tasks,
-                           (const double *)B->r, B->T1, B->T2, B->x, (const int *)B->done);
-        hipLaunchKernelGGL(k_pb_gemm<2>, g, blk, 0, st, (const ProbDev *)B->P, (const GemmTask *)B->tasks,
-                           (const double *)B->r, B->T1, B->T2, B->x, (const int *)B->done);
-        hipLaunchKernelGGL(k_pb_gemm<3>, g, blk, 0, st, (const ProbDev *)B->P, (const GemmTask *)B->tasks,
-                           (const double *)B->r, B->T1, B->T2, B->x, (const int *)B->done);
-    }
+    pb_enqueue(*B, B->b, B->x, tol, max_cycles, B->st);
     PBCHK(hipGetLastError());
     return 0;
 }
@@ -408,6 +469,84 @@ int pic_pbatch_sync(pic_pbatch *B) {
     if (!B) { g_pb_err = "invalid arguments"; return -2; }
     PBCHK(hipStreamSynchronize(B->st));
     return 0;
+}
+
+// The batch of pic_pbatch_create + pic_pbatch_solve + pic_pbatch_result in
+// one call that allocates no device memory: tables, eigenbases and work
+// vectors are laid out in the device's scratch array (PB_SCRATCH_BYTES),
+// right-hand sides and solution are the caller's device arrays.  The work
+// is ordered on `stream` and the call waits for it.  Calls that overlap on
+// one device race on the shared array.
+int pic_pbatch_solve_noalloc(int nprob, const int *nx, const int *ny, const int *bc, int device, const double *b_dev,
+                             double *x_dev, double tol, int max_cycles, double *rel_res, void *stream) {
+    if (!b_dev || !x_dev || tol <= 0 || max_cycles < 1) { g_pb_err = "invalid arguments"; return -2; }
+    if (device < 0 || device >= PB_MAX_DEVICES) { g_pb_err = "device ordinal out of range"; return -2; }
+    PbPlan pl;
+    if (const int rc = pb_plan(nprob, nx, ny, bc, pl)) return rc;
+    // layout: tables first, then matrices, then the work vectors; every
+    // region starts on a 256-byte boundary
+    size_t at = 0;
+    auto region = [&](size_t bytes) { const size_t o = at; at += (bytes + 255) & ~(size_t)255; return o; };
+    const size_t oP = region(pl.probs.size() * sizeof(ProbDev));
+    const size_t oT = region(pl.tasks.size() * sizeof(GemmTask));
+    const size_t oB = region(pl.blocks.size() * sizeof(int2));
+    const size_t oS = region(pl.bstart.size() * sizeof(int));
+    std::vector<size_t> oM(pl.mats.size());
+    for (size_t m = 0; m < pl.mats.size(); ++m) oM[m] = region(pl.mats[m].size() * sizeof(double));
+    const size_t host_bytes = at;   // everything above is copied from the host
+    const size_t oR = region((size_t)pl.N * sizeof(double));
+    const size_t o1 = region((size_t)pl.N * sizeof(double));
+    const size_t o2 = region((size_t)pl.N * sizeof(double));
+    const size_t oPart = region(2 * pl.blocks.size() * sizeof(double));
+    const size_t oRel = region((size_t)nprob * sizeof(double));
+    const size_t oDone = region((size_t)nprob * sizeof(int));
+    const size_t oStat = region((size_t)nprob * sizeof(int));
+    if (at > PB_SCRATCH_BYTES) {
+        g_pb_err = "batch needs " + std::to_string(at) + " bytes of work space, more than the " +
+                   std::to_string(PB_SCRATCH_BYTES) + " of the no-allocation solve";
+        return -3;
+    }
+    PBCHK(hipSetDevice(device));
+    unsigned char *base = nullptr;
+    if (pb_scratch(device, &base)) return -1;
+    std::vector<const double *> mdev(pl.mats.size());
+    for (size_t m = 0; m < pl.mats.size(); ++m) mdev[m] = (const double *)(base + oM[m]);
+    pl.bind(mdev);
+    std::vector<unsigned char> host(host_bytes, 0);
+    auto put = [&](size_t o, const void *src, size_t n) { if (n) memcpy(host.data() + o, src, n); };
+    put(oP, pl.probs.data(), pl.probs.size() * sizeof(ProbDev));
+    put(oT, pl.tasks.data(), pl.tasks.size() * sizeof(GemmTask));
+    put(oB, pl.blocks.data(), pl.blocks.size() * sizeof(int2));
+    put(oS, pl.bstart.data(), pl.bstart.size() * sizeof(int));
+    for (size_t m = 0; m < pl.mats.size(); ++m) put(oM[m], pl.mats[m].data(), pl.mats[m].size() * sizeof(double));
+    const hipStream_t st = (hipStream_t)stream;
+    PBCHK(hipMemcpyAsync(base, host.data(), host_bytes, hipMemcpyHostToDevice, st));
+    pic_pbatch B;   // a view of the scratch layout (owns nothing)
+    B.nprob = nprob;
+    B.N = pl.N;
+    B.dev = device;
+    B.P = (ProbDev *)(base + oP);
+    B.tasks = (GemmTask *)(base + oT);
+    B.ntask = (int)pl.tasks.size();
+    B.blocks = (int2 *)(base + oB);
+    B.nblk = (int)pl.blocks.size();
+    B.bstart = (int *)(base + oS);
+    B.r = (double *)(base + oR);
+    B.T1 = (double *)(base + o1);
+    B.T2 = (double *)(base + o2);
+    B.part = (double *)(base + oPart);
+    B.rel = (double *)(base + oRel);
+    B.done = (int *)(base + oDone);
+    B.status = (int *)(base + oStat);
+    pb_enqueue(B, b_dev, x_dev, tol, max_cycles, st);
+    PBCHK(hipGetLastError());
+    std::vector<int> stt(nprob);
+    if (rel_res) PBCHK(hipMemcpyAsync(rel_res, B.rel, (size_t)nprob * sizeof(double), hipMemcpyDeviceToHost, st));
+    PBCHK(hipMemcpyAsync(stt.data(), B.status, (size_t)nprob * sizeof(int), hipMemcpyDeviceToHost, st));
+    PBCHK(hipStreamSynchronize(st));
+    int bad = 0;
+    for (int v : stt) bad += v != 0;
+    return bad;
 }
 
 }  // extern "C"

@@ -82,6 +82,8 @@ EXPORTS = {
     "pic_pbatch_sync": (C.c_int, [C.c_void_p]),
     "pic_pbatch_result": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pic_pbatch_last_error": (C.c_char_p, []),
+    "pic_pbatch_solve_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "pic_setup_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "pic_setup_array_info": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
@@ -356,3 +358,23 @@ class PoissonBatch:
             self.close()
         except Exception:
             pass
+
+
+def pbatch_solve_noalloc(sizes, bcs, b_dev, x_dev, tol=1e-10, max_cycles=3, device=0, stream=None):
+    """The batched solve of PoissonBatch in one call that allocates no device
+    memory (include/picmcc.h, pic_pbatch_solve_noalloc): b_dev and x_dev are
+    device addresses (ints) of the right-hand sides and the solution, laid
+    out as for PoissonBatch.  Returns (per-problem relative residuals, number
+    of problems above tol)."""
+    n = len(sizes)
+    nx = np.array([int(a) for a, _ in sizes], dtype=np.int32)
+    ny = np.array([int(b) for _, b in sizes], dtype=np.int32)
+    bc = np.ascontiguousarray(np.asarray(bcs, dtype=np.int32).reshape(n, 4))
+    rel = np.empty(n, dtype=np.float64)
+    L = lib()
+    rc = L.pic_pbatch_solve_noalloc(n, _ptr(nx), _ptr(ny), _ptr(bc), int(device), C.c_void_p(int(b_dev)),
+                                    C.c_void_p(int(x_dev)), float(tol), int(max_cycles), _ptr(rel),
+                                    C.c_void_p(int(stream) if stream else 0))
+    if rc < 0:
+        raise PicError(rc, L.pic_pbatch_last_error().decode())
+    return rel, rc

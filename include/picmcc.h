@@ -132,6 +132,20 @@ int pic_pbatch_sync(pic_pbatch *b);
  * returns the number of problems above tol, or < 0 on error */
 int pic_pbatch_result(pic_pbatch *b, double *x, double *rel_res);
 const char *pic_pbatch_last_error(void);
+/* pic_pbatch_create + pic_pbatch_solve + pic_pbatch_result in one call that
+ * allocates no device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  b_dev and x_dev are device arrays of
+ * all unknowns, laid out as for pic_pbatch_create; rel_res (host, nprob
+ * entries) may be NULL; stream is a hipStream_t (NULL = the device's null
+ * stream).  Tables, eigenbases and work vectors live in a fixed 64 MiB
+ * device array that every call on a device shares (a batch needs about
+ * 24 bytes per unknown plus 16 n^2 bytes per distinct side length n, and 8
+ * bytes per unknown for 1/(lx+ly)); a larger batch returns -3.  Calls that
+ * overlap on one device race.  Waits for the solve; returns the number of
+ * problems above tol, or < 0 on error. */
+int pic_pbatch_solve_noalloc(int nprob, const int *nx, const int *ny, const int *bc, int device, const double *b_dev,
+                             double *x_dev, double tol, int max_cycles, double *rel_res, void *stream);
 
 /* ---------------- host-only setup (no device needed) ---------------- */
 /* deck -> geometry maps, field operator, Laplace solutions, tables and the
