@@ -499,22 +499,20 @@ __global__ __launch_bounds__(256) void k_mcc_insert(DevConst<R> k, PView<R> src,
 // ============================================================ fused step
 // Pass 1 (count) pushes every particle, applies the walls and the
 // null-collision test, and counts survivors per (cell, slot).  Colliding
-// particles are pooled across the 4 waves of a block and collided together,
-// so the long collision code runs on nearly full wavefronts; their final
-// state is written back in place, and a per-particle flag byte records the
-// move code, whether the particle collided and whether it made an
-// ionisation pair (whose velocities go to a stash indexed like the
-// particle).  Pass 2 (scatter) re-pushes only the particles that did not
-// collide (same bits), ranks everything with ballots in particle order and
-// writes the new, sorted arrays; it draws no random numbers.
+// particles are queued per wave and collided 64 at a time, so the long
+// collision code runs on full wavefronts; their final state is written back
+// in place, and a per-particle flag byte records the move code, whether the
+// particle collided and whether it made an ionisation pair (whose velocities
+// go to a stash indexed like the particle).  Pass 2 (scatter) re-pushes only
+// the particles that did not collide (same bits), ranks everything with
+// ballots in particle order and writes the new, sorted arrays; it draws no
+// random numbers.
 //
-// Both passes stage each wave's cell in chunks of FCHUNK particles through
-// LDS: every lane issues the loads of all its particles of the chunk at once
-// (one memory round trip per chunk), then walks them 64 at a time.  The
-// cell's 9 route entries and the collision probabilities of their
-// destination nodes are loaded once per wave and fetched per particle with a
+// Pass 2 stages each wave's cell in chunks of SCHUNK particles through LDS:
+// every lane issues the loads of all its particles of the chunk at once (one
+// memory round trip per chunk), then walks them 64 at a time.  The cell's 9
+// route entries are loaded once per wave and fetched per particle with a
 // lane shuffle.
-constexpr int FCHUNK = 256;
 constexpr int SCHUNK = 256;   // pass 2 chunk (larger chunks measured faster: more loads in flight)
 constexpr int FWAVES = 4;   // waves per 256-thread block
 enum { KIND_AR_ELECTRON = 0, KIND_AR_ION = 1 };
@@ -525,7 +523,7 @@ enum { FLAG_CODE = 15, FLAG_DONE = 16, FLAG_PROD = 32 };
 // EV_MAX go to the node counters directly.
 constexpr int EV_MAX = 10;
 
-template <typename R, int C = FCHUNK>
+template <typename R, int C = SCHUNK>
 struct FStage {
     R x[C], y[C], vx[C], vy[C], vz[C];
     uint32_t id[C];
@@ -539,21 +537,8 @@ __device__ inline void wave_lds_fence() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// collision pool of one count block
-template <typename R>
-struct FPool {
-    int q[FWAVES * FCHUNK];   // wave << 28 | jcoll << 12 | code << 8 | i
-    int qn;
-    int b0[FWAVES];           // first particle of each wave's current chunk
-    int dest[FWAVES][ROUTE_SLOTS], info[FWAVES][ROUTE_SLOTS];
-    R pa[FWAVES][ROUTE_SLOTS];
-    int prun[FWAVES][ROUTE_SLOTS];
-    int rounds[FWAVES];
-    int ev[FWAVES][ROUTE_SLOTS][EV_MAX];   // collision events per route slot, flushed at the end
-};
-
 // global -> LDS for particles [b0, b0+n), n >= 1.  Lanes past the end load the
-// last particle again (same cache line, never read back), so the 6*FCHUNK/64
+// last particle again (same cache line, never read back), so the 6*C/64
 // loads of a lane form one straight-line block the compiler can batch.
 template <typename R, int C>
 __device__ inline void stage_chunk(FStage<R, C> &S, const PView<R> &p, int b0, int n, int lane) {
@@ -570,25 +555,6 @@ __device__ inline void stage_chunk(FStage<R, C> &S, const PView<R> &p, int b0, i
         S.x[i] = tx[j]; S.y[i] = ty[j]; S.vx[i] = tvx[j]; S.vy[i] = tvy[j]; S.vz[i] = tvz[j]; S.id[i] = tid[j];
     }
 }
-
-// route entry `lane` of cell c and the collision probabilities of its
-// destination (electrons: P1 in pa, P2 in pb; ions: Pion in pb)
-template <typename R>
-struct WaveRoutes {
-    int dest, info;
-    R pa, pb;
-    __device__ void load(const DevConst<R> &k, int c, int lane, int kind) {
-        dest = -1; info = 0; pa = 0; pb = 0;
-        if (c >= 0 && lane < ROUTE_SLOTS) {
-            Route r = k.route[c * ROUTE_SLOTS + lane];
-            dest = r.dest; info = r.info;
-            if (dest >= 0) {
-                if (kind == KIND_AR_ELECTRON) { pa = k.P1[dest]; pb = k.P2[dest]; }
-                else pb = k.Pion[dest];
-            }
-        }
-    }
-};
 
 // null-collision test of one particle at its destination node: the index of
 // the colliding draw, or -1
@@ -612,130 +578,210 @@ __device__ inline int collide(const DevConst<R> &k, int dest, R pa, uint32_t id,
 
 // Pass 1.  flag[k] = move code | FLAG_DONE (collided: final state written
 // back in place) | FLAG_PROD (made an ionisation pair; the new electron's
-// and ion's velocities in stash[q * stash_stride + k], q = 0..5).  The waves
-// of a block walk their cells chunk by chunk in step; after each round the
-// block collides the pooled particles of its 4 cells together.
+// and ion's velocities in stash[q * stash_stride + k], q = 0..5).
+//
+// A wave takes CPW consecutive cells.  Their particles are one contiguous
+// range (cells of one column are consecutive nodes, and the node between
+// two columns starts no cell), so the wave walks the range 64 particles at a
+// time with no idle lanes at cell ends, the next 64 particles' loads in
+// flight while the current ones are pushed.  A lane finds its cell by
+// comparing its index with the cells' starts; the cells' E corners and route
+// entries sit in a per-wave LDS table.  Colliding particles go to a per-wave
+// LDS queue with their post-push state, and whenever 64 are queued they
+// collide together, one per lane, so the long collision code runs on full
+// wavefronts.  The waves of a block never wait for each other.
+constexpr int CPW = 4;          // cells per wave (at most 16: the queue keeps the cell in 4 bits)
+constexpr int CQCAP = 128;      // queued colliders per wave (< 64 left after each collide, + 64 per walk step)
+
+template <typename R>
+struct CountWave {
+    // collider queue: post-push state, particle index and (draw << 8 | cell << 4 | move code)
+    R qx[CQCAP], qy[CQCAP], qvx[CQCAP], qvy[CQCAP], qvz[CQCAP];
+    uint32_t qid[CQCAP];
+    int qkk[CQCAP], qmeta[CQCAP];
+    R e[CPW][8];                                 // E corners of each cell (ews ewn ees een yws ywn yes yen)
+    int dest[CPW * ROUTE_SLOTS], info[CPW * ROUTE_SLOTS];
+    R pa[CPW * ROUTE_SLOTS], pb[CPW * ROUTE_SLOTS];
+    int cnt[CPW * ROUTE_SLOTS], prun[CPW * ROUTE_SLOTS];
+    int ev[CPW * ROUTE_SLOTS][EV_MAX];           // collision events per (cell, route slot), flushed at the end
+};
+
 template <typename R, int KIND>
 __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, const StepClock *clk, int *cnt,
                                                      int *pcnt, unsigned long long *mccr, int *condcharge,
                                                      uint8_t *flag, R *stash, size_t stash_stride, int *err) {
-    __shared__ FStage<R> stage_all[FWAVES];
-    __shared__ FPool<R> pool;
+    __shared__ CountWave<R> waves[FWAVES];
+    CountWave<R> &W = waves[threadIdx.x >> 6];
     const uint32_t step = clk->step;
-    const int wv = threadIdx.x >> 6;
-    int lane;
-    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
-    FStage<R> &S = stage_all[wv];
+    const int lane = threadIdx.x & 63;
+    const int w0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)) * CPW;
     const unsigned long long lt = lanes_below(lane);
-    int k0 = 0, k1 = 0;
-    if (c >= 0) cell_range(p, c, k0, k1);
-    WaveRoutes<R> w;
-    w.load(k, c, lane, KIND);
-    if (lane < ROUTE_SLOTS) {
-        pool.dest[wv][lane] = w.dest; pool.info[wv][lane] = w.info; pool.pa[wv][lane] = w.pa;
-        pool.prun[wv][lane] = 0;
+    // cells of this wave (-1 past the last cell) and their particle ranges
+    int cell[CPW], lo[CPW + 1];
 #pragma unroll
-        for (int r = 0; r < EV_MAX; ++r) pool.ev[wv][lane][r] = 0;
+    for (int q = 0; q < CPW; ++q) {
+        const int wq = w0 + q;
+        cell[q] = wq < k.Csize ? (wq / k.ncy) * k.ngy + (wq % k.ncy) : -1;
     }
-    if (lane == 0) pool.rounds[wv] = (k1 - k0 + FCHUNK - 1) / FCHUNK;
-    if (threadIdx.x == 0) pool.qn = 0;
-    CellE<R> e;
-    if (k0 < k1) e.load(k.Ex, k.Ey, c, k.ngy);
+    {
+        int a0 = 0, a1 = 0;
+        if (cell[0] >= 0) cell_range(p, cell[0], a0, a1);
+        lo[0] = a0;
+        lo[1] = a1;
+#pragma unroll
+        for (int q = 1; q < CPW; ++q) {
+            int b0 = lo[q], b1 = lo[q];
+            if (cell[q] >= 0) cell_range(p, cell[q], b0, b1);
+            lo[q + 1] = max(b1, lo[q]);
+        }
+    }
+    const int hi = lo[CPW];
+    // per-wave tables
+    for (int t = lane; t < CPW * ROUTE_SLOTS; t += 64) {
+        const int q = t / ROUTE_SLOTS;
+        int d = -1, inf = 0;
+        R a = 0, b = 0;
+        if (cell[q] >= 0) {
+            const Route r = k.route[cell[q] * ROUTE_SLOTS + (t % ROUTE_SLOTS)];
+            d = r.dest;
+            inf = r.info;
+            if (d >= 0) {
+                if (KIND == KIND_AR_ELECTRON) { a = k.P1[d]; b = k.P2[d]; }
+                else b = k.Pion[d];
+            }
+        }
+        W.dest[t] = d; W.info[t] = inf; W.pa[t] = a; W.pb[t] = b;
+        W.cnt[t] = 0; W.prun[t] = 0;
+#pragma unroll
+        for (int r = 0; r < EV_MAX; ++r) W.ev[t][r] = 0;
+    }
+    if (lane < CPW * 8) {
+        const int q = lane >> 3, j = lane & 7;
+        R v = 0;
+        if (cell[q] >= 0 && lo[q + 1] > lo[q]) {
+            const int c = cell[q];
+            const int at = c + ((j & 3) == 1 ? 1 : (j & 3) == 2 ? k.ngy : (j & 3) == 3 ? k.ngy + 1 : 0);
+            v = j < 4 ? k.Ex[at] : k.Ey[at];
+        }
+        W.e[q][j] = v;
+    }
+    wave_lds_fence();
     const R As = k.Ascale[isp];
-    int run[ROUTE_SLOTS];
-#pragma unroll
-    for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] = 0;
-    __syncthreads();
-    int nrounds = 0;
-#pragma unroll
-    for (int q = 0; q < FWAVES; ++q) nrounds = max(nrounds, pool.rounds[q]);
-    for (int round = 0; round < nrounds; ++round) {
-        const int b0 = k0 + round * FCHUNK;
-        const int n = min(FCHUNK, k1 - b0);
-        if (lane == 0) pool.b0[wv] = b0;
-        if (n > 0) {
-            stage_chunk<R, FCHUNK>(S, p, b0, n, lane);
-            for (int j = 0; j * 64 < n; ++j) {
-                const int i = j * 64 + lane;
-                int slot = -1, jc = -1;
-                R x = S.x[i], y = S.y[i], vx = S.vx[i], vy = S.vy[i];
-                const uint32_t id = S.id[i];
-                // a stored in-cell offset is in [0,1), or just outside after a
-                // move of more than one cell (not handled, as in the reference);
-                // anything else is a corrupt particle record
-                if (i < n && !(x > (R)-4 && x < (R)4 && y > (R)-4 && y < (R)4)) atomicOr(err, 2);
-                const int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As,
-                                                  k.dt_dx, k.dt_dy, x, y, vx, vy);
-                const int dest = __shfl(w.dest, code, 64);
-                const int info = __shfl(w.info, code, 64);
-                const R pb = __shfl(w.pb, code, 64);
-                if (i < n) {
-                    if (dest >= 0) {
-                        apply_reflect<R>(info, x, y, vx, vy);
-                        slot = info & 15;
-                        jc = null_test<R, KIND>(k, pb, id, step);
-                    } else {
-                        wall_effects<R>(k, Route{dest, info}, isp, c, code, x, y, vx, vy, condcharge);
-                    }
-                }
-#pragma unroll
-                for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] += __popcll(__ballot(slot == s));
-                const unsigned long long cm = __ballot(jc >= 0);
-                int qbase = 0;
-                if (lane == 0 && cm) qbase = atomicAdd(&pool.qn, __popcll(cm));
-                qbase = __shfl(qbase, 0, 64);
-                if (jc >= 0) {
-                    S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy;   // post-push state (vz unchanged)
-                    pool.q[qbase + __popcll(cm & lt)] = (wv << 28) | (jc << 12) | (code << 8) | i;
-                } else if (i < n) {
-                    flag[b0 + i] = (uint8_t)code;
-                }
+    int qn = 0;   // wave-uniform queue length
+
+    // collide queue entries [q0, q0 + n), one per lane
+    auto collide_queue = [&](int q0, int n) {
+        if (lane < n) {
+            const int e = q0 + lane;
+            const int meta = W.qmeta[e];
+            const int code = meta & 15, kq = (meta >> 4) & 15, jc = meta >> 8;
+            const int ent = kq * ROUTE_SLOTS + code;
+            const int dest = W.dest[ent], info = W.info[ent];
+            const R pa = W.pa[ent];
+            R vx = W.qvx[e], vy = W.qvy[e], vz = W.qvz[e];
+            R evx, evy, evz, ivx, ivy, ivz;
+            const int ev = collide<R, KIND>(k, dest, pa, W.qid[e], step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy, ivz);
+            if (ev >= 0 && ev < EV_MAX) atomicAdd(&W.ev[ent][ev], 1);
+            else if (ev >= 0) atomicAdd(&mccr[(size_t)dest * k.TnRct + ev], 1ull);
+            const bool prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
+            const int kk = W.qkk[e];
+            p.x[kk] = W.qx[e]; p.y[kk] = W.qy[e]; p.vx[kk] = vx; p.vy[kk] = vy; p.vz[kk] = vz;
+            flag[kk] = (uint8_t)(code | FLAG_DONE | (prod ? FLAG_PROD : 0));
+            if (prod) {
+                atomicAdd(&W.prun[kq * ROUTE_SLOTS + (info & 15)], 1);
+                stash[kk] = evx; stash[stash_stride + kk] = evy; stash[2 * stash_stride + kk] = evz;
+                stash[3 * stash_stride + kk] = ivx; stash[4 * stash_stride + kk] = ivy;
+                stash[5 * stash_stride + kk] = ivz;
             }
         }
-        __syncthreads();
-        const int qn = pool.qn;
-        __syncthreads();
-        if (threadIdx.x == 0) pool.qn = 0;   // every thread has read qn; the next walk appends after the barrier below
-        for (int qb = 0; qb < qn; qb += FWAVES * 64) {
-            const int ei = qb + (int)threadIdx.x;
-            if (ei < qn) {
-                const int qe = pool.q[ei];
-                const int ww = qe >> 28, jc = (qe >> 12) & 0xffff, code = (qe >> 8) & 15, i = qe & 255;
-                const int dest = pool.dest[ww][code], info = pool.info[ww][code];
-                const R pa = pool.pa[ww][code];
-                FStage<R> &T = stage_all[ww];
-                R vx = T.vx[i], vy = T.vy[i], vz = T.vz[i];
-                R evx, evy, evz, ivx, ivy, ivz;
-                const int ev = collide<R, KIND>(k, dest, pa, T.id[i], step, jc, vx, vy, vz, evx, evy, evz, ivx, ivy,
-                                                ivz);
-                if (ev >= 0 && ev < EV_MAX) atomicAdd(&pool.ev[ww][code][ev], 1);
-                else if (ev >= 0) atomicAdd(&mccr[(size_t)dest * k.TnRct + ev], 1ull);
-                const bool prod = (KIND == KIND_AR_ELECTRON) && (ev == 3 || ev == 4);
-                const int kk = pool.b0[ww] + i;
-                p.x[kk] = T.x[i]; p.y[kk] = T.y[i]; p.vx[kk] = vx; p.vy[kk] = vy; p.vz[kk] = vz;
-                flag[kk] = (uint8_t)(code | FLAG_DONE | (prod ? FLAG_PROD : 0));
-                if (prod) {
-                    atomicAdd(&pool.prun[ww][info & 15], 1);
-                    stash[kk] = evx; stash[stash_stride + kk] = evy; stash[2 * stash_stride + kk] = evz;
-                    stash[3 * stash_stride + kk] = ivx; stash[4 * stash_stride + kk] = ivy;
-                    stash[5 * stash_stride + kk] = ivz;
-                }
+    };
+
+    // the walk, 64 particles at a time; lanes past the end reload the last
+    // particle (never used)
+    R x = 0, y = 0, vx = 0, vy = 0, vz = 0;
+    uint32_t id = 0;
+    if (lo[0] < hi) {
+        const int i = min(lo[0] + lane, hi - 1);
+        x = p.x[i]; y = p.y[i]; vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i]; id = p.id[i];
+    }
+    for (int b = lo[0]; b < hi; b += 64) {
+        R nx = 0, ny = 0, nvx = 0, nvy = 0, nvz = 0;
+        uint32_t nid = 0;
+        if (b + 64 < hi) {
+            const int i = min(b + 64 + lane, hi - 1);
+            nx = p.x[i]; ny = p.y[i]; nvx = p.vx[i]; nvy = p.vy[i]; nvz = p.vz[i]; nid = p.id[i];
+        }
+        const int i = b + lane;
+        const bool act = i < hi;
+        int kq = 0;
+#pragma unroll
+        for (int q = 1; q < CPW; ++q) kq += i >= lo[q] ? 1 : 0;
+        // a stored in-cell offset is in [0,1), or just outside after a move
+        // of more than one cell (not handled, as in the reference); anything
+        // else is a corrupt particle record
+        if (act && !(x > (R)-4 && x < (R)4 && y > (R)-4 && y < (R)4)) atomicOr(err, 2);
+        const int code = push_particle<R>(W.e[kq][0], W.e[kq][1], W.e[kq][2], W.e[kq][3], W.e[kq][4], W.e[kq][5],
+                                          W.e[kq][6], W.e[kq][7], As, k.dt_dx, k.dt_dy, x, y, vx, vy);
+        const int ent = kq * ROUTE_SLOTS + code;
+        const int dest = W.dest[ent], info = W.info[ent];
+        int slot = -1, jc = -1;
+        if (act) {
+            if (dest >= 0) {
+                apply_reflect<R>(info, x, y, vx, vy);
+                slot = info & 15;
+                jc = null_test<R, KIND>(k, W.pb[ent], id, step);
+            } else {
+                int c = cell[0];
+#pragma unroll
+                for (int q = 1; q < CPW; ++q) c = kq == q ? cell[q] : c;
+                wall_effects<R>(k, Route{dest, info}, isp, c, code, x, y, vx, vy, condcharge);
             }
         }
-        __syncthreads();
-    }
-    if (c >= 0 && lane == 0) {
+        // survivors per (cell, slot): the lanes of one cell are a contiguous run
+        const int kfirst = __builtin_amdgcn_readfirstlane(kq);
+        const int klast = __builtin_amdgcn_readfirstlane(__shfl(kq, min(63, hi - 1 - b), 64));
+        for (int q = kfirst; q <= klast; ++q) {
+            const unsigned long long mk = __ballot(act && kq == q);
 #pragma unroll
-        for (int s = 0; s < ROUTE_SLOTS; ++s) {
-            cnt[c * ROUTE_SLOTS + s] = run[s];
-            if (pcnt) pcnt[c * ROUTE_SLOTS + s] = pool.prun[wv][s];
+            for (int s2 = 0; s2 < ROUTE_SLOTS; ++s2) {
+                const int n = __popcll(__ballot(slot == s2) & mk);
+                if (n && lane == 0) W.cnt[q * ROUTE_SLOTS + s2] += n;
+            }
+        }
+        const unsigned long long cm = __ballot(jc >= 0);
+        if (jc >= 0) {
+            const int pos = qn + __popcll(cm & lt);
+            W.qx[pos] = x; W.qy[pos] = y; W.qvx[pos] = vx; W.qvy[pos] = vy; W.qvz[pos] = vz;
+            W.qid[pos] = id;
+            W.qkk[pos] = i;
+            W.qmeta[pos] = (jc << 8) | (kq << 4) | code;
+        } else if (act) {
+            flag[i] = (uint8_t)code;
+        }
+        qn += __popcll(cm);
+        wave_lds_fence();
+        if (qn >= 64) {
+            qn -= 64;
+            collide_queue(qn, 64);   // the newest 64; older entries stay in place
+            wave_lds_fence();
+        }
+        x = nx; y = ny; vx = nvx; vy = nvy; vz = nvz; id = nid;
+    }
+    if (qn > 0) collide_queue(0, qn);
+    wave_lds_fence();
+    // counts of this wave's cells, event counts per (cell, route slot, reaction)
+    for (int t = lane; t < CPW * ROUTE_SLOTS; t += 64) {
+        const int q = t / ROUTE_SLOTS;
+        if (cell[q] >= 0) {
+            const int o = cell[q] * ROUTE_SLOTS + (t % ROUTE_SLOTS);
+            cnt[o] = W.cnt[t];
+            if (pcnt) pcnt[o] = W.prun[t];
         }
     }
-    // event counts of this wave's cell, one lane per (route code, reaction)
-    for (int e = lane; e < ROUTE_SLOTS * EV_MAX; e += 64) {
-        const int code = e / EV_MAX, r = e % EV_MAX;
-        const int n = pool.ev[wv][code][r];
-        if (n) atomicAdd(&mccr[(size_t)pool.dest[wv][code] * k.TnRct + r], (unsigned long long)n);
+    for (int e = lane; e < CPW * ROUTE_SLOTS * EV_MAX; e += 64) {
+        const int t = e / EV_MAX, r = e % EV_MAX;
+        const int n = W.ev[t][r];
+        if (n) atomicAdd(&mccr[(size_t)W.dest[t] * k.TnRct + r], (unsigned long long)n);
     }
 }
 
@@ -1201,6 +1247,8 @@ __global__ void k_x_to_phi(int Gsize, const int *A_idx, const double *x, R *phi)
 
 // ============================================================ launchers
 static inline int cell_blocks(int Csize) { return (Csize + 3) / 4; }
+// count pass: CPW cells per wave, FWAVES waves per block
+static inline int count_blocks(int Csize) { return (Csize + CPW * FWAVES - 1) / (CPW * FWAVES); }
 static inline int node_blocks(int n) { return (n + 255) / 256; }
 
 template <typename R>
@@ -1235,10 +1283,10 @@ void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, const 
                         unsigned long long *mccr, int *cc, uint8_t *flag, R *stash, size_t stash_stride, int *err,
                         hipStream_t s) {
     if (isp == 0)
-        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
+        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(count_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
                            clk, cnt, pcnt, mccr, cc, flag, stash, stash_stride, err);
     else
-        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
+        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(count_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
                            clk, cnt, pcnt, mccr, cc, flag, stash, stash_stride, err);
 }
 template <typename R>
