@@ -222,7 +222,29 @@ def run_ensemble(n, rate_hz):
                                              "aggregate_particle_steps_per_s")}
     out["workload"] = (f"{n} Ar CCP cases, grids 32^2-96^2, 20-60 particles/cell, 10-500 mTorr log-uniform, "
                        f"50-300 V, 1-3 RF cycles of 400 steps, Poisson arrivals {rate_hz:g}/s, latency = finish - "
-                       "arrival incl. setup and queueing (Ar/O2 cases need the O2 collision step, not built)")
+                       "arrival incl. setup and queueing")
+    return out
+
+
+def run_serve5(k, rate_hz=8.0):
+    """The first k requests of the serving workload (512 Poisson arrivals at
+    rate_hz of argon and Ar/O2 cases, grids 64-256 cells a side, 20-200
+    particles per cell, 10-500 mTorr, 50-300 V, 3-20 RF cycles of 400 steps;
+    picmcc.serve.config5_requests), served concurrently, against the same
+    cases run one by one."""
+    from picmcc import serve
+    reqs = serve.config5_requests(512, rate_hz=rate_hz)[:k]
+    alone = [serve.run_alone(r) for r in reqs]
+    res = serve.EnsembleServer(max_active=16, setup_threads=2).run(reqs)
+    out = serve.summarize(res, alone)
+    out["gas_mix"] = {"argon": sum(1 for r in reqs if serve.case_gas(r.deck) == 0),
+                      "ar_o2": sum(1 for r in reqs if serve.case_gas(r.deck) == 2)}
+    out["one_by_one_failed"] = sum(1 for r in alone if r["error"])
+    out["workload"] = (f"first {k} of 512 Poisson arrivals at {rate_hz:g}/s: Ar or Ar/O2 (70-95 % Ar) CCP cases, "
+                       "grids 64-256 cells a side (dx=0.2mm), 20-200 particles/cell, 10-500 mTorr log-uniform, "
+                       "50-300 V at 13.56 MHz, 3-20 RF cycles of 400 steps; latency = finish - arrival incl. "
+                       "setup and queueing")
+    out["mode"] = "concurrent server, at most 16 resident, case setup on 2 worker threads"
     return out
 
 
@@ -274,6 +296,9 @@ def main():
                          "first-served server and one by one; 0 = skip)")
     ap.add_argument("--ensemble-rate", type=float, default=32.0, metavar="HZ",
                     help="mean arrival rate of the ensemble's cases")
+    ap.add_argument("--serve5", type=int, default=12, metavar="K",
+                    help="serve the first K requests of the 512-case argon and Ar/O2 serving workload, and run "
+                         "them one by one (0 = skip)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="first run a device copy of MB MiB (a kernel of known traffic, for counter calibration)")
     args = ap.parse_args()
@@ -291,6 +316,7 @@ def main():
     # the multi-GB allocations of the legs below, creating small cases is
     # measurably slower
     ensemble = run_ensemble(args.ensemble, args.ensemble_rate) if args.ensemble > 0 else None
+    serve5 = run_serve5(args.serve5) if args.serve5 > 0 else None
 
     ref = run_gpu(deck, FP32_REFINED, args.steps, args.warmup, timing_steps=min(args.steps, 100))
     value = particles_avg(ref) * args.steps / ref["seconds"]
@@ -396,6 +422,7 @@ def main():
     if args.aro2_steps > 0:
         line["aro2_chemistry"] = run_aro2(args.aro2_steps)
     line["ensemble"] = ensemble if ensemble is not None else {"status": "not run (--ensemble 0)"}
+    line["serving_workload"] = serve5 if serve5 is not None else {"status": "not run (--serve5 0)"}
     if args.cpu_steps > 0:
         cv, cdt = cpu_baseline(deck, args.cpu_steps)
         line["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",

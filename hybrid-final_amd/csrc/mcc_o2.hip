@@ -149,7 +149,7 @@ __device__ inline int o2_handle(const O2Const<R> &k, int isp, int grp, int c, St
     case O2T_IONSHARE: {
         engy -= k.Th_e[row.th];
         const R u = st.u<R>();
-        const R rengy = (R)10 * (R)dm_tan((double)(u * (R)dm_atan((double)(engy / (R)20))));
+        const R rengy = (R)10 * PMath<R>::tan(u * PMath<R>::atan(engy / (R)20));
         engy -= rengy;
         const R velr = rsqrt_ieee<R>(rabs<R>(rengy) / Es);
         const R vel2 = rsqrt_ieee<R>(rabs<R>(engy) / Es);

@@ -161,19 +161,17 @@ __device__ inline int push_particle(R ews, R ewn, R ees, R een, R yws, R ywn, R 
     R my = vy + dvy * As;
     lx += mx * dtdx;
     ly += my * dtdy;
-    int dxs = 0, dys = 0;
-    if (ly >= 1 || ly < 0 || lx >= 1 || lx < 0) {
-        if (ly >= 1) { dys = 1; ly -= 1; }
-        else if (ly < 0) { dys = -1; ly += 1; }
-        if (lx >= 1) { dxs = 1; lx -= 1; }
-        else if (lx < 0) { dxs = -1; lx += 1; }
-        // at most one cell per axis and step, as in the reference; an offset
-        // still outside [0,1) (a faster move) is clamped into the destination
-        // cell instead of being left outside it
-        const R top = one_minus_ulp<R>();
-        if (lx >= 1) lx = top; else if (lx < 0) lx = 0;
-        if (ly >= 1) ly = top; else if (ly < 0) ly = 0;
-    }
+    // at most one cell per axis and step, as in the reference; an offset
+    // still outside [0,1) (a faster move) is clamped into the destination
+    // cell instead of being left outside it.  Written as selects (no
+    // branches): for an offset inside [0,1) every select keeps its value.
+    const int dys = ly >= 1 ? 1 : (ly < 0 ? -1 : 0);
+    const int dxs = lx >= 1 ? 1 : (lx < 0 ? -1 : 0);
+    ly = ly >= 1 ? ly - 1 : (ly < 0 ? ly + 1 : ly);
+    lx = lx >= 1 ? lx - 1 : (lx < 0 ? lx + 1 : lx);
+    const R top = one_minus_ulp<R>();
+    lx = lx >= 1 ? top : (lx < 0 ? (R)0 : lx);
+    ly = ly >= 1 ? top : (ly < 0 ? (R)0 : ly);
     x = lx; y = ly; vx = mx; vy = my;
     return (dxs + 1) * 3 + (dys + 1);
 }
@@ -182,10 +180,11 @@ template <typename R>
 __device__ inline void apply_reflect(int info, R &x, R &y, R &vx, R &vy) {
     // a mirrored offset of 0 (a particle clamped onto the wall) would be 1:
     // keep it inside the cell like every other offset
-    if (info & 16) { x = 1 - x; if (x >= 1) x = one_minus_ulp<R>(); }
-    if (info & 32) { y = 1 - y; if (y >= 1) y = one_minus_ulp<R>(); }
-    if (info & 64) vx = -vx;
-    if (info & 128) vy = -vy;
+    const R mxr = 1 - x, myr = 1 - y;
+    x = (info & 16) ? (mxr >= 1 ? one_minus_ulp<R>() : mxr) : x;
+    y = (info & 32) ? (myr >= 1 ? one_minus_ulp<R>() : myr) : y;
+    vx = (info & 64) ? -vx : vx;
+    vy = (info & 128) ? -vy : vy;
 }
 
 // Dielectric surface-charge split (src/cuda_Sortboundary.cu:91-211).
@@ -245,6 +244,25 @@ __host__ __device__ inline int padded_index(int ID, int MCID, int ngy) {
     return (ID + ngy + 2 + x) + MCID;
 }
 
+// ---------------------------------------------------------------- elementary functions
+// In the path's precision: the FP32 path uses the single-precision
+// deterministic routines of detmath.h, the FP64 path the double ones.
+template <typename R> struct PMath;
+template <> struct PMath<float> {
+    static __device__ float log10(float x) { return dmf_log10(x); }
+    static __device__ float pow(float a, float b) { return dmf_pow(a, b); }
+    static __device__ void sincos(float x, float &s, float &c) { dmf_sincos(x, &s, &c); }
+    static __device__ float tan(float x) { return dmf_tan(x); }
+    static __device__ float atan(float x) { return dmf_atan(x); }
+};
+template <> struct PMath<double> {
+    static __device__ double log10(double x) { return dm_log10(x); }
+    static __device__ double pow(double a, double b) { return dm_pow(a, b); }
+    static __device__ void sincos(double x, double &s, double &c) { s = dm_sin(x); c = dm_cos(x); }
+    static __device__ double tan(double x) { return dm_tan(x); }
+    static __device__ double atan(double x) { return dm_atan(x); }
+};
+
 // ---------------------------------------------------------------- argon MCC
 // Cross section of reaction r at one energy (Argon_CrossSection,
 // src/cuda_mccAr.cu:707-777): log-spaced table, linear inside, a decaying
@@ -262,7 +280,7 @@ struct ArXs {
     __device__ void init(const K &k, R engy) {
         const int N = k.nlogx;
         if (engy == 0) { mode = 0; return; }
-        const R lengy = (R)dm_log10((double)engy);
+        const R lengy = PMath<R>::log10(engy);
         const R ee1 = k.idLOGX * (lengy - k.xe[0]);
         ee2 = (int)ee1;
         a1 = ee1 - (R)ee2;
@@ -297,10 +315,11 @@ template <typename R>
 __device__ inline void anewvel(R energy, R vel, R &nvx, R &nvy, R &nvz, R r1, R r2) {
     R coschi;
     if (energy < (R)1e-30) coschi = 1;
-    else coschi = (energy + 2 - 2 * (R)dm_pow((double)(energy + 1), (double)r1)) / energy;
+    else coschi = (energy + 2 - 2 * PMath<R>::pow(energy + 1, r1)) / energy;
     R sinchi = rsqrt_ieee<R>(rabs<R>(1 - coschi * coschi));
     R phi1 = (R)PIC_TWO_PI * r2;
-    R cosphi = (R)dm_cos((double)phi1), sinphi = (R)dm_sin((double)phi1);
+    R cosphi, sinphi;
+    PMath<R>::sincos(phi1, sinphi, cosphi);
     R r13 = nvx, r23 = nvy, r33 = nvz, up1, up2, up3;
     if (r33 == 1) { up1 = 0; up2 = 1; up3 = 0; } else { up1 = 0; up2 = 0; up3 = 1; }
     R r12 = r23 * up3 - r33 * up2;
@@ -321,7 +340,8 @@ __device__ inline void newvel_ionsc(R &vx, R &vy, R &vz, R vel, R r1, R r2) {
     R coschi = rsqrt_ieee<R>(r1);
     R sinchi = rsqrt_ieee<R>(rabs<R>(1 - coschi * coschi));
     R phi1 = (R)PIC_TWO_PI * r2;
-    R cosphi = (R)dm_cos((double)phi1), sinphi = (R)dm_sin((double)phi1);
+    R cosphi, sinphi;
+    PMath<R>::sincos(phi1, sinphi, cosphi);
     R r13 = vx / vel, r23 = vy / vel, r33 = vz / vel, up1, up2, up3;
     if (r33 == 1) { up1 = 0; up2 = 1; up3 = 0; } else { up1 = 0; up2 = 0; up3 = 1; }
     R r12 = r23 * up3 - r33 * up2;
@@ -342,8 +362,10 @@ __device__ inline void maxwell_draw(R &vx, R &vy, R &vz, R vsaven, R vti, R rphi
     R aphi = (R)PIC_TWO_PI * rphi;
     R costheta = 1 - 2 * rthe;
     R sintheta = rsqrt_ieee<R>(1 - costheta * costheta);
-    vx = vti * vsaven * sintheta * (R)dm_cos((double)aphi);
-    vy = vti * vsaven * sintheta * (R)dm_sin((double)aphi);
+    R cosa, sina;
+    PMath<R>::sincos(aphi, sina, cosa);
+    vx = vti * vsaven * sintheta * cosa;
+    vy = vti * vsaven * sintheta * sina;
     vz = vti * vsaven * costheta;
 }
 
@@ -446,7 +468,7 @@ __device__ inline int mcc_ar_electron_collide(const DevConst<R> &k, int node, R 
         } else if (engy > k.Th_e[3] && R2 <= (sum += xs.sigma(k, 3))) {
             type = 2; ev = 3; engy -= k.Th_e[3];
             R u = st.u<R>();
-            rengy = (R)10 * (R)dm_tan((double)(u * (R)dm_atan((double)(engy / (R)20))));
+            rengy = (R)10 * PMath<R>::tan(u * PMath<R>::atan(engy / (R)20));
             engy -= rengy;
             vel = rsqrt_ieee<R>(rabs<R>(rengy) / Es);
             vel2 = rsqrt_ieee<R>(rabs<R>(engy) / Es);
@@ -457,7 +479,7 @@ __device__ inline int mcc_ar_electron_collide(const DevConst<R> &k, int node, R 
         if (engy > k.Th_e[4] && R2 <= sum) {
             type = 2; ev = 4; engy -= k.Th_e[4];
             R u = st.u<R>();
-            rengy = (R)10 * (R)dm_tan((double)(u * (R)dm_atan((double)(engy / (R)20))));
+            rengy = (R)10 * PMath<R>::tan(u * PMath<R>::atan(engy / (R)20));
             engy -= rengy;
             vel = rsqrt_ieee<R>(rabs<R>(rengy) / Es);
             vel2 = rsqrt_ieee<R>(rabs<R>(engy) / Es);

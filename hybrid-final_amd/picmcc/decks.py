@@ -131,10 +131,11 @@ def oxygen_deck(gas=2, o2_ratio=0.2, ion_densities=None, neutral_densities=None,
     te, ti = cs["Electron"]["Temp(eV)"], cs["Ar+"]["Temp(eV)"]
     for name in ("Electron", "Ar+"):
         cs.pop(name)
+    cf = kw.get("capacity_factor", 3.0)
     for sid, (name, n) in enumerate(dens.items()):
         per_cell = ppc if name in ("Electron", "Ar+", "O2+") else max(1, ppc // 5)
         np2c = n * dx * dx * 1.0 / per_cell
-        nmax = int(3 * per_cell * ncx * ncy)
+        nmax = int(cf * per_cell * ncx * ncy)
         cs[name] = {"S_ID": sid, "Density": n, "Temp(eV)": te if name == "Electron" else ti,
                     "np2c": np2c, "Ratio": 1, "Max_np": nmax}
     d["SimulationMethod"]["ParticleLimit"] = max(c["Max_np"] for c in cs.values() if isinstance(c, dict)

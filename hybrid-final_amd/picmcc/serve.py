@@ -31,7 +31,8 @@ import time
 
 import numpy as np
 
-from . import FP32_REFINED, Case, _check, decks, lib
+from . import FP32_REFINED, Case, PicError, _check, decks, lib
+
 
 def device_free_bytes(device=0):
     """Free and total device memory, through the library's own HIP runtime."""
@@ -76,6 +77,48 @@ def ar_ensemble(n, seed=7, rate_hz=40.0, grid=(32, 96), ppc=(20, 60), torr=(0.01
         steps = d["SimulationMethod"]["RFCycles"] * steps_per_cycle
         out.append(Request(i, t, d, steps))
     return out
+
+
+def config5_requests(n=512, seed=2026, rate_hz=8.0, ar_fraction=0.5, grid=(64, 256), ppc=(20, 200),
+                     torr=(0.01, 0.5), volts=(50.0, 300.0), cycles=(3, 20), o2_share=(0.05, 0.30),
+                     steps_per_cycle=400, dx=2e-4, capacity_factor=4.0):
+    """The serving workload: Poisson arrivals (rate_hz, seeded) of discharge
+    cases, each argon or Ar/O2 (70-95 % argon), grid sides uniform over 64-256
+    cells (dx = 0.2 mm), 20-200 particles per cell, pressure log-uniform over
+    10-500 mTorr, 50-300 V at 13.56 MHz, 3-20 RF cycles of steps_per_cycle
+    steps, seeded Maxwellian loading.  Returns the Request list (arrival times
+    in seconds from the start)."""
+    rng = random.Random(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.expovariate(rate_hz)
+        ncx = rng.randint(grid[0], grid[1])
+        ncy = rng.randint(grid[0], grid[1])
+        p = math.exp(rng.uniform(math.log(torr[0]), math.log(torr[1])))
+        kw = dict(ncx=ncx, ncy=ncy, dx=dx, volts=rng.uniform(*volts), pres_torr=p, ppc=rng.randint(*ppc),
+                  steps_per_cycle=steps_per_cycle, cycles=rng.randint(*cycles), seed=1000 + i,
+                  philox_seed=5000 + i)
+        if rng.random() < ar_fraction:
+            d = decks.ccp_deck(capacity_factor=capacity_factor, **kw)
+        else:
+            d = decks.oxygen_deck(gas=2, o2_ratio=rng.uniform(*o2_share), capacity_factor=capacity_factor, **kw)
+        out.append(Request(i, t, d, d["SimulationMethod"]["RFCycles"] * steps_per_cycle))
+    return out
+
+
+def case_gas(deck):
+    return int(deck["GasSpecies"]["Type(0:Ar,1:O2,2:Ar/O2)"])
+
+
+def _sync_status(case):
+    """Waits for the case; 0, or the library's status when the case failed
+    (for example -30 when it outgrew its particle capacity)."""
+    try:
+        case.sync()
+        return 0
+    except PicError as e:
+        return e.code
 
 
 class _Active:
@@ -172,13 +215,13 @@ class EnsembleServer:
                     continue
                 a.inflight = False
             if a.remaining == 0:
-                a.case.sync()
+                err = _sync_status(a.case)
                 inf = a.case.info()
                 np1 = sum(inf.np[:inf.nsp])
                 rec = {"rid": a.req.rid, "arrival": a.req.arrival, "admit": a.admit,
                        "finish": time.perf_counter() - t0, "steps": a.req.steps,
                        "particle_steps": 0.5 * (a.np0 + np1) * a.req.steps, "footprint": a.footprint,
-                       "cells": (inf.ngx - 1) * (inf.ngy - 1)}
+                       "cells": (inf.ngx - 1) * (inf.ngy - 1), "gas": case_gas(a.req.deck), "error": err}
                 rec["latency"] = rec["finish"] - rec["arrival"]
                 if keep_state and keep_state(a.req.rid):
                     rec["state"] = {"particles": [a.case.particles(s) for s in range(inf.nsp)],
@@ -220,12 +263,13 @@ class SequentialServer:
             inf = case.info()
             np0 = sum(inf.np[:inf.nsp])
             case.step(req.steps)
-            case.sync()
+            err = _sync_status(case)
             inf = case.info()
             np1 = sum(inf.np[:inf.nsp])
             rec = {"rid": req.rid, "arrival": req.arrival, "admit": admit, "finish": time.perf_counter() - t0,
                    "steps": req.steps, "particle_steps": 0.5 * (np0 + np1) * req.steps,
-                   "footprint": max(f0 - f1, 0), "cells": (inf.ngx - 1) * (inf.ngy - 1)}
+                   "footprint": max(f0 - f1, 0), "cells": (inf.ngx - 1) * (inf.ngy - 1),
+                   "gas": case_gas(req.deck), "error": err}
             rec["latency"] = rec["finish"] - rec["arrival"]
             rec["service"] = rec["finish"] - rec["admit"]
             if keep_state and keep_state(req.rid):
@@ -242,10 +286,11 @@ def run_alone(req, precision=FP32_REFINED, keep_state=False):
     inf = case.info()
     np0 = sum(inf.np[:inf.nsp])
     case.step(req.steps)
-    case.sync()
+    err = _sync_status(case)
     inf = case.info()
     np1 = sum(inf.np[:inf.nsp])
-    rec = {"rid": req.rid, "seconds": time.perf_counter() - t0, "particle_steps": 0.5 * (np0 + np1) * req.steps}
+    rec = {"rid": req.rid, "seconds": time.perf_counter() - t0, "particle_steps": 0.5 * (np0 + np1) * req.steps,
+           "error": err}
     if keep_state:
         rec["state"] = {"particles": [case.particles(s) for s in range(inf.nsp)], "den": case.array("den")}
     case.close()
@@ -256,7 +301,8 @@ def summarize(results, one_by_one=None):
     lat = np.array([r["latency"] for r in results])
     ps = sum(r["particle_steps"] for r in results)
     span = max(r["finish"] for r in results) - min(r["arrival"] for r in results)
-    out = {"cases": len(results), "p50_latency_s": float(np.percentile(lat, 50)),
+    out = {"cases": len(results), "failed": sum(1 for r in results if r.get("error")),
+           "p50_latency_s": float(np.percentile(lat, 50)),
            "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": span,
            "busy_fraction": (sum(r["service"] for r in results) / span) if "service" in results[0] else None,
            "aggregate_particle_steps_per_s": ps / span,

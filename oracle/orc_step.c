@@ -18,6 +18,23 @@
 int orc_setup(orc_case *c);
 void orc_band_solve(int N, const double *lu, int bw, const double *b, double *x);
 
+/* elementary functions of the collision physics in the path precision:
+ * the FP32 build uses detmath.h's single-precision routines, the FP64 build
+ * the double ones (the GPU path makes the same choice) */
+#ifdef ORC_FLOAT
+#define R_LOG10(x) dmf_log10(x)
+#define R_POW(a, b) dmf_pow(a, b)
+#define R_SINCOS(x, s, c) dmf_sincos(x, s, c)
+#define R_TAN(x) dmf_tan(x)
+#define R_ATAN(x) dmf_atan(x)
+#else
+#define R_LOG10(x) dm_log10(x)
+#define R_POW(a, b) dm_pow(a, b)
+#define R_SINCOS(x, s, c) (*(s) = dm_sin(x), *(c) = dm_cos(x))
+#define R_TAN(x) dm_tan(x)
+#define R_ATAN(x) dm_atan(x)
+#endif
+
 static void *zalloc(size_t n, size_t s) {
     void *p = calloc(n ? n : 1, s);
     if (!p) { fprintf(stderr, "oracle: out of memory\n"); exit(2); }
@@ -438,7 +455,7 @@ int orc_sort(orc_case *c) {
 static real ar_sigma(const orc_case *c, int r, real engy) {
     if (engy == 0) return 0;
     int N = c->N_LOGX, T = c->TnRct;
-    real lengy = (real)dm_log10((double)engy);
+    real lengy = R_LOG10(engy);
     real ee1 = c->idLOGX * (lengy - c->xe[0]);
     int ee2 = (int)ee1;
     real a1 = ee1 - (real)ee2;
@@ -453,10 +470,11 @@ static real ar_sigma(const orc_case *c, int r, real engy) {
 static void anewvel(real energy, real vel, real *nvx, real *nvy, real *nvz, real r1, real r2) {
     real coschi;
     if (energy < (real)1e-30) coschi = 1;
-    else coschi = (energy + 2 - 2 * (real)dm_pow((double)(energy + 1), (double)r1)) / energy;
+    else coschi = (energy + 2 - 2 * R_POW(energy + 1, r1)) / energy;
     real sinchi = RSQRT(RABS(1 - coschi * coschi));
     real phi1 = TWO_PI_R * r2;
-    real cosphi = (real)dm_cos((double)phi1), sinphi = (real)dm_sin((double)phi1);
+    real cosphi, sinphi;
+    R_SINCOS(phi1, &sinphi, &cosphi);
     real r13 = *nvx, r23 = *nvy, r33 = *nvz, up1, up2, up3;
     if (r33 == 1) { up1 = 0; up2 = 1; up3 = 0; } else { up1 = 0; up2 = 0; up3 = 1; }
     real r12 = r23 * up3 - r33 * up2;
@@ -476,7 +494,8 @@ static void newvel_ionsc(real *vx, real *vy, real *vz, real vel, real r1, real r
     real coschi = RSQRT(r1);
     real sinchi = RSQRT(RABS(1 - coschi * coschi));
     real phi1 = TWO_PI_R * r2;
-    real cosphi = (real)dm_cos((double)phi1), sinphi = (real)dm_sin((double)phi1);
+    real cosphi, sinphi;
+    R_SINCOS(phi1, &sinphi, &cosphi);
     real r13 = *vx / vel, r23 = *vy / vel, r33 = *vz / vel, up1, up2, up3;
     if (r33 == 1) { up1 = 0; up2 = 1; up3 = 0; } else { up1 = 0; up2 = 0; up3 = 1; }
     real r12 = r23 * up3 - r33 * up2;
@@ -496,8 +515,10 @@ static void dmaxwellv(real *vx, real *vy, real *vz, real vsaven, real vti, real 
     real aphi = TWO_PI_R * Rphi;
     real costheta = 1 - 2 * Rthe;
     real sintheta = RSQRT(1 - costheta * costheta);
-    *vx = vti * vsaven * sintheta * (real)dm_cos((double)aphi);
-    *vy = vti * vsaven * sintheta * (real)dm_sin((double)aphi);
+    real cosa, sina;
+    R_SINCOS(aphi, &sina, &cosa);
+    *vx = vti * vsaven * sintheta * cosa;
+    *vy = vti * vsaven * sintheta * sina;
     *vz = vti * vsaven * costheta;
 }
 
@@ -571,7 +592,7 @@ int orc_mcc(orc_case *c) {
                 } else if (engy > c->Th_e[3] && R2 <= (sum += ar_sigma(c, 3, engy))) {
                     type = 2; engy -= c->Th_e[3];
                     real u = U(&st);
-                    rengy = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+                    rengy = (real)10 * R_TAN(u * R_ATAN(engy / (real)20));
                     engy -= rengy;
                     vel = RSQRT(RABS(rengy) / c->Escale[0]);
                     vel2 = RSQRT(RABS(engy) / c->Escale[0]);
@@ -583,7 +604,7 @@ int orc_mcc(orc_case *c) {
                 if (engy > c->Th_e[4] && R2 <= sum) {
                     type = 2; engy -= c->Th_e[4];
                     real u = U(&st);
-                    rengy = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+                    rengy = (real)10 * R_TAN(u * R_ATAN(engy / (real)20));
                     engy -= rengy;
                     vel = RSQRT(RABS(rengy) / c->Escale[0]);
                     vel2 = RSQRT(RABS(engy) / c->Escale[0]);
@@ -846,6 +867,27 @@ double orc_detmath(int fn, double x, double y) {
     }
 }
 
+/* the single-precision routines of detmath.h, element by element:
+ * fn 0 exp, 1 log, 2 log10, 3 sin, 4 cos, 5 tan, 6 atan, 7 pow(x, y) */
+void orc_detmathf(int fn, int n, const float *x, const float *y, float *out) {
+    int i;
+    for (i = 0; i < n; ++i) {
+        float v = 0.0f, s, c;
+        switch (fn) {
+        case 0: v = dmf_exp(x[i]); break;
+        case 1: v = dmf_log(x[i]); break;
+        case 2: v = dmf_log10(x[i]); break;
+        case 3: dmf_sincos(x[i], &s, &c); v = s; break;
+        case 4: dmf_sincos(x[i], &s, &c); v = c; break;
+        case 5: v = dmf_tan(x[i]); break;
+        case 6: v = dmf_atan(x[i]); break;
+        case 7: v = dmf_pow(x[i], y[i]); break;
+        default: break;
+        }
+        out[i] = v;
+    }
+}
+
 /* Test hook: one call of a collision kinematics routine of the MCC above,
  * with its random numbers given.  which = 0 anewvel {E, vel, ux, uy, uz, r1,
  * r2} -> {vx, vy, vz}; 1 newvel_ionsc {vx, vy, vz, vel, r1, r2} -> {vx, vy,
@@ -867,7 +909,7 @@ int orc_kinematics(int which, const double *in, double *out) {
         break;
     case 3: {
         real engy = (real)in[0], u = (real)in[1];
-        a = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+        a = (real)10 * R_TAN(u * R_ATAN(engy / (real)20));
         break;
     }
     default:
@@ -929,7 +971,7 @@ static const uint32_t o2_vtag[2] = {O2_VICTIM_TAG, ARO2_VICTIM_TAG};
 static real o2_sigma(const orc_o2_in *k, int r, real engy) {
     if (engy == 0) return 0;
     int N = k->nlogx, T = k->TnRct;
-    real lengy = (real)dm_log10((double)engy);
+    real lengy = R_LOG10(engy);
     real ee1 = k->idLOGX * (lengy - k->xe[0]);
     int ee2 = (int)ee1;
     real a1 = ee1 - (real)ee2;
@@ -997,7 +1039,7 @@ static void o2_collide(const orc_o2_in *k, int s, int grp, int g, orc_stream *st
         int j;
         engy -= k->Th_e[row->th];
         real u = U(st);
-        real rengy = (real)10 * (real)dm_tan((double)(u * (real)dm_atan((double)(engy / (real)20))));
+        real rengy = (real)10 * R_TAN(u * R_ATAN(engy / (real)20));
         engy -= rengy;
         real velr = RSQRT(RABS(rengy) / k->Escale[s]);
         real vel2 = RSQRT(RABS(engy) / k->Escale[s]);

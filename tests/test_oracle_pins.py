@@ -88,6 +88,52 @@ def test_detmath_pow():
     assert np.max(np.abs(got / np.power(a, b) - 1)) < 5e-15
 
 
+def _detmathf(fn, x, y=None):
+    L = lib("f32")
+    L.orc_detmathf.restype = None
+    L.orc_detmathf.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float32)
+    out = np.empty_like(x)
+    L.orc_detmathf(fn, len(x), x.ctypes.data, y.ctypes.data, out.ctypes.data)
+    return out
+
+
+def _ulps(got, want, floor=0.0):
+    """|got - want| in units of the float spacing at |want| (at least at floor)."""
+    sp = np.spacing(np.maximum(np.abs(want).astype(np.float32), np.float32(floor))).astype(np.float64)
+    return float(np.max(np.abs(got.astype(np.float64) - want) / sp))
+
+
+@pytest.mark.parametrize("name,fn,make,np_fn,floor,bound", [
+    ("log", 1, lambda r: np.float32(10.0) ** r.uniform(-30, 30, 100000).astype(np.float32), np.log, 0.0, 2.0),
+    ("log10", 2, lambda r: np.float32(10.0) ** r.uniform(-30, 30, 100000).astype(np.float32), np.log10, 0.0, 3.5),
+    ("exp", 0, lambda r: r.uniform(-103, 88, 100000).astype(np.float32), np.exp, 0.0, 1.5),
+    ("sin", 3, lambda r: r.uniform(-1e4, 1e4, 100000).astype(np.float32), np.sin, 1.0, 1.5),
+    ("cos", 4, lambda r: r.uniform(0, 2 * np.pi, 100000).astype(np.float32), np.cos, 1.0, 1.5),
+    ("tan", 5, lambda r: r.uniform(0, 1.56, 100000).astype(np.float32), np.tan, 0.0, 3.0),
+    ("atan", 6, lambda r: (np.float32(10.0) ** r.uniform(-6, 4, 100000)).astype(np.float32)
+     * np.where(r.random(100000) < 0.5, -1, 1).astype(np.float32), np.arctan, 0.0, 3.0)])
+def test_single_precision_detmath_against_libm(name, fn, make, np_fn, floor, bound):
+    """The FP32 path's elementary functions (detmath.h dmf_*) against libm in
+    double: sin and cos within 1.5 float ulps of 1 (absolute, as the
+    collision phases need), the others relative to the result."""
+    x = make(np.random.default_rng(fn + 100))
+    assert _ulps(_detmathf(fn, x), np_fn(x.astype(np.float64)), floor) <= bound
+
+
+def test_single_precision_pow():
+    """pow(1 + E, R) of the scattering angle: a, b drawn as the collisions
+    draw them; exp(b log a) loses |b log a| times the log's error (< 10 ulps
+    here)."""
+    r = np.random.default_rng(77)
+    a = r.uniform(1.0, 1001.0, 100000).astype(np.float32)
+    b = r.uniform(0.0, 1.0, 100000).astype(np.float32)
+    assert _ulps(_detmathf(7, a, b), np.power(a.astype(np.float64), b.astype(np.float64))) <= 10.0
+    assert _detmathf(0, [0.0])[0] == 1.0 and _detmathf(1, [1.0])[0] == 0.0
+    assert _detmathf(6, [1.0])[0] == np.float32(np.pi / 4)
+
+
 # ---------------------------------------------------------------- reference-built tables
 def _load_gold(name):
     with open(os.path.join(GOLD, name)) as f:
