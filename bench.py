@@ -14,6 +14,7 @@ Prints one JSON line (rank 0; this build runs on one GPU only).
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -57,6 +58,7 @@ def run_gpu(deck, precision, steps, warmup, timing_steps=0, field_solver=0):
         "stages_ms": {k: v / max(nst, 1) for k, v in stages.items()},
         "field_iters": it, "field_cycles": outer, "field_rel_res": rel,
         "solver_kind": c.field_solver_kind(),
+        "solver_shape": c.field_solver_shape(),
     }
     c.close()
     return out
@@ -115,20 +117,48 @@ def particles_avg(r):
     return 0.5 * (sum(r["np0"]) + sum(r["np1"]))
 
 
-def cpu_baseline(deck, steps):
-    """Oracle (FP64 CPU restatement, one thread) on `steps` steps of the workload."""
+def cpu_threads():
+    """Threads the CPU baseline uses: this process's CPU set, at most 16
+    (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(deck, steps, threads, repeats=3):
+    """The FP64 oracle's OpenMP step (cell tiles; bit-identical to the
+    sequential oracle) on `steps` steps of the workload, `repeats` times from
+    the same loaded state; returns the median rate and the run times."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
-    o = Oracle(deck, "f64")
-    o.orc_deposit()
-    n0 = o.scalar("np0") + o.scalar("np1")
-    t0 = time.perf_counter()
-    rc = o.orc_run(steps)
-    dt = time.perf_counter() - t0
-    n1 = o.scalar("np0") + o.scalar("np1")
-    if rc != 0:
-        raise RuntimeError(f"oracle run failed with {rc}")
-    return 0.5 * (n0 + n1) * steps / dt, dt
+    rates, times = [], []
+    for _ in range(repeats):
+        o = Oracle(deck, "f64_omp")
+        o.orc_deposit()
+        n0 = o.scalar("np0") + o.scalar("np1")
+        t0 = time.perf_counter()
+        rc = o.orc_run_par(steps, threads)
+        dt = time.perf_counter() - t0
+        n1 = o.scalar("np0") + o.scalar("np1")
+        o.close()
+        if rc != 0:
+            raise RuntimeError(f"oracle run failed with {rc}")
+        rates.append(0.5 * (n0 + n1) * steps / dt)
+        times.append(dt)
+    return float(statistics.median(rates)), times
 
 
 def run_pbatch(n, reps=3):
@@ -284,6 +314,8 @@ def main():
     ap.add_argument("--profile-cycles", type=int, default=10,
                     help="RF cycles of the FP32 vs FP64 time-averaged profile comparison (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-config1-steps", type=int, default=0,
+                    help="also time the CPU baseline on the 64x64 argon deck for this many steps (0 = skip)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--pbatch", type=int, default=256, metavar="N",
                     help="independent Poisson problems (64^2-512^2) in the batched-solve leg (0 = skip)")
@@ -384,7 +416,9 @@ def main():
         "fp64_plain": {"value": plain_value, "unit": "particle-steps/s",
                        "ms_per_step": plain["seconds"] / args.plain_steps * 1e3, "steps": args.plain_steps,
                        "field_solve": solver_label(plain, FP64_PLAIN),
-                       "field_iters_last": plain["field_iters"]} if plain else None,
+                       "field_iters_last": plain["field_iters"],
+                       "pcg_launch_tuned": {"blocks": plain["solver_shape"][0],
+                                            "threads_per_block": plain["solver_shape"][1]}} if plain else None,
         # headline ratio: the FP32-refined path against the reference
         # algorithm in FP64 (its Jacobi PCG dominates the FP64 step)
         "refined_over_plain": value / plain_value if plain_value > 0 else None,
@@ -400,7 +434,9 @@ def main():
                              "ms_per_step": rpcg["seconds"] / args.refined_pcg_steps * 1e3,
                              "steps": args.refined_pcg_steps, "field_solve": solver_label(rpcg, FP32_REFINED),
                              "field_iters_last": rpcg["field_iters"], "field_cycles_last": rpcg["field_cycles"],
-                             "field_rel_res_last": rpcg["field_rel_res"]} if rpcg else None,
+                             "field_rel_res_last": rpcg["field_rel_res"],
+                             "pcg_launch_tuned": {"blocks": rpcg["solver_shape"][0],
+                                                  "threads_per_block": rpcg["solver_shape"][1]}} if rpcg else None,
         "field_solve_last": {"cycles": ref["field_cycles"], "rel_residual": ref["field_rel_res"]},
         "stage_ms_per_step": {k: round(v, 5) for k, v in st.items()},
         # algorithmic HBM bytes of each particle stage over its time (particle
@@ -424,10 +460,20 @@ def main():
     line["ensemble"] = ensemble if ensemble is not None else {"status": "not run (--ensemble 0)"}
     line["serving_workload"] = serve5 if serve5 is not None else {"status": "not run (--serve5 0)"}
     if args.cpu_steps > 0:
-        cv, cdt = cpu_baseline(deck, args.cpu_steps)
-        line["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                                "sample": f"{args.cpu_steps} steps of the same workload on the FP64 oracle "
-                                          f"({cdt:.1f} s of CPU time, setup excluded)"}
+        th = cpu_threads()
+        cv, ct = cpu_baseline(deck, args.cpu_steps, th)
+        line["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": th, "kind": "port",
+                                "sample": f"median of 3 runs of {args.cpu_steps} steps of the same workload on the "
+                                          f"FP64 oracle's OpenMP step ({th} threads over cell tiles; run times "
+                                          + ", ".join(f"{t:.1f}" for t in ct) + " s, setup excluded)",
+                                "cpu": cpu_model(), "nproc": os.cpu_count()}
+    if args.cpu_config1_steps > 0:
+        th = cpu_threads()
+        cv, ct = cpu_baseline(decks.config1(), args.cpu_config1_steps, th)
+        line["cpu_baseline_64x64"] = {"value": cv, "unit": "particle-steps/s", "cores": th, "kind": "port",
+                                      "sample": f"median of 3 runs of {args.cpu_config1_steps} steps of the 64x64 "
+                                                "argon deck (409,600 particles)",
+                                      "run_s": ct, "cpu": cpu_model(), "nproc": os.cpu_count()}
     print(json.dumps(line), flush=True)
 
 

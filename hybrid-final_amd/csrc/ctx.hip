@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -68,6 +69,10 @@ struct CaseBase {
     virtual int deposit() = 0;
     virtual int diagnostic() = 0;
     virtual int eedf_sample(double emax) = 0;
+    virtual int diag_sample() = 0;
+    virtual int diag_history(double *out, int max_records, int *nrec, int *rec_len) = 0;
+    virtual int dump_save(const char *path) = 0;
+    virtual int dump_load(const char *path) = 0;
     virtual int step_fused(int n) = 0;
     virtual int sync() = 0;
     virtual int query() = 0;
@@ -80,6 +85,7 @@ struct CaseBase {
     virtual int set_array(const char *, const void *, size_t) = 0;
     virtual int set_step(long long) = 0;
     virtual int last_solve(int *, int *, double *) = 0;
+    virtual int solver_shape(int *, int *) = 0;
     virtual int set_timing(int) = 0;
     virtual int get_timing(double *, int, long long *) = 0;
     int rf_cycles = 0, dt_pic = 0;
@@ -131,6 +137,13 @@ struct Case : CaseBase {
     int *total2 = nullptr, *alive2 = nullptr;
     StepClock *clk = nullptr;   // device clock of the fused step (the step about to run)
     unsigned long long *eedf = nullptr;   // electron energy histogram (EEDF_BINS counts)
+    // diagnostics history (diag.hip): per-cell energy partials, absorption
+    // counts at the previous sample, a ring of DIAG_RING records
+    static constexpr int DIAG_RING = 4096;
+    double *diag_part = nullptr, *diag_hist = nullptr;
+    int *diag_cc_prev = nullptr;
+    long long diag_n = 0;
+    long long tstep_base = 0;   // step at which the counters (mccr, CondCharge, history) last started
     double eedf_emax = 0;
     long long eedf_n = 0;
     bool clk_dirty = true;      // host tstep changed outside the fused path
@@ -197,7 +210,7 @@ struct Case : CaseBase {
                         inv_ent, pcol, pvalf, pmdf, pvald, pmdd, b64, x64, d_res, prank, pcnt, pid_off, total, alive,
                         ptotal, pstart, scan_tmp, cellp, next_id, mccr, condcharge, d_err, sum_den, sum_phi, sum_ex,
                         sum_ey, total2, alive2, clk, eedf, lost, sigma_hits, o2prob, o2fluid, o2bg, o2bg2, o2bv2,
-                        o2pcount, o2pstart, o2delreq, o2total, o2delcnt};
+                        o2pcount, o2pstart, o2delreq, o2total, o2delcnt, diag_part, diag_hist, diag_cc_prev};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (int s = 0; s < CHEM_MAXSP; ++s) {
@@ -429,6 +442,12 @@ struct Case : CaseBase {
                     solver_kind = 1;
                 }
             }
+        }
+        // the persistent solver's launch shape, tuned on this operator when
+        // the PCG is the field solver (PIC_PCG_BLOCKS fixes the grid instead)
+        if (!spectral && !getenv("PIC_PCG_BLOCKS") && S.A_size > 0) {
+            if (int e2 = pcg_autotune(pw, op, precision == PIC_FP32_REFINED, st, nullptr))
+                return fail(e2, "field solver autotuning failed");
         }
         // particles
         int capmax = 0;
@@ -713,6 +732,379 @@ struct Case : CaseBase {
         launch_eedf<R>(cur[0], G, S.Escale[0], (R)emax, eedf, st);
         HIPCHK(hipGetLastError());
         ++eedf_n;
+        return 0;
+    }
+    int diag_sample() override {
+        const int ncond = S.CondNUMR, len = diag_record_len(nsp, ncond);
+        if (!diag_hist) {
+            HIPCHK(dalloc(&diag_part, (size_t)nsp * S.Csize));
+            HIPCHK(dalloc(&diag_hist, (size_t)DIAG_RING * len));
+            HIPCHK(dalloc(&diag_cc_prev, (size_t)nsp * ncond));
+            HIPCHK(hipMemsetAsync(diag_cc_prev, 0, (size_t)nsp * ncond * sizeof(int), st));
+        }
+        DiagConst d{};
+        d.G = G; d.ngy = S.ngy; d.ncy = S.ncy; d.Csize = S.Csize; d.nsp = nsp; d.ncond = ncond;
+        d.dt = (double)S.dt;
+        for (int sp = 0; sp < nsp; ++sp) { d.mass[sp] = (double)S.mass[sp]; d.np2c[sp] = (double)S.np2c[sp]; }
+        PSet<R> ps{};
+        for (int sp = 0; sp < nsp; ++sp) ps.s[sp] = cur[sp];
+        double *rec = diag_hist + (size_t)(diag_n % DIAG_RING) * len;
+        launch_diag_sample<R>(d, ps, diag_part, Sigma, Area, CondID, condcharge, diag_cc_prev, V, tstep, rec, st);
+        HIPCHK(hipGetLastError());
+        ++diag_n;
+        return 0;
+    }
+    // the retained records, oldest first
+    int diag_history(double *out, int max_records, int *nrec, int *rec_len) override {
+        const int len = diag_record_len(nsp, S.CondNUMR);
+        if (rec_len) *rec_len = len;
+        const long long have = diag_n < DIAG_RING ? diag_n : DIAG_RING;
+        const int n = (int)(have < max_records ? have : max_records);
+        if (nrec) *nrec = n;
+        if (!out || n == 0) return 0;
+        HIPCHK(hipStreamSynchronize(st));
+        const long long first = diag_n - n;   // the newest n records
+        for (int r = 0; r < n; ++r) {
+            const long long q = (first + r) % DIAG_RING;
+            HIPCHK(hipMemcpyAsync(out + (size_t)r * len, diag_hist + (size_t)q * len, len * sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    // ------------------------------------------------------------ V000 dump
+    // The reference's dump format, version 0.0.0 (writer src/cuda_SaveDump.cu:5-106
+    // and 142-155, reader src/load.c:5-120): 4-byte ints and floats, t as a
+    // double, everything in the order below.
+    struct DumpOut {
+        FILE *f;
+        bool ok = true;
+        template <typename T>
+        void put(const T *p, size_t n) {
+            if (ok && n && fwrite(p, sizeof(T), n, f) != n) ok = false;
+        }
+        void i32(int v) { put(&v, 1); }
+        void f32(float v) { put(&v, 1); }
+        void f64(double v) { put(&v, 1); }
+        void zeros(size_t n) {
+            static const float z[1024] = {0};
+            while (n) {
+                const size_t k = std::min<size_t>(n, 1024);
+                put(z, k);
+                n -= k;
+            }
+        }
+    };
+    template <typename T>
+    int fetch(std::vector<T> &h, const T *d, size_t n) {
+        h.resize(n);
+        if (n) HIPCHK(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    int dump_save(const char *path) override {
+        HIPCHK(hipStreamSynchronize(st));
+        const int ncx = S.ncx, ncy = S.ncy, ngx = S.ngx, ngy = S.ngy, nc = S.CondNUMR;
+        // everything the file needs, read back before the file is opened
+        std::vector<std::vector<uint32_t>> recs(nsp);
+        for (int s = 0; s < nsp; ++s) {
+            long long n = 0;
+            if (int e = np_of(s, n)) return e;
+            recs[s].resize((size_t)n * 6);
+            if (n == 0) continue;
+            uint32_t *d = nullptr;
+            HIPCHK(hipMalloc(&d, (size_t)n * 24));
+            launch_dump_pack<R>(cur[s], ncy, ngy, S.Csize, d, st);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpyAsync(recs[s].data(), d, (size_t)n * 24, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            (void)hipFree(d);
+            if (e != hipSuccess) return fail(-50, "dump: particle records: %s", hipGetErrorString(e));
+        }
+        std::vector<R> hden, hsig, htot, hphi, hex, hey, hsrc, hSig;
+        std::vector<double> hsd, hsp, hsx, hsy;
+        std::vector<unsigned long long> hmccr;
+        if (int e = fetch(hden, den, (size_t)nsp * G)) return e;
+        if (int e = fetch(hsig, sigma_sp, (size_t)nsp * G)) return e;
+        if (int e = fetch(htot, tot, G)) return e;
+        if (int e = fetch(hphi, phi, G)) return e;
+        if (int e = fetch(hex, Ex, G)) return e;
+        if (int e = fetch(hey, Ey, G)) return e;
+        if (int e = fetch(hsrc, Source, G)) return e;
+        if (int e = fetch(hSig, Sigma, G)) return e;
+        if (int e = fetch(hsd, sum_den, (size_t)nsp * G)) return e;
+        if (int e = fetch(hsp, sum_phi, G)) return e;
+        if (int e = fetch(hsx, sum_ex, G)) return e;
+        if (int e = fetch(hsy, sum_ey, G)) return e;
+        if (int e = fetch(hmccr, mccr, (size_t)G * S.TnRct)) return e;
+        int len = 0, nrec = 0;
+        if (int e = diag_history(nullptr, DIAG_RING, &nrec, &len)) return e;
+        std::vector<double> hist((size_t)nrec * len);
+        if (nrec)
+            if (int e = diag_history(hist.data(), nrec, &nrec, &len)) return e;
+
+        FILE *f = fopen(path, "wb");
+        if (!f) return fail(-60, "dump: cannot open '%s' for writing", path);
+        DumpOut w{f};
+        w.i32(0); w.i32(0); w.i32(0);   // version KEY2.KEY1.KEY0
+        // time (cstep counts completed RF cycles, src/cuda_main.cu:116)
+        w.f64(t);
+        w.i32((int)tstep);
+        w.i32(dt_pic > 0 ? (int)(tstep / dt_pic) : 0);
+        w.i32(gas);
+        for (int s = 0; s < nsp; ++s) {
+            w.f32((float)S.np2c[s]);
+            w.i32((int)(recs[s].size() / 6));
+            w.put(recs[s].data(), recs[s].size());
+        }
+        // fluid species: the metastable densities are the deck's (no fluid
+        // equations run here), their sources and fluxes zero
+        w.i32(0);
+        std::vector<float> row((size_t)std::max(ngy, ngx));
+        for (int fs = 0; fs < S.nfsp; ++fs) {
+            for (int i = 0; i < ncx; ++i) {
+                for (int j = 0; j < ncy; ++j) row[j] = (float)S.FluidN[(size_t)fs * G + i * ngy + j];
+                w.put(row.data(), ncy);
+                w.zeros(ncy);
+            }
+            w.zeros((size_t)ngx * ncy);
+            w.zeros((size_t)ncx * ngy);
+        }
+        // history: the retained diagnostics records (pic_diag_sample), oldest first
+        const int o_np = 2, o_q = 2 + 2 * nsp, o_ab = o_q + nc, o_v = o_ab + nsp * nc;
+        auto rec = [&](int r, int k) { return hist[(size_t)r * len + k]; };
+        auto spacing = [&](int r) {
+            const double prev = r > 0 ? rec(r - 1, 1) : (double)tstep_base;
+            return std::max(1.0, rec(r, 1) - prev);
+        };
+        w.i32(nrec);
+        w.i32(nrec >= 2 ? (int)spacing(nrec - 1) : 1);
+        for (int r = 0; r < nrec; ++r) w.f32((float)rec(r, 0));
+        for (int s = 0; s < nsp; ++s)
+            for (int r = 0; r < nrec; ++r) w.f32((float)rec(r, o_np + s));
+        for (int fs = 0; fs < S.nfsp; ++fs) {
+            double a = 0;
+            int m = 0;
+            for (int c = 0; c < S.Csize; ++c)
+                if (S.PlasmaRegion[c]) { a += (double)S.FluidN[(size_t)fs * G + (c / ncy) * ngy + c % ncy]; ++m; }
+            const float mean = m ? (float)(a / m) : 0.f;
+            for (int r = 0; r < nrec; ++r) w.f32(mean);
+        }
+        w.zeros(nrec);   // field iterations are not kept per record
+        const double dt = (double)S.dt;
+        for (int i = 0; i < nc; ++i) {
+            // current into conductor i: displacement (backward difference of
+            // its charge) minus conduction (absorbed charge per unit time),
+            // src/cuda_Diagnostic.cu:181-190
+            for (int r = 0; r < nrec; ++r) {
+                const double h = spacing(r) * dt;
+                double cond = 0;
+                for (int s = 0; s < nsp; ++s) cond += (double)S.q_density[s] * rec(r, o_ab + s * nc + i) / h;
+                double dis = 0;
+                if (r >= 2 && spacing(r) == spacing(r - 1))
+                    dis = 0.5 * (3 * rec(r, o_q + i) - 4 * rec(r - 1, o_q + i) + rec(r - 2, o_q + i)) / h;
+                else if (r >= 1)
+                    dis = (rec(r, o_q + i) - rec(r - 1, o_q + i)) / h;
+                w.f32((float)(dis - cond));
+            }
+            for (int s = 0; s < nsp; ++s)
+                for (int r = 0; r < nrec; ++r)
+                    w.f32((float)((double)S.q_density[s] * rec(r, o_ab + s * nc + i) / (spacing(r) * dt)));
+            for (int r = 0; r < nrec; ++r) w.f32((float)rec(r, o_v + i));
+            for (int r = 0; r < nrec; ++r) w.f32((float)rec(r, o_v + i));
+            for (int r = 0; r < nrec; ++r) w.f32((float)rec(r, o_q + i));
+        }
+        w.i32(0);   // steady-state history: none
+        // per species and node: density, its running average, surface charge
+        for (int s = 0; s < nsp; ++s)
+            for (int g = 0; g < G; ++g) {
+                const size_t a = (size_t)s * G + g;
+                w.f32((float)hden[a]);
+                w.f32(n_accum ? (float)(hsd[a] / (double)n_accum) : 0.f);
+                w.f32((float)hsig[a]);
+            }
+        // per node: background gas, Laplace and Poisson potentials, E
+        for (int g = 0; g < G; ++g) {
+            w.f32((float)S.Temp[g]);
+            w.f32((float)S.BackVel1[g]);
+            w.f32((float)S.BackDen1[g]);
+            w.f32(S.BackVel2.empty() ? 0.f : (float)S.BackVel2[g]);
+            w.f32(S.BackDen2.empty() ? 0.f : (float)S.BackDen2[g]);
+            w.f32((float)(htot[g] - hphi[g]));
+            w.f32((float)hphi[g]);
+            w.f32((float)hex[g]);
+            w.f32((float)hey[g]);
+        }
+        // potential and the averages (source and surface charge: current values)
+        std::vector<float> fl(G);
+        auto put_r = [&](const std::vector<R> &v) { for (int g = 0; g < G; ++g) fl[g] = (float)v[g]; w.put(fl.data(), G); };
+        auto put_avg = [&](const std::vector<double> &v) {
+            for (int g = 0; g < G; ++g) fl[g] = n_accum ? (float)(v[g] / (double)n_accum) : 0.f;
+            w.put(fl.data(), G);
+        };
+        put_r(htot);
+        put_avg(hsp);
+        put_r(hsrc);
+        put_r(hSig);
+        put_avg(hsx);
+        put_avg(hsy);
+        // collision rates per node and reaction (src/cuda_Diagnostic.cu:548-560)
+        const double steps = (double)std::max(1LL, tstep - tstep_base);
+        const double scale = (double)S.Denscale[0] / steps / dt;
+        std::vector<float> mr(hmccr.size());
+        for (size_t k = 0; k < hmccr.size(); ++k) mr[k] = (float)(scale * (double)hmccr[k]);
+        w.put(mr.data(), mr.size());
+        const bool ok = w.ok;
+        if (fclose(f) != 0 || !ok) return fail(-60, "dump: writing '%s' failed", path);
+        return 0;
+    }
+    struct DumpIn {
+        FILE *f;
+        long size = 0;   // file length: a skip may not pass it
+        bool ok = true;
+        template <typename T>
+        void get(T *p, size_t n) {
+            if (ok && n && fread(p, sizeof(T), n, f) != n) ok = false;
+        }
+        int i32() { int v = 0; get(&v, 1); return v; }
+        void skip(size_t nfloats) {
+            if (!ok || !nfloats) return;
+            const long at = ftell(f);
+            if (at < 0 || (size_t)(size - at) < nfloats * 4 || fseek(f, (long)(nfloats * 4), SEEK_CUR) != 0) ok = false;
+        }
+    };
+    int dump_load(const char *path) override {
+        FILE *f = fopen(path, "rb");
+        if (!f) return fail(-60, "dump: cannot open '%s'", path);
+        std::unique_ptr<FILE, int (*)(FILE *)> guard(f, fclose);
+        DumpIn r{f};
+        if (fseek(f, 0, SEEK_END) != 0 || (r.size = ftell(f)) < 0 || fseek(f, 0, SEEK_SET) != 0)
+            return fail(-60, "dump: cannot read '%s'", path);
+        const int ncx = S.ncx, ncy = S.ncy, ngx = S.ngx, ngy = S.ngy, nc = S.CondNUMR;
+        auto early = [&](const char *where) { return fail(-61, "dump '%s' ends early (%s)", path, where); };
+        const int k2 = r.i32(), k1 = r.i32(), k0 = r.i32();
+        if (!r.ok) return early("version");
+        if (k2 < 0 || k1 < 0 || k0 < 0) return fail(-62, "dump version %d.%d.%d is not readable", k2, k1, k0);
+        double tt = 0;
+        r.get(&tt, 1);
+        const int ts = r.i32();
+        (void)r.i32();   // completed RF cycles, derived from the step
+        const int g_dump = r.i32();
+        if (!r.ok) return early("time");
+        if (g_dump != gas) return fail(-63, "gas type of the dump (%d) differs from the deck's (%d)", g_dump, gas);
+        if (ts < 0) return fail(-62, "dump step %d is negative", ts);
+        std::vector<std::vector<uint32_t>> recs(nsp);
+        for (int s = 0; s < nsp; ++s) {
+            float np2c = 0;
+            r.get(&np2c, 1);
+            const int n = r.i32();
+            if (!r.ok) return early("species header");
+            if (np2c != (float)S.np2c[s])
+                return fail(-64, "species %d: np2c of the dump (%g) differs from the deck's (%g)", s, (double)np2c,
+                            (double)(float)S.np2c[s]);
+            if (n < 0 || n > cur[s].cap)
+                return fail(-30, "species %d: dump holds %d particles, capacity %d", s, n, cur[s].cap);
+            recs[s].resize((size_t)n * 6);
+            r.get(recs[s].data(), recs[s].size());
+            if (!r.ok) return early("particles");
+        }
+        (void)r.i32();   // fluid continuity flag
+        r.skip((size_t)S.nfsp * ((size_t)2 * ncx * ncy + (size_t)ngx * ncy + (size_t)ncx * ngy));
+        const int hc = r.i32();
+        (void)r.i32();
+        if (!r.ok) return early("fluid");
+        if (hc < 0) return fail(-62, "dump history length %d is negative", hc);
+        r.skip((size_t)hc * (2 + nsp + S.nfsp + (size_t)nc * (4 + nsp)));
+        const int hac = r.i32();
+        if (!r.ok) return early("history");
+        if (hac < 0) return fail(-62, "dump steady-state history length %d is negative", hac);
+        r.skip((size_t)hac * (1 + 2 * nsp));
+        std::vector<float> g3((size_t)nsp * G * 3), g9((size_t)G * 9), fpot(G);
+        r.get(g3.data(), g3.size());
+        r.get(g9.data(), g9.size());
+        r.get(fpot.data(), G);
+        r.skip((size_t)5 * G + (size_t)S.TnRct * G);
+        if (!r.ok) return early("node arrays");
+
+        // particles: bucketed by the cell of their absolute position, in file
+        // order within a cell (src/cuda_Particle.cu:27-58); the in-cell offset
+        // is the position minus the cell's corner, which float subtraction
+        // gives exactly.  A position on the far wall (x == ncx or y == ncy,
+        // reached when a rounded sum lands on it) is kept in the last cell at
+        // the largest offset below 1.
+        const R below_one = (R)1 - (sizeof(R) == 4 ? (R)FLT_EPSILON / 2 : (R)DBL_EPSILON / 2);
+        for (int s = 0; s < nsp; ++s) {
+            const size_t n = recs[s].size() / 6;
+            std::vector<int> node(n);
+            std::vector<R> ox(n), oy(n);
+            std::vector<int> cs(G + 1, 0);
+            for (size_t k = 0; k < n; ++k) {
+                float x, y;
+                memcpy(&x, &recs[s][k * 6 + 1], 4);
+                memcpy(&y, &recs[s][k * 6 + 2], 4);
+                if (!(x >= 0.f && x <= (float)ncx && y >= 0.f && y <= (float)ncy))
+                    return fail(-65, "species %d particle %zu lies outside the grid (%g, %g)", s, k, (double)x,
+                                (double)y);
+                int xi = (int)x, yi = (int)y;
+                ox[k] = xi < ncx ? (R)(x - (float)xi) : below_one;
+                oy[k] = yi < ncy ? (R)(y - (float)yi) : below_one;
+                xi = std::min(xi, ncx - 1);
+                yi = std::min(yi, ncy - 1);
+                node[k] = xi * ngy + yi;
+                ++cs[node[k] + 1];
+            }
+            for (int c = 0; c < G; ++c) cs[c + 1] += cs[c];
+            std::vector<int> fill(cs.begin(), cs.end() - 1);
+            std::vector<R> px(n), py(n), pvx(n), pvy(n), pvz(n);
+            std::vector<uint32_t> pid(n);
+            for (size_t k = 0; k < n; ++k) {
+                const int at = fill[node[k]]++;
+                float v[3];
+                memcpy(v, &recs[s][k * 6 + 3], 12);
+                px[at] = ox[k]; py[at] = oy[k];
+                pvx[at] = (R)v[0]; pvy[at] = (R)v[1]; pvz[at] = (R)v[2];
+            }
+            for (size_t k = 0; k < n; ++k) pid[k] = (uint32_t)k;
+            if (int e = set_particles(s, (long long)n, px.data(), py.data(), pvx.data(), pvy.data(), pvz.data(),
+                                      pid.data(), cs.data(), (uint32_t)n))
+                return e;
+        }
+        // surface charge per species; Poisson potential, E and the total potential
+        std::vector<R> hs((size_t)nsp * G), hp(G), hx(G), hy(G), ht(G);
+        for (size_t a = 0; a < hs.size(); ++a) hs[a] = (R)g3[a * 3 + 2];
+        for (int g = 0; g < G; ++g) {
+            hp[g] = (R)g9[(size_t)g * 9 + 6];
+            hx[g] = (R)g9[(size_t)g * 9 + 7];
+            hy[g] = (R)g9[(size_t)g * 9 + 8];
+            ht[g] = (R)fpot[g];
+        }
+        std::vector<double> hx64(S.A_size, 0.0);
+        for (int g = 0; g < G; ++g)
+            if (S.A_idx[g] >= 0 && S.A_idx[g] < S.A_size) hx64[S.A_idx[g]] = (double)hp[g];
+        HIPCHK(hipMemcpy(sigma_sp, hs.data(), hs.size() * sizeof(R), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(phi, hp.data(), G * sizeof(R), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(Ex, hx.data(), G * sizeof(R), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(Ey, hy.data(), G * sizeof(R), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(tot, ht.data(), G * sizeof(R), hipMemcpyHostToDevice));
+        if (S.A_size) HIPCHK(hipMemcpy(x64, hx64.data(), hx64.size() * sizeof(double), hipMemcpyHostToDevice));
+        // counters start again at the dump's step
+        HIPCHK(hipMemsetAsync(mccr, 0, (size_t)G * S.TnRct * sizeof(unsigned long long), st));
+        HIPCHK(hipMemsetAsync(condcharge, 0, (size_t)nsp * nc * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(lost, 0, (size_t)nsp * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(sigma_hits, 0, (size_t)nsp * G * sizeof(unsigned long long), st));
+        HIPCHK(hipMemsetAsync(sum_den, 0, (size_t)nsp * G * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(sum_phi, 0, (size_t)G * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(sum_ex, 0, (size_t)G * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(sum_ey, 0, (size_t)G * sizeof(double), st));
+        if (o2delcnt) HIPCHK(hipMemsetAsync(o2delcnt, 0, (size_t)G * sizeof(unsigned long long), st));
+        if (diag_cc_prev) HIPCHK(hipMemsetAsync(diag_cc_prev, 0, (size_t)nsp * nc * sizeof(int), st));
+        n_accum = 0;
+        diag_n = 0;
+        set_step(ts);
+        tstep_base = ts;
+        // the deposit that starts a run (src/cuda_main.cu:28): densities,
+        // source and the dielectric surface charge of the loaded state
+        if (int e = deposit()) return e;
+        HIPCHK(hipStreamSynchronize(st));
         return 0;
     }
     int diagnostic() override {
@@ -1054,6 +1446,15 @@ struct Case : CaseBase {
         t = (double)s * (double)S.dt;
         return 0;
     }
+    // launch shape of the persistent PCG (0, 0 when the separable solve is used)
+    int solver_shape(int *blocks, int *threads) override {
+        if (spectral || !pw) {
+            if (blocks) *blocks = 0;
+            if (threads) *threads = 0;
+            return 0;
+        }
+        return pcg_launch_shape(pw, blocks, threads);
+    }
     int last_solve(int *iters, int *outer, double *rel) override {
         PcgResult r{};
         HIPCHK(hipStreamSynchronize(st));
@@ -1260,6 +1661,21 @@ int pic_device_memory(int device, size_t *free_bytes, size_t *total_bytes) {
     return 0;
 }
 int pic_eedf_sample(pic_ctx *ctx, double emax_ev) { CTX_OR_FAIL(ctx); return ctx->c->eedf_sample(emax_ev); }
+int pic_dump_save(pic_ctx *ctx, const char *path) {
+    CTX_OR_FAIL(ctx);
+    if (!path) return fail(-3, "dump: no path");
+    return ctx->c->dump_save(path);
+}
+int pic_dump_load(pic_ctx *ctx, const char *path) {
+    CTX_OR_FAIL(ctx);
+    if (!path) return fail(-3, "dump: no path");
+    return ctx->c->dump_load(path);
+}
+int pic_diag_sample(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->diag_sample(); }
+int pic_diag_history(pic_ctx *ctx, double *out, int max_records, int *nrec, int *rec_len) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->diag_history(out, max_records, nrec, rec_len);
+}
 int pic_advance_time(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->advance_time(); }
 int pic_field_solve(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->field_solve(); }
 int pic_efield(pic_ctx *ctx) { CTX_OR_FAIL(ctx); return ctx->c->efield(); }
@@ -1335,6 +1751,10 @@ int pic_stage_times(pic_ctx *ctx, double *ms, int n, long long *steps) {
 int pic_field_solver_kind(pic_ctx *ctx) {
     CTX_OR_FAIL(ctx);
     return ctx->c->solver_kind;
+}
+int pic_field_solver_shape(pic_ctx *ctx, int *blocks, int *threads) {
+    CTX_OR_FAIL(ctx);
+    return ctx->c->solver_shape(blocks, threads);
 }
 
 // ------------------------------------------------------------------ host-only setup

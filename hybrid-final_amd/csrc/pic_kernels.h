@@ -170,8 +170,28 @@ struct PcgWork;   // device scratch owned by the context
 PcgWork *pcg_work_create(int N, int *err, int want_blocks, hipStream_t stream);
 void pcg_work_destroy(PcgWork *);
 int pcg_grid_blocks(PcgWork *);
+// per-step diagnostics (diag.hip)
+struct DiagConst {
+    int G, ngy, ncy, Csize, nsp, ncond;
+    double dt;
+    double mass[5], np2c[5];
+};
+int diag_record_len(int nsp, int ncond);
+template <typename R>
+void launch_diag_sample(const DiagConst &d, const PSet<R> &ps, double *part, const R *Sigma, const R *Area,
+                        const int *CondID, const int *condcharge, int *cc_prev, const R *V, long long step,
+                        double *rec, hipStream_t s);
+// V000 dump particle records of one species, in slot order (dump.hip)
+template <typename R>
+void launch_dump_pack(const PView<R> &p, int ncy, int ngy, int Csize, uint32_t *rec, hipStream_t s);
+
 // clears the grid barrier after an abandoned solve (stream-ordered after it)
 hipError_t pcg_work_reset(PcgWork *, hipStream_t);
+// picks the persistent solver's threads per block and grid size for this
+// operator by timing fixed-length trial solves (cached per size and path);
+// synchronises the stream
+int pcg_autotune(PcgWork *, const PcgOperator &A, int refined, hipStream_t st, double *best_us_per_iter);
+int pcg_launch_shape(const PcgWork *, int *blocks, int *threads);
 
 // x (FP64, warm start, in/out), b (FP64).  Writes the result to *res (device
 // memory) so the solve can run inside a captured graph.

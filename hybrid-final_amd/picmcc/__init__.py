@@ -72,6 +72,11 @@ EXPORTS = {
     "pic_last_field_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_double)]),
     "pic_field_solver_kind": (C.c_int, [C.c_void_p]),
+    "pic_field_solver_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "pic_diag_sample": (C.c_int, [C.c_void_p]),
+    "pic_dump_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pic_dump_load": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pic_diag_history": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "pic_stage_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "pic_stage_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_longlong)]),
     "pic_pbatch_create": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
@@ -260,6 +265,45 @@ class Case:
         n = C.c_longlong()
         _check(lib().pic_stage_times(self.h, ms, len(self.STAGES), C.byref(n)))
         return dict(zip(self.STAGES, list(ms))), n.value
+
+    def dump_save(self, path):
+        """Writes the case's state in the reference's V000 dump format."""
+        _check(lib().pic_dump_save(self.h, os.fsencode(path)))
+
+    def dump_load(self, path):
+        """Resumes the case from a V000 dump (same deck: gas, grid and np2c)."""
+        _check(lib().pic_dump_load(self.h, os.fsencode(path)))
+
+    def diag_sample(self):
+        """Appends a diagnostics record of the last completed step (no host synchronisation)."""
+        _check(lib().pic_diag_sample(self.h))
+
+    def diag_history(self):
+        """The retained diagnostics records, oldest first, as a dict of arrays:
+        t, step, np (records x nsp), ke (J, records x nsp), q_cond (C, records x
+        ncond), absorbed (records x nsp x ncond), v_cond (records x ncond)."""
+        L = lib()
+        n, ln = C.c_int(), C.c_int()
+        _check(L.pic_diag_history(self.h, None, 1 << 30, C.byref(n), C.byref(ln)))
+        buf = np.empty((max(n.value, 1), ln.value), dtype=np.float64)
+        _check(L.pic_diag_history(self.h, _ptr(buf), n.value, C.byref(n), C.byref(ln)))
+        buf = buf[:n.value]
+        inf = self.info()
+        nsp, nc = inf.nsp, inf.CondNUMR
+        o = 2
+        out = {"t": buf[:, 0], "step": buf[:, 1].astype(np.int64)}
+        out["np"] = buf[:, o:o + nsp].astype(np.int64); o += nsp
+        out["ke"] = buf[:, o:o + nsp]; o += nsp
+        out["q_cond"] = buf[:, o:o + nc]; o += nc
+        out["absorbed"] = buf[:, o:o + nsp * nc].reshape(-1, nsp, nc); o += nsp * nc
+        out["v_cond"] = buf[:, o:o + nc]
+        return out
+
+    def field_solver_shape(self):
+        """(blocks, threads per block) of the persistent PCG; (0, 0) with the separable solve."""
+        b, t = C.c_int(), C.c_int()
+        _check(lib().pic_field_solver_shape(self.h, C.byref(b), C.byref(t)))
+        return b.value, t.value
 
     def field_solver_kind(self):
         return lib().pic_field_solver_kind(self.h)

@@ -82,6 +82,38 @@ int pic_diagnostic(pic_ctx *ctx);    /* profile accumulators, src/cuda_Diagnosti
  * energies) to the "eedf" array; a different emax_ev restarts it.  The
  * reference's EEDF diagnostic is commented out (src/start.c:934-967). */
 int pic_eedf_sample(pic_ctx *ctx, double emax_ev);
+/* Diagnostics of the last completed step, in FP64 (src/cuda_Diagnostic.cu:37-267):
+ * appends one record to the case's history (enqueued, no host
+ * synchronisation; the newest 4096 records are kept).  Record layout,
+ * doubles: t, step, np[nsp], kinetic energy per species (sum of m v^2/2 x
+ * np2c, J per metre of depth), charge of each conductor (sum of Sigma x
+ * Area over its nodes, the reference's Surf_charge), particles of each
+ * (species, conductor) absorbed since the previous record, conductor
+ * voltages.  The reference's conduction current is q_density x absorbed /
+ * (steps between records x dt); its displacement current is the backward
+ * difference of the conductor charge. */
+int pic_diag_sample(pic_ctx *ctx);
+/* copies up to max_records of the history, oldest first, into out
+ * (max_records x *rec_len doubles; out may be NULL to query the sizes) */
+int pic_diag_history(pic_ctx *ctx, double *out, int max_records, int *nrec, int *rec_len);
+/* Checkpoint in the reference's dump format, version 0.0.0 (V000_DUMP,
+ * src/cuda_SaveDump.cu:5-106, with the version header of SaveDumpFile,
+ * src/cuda_SaveDump.cu:151-155): time, per species np2c, count and
+ * particles (node index, absolute position in cell units, velocity),
+ * fluid arrays, histories (from pic_diag_sample records), per-node
+ * densities and surface charge, fields, averages and collision rates.
+ * Synchronises the case's stream. */
+int pic_dump_save(pic_ctx *ctx, const char *path);
+/* Resume from such a file (LoadDumpFile / V000_LoadDUMP, src/load.c:5-120,
+ * then the particle bucketing of Set_Particle_cuda, src/cuda_Particle.cu:3-58,
+ * and the initial deposit, src/cuda_main.cu:28).  Restores the step and
+ * time, the particles (ids renumbered 0..n-1 in cell order), the surface
+ * charge per species, the Poisson and total potentials and E; counters and
+ * averages start again.  -63: the dump's gas type differs from the deck's;
+ * -64: np2c differs (the reference warns and adopts the dump's value);
+ * -30: more particles than the capacity; -61: the file ends early;
+ * -65: a particle outside the grid. */
+int pic_dump_load(pic_ctx *ctx, const char *path);
 /* one full timestep through the stage functions above, nsteps times */
 int pic_run_staged(pic_ctx *ctx, int nsteps);
 /* one full timestep on the fused path (push+walls+MCC counted, scanned and
@@ -113,6 +145,12 @@ int pic_stage_timing(pic_ctx *ctx, int enable);
 int pic_stage_times(pic_ctx *ctx, double *ms, int n, long long *steps);
 /* 0 = Jacobi PCG, 1 = separable fast solve (FP64 fast-Poisson inverse in FP64 refinement) */
 int pic_field_solver_kind(pic_ctx *ctx);
+/* launch shape of the persistent PCG (blocks in the grid, threads per block),
+ * chosen at pic_create by timing trial solves of the case's operator (kept
+ * per operator size for the process; the environment variable
+ * PIC_PCG_BLOCKS fixes the grid instead); 0, 0 with the separable solve.
+ * No counterpart in the reference, whose PCG_float launch is fixed. */
+int pic_field_solver_shape(pic_ctx *ctx, int *blocks, int *threads);
 
 /* ---------------- batched Poisson solves ---------------- */
 /* Many independent 5-point problems (unit spacing, operator = the reference

@@ -164,7 +164,8 @@ def deck_to_struct(deck):
 
 
 class Oracle:
-    """One oracle case.  precision is 'f32' or 'f64'."""
+    """One oracle case.  precision is 'f32', 'f64', or 'f64_omp' (the FP64
+    oracle built with OpenMP, which adds orc_run_par(nsteps, threads))."""
 
     def __init__(self, deck, precision="f64"):
         path = os.path.join(HERE, f"liborc_{precision}.so")
@@ -191,6 +192,9 @@ class Oracle:
         L.orc_destroy.restype = None
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_voltages.argtypes = [C.c_void_p, C.c_double]
+        if hasattr(L, "orc_run_par"):
+            L.orc_run_par.restype = C.c_int
+            L.orc_run_par.argtypes = [C.c_void_p, C.c_int, C.c_int]
         assert L.orc_sizeof_deck() == C.sizeof(OrcDeck), "orc_deck layout mismatch"
         self.real = np.float32 if precision == "f32" else np.float64
         self.deck_struct = deck_to_struct(deck)

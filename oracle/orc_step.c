@@ -93,6 +93,7 @@ void orc_destroy(orc_case *c) {
     for (s = 0; s < ORC_MAXSP; ++s) {
         orc_species *p = &c->sp[s];
         free(p->x); free(p->y); free(p->vx); free(p->vy); free(p->vz); free(p->id); free(p->cell_start);
+        free(p->sx); free(p->sy); free(p->svx); free(p->svy); free(p->svz); free(p->sid);
     }
     free(c);
 }
@@ -527,18 +528,25 @@ static void dmaxwellv(real *vx, real *vy, real *vz, real vsaven, real vti, real 
 
 typedef struct { real x, y, vx, vy, vz; int cell; } orc_prod;
 
-/* Appends products to each bucket in parent order; returns 0 or -30 on
- * capacity overflow. */
-int orc_mcc(orc_case *c) {
+/* Growable product lists of one MCC pass (electrons and ions pair up). */
+typedef struct { orc_prod *e, *i; int n, cap; } orc_prods;
+
+static void prods_init(orc_prods *P, int cap) {
+    P->cap = cap < 16 ? 16 : cap;
+    P->n = 0;
+    P->e = zalloc(P->cap, sizeof(orc_prod));
+    P->i = zalloc(P->cap, sizeof(orc_prod));
+}
+
+/* Collisions of the ions and electrons of cell ID; products go to P in
+ * parent order.  Touches only the cell's particles and its rows of mccr. */
+static void mcc_cell(orc_case *c, int ID, orc_prods *P) {
     orc_species *se = &c->sp[0], *si = &c->sp[1];
-    int G = c->Gsize, ID, k;
+    int k;
     uint64_t seed = (uint64_t)c->deck.rng_seed;
     uint32_t step = (uint32_t)c->tstep;
     int T = c->TnRct;
-    int maxp = se->n + 16;
-    orc_prod *pe = zalloc(maxp, sizeof(orc_prod)), *pi = zalloc(maxp, sizeof(orc_prod));
-    int npe = 0, npi = 0;
-    for (ID = 0; ID < G; ++ID) {
+    {
         /* ions (Direct_Argon_ArIon) */
         real Prob = c->Pion[ID];
         for (k = si->cell_start[ID]; k < si->cell_start[ID + 1]; ++k) {
@@ -616,7 +624,8 @@ int orc_mcc(orc_case *c) {
                 anewvel(engy, vel, &VX, &VY, &VZ, a, b);
                 se->vx[k] = VX; se->vy[k] = VY; se->vz[k] = VZ;
             } else if (type == 2) {
-                orc_prod *e = &pe[npe++], *ion = &pi[npi++];
+                orc_prod *e = &P->e[P->n], *ion = &P->i[P->n];
+                P->n++;
                 e->cell = ID; e->x = se->x[k]; e->y = se->y[k];
                 e->vx = VX; e->vy = VY; e->vz = VZ;
                 real a = U(&st), b = U(&st);
@@ -629,12 +638,42 @@ int orc_mcc(orc_case *c) {
                 anewvel(engy, vel2, &VX, &VY, &VZ, a, b);
                 se->vx[k] = VX; se->vy[k] = VY; se->vz[k] = VZ;
             }
-            if (npe >= maxp - 1) {
-                maxp *= 2;
-                pe = realloc(pe, maxp * sizeof(orc_prod)); pi = realloc(pi, maxp * sizeof(orc_prod));
+            if (P->n >= P->cap - 1) {
+                P->cap *= 2;
+                P->e = realloc(P->e, P->cap * sizeof(orc_prod)); P->i = realloc(P->i, P->cap * sizeof(orc_prod));
             }
         }
     }
+}
+
+/* The species' second array set (allocated on first use). */
+static void spare_get(orc_species *s, real **x, real **y, real **vx, real **vy, real **vz, uint32_t **id) {
+    if (!s->sx) {
+        s->sx = malloc(s->cap * sizeof(real)); s->sy = malloc(s->cap * sizeof(real));
+        s->svx = malloc(s->cap * sizeof(real)); s->svy = malloc(s->cap * sizeof(real));
+        s->svz = malloc(s->cap * sizeof(real)); s->sid = malloc(s->cap * sizeof(uint32_t));
+    }
+    *x = s->sx; *y = s->sy; *vx = s->svx; *vy = s->svy; *vz = s->svz; *id = s->sid;
+}
+
+/* makes the second set current and the current one the second */
+static void spare_swap(orc_species *s) {
+    real *t;
+    uint32_t *u;
+    t = s->x; s->x = s->sx; s->sx = t;
+    t = s->y; s->y = s->sy; s->sy = t;
+    t = s->vx; s->vx = s->svx; s->svx = t;
+    t = s->vy; s->vy = s->svy; s->svy = t;
+    t = s->vz; s->vz = s->svz; s->svz = t;
+    u = s->id; s->id = s->sid; s->sid = u;
+}
+
+/* Appends products to each bucket in parent order; returns 0 or -30 on
+ * capacity overflow.  reuse: rebuild into the species' second array set
+ * instead of fresh arrays. */
+static int mcc_append(orc_case *c, orc_prod *pe, orc_prod *pi, int npe, int npi, int reuse) {
+    orc_species *se = &c->sp[0], *si = &c->sp[1];
+    int G = c->Gsize;
     /* append products: bucket d = old bucket ++ products of d (parent order);
      * new ids in global (cell, parent) order */
     orc_species *sps[2] = {se, si};
@@ -644,15 +683,22 @@ int orc_mcc(orc_case *c) {
         orc_species *s = sps[s2];
         int np = nps[s2];
         if (s->n + np > s->cap) { rc = -30; continue; }
+        if (np == 0) continue;   /* buckets unchanged */
         int *pc = zalloc(G, sizeof(int)), d, q;
         for (q = 0; q < np; ++q) pc[lists[s2][q].cell]++;
         int *start = zalloc(G + 1, sizeof(int));
         for (d = 0; d < G; ++d) start[d + 1] = start[d] + (s->cell_start[d + 1] - s->cell_start[d]) + pc[d];
         int tot = start[G];
-        real *nx = zalloc(s->cap, sizeof(real)), *ny = zalloc(s->cap, sizeof(real));
-        real *nvx = zalloc(s->cap, sizeof(real)), *nvy = zalloc(s->cap, sizeof(real)), *nvz = zalloc(s->cap, sizeof(real));
-        uint32_t *nid = zalloc(s->cap, sizeof(uint32_t));
+        real *nx, *ny, *nvx, *nvy, *nvz;
+        uint32_t *nid;
+        if (reuse) spare_get(s, &nx, &ny, &nvx, &nvy, &nvz, &nid);
+        else {
+            nx = zalloc(s->cap, sizeof(real)); ny = zalloc(s->cap, sizeof(real));
+            nvx = zalloc(s->cap, sizeof(real)); nvy = zalloc(s->cap, sizeof(real)); nvz = zalloc(s->cap, sizeof(real));
+            nid = zalloc(s->cap, sizeof(uint32_t));
+        }
         int *fill = zalloc(G, sizeof(int));
+#pragma omp parallel for schedule(dynamic, 64)
         for (d = 0; d < G; ++d) {
             int o = s->cell_start[d], m = s->cell_start[d + 1] - o, r;
             for (r = 0; r < m; ++r) {
@@ -668,13 +714,25 @@ int orc_mcc(orc_case *c) {
             nx[pos] = p->x; ny[pos] = p->y; nvx[pos] = p->vx; nvy[pos] = p->vy; nvz[pos] = p->vz;
             nid[pos] = s->next_id++;
         }
-        free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->vz); free(s->id);
-        s->x = nx; s->y = ny; s->vx = nvx; s->vy = nvy; s->vz = nvz; s->id = nid;
+        if (reuse) spare_swap(s);
+        else {
+            free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->vz); free(s->id);
+            s->x = nx; s->y = ny; s->vx = nvx; s->vy = nvy; s->vz = nvz; s->id = nid;
+        }
         memcpy(s->cell_start, start, (G + 1) * sizeof(int));
         s->n = tot;
         free(pc); free(start); free(fill);
     }
-    free(pe); free(pi);
+    return rc;
+}
+
+int orc_mcc(orc_case *c) {
+    orc_prods P;
+    int ID, rc;
+    prods_init(&P, c->sp[0].n + 16);
+    for (ID = 0; ID < c->Gsize; ++ID) mcc_cell(c, ID, &P);
+    rc = mcc_append(c, P.e, P.i, P.n, P.n, 0);
+    free(P.e); free(P.i);
     return rc;
 }
 
@@ -765,6 +823,162 @@ int orc_run(orc_case *c, int nsteps) {
     for (i = 0; i < nsteps; ++i) { e = orc_step(c); if (e) return e; }
     return 0;
 }
+
+/* ================= OpenMP step (CPU baseline only) =========================
+ * The same step with the particle loops split over cell tiles: the push of
+ * every cell, the bucket rebuild of the particles that stay in their cell,
+ * and the collisions of every cell run in parallel; the wall crossings (which
+ * add to shared surface charge and counters) and the placement of crossers
+ * and collision products run in particle order on one thread, so the result
+ * is the sequential step's, bit for bit.  Built only with -fopenmp. */
+#ifdef _OPENMP
+#include <omp.h>
+
+static void orc_move_par(orc_case *c) {
+    int isp, ngy = c->ngy;
+    for (isp = 0; isp < c->nsp; ++isp) {
+        orc_species *s = &c->sp[isp];
+        int *tag = c->tag + (size_t)isp * s->cap;
+        real As = c->Ascale[isp];
+        int cell;
+#pragma omp parallel for schedule(dynamic, 64)
+        for (cell = 0; cell < c->Gsize; ++cell) {
+            int k0 = s->cell_start[cell], k1 = s->cell_start[cell + 1], k;
+            if (k0 == k1) continue;
+            real ews = c->Ex[cell], ewn = c->Ex[cell + 1], ees = c->Ex[cell + ngy], een = c->Ex[cell + ngy + 1];
+            real yws = c->Ey[cell], ywn = c->Ey[cell + 1], yes = c->Ey[cell + ngy], yen = c->Ey[cell + ngy + 1];
+            for (k = k0; k < k1; ++k) {
+                real lx = s->x[k], ly = s->y[k];
+                real dvx = ews * (1 - lx) * (1 - ly) + ewn * (1 - lx) * ly + ees * lx * (1 - ly) + een * lx * ly;
+                real dvy = yws * (1 - lx) * (1 - ly) + ywn * (1 - lx) * ly + yes * lx * (1 - ly) + yen * lx * ly;
+                real mx = s->vx[k] + dvx * As;
+                real my = s->vy[k] + dvy * As;
+                lx += mx * c->dt_dx;
+                ly += my * c->dt_dy;
+                int t = 0;
+                if (ly >= 1 || ly < 0 || lx >= 1 || lx < 0) {
+                    if (ly >= 1) { t += 1; ly -= 1; }
+                    else if (ly < 0) { t -= 1; ly += 1; }
+                    if (lx >= 1) { t += ngy; lx -= 1; }
+                    else if (lx < 0) { t -= ngy; lx += 1; }
+                    if (lx >= 1) lx = ORC_ONE_MINUS; else if (lx < 0) lx = 0;
+                    if (ly >= 1) ly = ORC_ONE_MINUS; else if (ly < 0) ly = 0;
+                }
+                s->x[k] = lx; s->y[k] = ly; s->vx[k] = mx; s->vy[k] = my;
+                tag[k] = t;
+            }
+        }
+    }
+}
+
+static int orc_sort_par(orc_case *c) {
+    int isp;
+    for (isp = 0; isp < c->nsp; ++isp) {
+        orc_species *s = &c->sp[isp];
+        int *tag = c->tag + (size_t)isp * s->cap;
+        int n = s->n, G = c->Gsize, cell, k;
+        int *destof = zalloc(n, sizeof(int));
+        char *cls = zalloc(n, 1);
+        int *scnt = zalloc(G, sizeof(int)), *acnt = zalloc(G, sizeof(int));
+        /* stayers in parallel; crossers (cls 3) resolved below in order */
+#pragma omp parallel for schedule(dynamic, 64)
+        for (cell = 0; cell < G; ++cell) {
+            int m = 0, kk;
+            for (kk = s->cell_start[cell]; kk < s->cell_start[cell + 1]; ++kk)
+                if (tag[kk] == 0) { destof[kk] = cell; cls[kk] = 0; ++m; }
+                else cls[kk] = 3;
+            scnt[cell] = m;
+        }
+        for (cell = 0; cell < G; ++cell)
+            for (k = s->cell_start[cell]; k < s->cell_start[cell + 1]; ++k) {
+                if (cls[k] != 3) continue;
+                int d;
+                if (resolve_crosser(c, isp, cell, tag[k], &s->x[k], &s->y[k], &s->vx[k], &s->vy[k], &d)) {
+                    destof[k] = d; cls[k] = 1; acnt[d]++;
+                } else { destof[k] = -1; cls[k] = 2; }
+            }
+        int *start = zalloc(G + 1, sizeof(int));
+        for (cell = 0; cell < G; ++cell) start[cell + 1] = start[cell] + scnt[cell] + acnt[cell];
+        int total = start[G];
+        real *nx, *ny, *nvx, *nvy, *nvz;
+        uint32_t *nid;
+        spare_get(s, &nx, &ny, &nvx, &nvy, &nvz, &nid);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (cell = 0; cell < G; ++cell) {
+            int pos = start[cell], kk;
+            for (kk = s->cell_start[cell]; kk < s->cell_start[cell + 1]; ++kk) {
+                if (cls[kk] != 0) continue;
+                nx[pos] = s->x[kk]; ny[pos] = s->y[kk];
+                nvx[pos] = s->vx[kk]; nvy[pos] = s->vy[kk]; nvz[pos] = s->vz[kk]; nid[pos] = s->id[kk];
+                ++pos;
+            }
+        }
+        int *afill = zalloc(G, sizeof(int));
+        for (k = 0; k < n; ++k) {
+            if (cls[k] != 1) continue;
+            int d = destof[k], pos = start[d] + scnt[d] + afill[d]++;
+            nx[pos] = s->x[k]; ny[pos] = s->y[k];
+            nvx[pos] = s->vx[k]; nvy[pos] = s->vy[k]; nvz[pos] = s->vz[k]; nid[pos] = s->id[k];
+        }
+        spare_swap(s);
+        memcpy(s->cell_start, start, (G + 1) * sizeof(int));
+        s->n = total;
+        free(destof); free(cls); free(scnt); free(acnt); free(start); free(afill);
+#pragma omp parallel for schedule(static)
+        for (k = 0; k < n; ++k) tag[k] = 0;   /* the push tags only [0, n) */
+    }
+    return 0;
+}
+
+/* collisions over contiguous cell tiles, one per thread; each tile's
+ * products are in (cell, parent) order, so the tiles concatenated in order
+ * give the sequential list */
+static int orc_mcc_par(orc_case *c) {
+    int nt = omp_get_max_threads(), t, G = c->Gsize, rc;
+    orc_prods *P = zalloc(nt, sizeof(orc_prods));
+#pragma omp parallel num_threads(nt)
+    {
+        int me = omp_get_thread_num(), ID;
+        int a = (int)((long)G * me / nt), b = (int)((long)G * (me + 1) / nt);
+        prods_init(&P[me], 64);
+        for (ID = a; ID < b; ++ID) mcc_cell(c, ID, &P[me]);
+    }
+    int n = 0;
+    for (t = 0; t < nt; ++t) n += P[t].n;
+    orc_prod *pe = zalloc(n + 1, sizeof(orc_prod)), *pi = zalloc(n + 1, sizeof(orc_prod));
+    for (t = 0, n = 0; t < nt; ++t) {
+        memcpy(pe + n, P[t].e, P[t].n * sizeof(orc_prod));
+        memcpy(pi + n, P[t].i, P[t].n * sizeof(orc_prod));
+        n += P[t].n;
+        free(P[t].e); free(P[t].i);
+    }
+    free(P);
+    rc = mcc_append(c, pe, pi, n, n, 1);
+    free(pe); free(pi);
+    return rc;
+}
+
+/* nsteps of the argon step with `threads` OpenMP threads (0: the runtime's
+ * default); returns 0, or the first step's error */
+int orc_run_par(orc_case *c, int nsteps, int threads) {
+    int i, e;
+    if (threads > 0) omp_set_num_threads(threads);
+    for (i = 0; i < nsteps; ++i) {
+        c->tstep++;
+        c->t = (double)c->tstep * (double)c->dt;
+        orc_field(c);
+        orc_efield(c);
+        orc_move_par(c);
+        if ((e = orc_sort_par(c))) return e;
+        if ((e = orc_mcc_par(c))) return e;
+        orc_deposit(c);
+        orc_accumulate(c);
+    }
+    return 0;
+}
+
+int orc_threads(void) { return omp_get_max_threads(); }
+#endif
 
 /* ================= accessors for the Python test harness ================= */
 typedef struct { const char *name; void *ptr; long count; int elsize; } orc_arr;

@@ -82,6 +82,10 @@ typedef struct {
     uint32_t *id;
     int *cell_start;              /* Gsize+1, bucket of node-indexed cell c */
     uint32_t next_id;
+    /* second set of particle arrays that the OpenMP step rebuilds buckets
+     * into and swaps with the first (NULL until used) */
+    real *sx, *sy, *svx, *svy, *svz;
+    uint32_t *sid;
 } orc_species;
 
 typedef struct {

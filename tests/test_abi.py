@@ -36,7 +36,9 @@ def test_python_mirror_binds():
     import picmcc
     for m in ("advance_time", "field_solve", "efield", "move", "sort_boundary", "mcc", "deposit", "diagnostic",
               "step", "run_staged", "set_step", "info", "particles", "set_particles", "array", "set_array", "sync",
-              "close", "eedf_sample", "stage_timing", "stage_times", "last_field_solve", "field_solver_kind"):
+              "close", "eedf_sample", "stage_timing", "stage_times", "last_field_solve", "field_solver_kind",
+              "field_solver_shape", "diag_sample", "diag_history",
+              "dump_save", "dump_load"):
         assert callable(getattr(picmcc.Case, m, None)), f"Case.{m} is missing"
     for m in ("set_rhs", "solve", "sync", "result", "close"):
         assert callable(getattr(picmcc.PoissonBatch, m, None)), f"PoissonBatch.{m} is missing"

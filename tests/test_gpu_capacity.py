@@ -59,3 +59,39 @@ def test_abandoned_field_solve_is_reported_and_cleared(monkeypatch):
         assert it > 0 and rel <= 1e-10
     finally:
         ok.close()
+
+
+def test_field_solver_launch_shape_is_tuned_and_kept(monkeypatch):
+    """The persistent PCG's launch shape is tuned at case creation on the
+    case's own operator: threads per block 128, 256 or 512, a grid that the
+    device holds at once and no larger than the unknowns need; a second case
+    of the same size reuses it; PIC_PCG_BLOCKS fixes the grid instead.  The
+    tuned solver converges to the requested tolerance."""
+    d = decks.small(ncx=48, ncy=40, ppc=10, volts=150.0)
+    monkeypatch.delenv("PIC_PCG_BLOCKS", raising=False)
+    a = Case(d, FP64_PLAIN)
+    b = Case(d, FP64_PLAIN)
+    try:
+        nb, nt = a.field_solver_shape()
+        assert nt in (128, 256, 512) and 1 <= nb <= min(nt, (a.info().A_size + nt - 1) // nt)
+        assert b.field_solver_shape() == (nb, nt)
+        for c in (a, b):
+            c.advance_time()
+            c.field_solve()
+            c.sync()
+            it, outer, rel = c.last_field_solve()
+            assert it > 0 and rel <= 1e-10
+    finally:
+        a.close()
+        b.close()
+    monkeypatch.setenv("PIC_PCG_BLOCKS", "7")
+    f = Case(d, FP64_PLAIN)
+    try:
+        assert f.field_solver_shape() == (7, 256)
+    finally:
+        f.close()
+    s = Case(d, FP32_REFINED)   # separable operator: no PCG
+    try:
+        assert s.field_solver_kind() == 1 and s.field_solver_shape() == (0, 0)
+    finally:
+        s.close()
