@@ -247,6 +247,13 @@ def run_ensemble(n, rate_hz):
     out = serve.summarize(serve.EnsembleServer(max_active=16, setup_threads=2).run(reqs), alone)
     out["mode"] = ("concurrent: continuous admission by device-memory budget (at most 16 resident), one stream "
                    "per case, graph replays, case setup on 2 worker threads")
+    srv = serve.EnsembleServer(max_active=16, setup_threads=2, batch=True)
+    bat = serve.summarize(srv.run(reqs))
+    out["batched"] = {k: bat[k] for k in ("p50_latency_s", "p99_latency_s", "makespan_s",
+                                          "aggregate_particle_steps_per_s")}
+    out["batched"]["mode"] = ("concurrent admission as above; the resident cases stepped together through one "
+                              "pic_batch (each kernel of the step launched once for all of them)")
+    out["batched"]["batch_rebuilds"] = srv.rebuilds
     seq = serve.summarize(serve.SequentialServer().run(reqs))
     out["sequential"] = {k: seq[k] for k in ("p50_latency_s", "p99_latency_s", "makespan_s", "busy_fraction",
                                              "aggregate_particle_steps_per_s")}
@@ -267,6 +274,11 @@ def run_serve5(k, rate_hz=8.0):
     alone = [serve.run_alone(r) for r in reqs]
     res = serve.EnsembleServer(max_active=16, setup_threads=2).run(reqs)
     out = serve.summarize(res, alone)
+    srv = serve.EnsembleServer(max_active=16, setup_threads=2, batch=True)
+    bat = serve.summarize(srv.run(reqs))
+    out["batched"] = {k: bat[k] for k in ("p50_latency_s", "p99_latency_s", "makespan_s",
+                                          "aggregate_particle_steps_per_s", "failed")}
+    out["batched"]["batch_rebuilds"] = srv.rebuilds
     out["gas_mix"] = {"argon": sum(1 for r in reqs if serve.case_gas(r.deck) == 0),
                       "ar_o2": sum(1 for r in reqs if serve.case_gas(r.deck) == 2)}
     out["one_by_one_failed"] = sum(1 for r in alone if r["error"])
@@ -392,6 +404,13 @@ def main():
             t = json.load(f)["per_launch_bytes"].get(dom)
         traffic = t["total"] if t else None
 
+    # the same kernel's shader counters (rocprofv3 SQ pass of the same workload)
+    sq = None
+    qpath = os.path.join(ROOT, "profiles", "sq_ar_ccp_256x256.json")
+    if args.config == "c2" and os.path.exists(qpath):
+        with open(qpath) as f:
+            sq = json.load(f).get(dom)
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -449,6 +468,15 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_, "ms_per_launch": ms},
     }
+    if sq:
+        # the measured limiter: an HBM-bound kernel below the HBM peak whose
+        # waves mostly wait is limited by memory latency; one whose VALU is
+        # mostly busy is limited by instruction issue
+        lim = ("memory latency" if sq["wait_per_wave_cycle"] > sq["valu_per_wave_cycle"] else "VALU issue")
+        line["roofline"]["counters"] = {"valu_busy": sq["valu_busy"], "valu_per_wave_cycle": sq["valu_per_wave_cycle"],
+                                        "wait_per_wave_cycle": sq["wait_per_wave_cycle"],
+                                        "source": "profiles/sq_ar_ccp_256x256.md"}
+        line["roofline"]["limiter"] = lim
     if args.profile_cycles > 0:
         line["profile_error_vs_fp64"] = profile_check(deck, args.profile_cycles)
     if args.pbatch > 0:

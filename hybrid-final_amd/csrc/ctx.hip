@@ -19,6 +19,7 @@
 
 #include "../../include/picmcc.h"
 #include "mcc_o2.h"
+#include "multi.h"
 #include "pic_kernels.h"
 #include "pic_setup.h"
 
@@ -89,6 +90,7 @@ struct CaseBase {
     virtual int set_timing(int) = 0;
     virtual int get_timing(double *, int, long long *) = 0;
     int rf_cycles = 0, dt_pic = 0;
+    int batch_refs = 0;    // batches this case belongs to (it may not be destroyed meanwhile)
     int solver_kind = 0;   // 0 Jacobi PCG, 1 separable fast solve
     int gas = 0;           // 0 Ar, 1 O2, 2 Ar/O2 (selects the collision step)
 };
@@ -1570,6 +1572,191 @@ struct HostSetup : HostSetupBase {
     }
     bool scalar(const std::string &n, double &v) override { return setup_scalar<R>(S, n, v); }
 };
+
+// ------------------------------------------------------------------ batches
+// A batch steps several cases of one precision together: every kernel of
+// the fused step runs once for all of them, over a grid that concatenates
+// the cases' own grids (multi.h).  The launches of one step of each case are
+// recorded twice at set-up (once per particle buffer parity), the records of
+// the same launch placed side by side in device memory, and each parity's
+// merged sequence captured in a graph.  A batch step then is one graph
+// launch on the batch stream, ordered after the work already enqueued on the
+// cases' streams and before work enqueued on them later.  The cases give
+// the same bits as when stepped alone.
+struct BatchBase {
+    virtual ~BatchBase() {}
+    virtual int step(int n) = 0;
+    virtual int sync() = 0;
+};
+
+template <typename R>
+struct Batch : BatchBase {
+    struct Merged {
+        MultiLaunchFn fn;
+        const void *jobs;
+        const uint32_t *map;
+        int total;
+    };
+    std::vector<Case<R> *> cs;
+    hipStream_t st = nullptr;
+    int device = 0;
+    unsigned char *dmem = nullptr;
+    std::vector<Merged> seq[2];
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<const void *> expect[2];   // each case's electron x buffer at the batch's parity
+    int parity = 0;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t done = nullptr;
+
+    ~Batch() override {
+        if (st) (void)hipStreamSynchronize(st);
+        for (auto &g : gexec)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto e : ev) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
+        if (dmem) (void)hipFree(dmem);
+        if (st) (void)hipStreamDestroy(st);
+        for (auto *c : cs) --c->batch_refs;
+    }
+
+    int build(const std::vector<Case<R> *> &cases) {
+        cs = cases;
+        for (auto *c : cs) ++c->batch_refs;
+        device = cs[0]->device;
+        for (size_t i = 0; i < cs.size(); ++i) {
+            Case<R> *c = cs[i];
+            if (c->device != device) return fail(-70, "batch: case %zu is on another device", i);
+            if (c->gas != 0) return fail(-70, "batch: case %zu is not an argon case (its step is not the fused one)", i);
+            if (!c->spectral) return fail(-70, "batch: case %zu does not use the separable field solve", i);
+            if (c->timing) return fail(-70, "batch: case %zu has stage timers on", i);
+            if (c->spec_cycles != cs[0]->spec_cycles)
+                return fail(-70, "batch: case %zu's field solve runs %d cycles, case 0's %d", i, c->spec_cycles,
+                            cs[0]->spec_cycles);
+        }
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        ev.resize(cs.size(), nullptr);
+        for (auto &e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        // record one step of every case at each parity (recording launches
+        // nothing; each recorded step flips the case's host parity, so two
+        // leave it where it was)
+        std::vector<std::vector<RecordedLaunch>> rec[2];
+        for (int q = 0; q < 2; ++q) {
+            rec[q].resize(cs.size());
+            for (size_t i = 0; i < cs.size(); ++i) {
+                Case<R> *c = cs[i];
+                if (q == 0) {
+                    HIPCHK(hipStreamSynchronize(c->st));
+                    if (int e = c->ensure_clock()) return e;
+                    HIPCHK(hipStreamSynchronize(c->st));
+                }
+                expect[q].push_back(c->cur[0].x);
+                Recorder r;
+                launch_recorder() = &r;
+                const int e = c->enqueue_step();
+                launch_recorder() = nullptr;
+                if (e) return e;
+                rec[q][i] = std::move(r.launches);
+            }
+        }
+        // lay out the job records and block maps of both parities
+        struct Place { size_t jobs, map; int total; };
+        std::vector<Place> places[2];
+        size_t bytes = 0;
+        for (int q = 0; q < 2; ++q) {
+            const auto &r0 = rec[q][0];
+            for (size_t i = 1; i < cs.size(); ++i)
+                if (rec[q][i].size() != r0.size()) return fail(-70, "batch: case %zu's step has another launch sequence", i);
+            for (size_t l = 0; l < r0.size(); ++l) {
+                Place pl{};
+                const size_t stride = r0[l].job_size;   // the kernel indexes the records as an array
+                bytes = (bytes + 255) / 256 * 256;
+                pl.jobs = bytes;
+                bytes += stride * cs.size();
+                bytes = (bytes + 255) / 256 * 256;
+                pl.map = bytes;
+                int total = 0;
+                for (size_t i = 0; i < cs.size(); ++i) {
+                    const RecordedLaunch &x = rec[q][i][l];
+                    if (x.fn != r0[l].fn || x.job_size != r0[l].job_size)
+                        return fail(-70, "batch: case %zu's step has another launch sequence", i);
+                    if (x.blocks >= (1 << MULTI_LB_BITS)) return fail(-70, "batch: a launch of case %zu is too large", i);
+                    total += x.blocks;
+                }
+                pl.total = total;
+                bytes += (size_t)total * sizeof(uint32_t);
+                places[q].push_back(pl);
+            }
+        }
+        if (cs.size() >= (1u << (32 - MULTI_LB_BITS))) return fail(-70, "batch: too many cases");
+        std::vector<unsigned char> host(bytes, 0);
+        for (int q = 0; q < 2; ++q)
+            for (size_t l = 0; l < places[q].size(); ++l) {
+                const Place &pl = places[q][l];
+                const size_t stride = rec[q][0][l].job_size;
+                uint32_t *map = (uint32_t *)(host.data() + pl.map);
+                int b = 0;
+                for (size_t i = 0; i < cs.size(); ++i) {
+                    const RecordedLaunch &x = rec[q][i][l];
+                    memcpy(host.data() + pl.jobs + i * stride, x.job.data(), x.job_size);
+                    for (int k = 0; k < x.blocks; ++k) map[b++] = ((uint32_t)i << MULTI_LB_BITS) | (uint32_t)k;
+                }
+            }
+        HIPCHK(hipMalloc(&dmem, bytes ? bytes : 1));
+        HIPCHK(hipMemcpy(dmem, host.data(), bytes, hipMemcpyHostToDevice));
+        for (int q = 0; q < 2; ++q) {
+            for (size_t l = 0; l < places[q].size(); ++l) {
+                const Place &pl = places[q][l];
+                seq[q].push_back(Merged{rec[q][0][l].fn, dmem + pl.jobs, (const uint32_t *)(dmem + pl.map), pl.total});
+            }
+        }
+        // one graph per parity
+        for (int q = 0; q < 2; ++q) {
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (const Merged &m : seq[q]) m.fn(m.jobs, m.map, m.total, st);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                HIPCHK(ce);
+            }
+            const hipError_t ie = hipGraphInstantiate(&gexec[q], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(ie);
+        }
+        return 0;
+    }
+
+    int step(int n) override {
+        if (n <= 0) return 0;
+        HIPCHK(hipSetDevice(device));
+        for (size_t i = 0; i < cs.size(); ++i) {
+            if (cs[i]->cur[0].x != expect[parity][i])
+                return fail(-71, "batch: case %zu was stepped outside the batch; make a new batch", i);
+            HIPCHK(hipEventRecord(ev[i], cs[i]->st));
+            HIPCHK(hipStreamWaitEvent(st, ev[i], 0));
+        }
+        for (int k = 0; k < n; ++k) {
+            HIPCHK(hipGraphLaunch(gexec[parity], st));
+            parity ^= 1;
+            for (auto *c : cs) {
+                for (int s = 0; s < c->nsp; ++s) c->swap_species(s);
+                c->host_tick();
+                ++c->n_accum;
+            }
+        }
+        HIPCHK(hipEventRecord(done, st));
+        for (auto *c : cs) HIPCHK(hipStreamWaitEvent(c->st, done, 0));
+        return 0;
+    }
+    int sync() override {
+        HIPCHK(hipStreamSynchronize(st));
+        for (auto *c : cs)
+            if (int e = c->sync()) return e;
+        return 0;
+    }
+};
 }  // namespace pic
 
 // ================================================================== C ABI
@@ -1625,7 +1812,54 @@ int pic_create_from_file(const char *path, const pic_options *opt, pic_ctx **out
 }
 
 int pic_destroy(pic_ctx *ctx) {
+    if (ctx && ctx->c && ctx->c->batch_refs > 0)
+        return fail(-72, "the case belongs to a batch: destroy the batch first");
     delete ctx;
+    return 0;
+}
+
+struct pic_batch {
+    std::unique_ptr<BatchBase> b;
+};
+
+int pic_batch_create(pic_ctx *const *cases, int n, pic_batch **out) {
+    if (!cases || n <= 0 || !out) return fail(-1, "batch: no cases");
+    *out = nullptr;
+    std::vector<Case<float> *> f;
+    std::vector<Case<double> *> d;
+    for (int i = 0; i < n; ++i) {
+        if (!cases[i] || !cases[i]->c) return fail(-1, "batch: case %d is null", i);
+        for (int j = 0; j < i; ++j)
+            if (cases[j] == cases[i]) return fail(-70, "batch: case %d is listed twice", i);
+        if (auto *cf = dynamic_cast<Case<float> *>(cases[i]->c.get())) f.push_back(cf);
+        else if (auto *cd = dynamic_cast<Case<double> *>(cases[i]->c.get())) d.push_back(cd);
+    }
+    if (!f.empty() && !d.empty()) return fail(-70, "batch: the cases have different precisions");
+    std::unique_ptr<pic_batch> h(new pic_batch());
+    int e = 0;
+    if (!f.empty()) {
+        auto *b = new Batch<float>();
+        h->b.reset(b);
+        e = b->build(f);
+    } else {
+        auto *b = new Batch<double>();
+        h->b.reset(b);
+        e = b->build(d);
+    }
+    if (e) return e;
+    *out = h.release();
+    return 0;
+}
+int pic_batch_step(pic_batch *b, int nsteps) {
+    if (!b || !b->b) return fail(-1, "null batch");
+    return b->b->step(nsteps);
+}
+int pic_batch_sync(pic_batch *b) {
+    if (!b || !b->b) return fail(-1, "null batch");
+    return b->b->sync();
+}
+int pic_batch_destroy(pic_batch *b) {
+    delete b;
     return 0;
 }
 

@@ -13,11 +13,17 @@
 
 #include <algorithm>
 
+#include "multi.h"
 #include "pic_device.h"
 #include "pic_kernels.h"
 #include "pic_setup.h"
 
 namespace pic {
+
+Recorder *&launch_recorder() {
+    static thread_local Recorder *r = nullptr;
+    return r;
+}
 
 namespace {
 
@@ -296,12 +302,21 @@ __device__ inline int block_sum_256(int v, int *sh) {
     return t;
 }
 
-__global__ __launch_bounds__(BASES_TILE) void k_pair_totals(int G, int nt, const int *inv_start, const int *inv_ent,
-                                                            const int *cnt_e, const int *cnt_i, const int *prod_cnt,
-                                                            int *total_e, int *alive_e, int *total_i, int *alive_i,
-                                                            int *prod_total, int *sums) {
+struct PairTotalsJob {
+    int G, nt;
+    const int *inv_start, *inv_ent, *cnt_e, *cnt_i, *prod_cnt;
+    int *total_e, *alive_e, *total_i, *alive_i, *prod_total, *sums;
+};
+
+__device__ __forceinline__ void pair_totals_body(const PairTotalsJob &J, int bx) {
+    const int G = J.G, nt = J.nt;
+    const int *inv_start = J.inv_start, *inv_ent = J.inv_ent, *cnt_e = J.cnt_e, *cnt_i = J.cnt_i;
+    const int *prod_cnt = J.prod_cnt;
+    int *total_e = J.total_e, *alive_e = J.alive_e, *total_i = J.total_i, *alive_i = J.alive_i;
+    int *prod_total = J.prod_total, *sums = J.sums;
+
     __shared__ int sh[4];
-    const int d = blockIdx.x * BASES_TILE + threadIdx.x;
+    const int d = bx * BASES_TILE + threadIdx.x;
     int te = 0, ti = 0, tp = 0;
     if (d < G) {
         DestEntries E;
@@ -317,17 +332,33 @@ __global__ __launch_bounds__(BASES_TILE) void k_pair_totals(int G, int nt, const
     te = block_sum_256(te, sh);
     ti = block_sum_256(ti, sh);
     tp = block_sum_256(tp, sh);
-    if (threadIdx.x == 0) { sums[blockIdx.x] = te; sums[nt + blockIdx.x] = ti; sums[2 * nt + blockIdx.x] = tp; }
+    if (threadIdx.x == 0) { sums[bx] = te; sums[nt + bx] = ti; sums[2 * nt + bx] = tp; }
 }
 
-__global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases(int G, int nt, const int *sums, const int *total_e,
-                                                                const int *total_i, const int *prod_total,
-                                                                const int *alive_e, const int *alive_i,
-                                                                const int *inv_start, const int *inv_ent,
-                                                                const int *cnt_e, const int *cnt_i,
-                                                                const int *prod_cnt, int *start_e, int *start_i,
-                                                                int *prod_start, int *base_e, int *base_i,
-                                                                int *pbase_e, int *pbase_i, int *pid_off) {
+__global__ __launch_bounds__(BASES_TILE) void k_pair_totals(PairTotalsJob J) { pair_totals_body(J, blockIdx.x); }
+__global__ __launch_bounds__(BASES_TILE) void k_pair_totals_m(const PairTotalsJob *J, const uint32_t *map) {
+    int lb;
+    const PairTotalsJob &a = J[multi_job(map, lb)];
+    pair_totals_body(a, lb);
+}
+
+
+struct PairBasesJob {
+    int G, nt;
+    const int *sums, *total_e, *total_i, *prod_total, *alive_e, *alive_i;
+    const int *inv_start, *inv_ent, *cnt_e, *cnt_i, *prod_cnt;
+    int *start_e, *start_i, *prod_start, *base_e, *base_i, *pbase_e, *pbase_i, *pid_off;
+};
+
+__device__ __forceinline__ void pair_scan_bases_body(const PairBasesJob &J, int bx) {
+    const int G = J.G, nt = J.nt;
+    const int *sums = J.sums, *total_e = J.total_e, *total_i = J.total_i, *prod_total = J.prod_total;
+    const int *alive_e = J.alive_e, *alive_i = J.alive_i;
+    const int *inv_start = J.inv_start, *inv_ent = J.inv_ent, *cnt_e = J.cnt_e, *cnt_i = J.cnt_i;
+    const int *prod_cnt = J.prod_cnt;
+    int *start_e = J.start_e, *start_i = J.start_i, *prod_start = J.prod_start, *base_e = J.base_e;
+    int *base_i = J.base_i, *pbase_e = J.pbase_e, *pbase_i = J.pbase_i, *pid_off = J.pid_off;
+
     __shared__ int wtot[3][4];
     __shared__ int sh_off[3], sh_tot[3];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -339,7 +370,7 @@ __global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases(int G, int nt, c
             for (int t = lane; t < nt; t += 64) {
                 const int v = sums[a * nt + t];
                 all += v;
-                if (t < (int)blockIdx.x) before += v;
+                if (t < (int)bx) before += v;
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
@@ -349,7 +380,7 @@ __global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases(int G, int nt, c
             if (lane == 0) { sh_off[a] = before; sh_tot[a] = all; }
         }
     }
-    const int d = blockIdx.x * BASES_TILE + threadIdx.x;
+    const int d = bx * BASES_TILE + threadIdx.x;
     const bool in = d < G;
     DestEntries E;
     if (in) E.load(d, inv_start, inv_ent, cnt_e, cnt_i, prod_cnt);
@@ -387,10 +418,18 @@ __global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases(int G, int nt, c
             se += E.ce[q]; si += E.ci[q]; sp += E.cp[q];
         }
     }
-    if (blockIdx.x == nt - 1 && threadIdx.x == 0) {
+    if (bx == nt - 1 && threadIdx.x == 0) {
         start_e[G] = sh_tot[0]; start_i[G] = sh_tot[1]; prod_start[G] = sh_tot[2];
     }
 }
+
+__global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases(PairBasesJob J) { pair_scan_bases_body(J, blockIdx.x); }
+__global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases_m(const PairBasesJob *J, const uint32_t *map) {
+    int lb;
+    const PairBasesJob &a = J[multi_job(map, lb)];
+    pair_scan_bases_body(a, lb);
+}
+
 
 // ============================================================ staged: Sort scatter
 template <typename R>
@@ -606,15 +645,38 @@ struct CountWave {
     int ev[CPW * ROUTE_SLOTS][EV_MAX];           // collision events per (cell, route slot), flushed at the end
 };
 
+template <typename R>
+struct CountJob {
+    DevConst<R> k;
+    PView<R> p;
+    int isp;
+    const StepClock *clk;
+    int *cnt, *pcnt;
+    unsigned long long *mccr;
+    int *condcharge;
+    uint8_t *flag;
+    R *stash;
+    size_t stash_stride;
+    int *err;
+};
+
 template <typename R, int KIND>
-__global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, int isp, const StepClock *clk, int *cnt,
-                                                     int *pcnt, unsigned long long *mccr, int *condcharge,
-                                                     uint8_t *flag, R *stash, size_t stash_stride, int *err) {
+__device__ __forceinline__ void fused_count_body(const CountJob<R> &J, int bx) {
+    const DevConst<R> &k = J.k;
+    const PView<R> &p = J.p;
+    const int isp = J.isp;
+    const StepClock *clk = J.clk;
+    int *cnt = J.cnt, *pcnt = J.pcnt, *condcharge = J.condcharge, *err = J.err;
+    unsigned long long *mccr = J.mccr;
+    uint8_t *flag = J.flag;
+    R *stash = J.stash;
+    const size_t stash_stride = J.stash_stride;
+
     __shared__ CountWave<R> waves[FWAVES];
     CountWave<R> &W = waves[threadIdx.x >> 6];
     const uint32_t step = clk->step;
     const int lane = threadIdx.x & 63;
-    const int w0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)) * CPW;
+    const int w0 = __builtin_amdgcn_readfirstlane((int)((bx * blockDim.x + threadIdx.x) >> 6)) * CPW;
     const unsigned long long lt = lanes_below(lane);
     // cells of this wave (-1 past the last cell) and their particle ranges
     int cell[CPW], lo[CPW + 1];
@@ -785,21 +847,51 @@ __global__ __launch_bounds__(256) void k_fused_count(DevConst<R> k, PView<R> p, 
     }
 }
 
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_fused_count(CountJob<R> J) { fused_count_body<R, KIND>(J, blockIdx.x); }
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_fused_count_m(const CountJob<R> *J, const uint32_t *map) {
+    int lb;
+    const CountJob<R> &a = J[multi_job(map, lb)];
+    fused_count_body<R, KIND>(a, lb);
+}
+
+
 // Pass 2: write each particle and product to its final slot.  For
 // electrons, ionisation products go to both the electron and the ion arrays.
 // Particles that did not collide are pushed again (same bits as pass 1);
 // collided ones already hold their final state.  The flag bytes are staged
 // with the particles.
+template <typename R>
+struct ScatterJob {
+    DevConst<R> k;
+    PView<R> p, q, qion;
+    int isp;
+    const uint8_t *flag;
+    const R *stash;
+    size_t stash_stride;
+    const int *base, *pbase_e, *pbase_i, *pid_off;
+    const uint32_t *next_id;
+    int cap, cap_ion;
+    int *err;
+};
+
 template <typename R, int KIND>
-__global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p, PView<R> q, PView<R> qion, int isp,
-                                                       const uint8_t *flag, const R *stash, size_t stash_stride,
-                                                       const int *base, const int *pbase_e, const int *pbase_i,
-                                                       const int *pid_off, const uint32_t *next_id, int cap,
-                                                       int cap_ion, int *err) {
+__device__ __forceinline__ void fused_scatter_body(const ScatterJob<R> &J, int bx) {
+    const DevConst<R> &k = J.k;
+    const PView<R> &p = J.p, &q = J.q, &qion = J.qion;
+    const int isp = J.isp, cap = J.cap, cap_ion = J.cap_ion;
+    const uint8_t *flag = J.flag;
+    const R *stash = J.stash;
+    const size_t stash_stride = J.stash_stride;
+    const int *base = J.base, *pbase_e = J.pbase_e, *pbase_i = J.pbase_i, *pid_off = J.pid_off;
+    const uint32_t *next_id = J.next_id;
+    int *err = J.err;
+
     __shared__ FStage<R, SCHUNK> stage_all[FWAVES];
     __shared__ uint8_t flag_all[FWAVES][SCHUNK];
     int lane;
-    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    int c = wave_cell_b(k.ncy, k.ngy, k.Csize, lane, bx);
     if (c < 0) return;
     int k0, k1;
     cell_range(p, c, k0, k1);
@@ -890,18 +982,40 @@ __global__ __launch_bounds__(256) void k_fused_scatter(DevConst<R> k, PView<R> p
     }
 }
 
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_fused_scatter(ScatterJob<R> J) { fused_scatter_body<R, KIND>(J, blockIdx.x); }
+template <typename R, int KIND>
+__global__ __launch_bounds__(256) void k_fused_scatter_m(const ScatterJob<R> *J, const uint32_t *map) {
+    int lb;
+    const ScatterJob<R> &a = J[multi_job(map, lb)];
+    fused_scatter_body<R, KIND>(a, lb);
+}
+
+
 // ============================================================ deposit
 // CIC weights summed per cell with a wave reduction (src/cuda_Deposit.cu:289-317);
 // blockIdx.y = species.  A lane's particles of a cell are loaded at once
 // (up to 4 per lane per chunk) and summed in the same order as a plain
 // 64-strided walk.
 template <typename R>
-__global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PSet<R> ps, R *part) {
+struct DepCellsJob {
+    DevConst<R> k;
+    PSet<R> ps;
+    R *part;
+    int gx;   // blocks along x of the job's own grid
+};
+
+template <typename R>
+__device__ __forceinline__ void deposit_cells_body(const DepCellsJob<R> &J, int bx, int by) {
+    const DevConst<R> &k = J.k;
+    const PSet<R> &ps = J.ps;
+    R *part = J.part;
+
     int lane;
-    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    int c = wave_cell_b(k.ncy, k.ngy, k.Csize, lane, bx);
     if (c < 0) return;
-    const PView<R> &p = ps.s[blockIdx.y];
-    R *pp = part + (size_t)blockIdx.y * k.Gsize * 4;
+    const PView<R> &p = ps.s[by];
+    R *pp = part + (size_t)by * k.Gsize * 4;
     int k0, k1;
     cell_range(p, c, k0, k1);
     R WS = 0, WN = 0, ES = 0, EN = 0;
@@ -931,6 +1045,18 @@ __global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PSet<R> ps
     }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_deposit_cells(DepCellsJob<R> J) {
+    deposit_cells_body<R>(J, blockIdx.x, blockIdx.y);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_deposit_cells_m(const DepCellsJob<R> *J, const uint32_t *map) {
+    int lb;
+    const DepCellsJob<R> &a = J[multi_job(map, lb)];
+    deposit_cells_body<R>(a, lb % a.gx, lb / a.gx);
+}
+
+
 // Node density from the four adjacent cells, boundary factors
 // (src/cuda_Deposit.cu:277-288), then charge density, source and PCG rhs
 // (src/cuda_Deposit.cu:30-104).  part holds nsp x Gsize x 4.  This step's
@@ -940,9 +1066,21 @@ __global__ __launch_bounds__(256) void k_deposit_cells(DevConst<R> k, PSet<R> ps
 // to the FP64 profile sums (src/cuda_Diagnostic.cu:505-547) and thread 0
 // advances the particle id counters past this step's ionisation products.
 template <typename R>
-__global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_sp, R *Sigma, R *Source, R *b,
-                                DepositExtras<R> ex) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+struct DepNodesJob {
+    NodeConst<R> n;
+    const R *part;
+    R *den, *sigma_sp, *Sigma, *Source, *b;
+    DepositExtras<R> ex;
+};
+
+template <typename R>
+__device__ __forceinline__ void deposit_nodes_body(const DepNodesJob<R> &J, int bx) {
+    const NodeConst<R> &n = J.n;
+    const R *part = J.part;
+    R *den = J.den, *sigma_sp = J.sigma_sp, *Sigma = J.Sigma, *Source = J.Source, *b = J.b;
+    const DepositExtras<R> &ex = J.ex;
+
+    int g = bx * blockDim.x + threadIdx.x;
     if (ex.on && g == 0) {
         const uint32_t add = (uint32_t)*ex.prod_total_end;
         for (int s = 0; s < n.nsp; ++s) ex.next_id[s] += add;
@@ -1013,13 +1151,39 @@ __global__ void k_deposit_nodes(NodeConst<R> n, const R *part, R *den, R *sigma_
     }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_deposit_nodes(DepNodesJob<R> J) { deposit_nodes_body<R>(J, blockIdx.x); }
+template <typename R>
+__global__ __launch_bounds__(256) void k_deposit_nodes_m(const DepNodesJob<R> *J, const uint32_t *map) {
+    int lb;
+    const DepNodesJob<R> &a = J[multi_job(map, lb)];
+    deposit_nodes_body<R>(a, lb);
+}
+
+
 // Fused step: node potential straight from the field solution, with the
 // electrode voltages of this time evaluated per thread:
 // phi = x[A_idx-1] (0 on fixed nodes), tot = sum_c V_c(t) lap_c + phi.
 template <typename R>
-__global__ void k_phi_potential(NodeConst<R> n, VoltConst v, const StepClock *clk, const double *x, const R *lap,
-                                R *V, R *phi, R *tot) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+struct PhiPotJob {
+    NodeConst<R> n;
+    VoltConst v;
+    const StepClock *clk;
+    const double *x;
+    const R *lap;
+    R *V, *phi, *tot;
+};
+
+template <typename R>
+__device__ __forceinline__ void phi_potential_body(const PhiPotJob<R> &J, int bx) {
+    const NodeConst<R> &n = J.n;
+    const VoltConst &v = J.v;
+    const StepClock *clk = J.clk;
+    const double *x = J.x;
+    const R *lap = J.lap;
+    R *V = J.V, *phi = J.phi, *tot = J.tot;
+
+    int g = bx * blockDim.x + threadIdx.x;
     if (g >= n.Gsize) return;
     const double t = clk->t;
     const int a = n.A_idx[g];
@@ -1034,6 +1198,16 @@ __global__ void k_phi_potential(NodeConst<R> n, VoltConst v, const StepClock *cl
     tp += (R)1 * ph;
     tot[g] = tp;
 }
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_potential(PhiPotJob<R> J) { phi_potential_body<R>(J, blockIdx.x); }
+template <typename R>
+__global__ __launch_bounds__(256) void k_phi_potential_m(const PhiPotJob<R> *J, const uint32_t *map) {
+    int lb;
+    const PhiPotJob<R> &a = J[multi_job(map, lb)];
+    phi_potential_body<R>(a, lb);
+}
+
 
 // ============================================================ EEDF
 // Electron energy histogram: E = Escale v^2 (the MCC's energy, in eV) in
@@ -1086,8 +1260,19 @@ __device__ inline void eps_idx(int TID, int ngx, int ngy, int &ET, int &EL, int 
 // from the Poisson potential and the last deposit) and node E
 // (src/cuda_Field.cu:698-846, from the total potential).
 template <typename R>
-__global__ void k_efield(NodeConst<R> n, const R *phi, const R *tot, const R *den, R *Sigma, R *Ex, R *Ey) {
-    int TID = blockIdx.x * blockDim.x + threadIdx.x;
+struct EfieldJob {
+    NodeConst<R> n;
+    const R *phi, *tot, *den;
+    R *Sigma, *Ex, *Ey;
+};
+
+template <typename R>
+__device__ __forceinline__ void efield_body(const EfieldJob<R> &J, int bx) {
+    const NodeConst<R> &n = J.n;
+    const R *phi = J.phi, *tot = J.tot, *den = J.den;
+    R *Sigma = J.Sigma, *Ex = J.Ex, *Ey = J.Ey;
+
+    int TID = bx * blockDim.x + threadIdx.x;
     if (TID >= n.Gsize) return;
     const int ngx = n.ngx, ngy = n.ngy;
     const int x = TID / ngy, y = TID % ngy;
@@ -1188,6 +1373,16 @@ __global__ void k_efield(NodeConst<R> n, const R *phi, const R *tot, const R *de
     Ey[TID] = ey;
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_efield(EfieldJob<R> J) { efield_body<R>(J, blockIdx.x); }
+template <typename R>
+__global__ __launch_bounds__(256) void k_efield_m(const EfieldJob<R> *J, const uint32_t *map) {
+    int lb;
+    const EfieldJob<R> &a = J[multi_job(map, lb)];
+    efield_body<R>(a, lb);
+}
+
+
 // Conductor voltages at t = step * dt: V = sum DC + AC sin(w t + phase)
 template <typename R>
 __global__ void k_voltages(VoltConst v, double t, R *V) {
@@ -1282,37 +1477,54 @@ template <typename R>
 void launch_fused_count(const DevConst<R> &k, const PView<R> &p, int isp, const StepClock *clk, int *cnt, int *pcnt,
                         unsigned long long *mccr, int *cc, uint8_t *flag, R *stash, size_t stash_stride, int *err,
                         hipStream_t s) {
+    const CountJob<R> J{k, p, isp, clk, cnt, pcnt, mccr, cc, flag, stash, stash_stride, err};
+    const int nb = count_blocks(k.Csize);
+    if (Recorder *r = launch_recorder()) {
+        r->add(isp == 0 ? &launch_m<CountJob<R>, k_fused_count_m<R, KIND_AR_ELECTRON>, 256>
+                        : &launch_m<CountJob<R>, k_fused_count_m<R, KIND_AR_ION>, 256>,
+               J, nb);
+        return;
+    }
     if (isp == 0)
-        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(count_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           clk, cnt, pcnt, mccr, cc, flag, stash, stash_stride, err);
+        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ELECTRON>), dim3(nb), dim3(256), 0, s, J);
     else
-        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(count_blocks(k.Csize)), dim3(256), 0, s, k, p, isp,
-                           clk, cnt, pcnt, mccr, cc, flag, stash, stash_stride, err);
+        hipLaunchKernelGGL((k_fused_count<R, KIND_AR_ION>), dim3(nb), dim3(256), 0, s, J);
 }
 template <typename R>
 void launch_fused_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const PView<R> &qion, int isp,
                           const uint8_t *flag, const R *stash, size_t stash_stride, const int *base,
                           const int *pbase_e, const int *pbase_i, const int *pid_off, const uint32_t *next_id,
                           int cap, int cap_ion, int *err, hipStream_t s) {
+    const ScatterJob<R> J{k, p, q, qion, isp, flag, stash, stash_stride, base, pbase_e, pbase_i, pid_off, next_id,
+                          cap, cap_ion, err};
+    const int nb = cell_blocks(k.Csize);
+    if (Recorder *r = launch_recorder()) {
+        r->add(isp == 0 ? &launch_m<ScatterJob<R>, k_fused_scatter_m<R, KIND_AR_ELECTRON>, 256>
+                        : &launch_m<ScatterJob<R>, k_fused_scatter_m<R, KIND_AR_ION>, 256>,
+               J, nb);
+        return;
+    }
     if (isp == 0)
-        hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ELECTRON>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p,
-                           q, qion, isp, flag, stash, stash_stride, base, pbase_e, pbase_i, pid_off, next_id, cap,
-                           cap_ion, err);
+        hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ELECTRON>), dim3(nb), dim3(256), 0, s, J);
     else
-        hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ION>), dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, q,
-                           qion, isp, flag, stash, stash_stride, base, pbase_e, pbase_i, pid_off, next_id, cap,
-                           cap_ion, err);
+        hipLaunchKernelGGL((k_fused_scatter<R, KIND_AR_ION>), dim3(nb), dim3(256), 0, s, J);
 }
 template <typename R>
 void launch_deposit(const DevConst<R> &k, const NodeConst<R> &n, const PView<R> *ps, R *part, R *den, R *sigma_sp,
                     R *Sigma, R *Source, R *b, const DepositExtras<R> *ex, hipStream_t s) {
     PSet<R> set{};
     for (int isp = 0; isp < n.nsp; ++isp) set.s[isp] = ps[isp];
-    hipLaunchKernelGGL(k_deposit_cells<R>, dim3(cell_blocks(k.Csize), n.nsp), dim3(256), 0, s, k, set, part);
     DepositExtras<R> e{};
     if (ex) e = *ex;
-    hipLaunchKernelGGL(k_deposit_nodes<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, (const R *)part, den,
-                       sigma_sp, Sigma, Source, b, e);
+    const DepCellsJob<R> Jc{k, set, part, cell_blocks(k.Csize)};
+    const DepNodesJob<R> Jn{n, (const R *)part, den, sigma_sp, Sigma, Source, b, e};
+    if (Recorder *r = launch_recorder()) {
+        r->add(&launch_m<DepCellsJob<R>, k_deposit_cells_m<R>, 256>, Jc, Jc.gx * n.nsp);
+        r->add(&launch_m<DepNodesJob<R>, k_deposit_nodes_m<R>, 256>, Jn, node_blocks(n.Gsize));
+        return;
+    }
+    hipLaunchKernelGGL(k_deposit_cells<R>, dim3(Jc.gx, n.nsp), dim3(256), 0, s, Jc);
+    hipLaunchKernelGGL(k_deposit_nodes<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, Jn);
 }
 template <typename R>
 void launch_eedf(const PView<R> &p, int Gsize, R escale, R emax, unsigned long long *hist, hipStream_t s) {
@@ -1321,20 +1533,30 @@ void launch_eedf(const PView<R> &p, int Gsize, R escale, R emax, unsigned long l
 template <typename R>
 void launch_phi_potential(const NodeConst<R> &n, const VoltConst &v, const StepClock *clk, const double *x,
                           const R *lap, R *V, R *phi, R *tot, hipStream_t s) {
-    hipLaunchKernelGGL(k_phi_potential<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, v, clk, x, lap, V, phi, tot);
+    const PhiPotJob<R> J{n, v, clk, x, lap, V, phi, tot};
+    if (Recorder *r = launch_recorder()) {
+        r->add(&launch_m<PhiPotJob<R>, k_phi_potential_m<R>, 256>, J, node_blocks(n.Gsize));
+        return;
+    }
+    hipLaunchKernelGGL(k_phi_potential<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, J);
 }
 template <typename R>
 void launch_efield_only(const NodeConst<R> &n, const R *phi, const R *tot, const R *den, R *Sigma, R *Ex, R *Ey,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_efield<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, phi, tot, den, Sigma, Ex, Ey);
+    const EfieldJob<R> J{n, phi, tot, den, Sigma, Ex, Ey};
+    if (Recorder *r = launch_recorder()) {
+        r->add(&launch_m<EfieldJob<R>, k_efield_m<R>, 256>, J, node_blocks(n.Gsize));
+        return;
+    }
+    hipLaunchKernelGGL(k_efield<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, J);
 }
 template <typename R>
 void launch_efield(const NodeConst<R> &n, const VoltConst &v, double t, const R *lap, R *V, const R *phi, R *tot,
                    const R *den, R *Sigma, R *Ex, R *Ey, hipStream_t s) {
     hipLaunchKernelGGL(k_voltages<R>, dim3(1), dim3(64), 0, s, v, t, V);
     hipLaunchKernelGGL(k_potential<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, lap, (const R *)V, phi, tot);
-    hipLaunchKernelGGL(k_efield<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s, n, phi, (const R *)tot, den, Sigma, Ex,
-                       Ey);
+    hipLaunchKernelGGL(k_efield<R>, dim3(node_blocks(n.Gsize)), dim3(256), 0, s,
+                       EfieldJob<R>{n, phi, (const R *)tot, den, Sigma, Ex, Ey});
 }
 template <typename R>
 void launch_accumulate(int Gsize, int nsp, const R *den, const R *tot, const R *Ex, const R *Ey, double *sden,
@@ -1381,12 +1603,17 @@ void launch_bases_pair(int G, const int *inv_start, const int *inv_ent, const in
                        int *start_e, int *start_i, int *prod_start, int *base_e, int *base_i, int *pbase_e,
                        int *pbase_i, int *pid_off, int *tmp, hipStream_t s) {
     const int nt = std::max(1, (G + BASES_TILE - 1) / BASES_TILE);
-    hipLaunchKernelGGL(k_pair_totals, dim3(nt), dim3(BASES_TILE), 0, s, G, nt, inv_start, inv_ent, cnt_e, cnt_i, prod_cnt,
-                       total_e, alive_e, total_i, alive_i, prod_total, tmp);
-    hipLaunchKernelGGL(k_pair_scan_bases, dim3(nt), dim3(BASES_TILE), 0, s, G, nt, (const int *)tmp, (const int *)total_e,
-                       (const int *)total_i, (const int *)prod_total, (const int *)alive_e, (const int *)alive_i,
-                       inv_start, inv_ent, cnt_e, cnt_i, prod_cnt, start_e, start_i, prod_start, base_e, base_i,
-                       pbase_e, pbase_i, pid_off);
+    const PairTotalsJob Jt{G, nt, inv_start, inv_ent, cnt_e, cnt_i, prod_cnt, total_e, alive_e, total_i, alive_i,
+                           prod_total, tmp};
+    const PairBasesJob Jb{G, nt, tmp, total_e, total_i, prod_total, alive_e, alive_i, inv_start, inv_ent, cnt_e,
+                          cnt_i, prod_cnt, start_e, start_i, prod_start, base_e, base_i, pbase_e, pbase_i, pid_off};
+    if (Recorder *r = launch_recorder()) {
+        r->add(&launch_m<PairTotalsJob, k_pair_totals_m, BASES_TILE>, Jt, nt);
+        r->add(&launch_m<PairBasesJob, k_pair_scan_bases_m, BASES_TILE>, Jb, nt);
+        return;
+    }
+    hipLaunchKernelGGL(k_pair_totals, dim3(nt), dim3(BASES_TILE), 0, s, Jt);
+    hipLaunchKernelGGL(k_pair_scan_bases, dim3(nt), dim3(BASES_TILE), 0, s, Jb);
 }
 // tile sums of up to 3 arrays: for launch_scan_multi (SCAN_TILE tiles) and
 // launch_bases_pair (BASES_TILE tiles)

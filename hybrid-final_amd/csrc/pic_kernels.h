@@ -27,13 +27,16 @@ struct NodeConst {
 };
 
 // One 64-lane wavefront per cell: the wave's cell (named by its lower-left
-// node), or -1 past the last cell.
-__device__ inline int wave_cell(int ncy, int ngy, int Csize, int &lane) {
+// node), or -1 past the last cell; bx is the block's index in its grid.
+__device__ inline int wave_cell_b(int ncy, int ngy, int Csize, int &lane, int bx) {
     lane = threadIdx.x & 63;
     // wave-uniform: computed in scalar registers
-    const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int w = __builtin_amdgcn_readfirstlane((int)((bx * blockDim.x + threadIdx.x) >> 6));
     if (w >= Csize) return -1;
     return (w / ncy) * ngy + (w % ncy);
+}
+__device__ inline int wave_cell(int ncy, int ngy, int Csize, int &lane) {
+    return wave_cell_b(ncy, ngy, Csize, lane, (int)blockIdx.x);
 }
 
 // [k0, k1) of cell c, clamped to the array capacity: after a capacity

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dgemm_mfma.h"
+#include "multi.h"
 #include "pic_kernels.h"
 
 namespace pic {
@@ -35,15 +36,30 @@ constexpr int GT = 16;   // GEMM output tile edge
 // residual of a solve (b32 != null) also starts the solve: it converts the
 // right-hand side (FP32 or FP64) into b and resets the solve state.
 template <typename RB>
-__global__ __launch_bounds__(RES_BLOCK) void k_residual64(PcgOperator A, const RB *b32, double *b, const double *x,
-                                                          double *r, double *partials, SpecState *s) {
+struct ResidJob {
+    PcgOperator A;
+    const RB *b32;
+    double *b;
+    const double *x;
+    double *r, *partials;
+    SpecState *s;
+};
+
+template <typename RB>
+__device__ __forceinline__ void residual64_body(const ResidJob<RB> &J, int bx) {
+    const PcgOperator &A = J.A;
+    const RB *b32 = J.b32;
+    double *b = J.b, *r = J.r, *partials = J.partials;
+    const double *x = J.x;
+    SpecState *s = J.s;
+
     if (b32) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { s->done = 0; s->outer = 0; s->rel = 0.0; s->status = 0; }
+        if (bx == 0 && threadIdx.x == 0) { s->done = 0; s->outer = 0; s->rel = 0.0; s->status = 0; }
     } else if (s->done) {
         return;
     }
     __shared__ double sh[2 * RES_BLOCK];
-    const int i = blockIdx.x * RES_BLOCK + threadIdx.x;
+    const int i = bx * RES_BLOCK + threadIdx.x;
     double rr = 0.0, bb = 0.0;
     if (i < A.N) {
         if (b32) b[i] = (double)b32[i];
@@ -69,14 +85,38 @@ __global__ __launch_bounds__(RES_BLOCK) void k_residual64(PcgOperator A, const R
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = sh[0];
-        partials[2 * blockIdx.x + 1] = sh[RES_BLOCK];
+        partials[2 * bx] = sh[0];
+        partials[2 * bx + 1] = sh[RES_BLOCK];
     }
 }
 
+template <typename RB>
+__global__ __launch_bounds__(RES_BLOCK) void k_residual64(ResidJob<RB> J) { residual64_body<RB>(J, blockIdx.x); }
+template <typename RB>
+__global__ __launch_bounds__(RES_BLOCK) void k_residual64_m(const ResidJob<RB> *J, const uint32_t *map) {
+    int lb;
+    const ResidJob<RB> &a = J[multi_job(map, lb)];
+    residual64_body<RB>(a, lb);
+}
+
+
 // one block: fixed-order sum of the partials, convergence decision
-__global__ __launch_bounds__(RES_BLOCK) void k_spec_finalize(const double *partials, int nblk, double tol, int last,
-                                                             SpecState *s, PcgResult *res) {
+struct FinalizeJob {
+    const double *partials;
+    int nblk;
+    double tol;
+    int last;
+    SpecState *s;
+    PcgResult *res;
+};
+
+__device__ __forceinline__ void spec_finalize_body(const FinalizeJob &J) {
+    const double *partials = J.partials;
+    const int nblk = J.nblk, last = J.last;
+    const double tol = J.tol;
+    SpecState *s = J.s;
+    PcgResult *res = J.res;
+
     if (s->done) return;
     __shared__ double sh[2 * RES_BLOCK];
     double rr = 0.0, bb = 0.0;
@@ -113,18 +153,40 @@ __global__ __launch_bounds__(RES_BLOCK) void k_spec_finalize(const double *parti
     }
 }
 
+__global__ __launch_bounds__(RES_BLOCK) void k_spec_finalize(FinalizeJob J) { spec_finalize_body(J); }
+__global__ __launch_bounds__(RES_BLOCK) void k_spec_finalize_m(const FinalizeJob *J, const uint32_t *map) {
+    int lb;
+    spec_finalize_body(J[multi_job(map, lb)]);
+}
+
+
 // C = A B (row-major, A: M x K, B: K x N) in FP64 on the matrix cores
 // (dgemm_tile_mfma), one 16x16 output tile per 256-thread block with K split
 // over its 4 waves.  MODE 0: store; MODE 1: store C .* scale; MODE 2:
 // x64 += C.
+struct GemmJob {
+    int M, N, K;
+    const double *A, *B;
+    double *C;
+    const double *scale;
+    double *x64;
+    const SpecState *s;
+    int gx;   // blocks along x of the job's own grid
+};
+
 template <int MODE>
-__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double *__restrict__ A,
-                                               const double *__restrict__ B, double *__restrict__ C,
-                                               const double *__restrict__ scale, double *__restrict__ x64,
-                                               const SpecState *s) {
+__device__ __forceinline__ void dgemm_body(const GemmJob &J, int bx, int by) {
+    const int M = J.M, N = J.N, K = J.K;
+    const double *__restrict__ A = J.A;
+    const double *__restrict__ B = J.B;
+    double *__restrict__ C = J.C;
+    const double *__restrict__ scale = J.scale;
+    double *__restrict__ x64 = J.x64;
+    const SpecState *s = J.s;
+
     if (s->done) return;   // uniform over the grid: no thread reaches a barrier alone
     __shared__ double part[4][256];
-    const int r0 = blockIdx.y * GT, c0 = blockIdx.x * GT;
+    const int r0 = by * GT, c0 = bx * GT;
     const v4d acc = dgemm_tile_mfma(M, N, K, A, B, r0, c0);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -142,6 +204,16 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
     else x64[o] += sum;
 }
 
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dgemm(GemmJob J) { dgemm_body<MODE>(J, blockIdx.x, blockIdx.y); }
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dgemm_m(const GemmJob *J, const uint32_t *map) {
+    int lb;
+    const GemmJob &a = J[multi_job(map, lb)];
+    dgemm_body<MODE>(a, lb % a.gx, lb / a.gx);
+}
+
+
 }  // namespace
 
 template <typename RB>
@@ -153,21 +225,36 @@ void spectral_solve_refined(const PcgOperator &A, const SpecDev &sp, const RB *b
     const dim3 g((sp.ny + GT - 1) / GT, (sp.nx + GT - 1) / GT);
     const dim3 blk(GT * GT);
     const SpecState *cs = state;
+    Recorder *rec = launch_recorder();
     for (int k = 0; k <= max_cycles; ++k) {
-        hipLaunchKernelGGL(k_residual64<RB>, dim3(nblk), dim3(RES_BLOCK), 0, st, A, k == 0 ? b32 : (const RB *)nullptr,
-                           b, (const double *)x, sp.r, partials, state);
-        hipLaunchKernelGGL(k_spec_finalize, dim3(1), dim3(RES_BLOCK), 0, st, (const double *)partials, nblk, tol,
-                           k == max_cycles ? 1 : 0, state, res);
+        const ResidJob<RB> Jr{A, k == 0 ? b32 : (const RB *)nullptr, b, (const double *)x, sp.r, partials, state};
+        const FinalizeJob Jf{(const double *)partials, nblk, tol, k == max_cycles ? 1 : 0, state, res};
+        if (rec) {
+            rec->add(&launch_m<ResidJob<RB>, k_residual64_m<RB>, RES_BLOCK>, Jr, nblk);
+            rec->add(&launch_m<FinalizeJob, k_spec_finalize_m, RES_BLOCK>, Jf, 1);
+        } else {
+            hipLaunchKernelGGL(k_residual64<RB>, dim3(nblk), dim3(RES_BLOCK), 0, st, Jr);
+            hipLaunchKernelGGL(k_spec_finalize, dim3(1), dim3(RES_BLOCK), 0, st, Jf);
+        }
         if (k == max_cycles) break;
         // T1 = Vx^-1 R ; T2 = (T1 Vy^-T) .* dinv ; T1 = Vx T2 ; x += T1 Vy^T
-        hipLaunchKernelGGL(k_dgemm<0>, g, blk, 0, st, sp.nx, sp.ny, sp.nx, sp.Vxi, (const double *)sp.r, sp.T1,
-                           (const double *)nullptr, (double *)nullptr, cs);
-        hipLaunchKernelGGL(k_dgemm<1>, g, blk, 0, st, sp.nx, sp.ny, sp.ny, (const double *)sp.T1, sp.VyiT, sp.T2,
-                           sp.dinv, (double *)nullptr, cs);
-        hipLaunchKernelGGL(k_dgemm<0>, g, blk, 0, st, sp.nx, sp.ny, sp.nx, sp.Vx, (const double *)sp.T2, sp.T1,
-                           (const double *)nullptr, (double *)nullptr, cs);
-        hipLaunchKernelGGL(k_dgemm<2>, g, blk, 0, st, sp.nx, sp.ny, sp.ny, (const double *)sp.T1, sp.VyT,
-                           (double *)nullptr, (const double *)nullptr, x, cs);
+        const int gx = (int)g.x;
+        const GemmJob J0{sp.nx, sp.ny, sp.nx, sp.Vxi, (const double *)sp.r, sp.T1, nullptr, nullptr, cs, gx};
+        const GemmJob J1{sp.nx, sp.ny, sp.ny, (const double *)sp.T1, sp.VyiT, sp.T2, sp.dinv, nullptr, cs, gx};
+        const GemmJob J2{sp.nx, sp.ny, sp.nx, sp.Vx, (const double *)sp.T2, sp.T1, nullptr, nullptr, cs, gx};
+        const GemmJob J3{sp.nx, sp.ny, sp.ny, (const double *)sp.T1, sp.VyT, nullptr, nullptr, x, cs, gx};
+        if (rec) {
+            const int nb = (int)(g.x * g.y);
+            rec->add(&launch_m<GemmJob, k_dgemm_m<0>, 256>, J0, nb);
+            rec->add(&launch_m<GemmJob, k_dgemm_m<1>, 256>, J1, nb);
+            rec->add(&launch_m<GemmJob, k_dgemm_m<0>, 256>, J2, nb);
+            rec->add(&launch_m<GemmJob, k_dgemm_m<2>, 256>, J3, nb);
+        } else {
+            hipLaunchKernelGGL(k_dgemm<0>, g, blk, 0, st, J0);
+            hipLaunchKernelGGL(k_dgemm<1>, g, blk, 0, st, J1);
+            hipLaunchKernelGGL(k_dgemm<0>, g, blk, 0, st, J2);
+            hipLaunchKernelGGL(k_dgemm<2>, g, blk, 0, st, J3);
+        }
     }
 }
 template void spectral_solve_refined<float>(const PcgOperator &, const SpecDev &, const float *, double *, double *,

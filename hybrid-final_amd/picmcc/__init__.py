@@ -75,6 +75,10 @@ EXPORTS = {
     "pic_field_solver_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "pic_diag_sample": (C.c_int, [C.c_void_p]),
     "pic_dump_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pic_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "pic_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "pic_batch_sync": (C.c_int, [C.c_void_p]),
+    "pic_batch_destroy": (C.c_int, [C.c_void_p]),
     "pic_dump_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "pic_diag_history": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "pic_stage_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -208,7 +212,7 @@ class Case:
     # ---- lifecycle
     def close(self):
         if getattr(self, "h", None):
-            lib().pic_destroy(self.h)
+            _check(lib().pic_destroy(self.h))   # refused while the case is in a batch
             self.h = None
 
     def __del__(self):
@@ -347,6 +351,36 @@ class Case:
         _check(L.pic_array_info(self.h, name.encode(), C.byref(n), C.byref(es), C.byref(fl)))
         a = np.ascontiguousarray(values, dtype=_dtype(name, es.value, fl.value))
         _check(L.pic_set_array(self.h, name.encode(), _ptr(a), a.nbytes))
+
+
+class Batch:
+    """Several cases stepped together by shared launches (pic_batch_*): one
+    graph launch per step for all of them.  The cases keep working as
+    before; stepping one outside the batch invalidates the batch."""
+
+    def __init__(self, cases):
+        self.cases = list(cases)
+        arr = (C.c_void_p * len(self.cases))(*[c.h for c in self.cases])
+        h = C.c_void_p()
+        _check(lib().pic_batch_create(arr, len(self.cases), C.byref(h)))
+        self.h = h.value
+
+    def step(self, n=1):
+        _check(lib().pic_batch_step(self.h, int(n)))
+
+    def sync(self):
+        _check(lib().pic_batch_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pic_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class PoissonBatch:

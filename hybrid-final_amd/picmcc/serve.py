@@ -31,7 +31,7 @@ import time
 
 import numpy as np
 
-from . import FP32_REFINED, Case, PicError, _check, decks, lib
+from . import FP32_REFINED, Batch, Case, PicError, _check, decks, lib
 
 
 def device_free_bytes(device=0):
@@ -122,12 +122,13 @@ def _sync_status(case):
 
 
 class _Active:
-    __slots__ = ("req", "case", "remaining", "inflight", "admit", "footprint", "np0")
+    __slots__ = ("req", "case", "remaining", "inflight", "admit", "footprint", "np0", "batchable")
 
     def __init__(self, req, case, admit, footprint):
         self.req, self.case, self.admit, self.footprint = req, case, admit, footprint
         self.remaining = req.steps
         self.inflight = False
+        self.batchable = None   # decided when a batching server first sees it
         inf = case.info()
         self.np0 = sum(inf.np[:inf.nsp])
 
@@ -153,15 +154,26 @@ class EnsembleServer:
     memory is allocated by several creations at once, so a free-memory
     difference would not belong to one case).
 
+    batch: step the active argon cases on the separable field solve
+    together through one pic_batch (each kernel of the step launched once for
+    all of them; see picmcc.Batch), rebuilt whenever one joins or leaves; the
+    batch advances by the smallest remaining step count of its members, at
+    most slice_steps.  Other cases get their own slices as before.
+
     run() raises if CONCURRENT is off (DISABLED_REASON)."""
 
-    def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=16, setup_threads=2):
+    def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=16, setup_threads=2,
+                 batch=False):
         self.precision = precision
         self.slice = slice_steps
         self.max_active = max_active
         self.setup_threads = setup_threads
+        self.batch = batch
         free, _ = device_free_bytes()
         self.budget = int(free * mem_fraction)
+        self._b = None          # the current Batch
+        self._members = []      # its _Active entries
+        self.rebuilds = 0
 
     def run(self, requests, keep_state=None):
         """Serve `requests` in real time; returns one record per case.
@@ -194,6 +206,8 @@ class EnsembleServer:
                     if fut.done():
                         creating.remove(item)
                         active.append(_Active(req, fut.result(), admit, est))
+                if self.batch:
+                    used -= self._feed_batch(active, results, keep_state, t0)
                 used -= self._feed(active, results, keep_state, t0)
                 busy = any(a.inflight for a in active) or bool(creating)
                 if not busy and pending:
@@ -203,33 +217,80 @@ class EnsembleServer:
         finally:
             if pool is not None:
                 pool.shutdown(wait=True)
+            if self._b is not None:
+                self._b.close()
+                self._b, self._members = None, []
         return results
+
+    def _retire(self, a, active, results, keep_state, t0):
+        err = _sync_status(a.case)
+        inf = a.case.info()
+        np1 = sum(inf.np[:inf.nsp])
+        rec = {"rid": a.req.rid, "arrival": a.req.arrival, "admit": a.admit,
+               "finish": time.perf_counter() - t0, "steps": a.req.steps,
+               "particle_steps": 0.5 * (a.np0 + np1) * a.req.steps, "footprint": a.footprint,
+               "cells": (inf.ngx - 1) * (inf.ngy - 1), "gas": case_gas(a.req.deck), "error": err}
+        rec["latency"] = rec["finish"] - rec["arrival"]
+        if keep_state and keep_state(a.req.rid):
+            rec["state"] = {"particles": [a.case.particles(s) for s in range(inf.nsp)],
+                            "den": a.case.array("den")}
+        a.case.close()
+        active.remove(a)
+        results.append(rec)
+        return a.footprint
+
+    def _feed_batch(self, active, results, keep_state, t0):
+        """Steps the batchable cases together; returns the footprint freed by
+        the ones that finished."""
+        if self._members and any(not m.case.done() for m in self._members):
+            return 0   # the last batch step is still running
+        for m in self._members:
+            m.inflight = False
+        for a in active:
+            if a.batchable is None:
+                a.batchable = case_gas(a.req.deck) == 0 and a.case.field_solver_kind() == 1
+        want = [a for a in active if a.batchable]
+        freed = 0
+        done = [a for a in want if a.remaining == 0]
+        keep = [a for a in want if a.remaining > 0]
+        if done or [id(a) for a in keep] != [id(m) for m in self._members]:
+            if self._b is not None:
+                self._b.close()
+                self._b = None
+            for a in done:
+                freed += self._retire(a, active, results, keep_state, t0)
+            self._members = keep
+            if keep:
+                try:
+                    self._b = Batch([a.case for a in keep])
+                    self.rebuilds += 1
+                except PicError:
+                    # a case the batch refuses (for example one whose field
+                    # solve runs another cycle count) is served on its own
+                    for a in keep:
+                        a.batchable = False
+                    self._members = []
+        if self._b is not None:
+            k = min(self.slice, min(a.remaining for a in self._members))
+            self._b.step(k)
+            for a in self._members:
+                a.remaining -= k
+                a.inflight = True
+        return freed
 
     def _feed(self, active, results, keep_state, t0):
         """Gives every idle case its next slice of steps and retires the cases
         that have finished; returns the footprint freed by the retired ones."""
         freed = 0
         for a in list(active):
+            if a.batchable:
+                continue   # stepped by the batch
             if a.inflight:
                 if not a.case.done():
                     continue
                 a.inflight = False
             if a.remaining == 0:
-                err = _sync_status(a.case)
-                inf = a.case.info()
-                np1 = sum(inf.np[:inf.nsp])
-                rec = {"rid": a.req.rid, "arrival": a.req.arrival, "admit": a.admit,
-                       "finish": time.perf_counter() - t0, "steps": a.req.steps,
-                       "particle_steps": 0.5 * (a.np0 + np1) * a.req.steps, "footprint": a.footprint,
-                       "cells": (inf.ngx - 1) * (inf.ngy - 1), "gas": case_gas(a.req.deck), "error": err}
-                rec["latency"] = rec["finish"] - rec["arrival"]
-                if keep_state and keep_state(a.req.rid):
-                    rec["state"] = {"particles": [a.case.particles(s) for s in range(inf.nsp)],
-                                    "den": a.case.array("den")}
-                a.case.close()
-                freed += a.footprint
-                active.remove(a)
-                results.append(rec)
+                freed += self._retire(a, active, results, keep_state, t0)
                 continue
             k = min(self.slice, a.remaining)
             a.case.step(k)

@@ -114,6 +114,24 @@ int pic_dump_save(pic_ctx *ctx, const char *path);
  * -30: more particles than the capacity; -61: the file ends early;
  * -65: a particle outside the grid. */
 int pic_dump_load(pic_ctx *ctx, const char *path);
+/* ---------------- batches: several cases stepped by shared launches ----------------
+ * No counterpart in the reference (which runs one case per process).  A
+ * batch steps its cases together: each kernel of the fused step runs once
+ * for all of them over a grid that concatenates the cases' grids, from job
+ * records prepared at creation; one step of the batch is one graph launch.
+ * The cases must share device and precision, be argon cases on the
+ * separable field solve (pic_field_solver_kind 1) with equal solve cycles,
+ * and have stage timers off (-70 otherwise).  Each case gives the same bits
+ * as when stepped alone.  A batch step is ordered after the work already
+ * enqueued on the cases' streams and before work enqueued on them later, so
+ * pic_sync / pic_get_* on a case see the batch's steps.  A case stepped
+ * outside its batch invalidates the batch (pic_batch_step returns -71); a
+ * case may not be destroyed while in a batch (pic_destroy returns -72). */
+typedef struct pic_batch pic_batch;
+int pic_batch_create(pic_ctx *const *cases, int n, pic_batch **out);
+int pic_batch_step(pic_batch *b, int nsteps);   /* enqueued, no host synchronisation */
+int pic_batch_sync(pic_batch *b);               /* waits; reports each case's errors (pic_sync) */
+int pic_batch_destroy(pic_batch *b);            /* the cases stay valid */
 /* one full timestep through the stage functions above, nsteps times */
 int pic_run_staged(pic_ctx *ctx, int nsteps);
 /* one full timestep on the fused path (push+walls+MCC counted, scanned and

@@ -42,4 +42,6 @@ def test_python_mirror_binds():
         assert callable(getattr(picmcc.Case, m, None)), f"Case.{m} is missing"
     for m in ("set_rhs", "solve", "sync", "result", "close"):
         assert callable(getattr(picmcc.PoissonBatch, m, None)), f"PoissonBatch.{m} is missing"
+    for m in ("step", "sync", "close"):
+        assert callable(getattr(picmcc.Batch, m, None)), f"Batch.{m} is missing"
     assert callable(picmcc.pbatch_solve_noalloc)

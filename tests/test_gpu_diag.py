@@ -50,13 +50,22 @@ def snapshot(c):
             "v": c.array("V_cond")[:nc].astype(np.float64), "tstep": int(inf.tstep), "dt": float(inf.dt)}
 
 
+def deck():
+    """Thick electrodes (conductor cells, which absorb) and a dielectric block
+    (surface charge on the conductors' neighbours): every record field moves."""
+    from test_setup_parity import deck_variants
+    d = deck_variants()["thick_electrodes_dielectric"]
+    d["Geometry"]["Source"][0]["AC(V)"] = 150.0
+    return d
+
+
 @pytest.mark.parametrize("prec,rtol", [(FP32_REFINED, 2e-6), (FP64_PLAIN, 1e-11)], ids=["fp32", "fp64"])
 def test_diagnostics_records_match_the_case_arrays(prec, rtol):
-    d = decks.small(ncx=32, ncy=24, ppc=40)
+    d = deck()
     np2c = find(d, "np2c")
     c = Case(d, prec)
     snaps = []
-    for k in (3, 5, 4):
+    for k in (5, 9, 6):
         c.step(k)
         c.diag_sample()
         snaps.append(snapshot(c))
@@ -75,7 +84,8 @@ def test_diagnostics_records_match_the_case_arrays(prec, rtol):
         assert np.array_equal(h["absorbed"][r].astype(np.int64), sn["cc"] - cc_prev), f"record {r}: absorbed"
         cc_prev = sn["cc"]
         assert np.array_equal(h["v_cond"][r], sn["v"]), f"record {r}: voltages"
-    assert h["absorbed"].sum() > 0
+    assert h["absorbed"].sum() > 0, "no particle reached a conductor"
+    assert np.any(h["q_cond"] != 0)
     # more steps: the history grows at its end and keeps the earlier records
     c.step(2)
     c.diag_sample()

@@ -4,7 +4,9 @@ Every case must give the same bits alone and when served (the step is
 deterministic per case), every request must finish, and the latency
 statistics must be consistent with the arrivals.  The served ensembles are
 argon-only and mixed argon and Ar/O2 (the serving workload's generator at a
-small grid size), served concurrently and first come first served.
+small grid size), served concurrently (each case on its own stream, or the
+argon cases stepped together through a pic_batch) and first come first
+served.
 
 A served case is not compared with the oracle's orc_run directly: the two
 deposits sum the charge in different orders, so the potentials differ at the
@@ -46,13 +48,16 @@ def test_sequential_serving_is_fifo_and_exact():
 
 
 @pytest.mark.skipif(not serve.CONCURRENT, reason=serve.DISABLED_REASON)
-def test_results_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("batch", [False, True], ids=["streams", "batched"])
+def test_results_same_alone_and_in_the_mix(batch):
     # dt = T_RF/400: these decks stay within their particle capacity (checked
     # with the CPU oracle); a much coarser dt makes them run away
     reqs = serve.ar_ensemble(6, seed=3, rate_hz=200.0, grid=(16, 40), ppc=(10, 30), cycles=(1, 1),
                              steps_per_cycle=400)
-    srv = serve.EnsembleServer(slice_steps=100, max_active=4)
+    srv = serve.EnsembleServer(slice_steps=100, max_active=4, batch=batch)
     res = srv.run(reqs, keep_state=lambda rid: True)
+    if batch:
+        assert srv.rebuilds >= 2, "the batched server never batched"
     assert sorted(r["rid"] for r in res) == list(range(6))
     for r in res:
         assert r["finish"] >= r["admit"] >= 0.0
@@ -69,14 +74,15 @@ def test_results_same_alone_and_in_the_mix():
     assert summ["p99_latency_s"] >= summ["p50_latency_s"] > 0.0
 
 
-def test_mixed_argon_and_aro2_cases_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("batch", [False, True], ids=["streams", "batched"])
+def test_mixed_argon_and_aro2_cases_same_alone_and_in_the_mix(batch):
     """Argon and Ar/O2 cases of the serving workload's generator (grids of
     24-40 cells, one RF cycle at dt = T/400), served four at a time: every
     species of every case has the same bits as the case run alone."""
     reqs = serve.config5_requests(6, seed=11, rate_hz=100.0, grid=(24, 40), ppc=(10, 20), cycles=(1, 1),
                                   dx=5e-4)
     assert {serve.case_gas(r.deck) for r in reqs} == {0, 2}
-    res = serve.EnsembleServer(slice_steps=100, max_active=4).run(reqs, keep_state=lambda rid: True)
+    res = serve.EnsembleServer(slice_steps=100, max_active=4, batch=batch).run(reqs, keep_state=lambda rid: True)
     assert sorted(r["rid"] for r in res) == list(range(6))
     assert all(r["error"] == 0 for r in res)
     by_id = {r["rid"]: r for r in res}
