@@ -117,6 +117,11 @@ def particles_avg(r):
     return 0.5 * (sum(r["np0"]) + sum(r["np1"]))
 
 
+def progress(msg):
+    """One line on stderr per finished leg (the JSON line goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_threads():
     """Threads the CPU baseline uses: this process's CPU set, at most 16
     (the host share of one GPU)."""
@@ -244,22 +249,64 @@ def run_ensemble(n, rate_hz):
     # one by one first: it also takes the process's one-time costs (first
     # graph captures and kernel loads) out of the served runs
     alone = [serve.run_alone(r) for r in reqs]
+    progress(f"ensemble: {n} cases one by one")
     out = serve.summarize(serve.EnsembleServer(max_active=16, setup_threads=2).run(reqs), alone)
+    progress("ensemble: concurrent server")
     out["mode"] = ("concurrent: continuous admission by device-memory budget (at most 16 resident), one stream "
                    "per case, graph replays, case setup on 2 worker threads")
     srv = serve.EnsembleServer(max_active=16, setup_threads=2, batch=True)
     bat = serve.summarize(srv.run(reqs))
     out["batched"] = {k: bat[k] for k in ("p50_latency_s", "p99_latency_s", "makespan_s",
                                           "aggregate_particle_steps_per_s")}
+    progress("ensemble: batched server")
+    out["batch_stepping"] = run_batch_stepping()
+    progress("ensemble: batch stepping")
     out["batched"]["mode"] = ("concurrent admission as above; the resident cases stepped together through one "
                               "pic_batch (each kernel of the step launched once for all of them)")
     out["batched"]["batch_rebuilds"] = srv.rebuilds
     seq = serve.summarize(serve.SequentialServer().run(reqs))
+    progress("ensemble: sequential server")
     out["sequential"] = {k: seq[k] for k in ("p50_latency_s", "p99_latency_s", "makespan_s", "busy_fraction",
                                              "aggregate_particle_steps_per_s")}
     out["workload"] = (f"{n} Ar CCP cases, grids 32^2-96^2, 20-60 particles/cell, 10-500 mTorr log-uniform, "
                        f"50-300 V, 1-3 RF cycles of 400 steps, Poisson arrivals {rate_hz:g}/s, latency = finish - "
                        "arrival incl. setup and queueing")
+    return out
+
+
+def run_batch_stepping(steps=100):
+    """Stepping only (no setup, no admission): 16 argon cases of one size,
+    each on its own stream, against the same cases in one pic_batch."""
+    from picmcc import Batch, Case, FP32_REFINED, decks
+    out = []
+    for n, ncx, ppc in [(16, 32, 20), (16, 48, 30), (16, 64, 50), (8, 96, 50)]:
+        ds = [decks.ccp_deck(ncx=ncx, ncy=ncx, ppc=ppc, volts=150.0 + 5 * i, seed=100 + i, philox_seed=200 + i)
+              for i in range(n)]
+        cs = [Case(d, FP32_REFINED) for d in ds]
+        for c in cs:
+            c.step(2)
+        for c in cs:
+            c.sync()
+        npart = sum(sum(c.info().np[:2]) for c in cs)
+        t0 = time.perf_counter()
+        for c in cs:
+            c.step(steps)
+        for c in cs:
+            c.sync()
+        ts = time.perf_counter() - t0
+        b = Batch(cs)
+        b.step(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        tb = time.perf_counter() - t0
+        b.close()
+        for c in cs:
+            c.close()
+        out.append({"cases": n, "grid": f"{ncx}x{ncx}", "particles_per_cell": ppc, "particles": npart,
+                    "streams_ms_per_step": ts / steps * 1e3, "batch_ms_per_step": tb / steps * 1e3,
+                    "speedup": ts / tb})
     return out
 
 
@@ -271,14 +318,19 @@ def run_serve5(k, rate_hz=8.0):
     cases run one by one."""
     from picmcc import serve
     reqs = serve.config5_requests(512, rate_hz=rate_hz)[:k]
-    alone = [serve.run_alone(r) for r in reqs]
+    alone = []
+    for r in reqs:
+        alone.append(serve.run_alone(r))
+        progress(f"serving workload: case {r.rid} alone, {alone[-1]['seconds']:.1f} s")
     res = serve.EnsembleServer(max_active=16, setup_threads=2).run(reqs)
+    progress("serving workload: concurrent server")
     out = serve.summarize(res, alone)
     srv = serve.EnsembleServer(max_active=16, setup_threads=2, batch=True)
     bat = serve.summarize(srv.run(reqs))
     out["batched"] = {k: bat[k] for k in ("p50_latency_s", "p99_latency_s", "makespan_s",
                                           "aggregate_particle_steps_per_s", "failed")}
     out["batched"]["batch_rebuilds"] = srv.rebuilds
+    progress("serving workload: batched server")
     out["gas_mix"] = {"argon": sum(1 for r in reqs if serve.case_gas(r.deck) == 0),
                       "ar_o2": sum(1 for r in reqs if serve.case_gas(r.deck) == 2)}
     out["one_by_one_failed"] = sum(1 for r in alone if r["error"])
@@ -363,10 +415,12 @@ def main():
     serve5 = run_serve5(args.serve5) if args.serve5 > 0 else None
 
     ref = run_gpu(deck, FP32_REFINED, args.steps, args.warmup, timing_steps=min(args.steps, 100))
+    progress("FP32 refined steps")
     value = particles_avg(ref) * args.steps / ref["seconds"]
     plain, plain_value = None, 0.0
     if args.plain_steps > 0:
         plain = run_gpu(deck, FP64_PLAIN, args.plain_steps, 2)
+        progress("FP64 plain steps")
         plain_value = particles_avg(plain) * args.plain_steps / plain["seconds"]
     # the same precision comparison with the same field solver on both sides
     same, same_value = None, 0.0
@@ -374,11 +428,13 @@ def main():
         same = run_gpu(deck, FP64_PLAIN, args.same_solver_steps, 5, timing_steps=min(args.same_solver_steps, 50),
                        field_solver=2)
         same_value = particles_avg(same) * args.same_solver_steps / same["seconds"]
+        progress("FP64 steps on the separable solve")
     # the FP32 Jacobi PCG inside FP64 refinement (the solver for operators
     # that are not separable) on the same workload
     rpcg, rpcg_value = None, 0.0
     if args.refined_pcg_steps > 0:
         rpcg = run_gpu(deck, FP32_REFINED, args.refined_pcg_steps, 2, field_solver=1)
+        progress("FP32 refined PCG steps")
         rpcg_value = particles_avg(rpcg) * args.refined_pcg_steps / rpcg["seconds"]
 
     # roofline of the dominant particle kernel (HIP events on the case stream)
@@ -479,12 +535,16 @@ def main():
         line["roofline"]["limiter"] = lim
     if args.profile_cycles > 0:
         line["profile_error_vs_fp64"] = profile_check(deck, args.profile_cycles)
+        progress("profile check")
     if args.pbatch > 0:
         line["batched_poisson"] = run_pbatch(args.pbatch)
+        progress("batched Poisson")
     if args.o2_steps > 0:
         line["o2_chemistry"] = run_o2(args.o2_steps)
+        progress("O2 steps")
     if args.aro2_steps > 0:
         line["aro2_chemistry"] = run_aro2(args.aro2_steps)
+        progress("Ar/O2 steps")
     line["ensemble"] = ensemble if ensemble is not None else {"status": "not run (--ensemble 0)"}
     line["serving_workload"] = serve5 if serve5 is not None else {"status": "not run (--serve5 0)"}
     if args.cpu_steps > 0:

@@ -154,21 +154,26 @@ class EnsembleServer:
     memory is allocated by several creations at once, so a free-memory
     difference would not belong to one case).
 
-    batch: step the active argon cases on the separable field solve
-    together through one pic_batch (each kernel of the step launched once for
-    all of them; see picmcc.Batch), rebuilt whenever one joins or leaves; the
-    batch advances by the smallest remaining step count of its members, at
-    most slice_steps.  Other cases get their own slices as before.
+    batch: step the small active argon cases on the separable field solve
+    (at most batch_max_particles particles at admission) together through
+    one pic_batch (each kernel of the step launched once for all of them; see
+    picmcc.Batch), rebuilt whenever one joins or leaves; the batch advances
+    by the smallest remaining step count of its members, at most
+    slice_steps.  Other cases get their own slices as before.  Small cases
+    are launch-bound and gain from the shared launches; larger ones gain
+    more from overlapping on their own streams, where one case's kernels
+    run beside another's instead of after a grid-wide stage boundary.
 
     run() raises if CONCURRENT is off (DISABLED_REASON)."""
 
     def __init__(self, precision=FP32_REFINED, mem_fraction=0.8, slice_steps=50, max_active=16, setup_threads=2,
-                 batch=False):
+                 batch=False, batch_max_particles=400_000):
         self.precision = precision
         self.slice = slice_steps
         self.max_active = max_active
         self.setup_threads = setup_threads
         self.batch = batch
+        self.batch_max_particles = batch_max_particles
         free, _ = device_free_bytes()
         self.budget = int(free * mem_fraction)
         self._b = None          # the current Batch
@@ -248,7 +253,8 @@ class EnsembleServer:
             m.inflight = False
         for a in active:
             if a.batchable is None:
-                a.batchable = case_gas(a.req.deck) == 0 and a.case.field_solver_kind() == 1
+                a.batchable = (case_gas(a.req.deck) == 0 and a.case.field_solver_kind() == 1
+                               and a.np0 <= self.batch_max_particles)
         want = [a for a in active if a.batchable]
         freed = 0
         done = [a for a in want if a.remaining == 0]
