@@ -73,6 +73,7 @@ struct CaseBase {
     virtual int diag_sample() = 0;
     virtual int diag_history(double *out, int max_records, int *nrec, int *rec_len) = 0;
     virtual int dump_save(const char *path) = 0;
+    virtual int fluid_enable(int on) = 0;
     virtual int dump_load(const char *path) = 0;
     virtual int step_fused(int n) = 0;
     virtual int sync() = 0;
@@ -146,6 +147,10 @@ struct Case : CaseBase {
     int *diag_cc_prev = nullptr;
     long long diag_n = 0;
     long long tstep_base = 0;   // step at which the counters (mccr, CondCharge, history) last started
+    // metastable fluid (fluid.hip), allocated when first switched on
+    bool fluid_on = false;
+    FluidDev<R> fl{};
+    std::vector<void *> fl_bufs;
     double eedf_emax = 0;
     long long eedf_n = 0;
     bool clk_dirty = true;      // host tstep changed outside the fused path
@@ -214,6 +219,8 @@ struct Case : CaseBase {
                         sum_ey, total2, alive2, clk, eedf, lost, sigma_hits, o2prob, o2fluid, o2bg, o2bg2, o2bv2,
                         o2pcount, o2pstart, o2delreq, o2total, o2delcnt, diag_part, diag_hist, diag_cc_prev};
         for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        for (void *p : fl_bufs)
             if (p) (void)hipFree(p);
         for (int s = 0; s < CHEM_MAXSP; ++s) {
             if (o2fate[s]) (void)hipFree(o2fate[s]);
@@ -1109,7 +1116,80 @@ struct Case : CaseBase {
         HIPCHK(hipStreamSynchronize(st));
         return 0;
     }
+    // ------------------------------------------------------------ metastable fluid
+    template <typename T>
+    int fl_upload(T *&dst, const std::vector<T> &src, size_t n = 0) {
+        const size_t c = n ? n : src.size();
+        T *p = nullptr;
+        HIPCHK(dalloc(&p, c ? c : 1));
+        fl_bufs.push_back(p);
+        if (!src.empty()) HIPCHK(hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+        else HIPCHK(hipMemset(p, 0, (c ? c : 1) * sizeof(T)));
+        dst = p;
+        return 0;
+    }
+    int fluid_alloc() {
+        if (fl.state) return 0;
+        HIPCHK(hipStreamSynchronize(st));
+        FluidDev<R> d{};
+        d.ncx = S.ncx; d.ncy = S.ncy; d.ngx = S.ngx; d.ngy = S.ngy; d.G = G; d.nf = S.nfsp;
+        d.nx = S.fx_n; d.ny = S.fy_n; d.n_ave = S.deck.n_ave; d.TnRct = S.TnRct;
+        d.dtc = S.dtc; d.dx = S.dx; d.dy = S.dy; d.dt = S.dt; d.denscale_e = S.Denscale[0];
+        d.RR7 = S.RR[7]; d.RR9 = S.RR[9];
+        d.dtm = (double)S.dt_mcc; d.sigv1 = (double)S.sigv[1];
+        int *ip = nullptr;
+        R *rp = nullptr;
+        if (int e = fl_upload(ip, S.fx_x1)) return e; d.x_x1 = ip;
+        if (int e = fl_upload(ip, S.fx_x2)) return e; d.x_x2 = ip;
+        if (int e = fl_upload(ip, S.fx_y)) return e; d.x_y = ip;
+        if (int e = fl_upload(ip, S.fy_x)) return e; d.y_x = ip;
+        if (int e = fl_upload(ip, S.fy_y1)) return e; d.y_y1 = ip;
+        if (int e = fl_upload(ip, S.fy_y2)) return e; d.y_y2 = ip;
+        if (int e = fl_upload(ip, S.PlasmaRegion)) return e; d.plasma = ip;
+        if (int e = fl_upload(rp, S.fx_g1)) return e; d.x_g1 = rp;
+        if (int e = fl_upload(rp, S.fx_g2)) return e; d.x_g2 = rp;
+        if (int e = fl_upload(rp, S.fy_g1)) return e; d.y_g1 = rp;
+        if (int e = fl_upload(rp, S.fy_g2)) return e; d.y_g2 = rp;
+        if (int e = fl_upload(rp, S.fl_ax)) return e; d.ax = rp;
+        if (int e = fl_upload(rp, S.fl_bx)) return e; d.bx = rp;
+        if (int e = fl_upload(rp, S.fl_ay)) return e; d.ay = rp;
+        if (int e = fl_upload(rp, S.fl_by)) return e; d.by = rp;
+        if (int e = fl_upload(rp, S.BackDen1)) return e; d.BackDen1 = rp;
+        if (int e = fl_upload(d.den, S.fl_den)) return e;
+        if (int e = fl_upload(d.src, std::vector<R>(), S.fl_den.size())) return e;
+        if (int e = fl_upload(d.flux_x, S.fl_flux_x)) return e;
+        if (int e = fl_upload(d.flux_y, S.fl_flux_y)) return e;
+        const size_t scratch = std::max((size_t)S.fx_n * S.nfsp * S.ncx, (size_t)S.fy_n * S.nfsp * S.ncy);
+        if (int e = fl_upload(d.xs, std::vector<R>(), scratch)) return e;
+        if (int e = fl_upload(d.gs, std::vector<R>(), scratch)) return e;
+        if (int e = fl_upload(d.win, std::vector<R>(), (size_t)G)) return e;
+        if (int e = fl_upload(d.nodeN, std::vector<R>(S.FluidN.begin(), S.FluidN.begin() + G))) return e;
+        if (int e = fl_upload(d.mark, std::vector<unsigned long long>(), (size_t)G * S.TnRct)) return e;
+        // the event counts so far are not part of the first window
+        HIPCHK(hipMemcpy(d.mark, mccr, (size_t)G * S.TnRct * sizeof(unsigned long long), hipMemcpyDeviceToDevice));
+        if (int e = fl_upload(d.state, std::vector<FluidState>(1, FluidState{0, 0}))) return e;
+        fl = d;
+        return 0;
+    }
+    int fluid_enable(int on) override {
+        if (on && gas != 0)
+            return fail(-75, "the metastable fluid coupling is built for argon cases (gas type 0)");
+        if (on && S.deck.n_ave <= 0) return fail(-75, "the metastable fluid needs AverageIter > 0");
+        if (on)
+            if (int e = fluid_alloc()) return e;
+        if (fluid_on != (on != 0)) {
+            // the captured step graphs hold the old launch sequence
+            HIPCHK(hipStreamSynchronize(st));
+            for (int q = 0; q < 2; ++q)
+                if (gexec[q]) { (void)hipGraphExecDestroy(gexec[q]); gexec[q] = nullptr; gkey[q] = nullptr; }
+        }
+        fluid_on = on != 0;
+        return 0;
+    }
     int diagnostic() override {
+        // the continuity solve and the fluid's averaging come between the
+        // deposit and the diagnostic (src/cuda_main.cu:84-92)
+        if (fluid_on) launch_fluid_step<R>(fl, den, mccr, P1, P2, st);
         launch_accumulate<R>(G, nsp, den, tot, Ex, Ey, sum_den, sum_phi, sum_ex, sum_ey, st);
         ++n_accum;
         HIPCHK(hipGetLastError());
@@ -1154,6 +1234,7 @@ struct Case : CaseBase {
         DepositExtras<R> ex{1, sum_den, sum_phi, sum_ex, sum_ey, tot, Ex, Ey, next_id, pstart + G, clk,
                             (double)S.dt};
         launch_deposit<R>(kc, nc, cur, part, den, sigma_sp, Sigma, Source, brhs, &ex, st);
+        if (fluid_on) launch_fluid_step<R>(fl, den, mccr, P1, P2, st);
         HIPCHK(hipGetLastError());
         mark(8);
         mark(9);
@@ -1369,6 +1450,17 @@ struct Case : CaseBase {
         if (n == "sum_ex") return D(sum_ex, G, 8, 1);
         if (n == "sum_ey") return D(sum_ey, G, 8, 1);
         if (n == "next_id") return D(next_id, nsp, 4, 0);
+        if (fl.state) {
+            const size_t NC = (size_t)S.nfsp * S.ncx * S.ncy;
+            if (n == "fl_den") return D(fl.den, NC, rs, 1);
+            if (n == "fl_src") return D(fl.src, NC, rs, 1);
+            if (n == "fl_flux_x") return D(fl.flux_x, (size_t)S.nfsp * S.ngx * S.ncy, rs, 1);
+            if (n == "fl_flux_y") return D(fl.flux_y, (size_t)S.nfsp * S.ncx * S.ngy, rs, 1);
+            if (n == "fl_win") return D(fl.win, G, rs, 1);
+            if (n == "FluidN") return D(fl.nodeN, G, rs, 1, false);
+            if (n == "P2") return D(P2, G, rs, 1, false);
+            if (n == "fl_state") return D(fl.state, 2, 4, 0, false);
+        }
         if (n == "detach_removed") {
             if (!o2delcnt) return false;
             return D(o2delcnt, G, 8, 0);
@@ -1516,6 +1608,24 @@ static bool setup_lookup(Setup<R> &S, const std::string &n, const void *&p, size
     if (n == "P2") return H(S.P2.data(), S.P2.size(), rs, 1);
     if (n == "Pion") return H(S.Pion.data(), S.Pion.size(), rs, 1);
     if (n == "vsave") return H(S.vsave.data(), S.vsave.size(), rs, 1);
+    if (n == "fl_x1") return H(S.fx_x1.data(), S.fx_x1.size(), 4, 0);
+    if (n == "fl_x2") return H(S.fx_x2.data(), S.fx_x2.size(), 4, 0);
+    if (n == "fl_xy") return H(S.fx_y.data(), S.fx_y.size(), 4, 0);
+    if (n == "fl_yx") return H(S.fy_x.data(), S.fy_x.size(), 4, 0);
+    if (n == "fl_y1") return H(S.fy_y1.data(), S.fy_y1.size(), 4, 0);
+    if (n == "fl_y2") return H(S.fy_y2.data(), S.fy_y2.size(), 4, 0);
+    if (n == "fl_xg1") return H(S.fx_g1.data(), S.fx_g1.size(), rs, 1);
+    if (n == "fl_xg2") return H(S.fx_g2.data(), S.fx_g2.size(), rs, 1);
+    if (n == "fl_yg1") return H(S.fy_g1.data(), S.fy_g1.size(), rs, 1);
+    if (n == "fl_yg2") return H(S.fy_g2.data(), S.fy_g2.size(), rs, 1);
+    if (n == "fl_D") return H(S.fl_D.data(), S.fl_D.size(), rs, 1);
+    if (n == "fl_den") return H(S.fl_den.data(), S.fl_den.size(), rs, 1);
+    if (n == "fl_ax") return H(S.fl_ax.data(), S.fl_ax.size(), rs, 1);
+    if (n == "fl_bx") return H(S.fl_bx.data(), S.fl_bx.size(), rs, 1);
+    if (n == "fl_ay") return H(S.fl_ay.data(), S.fl_ay.size(), rs, 1);
+    if (n == "fl_by") return H(S.fl_by.data(), S.fl_by.size(), rs, 1);
+    if (n == "fl_flux_x") return H(S.fl_flux_x.data(), S.fl_flux_x.size(), rs, 1);
+    if (n == "fl_flux_y") return H(S.fl_flux_y.data(), S.fl_flux_y.size(), rs, 1);
     if (n.size() > 4 && n.compare(0, 2, "sp") == 0 && n[3] == '.') {
         int s = n[2] - '0';
         if (s < 0 || s >= S.nsp) return false;
@@ -1550,6 +1660,10 @@ static bool setup_scalar(Setup<R> &S, const std::string &n, double &v) {
     if (n == "idLOGX") { v = (double)S.idLOGX; return true; }
     if (n == "idx") { v = (double)S.idx; return true; }
     if (n == "idy") { v = (double)S.idy; return true; }
+    if (n == "dtc") { v = (double)S.dtc; return true; }
+    if (n == "fl_nx") { v = S.fx_n; return true; }
+    if (n == "fl_ny") { v = S.fy_n; return true; }
+    if (n == "nfsp") { v = S.nfsp; return true; }
     if (n.compare(0, 4, "np2c") == 0 && n.size() == 5) { v = (double)S.np2c[n[4] - '0']; return true; }
     if (n.compare(0, 2, "np") == 0 && n.size() == 3) { v = S.sp[n[2] - '0'].n; return true; }
     if (n.compare(0, 6, "Ascale") == 0 && n.size() == 7) { v = (double)S.Ascale[n[6] - '0']; return true; }
@@ -1629,6 +1743,7 @@ struct Batch : BatchBase {
             if (c->gas != 0) return fail(-70, "batch: case %zu is not an argon case (its step is not the fused one)", i);
             if (!c->spectral) return fail(-70, "batch: case %zu does not use the separable field solve", i);
             if (c->timing) return fail(-70, "batch: case %zu has stage timers on", i);
+            if (c->fluid_on) return fail(-70, "batch: case %zu runs the metastable fluid", i);
             if (c->spec_cycles != cs[0]->spec_cycles)
                 return fail(-70, "batch: case %zu's field solve runs %d cycles, case 0's %d", i, c->spec_cycles,
                             cs[0]->spec_cycles);
@@ -1895,6 +2010,7 @@ int pic_device_memory(int device, size_t *free_bytes, size_t *total_bytes) {
     return 0;
 }
 int pic_eedf_sample(pic_ctx *ctx, double emax_ev) { CTX_OR_FAIL(ctx); return ctx->c->eedf_sample(emax_ev); }
+int pic_fluid_enable(pic_ctx *ctx, int on) { CTX_OR_FAIL(ctx); return ctx->c->fluid_enable(on); }
 int pic_dump_save(pic_ctx *ctx, const char *path) {
     CTX_OR_FAIL(ctx);
     if (!path) return fail(-3, "dump: no path");

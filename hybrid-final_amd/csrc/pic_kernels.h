@@ -184,6 +184,24 @@ template <typename R>
 void launch_diag_sample(const DiagConst &d, const PSet<R> &ps, double *part, const R *Sigma, const R *Area,
                         const int *CondID, const int *condcharge, int *cc_prev, const R *V, long long step,
                         double *rec, hipStream_t s);
+// metastable fluid (fluid.hip): device state and arrays of one case
+struct FluidState {
+    int nave, conti;
+};
+template <typename R>
+struct FluidDev {
+    int ncx, ncy, ngx, ngy, G, nf, nx, ny, n_ave, TnRct;
+    R dtc, dx, dy, dt, denscale_e, RR7, RR9;
+    double dtm, sigv1;
+    const int *x_x1, *x_x2, *x_y, *y_x, *y_y1, *y_y2, *plasma;
+    const R *x_g1, *x_g2, *y_g1, *y_g2, *ax, *bx, *ay, *by, *BackDen1;
+    R *den, *src, *flux_x, *flux_y, *xs, *gs, *win, *nodeN;
+    unsigned long long *mark;
+    FluidState *state;
+};
+template <typename R>
+void launch_fluid_step(const FluidDev<R> &d, const R *den_e, const unsigned long long *mccr, const R *P1, R *P2,
+                       hipStream_t s);
 // V000 dump particle records of one species, in slot order (dump.hip)
 template <typename R>
 void launch_dump_pack(const PView<R> &p, int ncy, int ngy, int Csize, uint32_t *rec, hipStream_t s);

@@ -1,6 +1,7 @@
 // Host-side case setup; see pic_setup.h for the reference lines restated.
 #include "pic_setup.h"
 
+#include <cfloat>
 #include <cmath>
 #include <algorithm>
 #include <cstring>
@@ -109,6 +110,7 @@ int parse_deck(const std::string &text, Deck &d, std::string &err) {
     }
     const JValue &sm = root.object("SimulationMethod");
     d.dt_pic = (int)sm.number("TimeStep_PIC");
+    d.dt_conti = (int)sm.number("TimeStep_Conti");
     d.pcg_tol = sm.number("PCGMarginOfError");
     d.n_ave = (int)sm.number("AverageIter");
     d.n_smt = (int)sm.number("DensitySmoothing");
@@ -851,6 +853,170 @@ int load(Setup<R> &s, std::string &err) {
 
 }  // namespace
 
+// ------------------------------------------------ metastable fluid set-up
+// Collision integral of Ar* diffusion against T* = kT/epsilon, by the
+// reference's table and loop (src/cuda_Fluid.cu:261-283).  That loop has no
+// break, so inside the table every T* takes the last interval's line,
+// omega[25..26]; the same is done here.
+static float ar_meta_omega(float value) {
+    static const float tstar[27] = {0.250f, 0.267f, 0.286f, 0.308f, 0.333f, 0.364f, 0.400f, 0.444f, 0.500f,
+                                    0.571f, 0.667f, 0.800f, 1.000f, 1.330f, 2.000f, 2.500f, 2.860f, 3.330f,
+                                    4.000f, 5.000f, 6.670f, 10.00f, 12.50f, 16.67f, 25.00f, 50.00f, 100.0f};
+    static const float omega[27] = {1.8714f, 1.8344f, 1.7962f, 1.7560f, 1.7142f, 1.6702f, 1.6238f, 1.5748f, 1.5228f,
+                                    1.4676f, 1.4090f, 1.3466f, 1.2802f, 1.2096f, 1.1348f, 1.1042f, 1.0888f, 1.0738f,
+                                    1.0592f, 1.0450f, 1.0318f, 1.0198f, 1.0154f, 1.0112f, 1.0072f, 1.0034f, 1.000f};
+    float r = (float)DBL_MIN;
+    if (value < tstar[0]) return omega[0];
+    if (value > tstar[26]) return omega[26];
+    for (int i = 1; i < 27; ++i)
+        if (value < tstar[i]) {
+            const float t = (value - tstar[i - 1]) / (tstar[i] - tstar[i - 1]);
+            r = omega[i] * t + omega[i - 1] * (1 - t);
+        }
+    return r;
+}
+
+template <typename R>
+void fluid_setup(Setup<R> &s) {
+    const int ncx = s.ncx, ncy = s.ncy, ngx = s.ngx, ngy = s.ngy, nf = s.nfsp;
+    s.dtc = s.dt * (R)s.deck.dt_conti;
+    auto plasma = [&](int i, int j) { return s.PlasmaRegion[i * ncy + j] == 1; };
+    // lines (src/Fluid.c:188-256): along x, y outer; along y, x outer
+    s.fx_x1.clear(); s.fx_x2.clear(); s.fx_y.clear(); s.fy_x.clear(); s.fy_y1.clear(); s.fy_y2.clear();
+    for (int j = 0; j < ncy; ++j)
+        for (int i = 0; i < ncx; ++i) {
+            if (!plasma(i, j)) continue;
+            if (i == 0 || !plasma(i - 1, j)) { s.fx_x1.push_back(i); s.fx_y.push_back(j); }
+            if (i == ncx - 1 || !plasma(i + 1, j)) s.fx_x2.push_back(i);
+        }
+    for (int i = 0; i < ncx; ++i)
+        for (int j = 0; j < ncy; ++j) {
+            if (!plasma(i, j)) continue;
+            if (j == 0 || !plasma(i, j - 1)) { s.fy_x.push_back(i); s.fy_y1.push_back(j); }
+            if (j == ncy - 1 || !plasma(i, j + 1)) s.fy_y2.push_back(j);
+        }
+    s.fx_n = (int)s.fx_x2.size();
+    s.fy_n = (int)s.fy_y2.size();
+    // wall flux factors (src/Fluid.c:113-186): 0 at a Neumann node on the
+    // domain edge, the species' quenching probability everywhere else
+    s.fx_g1.assign((size_t)nf * s.fx_n, 0); s.fx_g2.assign((size_t)nf * s.fx_n, 0);
+    s.fy_g1.assign((size_t)nf * s.fy_n, 0); s.fy_g2.assign((size_t)nf * s.fy_n, 0);
+    auto neu = [&](int g) { return s.Boundary[g] == NEUMANN; };
+    for (int f = 0; f < nf; ++f) {
+        const R gam = (R)s.deck.fl_quench[f];
+        for (int k = 0; k < s.fx_n; ++k) {
+            const int x1 = s.fx_x1[k], x2 = s.fx_x2[k], yy = s.fx_y[k];
+            const bool n1 = neu(x1 * ngy + yy) || neu(x1 * ngy + yy + 1);
+            const bool n2 = neu((x2 + 1) * ngy + yy) || neu((x2 + 1) * ngy + yy + 1);
+            s.fx_g1[(size_t)f * s.fx_n + k] = (n1 && x1 == 0) ? (R)0 : gam;
+            s.fx_g2[(size_t)f * s.fx_n + k] = (n2 && x2 == ncx - 1) ? (R)0 : gam;
+        }
+        for (int k = 0; k < s.fy_n; ++k) {
+            const int xx = s.fy_x[k], y1 = s.fy_y1[k], y2 = s.fy_y2[k];
+            const bool n1 = neu(xx * ngy + y1) || neu((xx + 1) * ngy + y1);
+            const bool n2 = neu(xx * ngy + y2 + 1) || neu((xx + 1) * ngy + y2 + 1);
+            s.fy_g1[(size_t)f * s.fy_n + k] = (n1 && y1 == 0) ? (R)0 : gam;
+            s.fy_g2[(size_t)f * s.fy_n + k] = (n2 && y2 == ncy - 1) ? (R)0 : gam;
+        }
+    }
+    // diffusion coefficients per cell (src/cuda_Fluid.cu:199-260), with the
+    // reference's mixed float/double arithmetic
+    const size_t C = (size_t)ncx * ncy;
+    s.fl_D.assign((size_t)nf * C, 0);
+    const R tp = (R)s.deck.total_pres;
+    for (int f = 0; f < nf; ++f)
+        for (int i = 0; i < ncx; ++i)
+            for (int j = 0; j < ncy; ++j) {
+                const int g = i * ngy + j;
+                const R T = (R)(0.25 * (double)(s.Temp[g] + s.Temp[g + 1] + s.Temp[g + ngy] + s.Temp[g + ngy + 1]));
+                double D;
+                const double T3 = (double)(T * T * T * (R)24);
+                if (s.deck.gas == 0) {
+                    const R om = (R)ar_meta_omega((float)(T / (R)124.0f));
+                    const R den = tp * (R)11.682724f * om;
+                    D = 1.997279e-4 * std::sqrt((double)T / 39.948) * (double)T / (double)den * 20;
+                } else {
+                    // species: O2 mixtures O2A, O2B (O2), OP, OD (O); Ar/O2 Ar* first
+                    const int k = s.deck.gas == 2 ? f - 1 : f;   // -1: Ar*
+                    if (k < 0)
+                        D = 2.63e-3 * std::sqrt(T3 / 39.948 / 39.948) / (double)tp * (double)(R)0.08559f;
+                    else if (k <= 1)
+                        D = 2.63e-3 * std::sqrt((double)(T * T * T * (R)24 / (R)32 / (R)32)) / (double)tp * (double)(R)0.08650f;
+                    else
+                        D = 2.63e-3 * std::sqrt((double)(T * T * T * (R)24 / (R)16 / (R)16)) / (double)tp * (double)(R)0.10892f;
+                }
+                s.fl_D[(size_t)f * C + (size_t)i * ncy + j] = (R)D;
+            }
+    // Scharfetter-Gummel coefficients (src/cuda_Fluid.cu:179-198)
+    const size_t FX = (size_t)ngx * ncy, FY = (size_t)ncx * ngy;
+    s.fl_ax.assign((size_t)nf * FX, 0); s.fl_bx.assign((size_t)nf * FX, 0);
+    s.fl_ay.assign((size_t)nf * FY, 0); s.fl_by.assign((size_t)nf * FY, 0);
+    for (int f = 0; f < nf; ++f) {
+        const R *D = s.fl_D.data() + (size_t)f * C;
+        R *ax = s.fl_ax.data() + (size_t)f * FX, *bx = s.fl_bx.data() + (size_t)f * FX;
+        R *ay = s.fl_ay.data() + (size_t)f * FY, *by = s.fl_by.data() + (size_t)f * FY;
+        const double m = (double)s.fl_mass[f];
+        auto vth = [&](int g) { return 0.25 * std::sqrt((double)s.Temp[g] * 1.38e-23 / m); };
+        for (int k = 0; k < s.fx_n; ++k) {
+            const int x1 = s.fx_x1[k], x2 = s.fx_x2[k], yy = s.fx_y[k];
+            const double g1 = (double)s.fx_g1[(size_t)f * s.fx_n + k], g2 = (double)s.fx_g2[(size_t)f * s.fx_n + k];
+            bx[x1 * ncy + yy] = (R)(-g1 * (-vth(x1 * ngy + yy)));
+            ax[x1 * ncy + yy] = 0;
+            for (int i = x1 + 1; i <= x2; ++i) {
+                // the reference averages D[i-1] with itself here
+                ax[i * ncy + yy] = (D[(i - 1) * ncy + yy] + D[(i - 1) * ncy + yy]) / (R)2 / s.dx;
+                bx[i * ncy + yy] = ax[i * ncy + yy];
+            }
+            ax[(x2 + 1) * ncy + yy] = (R)(g2 * vth(x2 * ngy + yy));
+            bx[(x2 + 1) * ncy + yy] = 0;
+        }
+        for (int k = 0; k < s.fy_n; ++k) {
+            const int xx = s.fy_x[k], y1 = s.fy_y1[k], y2 = s.fy_y2[k];
+            const double g1 = (double)s.fy_g1[(size_t)f * s.fy_n + k], g2 = (double)s.fy_g2[(size_t)f * s.fy_n + k];
+            by[xx * ngy + y1] = (R)(-g1 * (-vth(xx * ngy + y1)));
+            ay[xx * ngy + y1] = 0;
+            for (int j = y1 + 1; j <= y2; ++j) {
+                ay[xx * ngy + j] = (D[xx * ncy + j - 1] + D[xx * ncy + j]) / (R)2 / s.dy;
+                by[xx * ngy + j] = ay[xx * ngy + j];
+            }
+            ay[xx * ngy + y2 + 1] = (R)(g2 * vth(xx * ngy + y2));
+            by[xx * ngy + y2 + 1] = 0;
+        }
+    }
+    // initial density (uniform load over the plasma, src/Fluid.c:35-67) and
+    // face fluxes (src/cuda_Fluid.cu:51-90)
+    s.fl_den.assign((size_t)nf * C, 0);
+    for (int f = 0; f < nf; ++f)
+        for (int i = 0; i < ncx; ++i)
+            for (int j = 0; j < ncy; ++j)
+                if (s.PlasmaRegion[i * ncy + j] != 0 && s.deck.fl_loadtype == 0)
+                    s.fl_den[(size_t)f * C + (size_t)i * ncy + j] = s.fl_dens[f];
+    s.fl_flux_x.assign((size_t)nf * FX, 0);
+    s.fl_flux_y.assign((size_t)nf * FY, 0);
+    for (int f = 0; f < nf; ++f) {
+        const R *n = s.fl_den.data() + (size_t)f * C;
+        const R *ax = s.fl_ax.data() + (size_t)f * FX, *bx = s.fl_bx.data() + (size_t)f * FX;
+        const R *ay = s.fl_ay.data() + (size_t)f * FY, *by = s.fl_by.data() + (size_t)f * FY;
+        R *fx = s.fl_flux_x.data() + (size_t)f * FX, *fy = s.fl_flux_y.data() + (size_t)f * FY;
+        for (int k = 0; k < s.fx_n; ++k) {
+            const int x1 = s.fx_x1[k], x2 = s.fx_x2[k], yy = s.fx_y[k];
+            const R g1 = s.fx_g1[(size_t)f * s.fx_n + k], g2 = s.fx_g2[(size_t)f * s.fx_n + k];
+            fx[x1 * ncy + yy] = -g1 * bx[x1 * ncy + yy] * n[x1 * ncy + yy];
+            for (int i = x1 + 1; i <= x2; ++i)
+                fx[i * ncy + yy] = ax[i * ncy + yy] * n[(i - 1) * ncy + yy] - bx[i * ncy + yy] * n[i * ncy + yy];
+            fx[(x2 + 1) * ncy + yy] = g2 * ax[(x2 + 1) * ncy + yy] * n[x2 * ncy + yy];
+        }
+        for (int k = 0; k < s.fy_n; ++k) {
+            const int xx = s.fy_x[k], y1 = s.fy_y1[k], y2 = s.fy_y2[k];
+            const R g1 = s.fy_g1[(size_t)f * s.fy_n + k], g2 = s.fy_g2[(size_t)f * s.fy_n + k];
+            fy[xx * ngy + y1] = -g1 * by[xx * ngy + y1] * n[xx * ncy + y1];
+            for (int j = y1 + 1; j <= y2; ++j)
+                fy[xx * ngy + j] = ay[xx * ngy + j] * n[xx * ncy + j - 1] - by[xx * ngy + j] * n[xx * ncy + j];
+            fy[xx * ngy + y2 + 1] = g2 * ay[xx * ngy + y2 + 1] * n[xx * ncy + y2];
+        }
+    }
+}
+
 template <typename R>
 int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool do_load) {
     s.deck = d;
@@ -869,6 +1035,7 @@ int build_setup(const Deck &d, Setup<R> &s, std::string &err, bool do_load) {
     }
     mass_ratios(s);
     maxwell_table(s);
+    fluid_setup(s);
     if (do_load) {
         e = load(s, err);
         if (e) return e;

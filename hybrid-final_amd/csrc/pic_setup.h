@@ -35,7 +35,7 @@ struct Deck {
     double sp_x0 = 0, sp_x1 = 0, sp_y0 = 0, sp_y1 = 0;
     double sp_dens[MAXSP] = {0}, sp_temp[MAXSP] = {0}, sp_np2c[MAXSP] = {0};
     int sp_ratio[MAXSP] = {0}, sp_maxnp[MAXSP] = {0};
-    int dt_pic = 0;
+    int dt_pic = 0, dt_conti = 0;
     double pcg_tol = 0;
     int n_ave = 0, n_smt = 0, np_limit = 0;
     double coll_flag[100] = {0};
@@ -89,6 +89,17 @@ struct Setup {
     int nvel = 0;
     std::vector<R> vsave;
     SpeciesInit<R> sp[MAXSP];
+    // metastable fluid (src/Fluid.c:3-256, src/cuda_Fluid.cu:3-33, 179-301):
+    // lines of consecutive plasma cells along x (fixed y) and along y (fixed
+    // x); per fluid species the wall flux factor at each line end, the
+    // diffusion coefficient per cell, the Scharfetter-Gummel coefficients per
+    // cell face, the initial density and face fluxes.  Layouts per species:
+    // cell arrays [ncx][ncy], x faces [ngx][ncy], y faces [ncx][ngy].
+    R dtc = 0;
+    int fx_n = 0, fy_n = 0;
+    std::vector<int> fx_x1, fx_x2, fx_y, fy_x, fy_y1, fy_y2;
+    std::vector<R> fx_g1, fx_g2, fy_g1, fy_g2;           // nfsp x lines
+    std::vector<R> fl_D, fl_den, fl_ax, fl_bx, fl_flux_x, fl_ay, fl_by, fl_flux_y;
 };
 
 template <typename R>

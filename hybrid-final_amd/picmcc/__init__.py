@@ -75,6 +75,7 @@ EXPORTS = {
     "pic_field_solver_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "pic_diag_sample": (C.c_int, [C.c_void_p]),
     "pic_dump_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pic_fluid_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "pic_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "pic_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "pic_batch_sync": (C.c_int, [C.c_void_p]),
@@ -269,6 +270,10 @@ class Case:
         n = C.c_longlong()
         _check(lib().pic_stage_times(self.h, ms, len(self.STAGES), C.byref(n)))
         return dict(zip(self.STAGES, list(ms))), n.value
+
+    def fluid(self, on=True):
+        """Switches the metastable fluid coupling on or off (argon cases)."""
+        _check(lib().pic_fluid_enable(self.h, int(bool(on))))
 
     def dump_save(self, path):
         """Writes the case's state in the reference's V000 dump format."""

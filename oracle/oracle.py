@@ -47,6 +47,7 @@ class OrcDeck(C.Structure):
         ("sp_dens", C.c_double * MAXSP), ("sp_temp", C.c_double * MAXSP), ("sp_np2c", C.c_double * MAXSP),
         ("sp_ratio", C.c_int * MAXSP), ("sp_maxnp", C.c_int * MAXSP),
         ("dt_pic", C.c_int),
+        ("dt_conti", C.c_int),
         ("pcg_tol", C.c_double),
         ("n_ave", C.c_int), ("n_smt", C.c_int), ("np_limit", C.c_int),
         ("coll_flag", C.c_double * MAXRCT),
@@ -149,6 +150,7 @@ def deck_to_struct(deck):
         d.sp_ratio[k], d.sp_maxnp[k] = _int(_num(o, "Ratio")), _int(_num(o, "Max_np"))
     sm = _obj(deck, "SimulationMethod")
     d.dt_pic = _int(_num(sm, "TimeStep_PIC"))
+    d.dt_conti = _int(_num(sm, "TimeStep_Conti"))
     d.pcg_tol = _num(sm, "PCGMarginOfError")
     d.n_ave = _int(_num(sm, "AverageIter"))
     d.n_smt = _int(_num(sm, "DensitySmoothing"))
@@ -179,7 +181,8 @@ class Oracle:
         L.orc_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_long), C.POINTER(C.c_int)]
         L.orc_scalar.restype = C.c_double
         L.orc_scalar.argtypes = [C.c_void_p, C.c_char_p]
-        for fn in ["orc_field", "orc_efield", "orc_move", "orc_deposit", "orc_accumulate"]:
+        for fn in ["orc_field", "orc_efield", "orc_move", "orc_deposit", "orc_accumulate", "orc_fluid_step",
+                   "orc_fluid_continuity", "orc_fluid_diag"]:
             getattr(L, fn).restype = None
             getattr(L, fn).argtypes = [C.c_void_p]
         for fn in ["orc_sort", "orc_mcc", "orc_step"]:
@@ -189,6 +192,8 @@ class Oracle:
         L.orc_run.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_np.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_set_tstep.argtypes = [C.c_void_p, C.c_long]
+        L.orc_set_fluid.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_fluid.restype = None
         L.orc_destroy.restype = None
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_voltages.argtypes = [C.c_void_p, C.c_double]
@@ -211,7 +216,8 @@ class Oracle:
             raise KeyError(name)
         dt = {4: np.int32, 8: np.int64}
         if name in ("Boundary", "CondID", "Face", "DensRegion", "PlasmaRegion", "StructIdx",
-                    "A_idx", "Ai", "Aj") or name.endswith((".cell_start", ".tag")):
+                    "A_idx", "Ai", "Aj", "fl_x1", "fl_x2", "fl_xy", "fl_yx", "fl_y1", "fl_y2") or \
+                name.endswith((".cell_start", ".tag")):
             dtype = np.int32
         elif name.endswith(".id"):
             dtype = np.uint32

@@ -4,6 +4,7 @@
  * direct FP64 solves, cross-section tables, null-collision constants,
  * Maxwellian table and the Park-Miller loader.
  */
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -691,6 +692,170 @@ static int load_particles(orc_case *c) {
     return 0;
 }
 
+/* ================= metastable fluid: src/Fluid.c:3-256, src/cuda_Fluid.cu:3-33, 51-90, 179-301 ===== */
+/* Ar_meta_omega (src/cuda_Fluid.cu:261-283), loop without a break included */
+static float meta_omega(float value) {
+    static const float ts[27] = {0.250f, 0.267f, 0.286f, 0.308f, 0.333f, 0.364f, 0.400f, 0.444f, 0.500f,
+                                 0.571f, 0.667f, 0.800f, 1.000f, 1.330f, 2.000f, 2.500f, 2.860f, 3.330f,
+                                 4.000f, 5.000f, 6.670f, 10.00f, 12.50f, 16.67f, 25.00f, 50.00f, 100.0f};
+    static const float om[27] = {1.8714f, 1.8344f, 1.7962f, 1.7560f, 1.7142f, 1.6702f, 1.6238f, 1.5748f, 1.5228f,
+                                 1.4676f, 1.4090f, 1.3466f, 1.2802f, 1.2096f, 1.1348f, 1.1042f, 1.0888f, 1.0738f,
+                                 1.0592f, 1.0450f, 1.0318f, 1.0198f, 1.0154f, 1.0112f, 1.0072f, 1.0034f, 1.000f};
+    float t, r = (float)DBL_MIN;
+    int i;
+    if (value < ts[0]) r = om[0];
+    else if (value > ts[26]) r = om[26];
+    else
+        for (i = 1; i < 27; i++)
+            if (value < ts[i]) { t = (value - ts[i - 1]) / (ts[i] - ts[i - 1]); r = om[i] * t + om[i - 1] * (1 - t); }
+    return r;
+}
+float orc_meta_omega(float v) { return meta_omega(v); }
+
+/* Cal_XRegion_check / Cal_YRegion_check / Set_Con_Region (src/Fluid.c:188-256) */
+static void fluid_lines(orc_case *c) {
+    int i, j, k, ncx = c->ncx, ncy = c->ncy;
+#define PL(i, j) (c->PlasmaRegion[(i) * ncy + (j)] == 1)
+    c->nlx = 0;
+    for (j = 0; j < ncy; j++)
+        for (i = 0; i < ncx; i++)
+            if (PL(i, j) && (i == ncx - 1 || !PL(i + 1, j))) c->nlx++;
+    c->nly = 0;
+    for (i = 0; i < ncx; i++)
+        for (j = 0; j < ncy; j++)
+            if (PL(i, j) && (j == ncy - 1 || !PL(i, j + 1))) c->nly++;
+    c->lx_x1 = xcalloc(c->nlx + 1, sizeof(int)); c->lx_x2 = xcalloc(c->nlx + 1, sizeof(int));
+    c->lx_y = xcalloc(c->nlx + 1, sizeof(int));
+    c->ly_x = xcalloc(c->nly + 1, sizeof(int)); c->ly_y1 = xcalloc(c->nly + 1, sizeof(int));
+    c->ly_y2 = xcalloc(c->nly + 1, sizeof(int));
+    k = 0;
+    for (j = 0; j < ncy; j++)
+        for (i = 0; i < ncx; i++) {
+            if (!PL(i, j)) continue;
+            if (i == 0 || !PL(i - 1, j)) { c->lx_x1[k] = i; c->lx_y[k] = j; }
+            if (i == ncx - 1 || !PL(i + 1, j)) { c->lx_x2[k] = i; k++; }
+        }
+    k = 0;
+    for (i = 0; i < ncx; i++)
+        for (j = 0; j < ncy; j++) {
+            if (!PL(i, j)) continue;
+            if (j == 0 || !PL(i, j - 1)) { c->ly_x[k] = i; c->ly_y1[k] = j; }
+            if (j == ncy - 1 || !PL(i, j + 1)) { c->ly_y2[k] = j; k++; }
+        }
+#undef PL
+}
+
+/* Set_Con_Boundary (src/Fluid.c:113-186): every branch gives Gamma1 except a
+ * Neumann node at a line end on the domain edge, which gives 0 */
+static void fluid_ends(orc_case *c) {
+    int f, k, ngy = c->ngy;
+    c->lx_f1 = xcalloc((size_t)c->nfsp * c->nlx + 1, sizeof(real)); c->lx_f2 = xcalloc((size_t)c->nfsp * c->nlx + 1, sizeof(real));
+    c->ly_f1 = xcalloc((size_t)c->nfsp * c->nly + 1, sizeof(real)); c->ly_f2 = xcalloc((size_t)c->nfsp * c->nly + 1, sizeof(real));
+    for (f = 0; f < c->nfsp; f++) {
+        real g = (real)(float)c->deck.fl_quench[f];
+        for (k = 0; k < c->nlx; k++) {
+            int x1 = c->lx_x1[k], x2 = c->lx_x2[k], yy = c->lx_y[k];
+            int a = c->Boundary[x1 * ngy + yy] == ORC_NEUMANN || c->Boundary[x1 * ngy + yy + 1] == ORC_NEUMANN;
+            int b = c->Boundary[(x2 + 1) * ngy + yy] == ORC_NEUMANN || c->Boundary[(x2 + 1) * ngy + yy + 1] == ORC_NEUMANN;
+            c->lx_f1[f * c->nlx + k] = (a && x1 == 0) ? 0 : g;
+            c->lx_f2[f * c->nlx + k] = (b && x2 == c->ncx - 1) ? 0 : g;
+        }
+        for (k = 0; k < c->nly; k++) {
+            int xx = c->ly_x[k], y1 = c->ly_y1[k], y2 = c->ly_y2[k];
+            int a = c->Boundary[xx * ngy + y1] == ORC_NEUMANN || c->Boundary[(xx + 1) * ngy + y1] == ORC_NEUMANN;
+            int b = c->Boundary[xx * ngy + y2 + 1] == ORC_NEUMANN || c->Boundary[(xx + 1) * ngy + y2 + 1] == ORC_NEUMANN;
+            c->ly_f1[f * c->nly + k] = (a && y1 == 0) ? 0 : g;
+            c->ly_f2[f * c->nly + k] = (b && y2 == c->ncy - 1) ? 0 : g;
+        }
+    }
+}
+
+/* Cal_D_Argon / Cal_D_Oxygen / Cal_D_ArO2 (src/cuda_Fluid.cu:199-260) */
+static void fluid_diffusion(orc_case *c) {
+    int f, i, j, ncx = c->ncx, ncy = c->ncy, ngy = c->ngy;
+    real P = (real)c->deck.total_pres;
+    c->fD = xcalloc((size_t)c->nfsp * ncx * ncy, sizeof(real));
+    for (f = 0; f < c->nfsp; f++)
+        for (i = 0; i < ncx; i++)
+            for (j = 0; j < ncy; j++) {
+                int G = i * ngy + j;
+                real T = (real)(0.25 * (c->Temp[G] + c->Temp[G + 1] + c->Temp[G + ngy] + c->Temp[G + ngy + 1]));
+                double D;
+                int kind = c->deck.gas == 0 ? 0 : c->deck.gas == 1 ? (f <= 1 ? 1 : 2) : (f == 0 ? 3 : f <= 2 ? 1 : 2);
+                if (kind == 0) {
+                    real omega = (real)meta_omega((float)(T / (real)124.0f));
+                    D = 1.997279e-4 * sqrt(T / 39.948) * T / (P * (real)11.682724f * omega) * 20;
+                } else if (kind == 1) {
+                    D = 2.63e-3 * sqrt((double)(T * T * T * 24 / 32 / 32)) / P * (real)0.08650f;
+                } else if (kind == 2) {
+                    D = 2.63e-3 * sqrt((double)(T * T * T * 24 / 16 / 16)) / P * (real)0.10892f;
+                } else {
+                    D = 2.63e-3 * sqrt((double)(T * T * T * 24) / 39.948 / 39.948) / P * (real)0.08559f;
+                }
+                c->fD[(size_t)f * ncx * ncy + i * ncy + j] = (real)D;
+            }
+}
+
+/* calculate_gummel_coef_x/_y (src/cuda_Fluid.cu:179-198): interior x faces
+ * average D[i-1] with itself, as the reference does */
+static void fluid_gummel(orc_case *c) {
+    int f, k, i, j, ncx = c->ncx, ncy = c->ncy, ngx = c->ngx, ngy = c->ngy;
+    size_t NX = (size_t)ngx * ncy, NY = (size_t)ncx * ngy, NC = (size_t)ncx * ncy;
+    c->fax = xcalloc(c->nfsp * NX, sizeof(real)); c->fbx = xcalloc(c->nfsp * NX, sizeof(real));
+    c->fay = xcalloc(c->nfsp * NY, sizeof(real)); c->fby = xcalloc(c->nfsp * NY, sizeof(real));
+    for (f = 0; f < c->nfsp; f++) {
+        real *ax = c->fax + f * NX, *bx = c->fbx + f * NX, *ay = c->fay + f * NY, *by = c->fby + f * NY;
+        const real *D = c->fD + f * NC;
+        double m = (double)c->fl_mass[f];
+        for (k = 0; k < c->nlx; k++) {
+            int x1 = c->lx_x1[k], x2 = c->lx_x2[k], yy = c->lx_y[k];
+            bx[x1 * ncy + yy] = (real)(-c->lx_f1[f * c->nlx + k] * (-0.25 * sqrt(c->Temp[x1 * ngy + yy] * 1.38e-23 / m)));
+            ax[x1 * ncy + yy] = 0;
+            for (i = x1 + 1; i <= x2; i++) {
+                ax[i * ncy + yy] = (D[(i - 1) * ncy + yy] + D[(i - 1) * ncy + yy]) / 2 / c->dx;
+                bx[i * ncy + yy] = ax[i * ncy + yy];
+            }
+            ax[(x2 + 1) * ncy + yy] = (real)(c->lx_f2[f * c->nlx + k] * (0.25 * sqrt(c->Temp[x2 * ngy + yy] * 1.38e-23 / m)));
+            bx[(x2 + 1) * ncy + yy] = 0;
+        }
+        for (k = 0; k < c->nly; k++) {
+            int xx = c->ly_x[k], y1 = c->ly_y1[k], y2 = c->ly_y2[k];
+            by[xx * ngy + y1] = (real)(-c->ly_f1[f * c->nly + k] * (-0.25 * sqrt(c->Temp[xx * ngy + y1] * 1.38e-23 / m)));
+            ay[xx * ngy + y1] = 0;
+            for (j = y1 + 1; j <= y2; j++) {
+                ay[xx * ngy + j] = (D[xx * ncy + j - 1] + D[xx * ncy + j]) / 2 / c->dy;
+                by[xx * ngy + j] = ay[xx * ngy + j];
+            }
+            ay[xx * ngy + y2 + 1] = (real)(c->ly_f2[f * c->nly + k] * (0.25 * sqrt(c->Temp[xx * ngy + y2] * 1.38e-23 / m)));
+            by[xx * ngy + y2 + 1] = 0;
+        }
+    }
+}
+
+void orc_fluid_flux_x(orc_case *c, int f);
+void orc_fluid_flux_y(orc_case *c, int f);
+
+static void fluid_setup(orc_case *c) {
+    int f, i, j, ncx = c->ncx, ncy = c->ncy;
+    size_t NC = (size_t)ncx * ncy;
+    c->dtc = c->dt * (real)c->deck.dt_conti;
+    fluid_lines(c);
+    fluid_ends(c);
+    fluid_diffusion(c);
+    fluid_gummel(c);
+    c->fden = xcalloc(c->nfsp * NC, sizeof(real));
+    c->fsrc = xcalloc(c->nfsp * NC, sizeof(real));
+    c->fflx = xcalloc((size_t)c->nfsp * c->ngx * ncy, sizeof(real));
+    c->ffly = xcalloc((size_t)c->nfsp * ncx * c->ngy, sizeof(real));
+    for (f = 0; f < c->nfsp; f++)
+        for (i = 0; i < ncx; i++)
+            for (j = 0; j < ncy; j++)
+                if (c->PlasmaRegion[i * ncy + j] != 0 && c->deck.fl_loadtype == 0) c->fden[f * NC + i * ncy + j] = c->fl_dens[f];
+    for (f = 0; f < c->nfsp; f++) { orc_fluid_flux_x(c, f); orc_fluid_flux_y(c, f); }
+    c->win_den = xcalloc(c->Gsize, sizeof(real));
+    c->mccr_mark = xcalloc((size_t)c->Gsize * (c->TnRct > 0 ? c->TnRct : 1), sizeof(int64_t));
+}
+
 int orc_setup(orc_case *c) {
     int e = derive_constants(c);
     if (e) return e;
@@ -702,6 +867,7 @@ int orc_setup(orc_case *c) {
     argon_tables(c);
     null_collision(c);
     maxwell_table(c);
+    fluid_setup(c);
     e = load_particles(c);
     if (e) return e;
     return 0;

@@ -88,7 +88,10 @@ void orc_destroy(orc_case *c) {
                    c->temp_b, c->lap_phi, c->BackDen1, c->BackVel1, c->BackDen2, c->BackVel2, c->FluidN,
                    c->Efield_Flag, c->xe, c->xee, c->cx, c->P1, c->P2, c->Pion, c->vsave, c->tag, c->phi, c->TotPot,
                    c->Sigma, c->Source, c->Ex, c->Ey, c->den, c->sigma_sp, c->b, c->xsol, c->V_cond, c->CondCharge,
-                   c->Lost, c->mccr, c->sum_den, c->sum_phi, c->sum_ex, c->sum_ey, c->lu};
+                   c->Lost, c->mccr, c->sum_den, c->sum_phi, c->sum_ex, c->sum_ey, c->lu,
+                   c->lx_x1, c->lx_x2, c->lx_y, c->ly_x, c->ly_y1, c->ly_y2, c->lx_f1, c->lx_f2, c->ly_f1,
+                   c->ly_f2, c->fD, c->fden, c->fsrc, c->fax, c->fbx, c->fflx, c->fay, c->fby, c->ffly,
+                   c->win_den, c->mccr_mark};
     for (s = 0; s < (int)(sizeof own / sizeof own[0]); ++s) free(own[s]);
     for (s = 0; s < ORC_MAXSP; ++s) {
         orc_species *p = &c->sp[s];
@@ -802,6 +805,8 @@ void orc_accumulate(orc_case *c) {
     c->n_accum++;
 }
 
+void orc_fluid_step(orc_case *c);
+
 /* ================= one timestep: src/cuda_main.cu:112-235 ================= */
 int orc_step(orc_case *c) {
     c->tstep++;
@@ -814,6 +819,7 @@ int orc_step(orc_case *c) {
     e = orc_mcc(c);
     if (e) return e;
     orc_deposit(c);
+    if (c->fluid_on) orc_fluid_step(c);
     orc_accumulate(c);
     return 0;
 }
@@ -822,6 +828,176 @@ int orc_run(orc_case *c, int nsteps) {
     int i, e;
     for (i = 0; i < nsteps; ++i) { e = orc_step(c); if (e) return e; }
     return 0;
+}
+
+/* ================= metastable fluid step: src/cuda_Fluid.cu:34-178,
+ *                   src/cuda_Diagnostic.cu:37-110, 534-560, 634-642 ====== */
+/* Calculate_Flux_x / _y (src/cuda_Fluid.cu:51-90) */
+void orc_fluid_flux_x(orc_case *c, int f) {
+    int k, i, ncy = c->ncy;
+    size_t NX = (size_t)c->ngx * ncy, NC = (size_t)c->ncx * ncy;
+    const real *n = c->fden + f * NC, *ax = c->fax + f * NX, *bx = c->fbx + f * NX;
+    real *fx = c->fflx + f * NX;
+    for (k = 0; k < c->nlx; k++) {
+        int x1 = c->lx_x1[k], x2 = c->lx_x2[k], yy = c->lx_y[k];
+        fx[x1 * ncy + yy] = -c->lx_f1[f * c->nlx + k] * bx[x1 * ncy + yy] * n[x1 * ncy + yy];
+        for (i = x1 + 1; i <= x2; i++) {
+            real left = n[(i - 1) * ncy + yy], right = n[i * ncy + yy];
+            fx[i * ncy + yy] = ax[i * ncy + yy] * left - bx[i * ncy + yy] * right;
+        }
+        fx[(x2 + 1) * ncy + yy] = c->lx_f2[f * c->nlx + k] * ax[(x2 + 1) * ncy + yy] * n[x2 * ncy + yy];
+    }
+}
+void orc_fluid_flux_y(orc_case *c, int f) {
+    int k, j, ncy = c->ncy, ngy = c->ngy;
+    size_t NY = (size_t)c->ncx * ngy, NC = (size_t)c->ncx * ncy;
+    const real *n = c->fden + f * NC, *ay = c->fay + f * NY, *by = c->fby + f * NY;
+    real *fy = c->ffly + f * NY;
+    for (k = 0; k < c->nly; k++) {
+        int xx = c->ly_x[k], y1 = c->ly_y1[k], y2 = c->ly_y2[k];
+        fy[xx * ngy + y1] = -c->ly_f1[f * c->nly + k] * by[xx * ngy + y1] * n[xx * ncy + y1];
+        for (j = y1 + 1; j <= y2; j++) {
+            real lower = n[xx * ncy + j - 1], upper = n[xx * ncy + j];
+            fy[xx * ngy + j] = ay[xx * ngy + j] * lower - by[xx * ngy + j] * upper;
+        }
+        fy[xx * ngy + y2 + 1] = c->ly_f2[f * c->nly + k] * ay[xx * ngy + y2 + 1] * n[xx * ncy + y2];
+    }
+}
+
+/* tridiag (src/cuda_Fluid.cu:164-178): Thomas algorithm; 0 when a pivot
+ * vanishes (the reference then skips the back substitution) */
+int orc_tridiag(const real *a, const real *b, const real *cc, const real *d, real *x, real *gam, int n) {
+    int j;
+    real bet;
+    x[0] = d[0] / (bet = b[0]);
+    for (j = 1; j < n; j++) {
+        gam[j - 1] = cc[j - 1] / bet;
+        bet = b[j] - a[j] * cc[j - 1] / bet;
+        x[j] = (d[j] - a[j] * x[j - 1]) / bet;
+        if (bet == 0) return 0;
+    }
+    for (j = n - 2; j >= 0; j--) x[j] -= gam[j] * x[j + 1];
+    return 1;
+}
+
+/* Solve_Density_x / _y (src/cuda_Fluid.cu:91-163).  A line whose system
+ * meets a zero pivot keeps its density (the reference reads uninitialised
+ * values there). */
+static void fluid_solve_x(orc_case *c, int f) {
+    int k, i, ncy = c->ncy, ngy = c->ngy;
+    size_t NX = (size_t)c->ngx * ncy, NY = (size_t)c->ncx * ngy, NC = (size_t)c->ncx * ncy;
+    real *n = c->fden + f * NC;
+    const real *ax = c->fax + f * NX, *bx = c->fbx + f * NX, *fy = c->ffly + f * NY, *src = c->fsrc + f * NC;
+    real del_f = c->dtc / c->dx;
+    int m = c->ngx + 1;
+    real *sub = malloc(m * sizeof(real)), *dia = malloc(m * sizeof(real)), *sup = malloc(m * sizeof(real));
+    real *rhs = malloc(m * sizeof(real)), *x = malloc(m * sizeof(real)), *gam = malloc(m * sizeof(real));
+    for (k = 0; k < c->nlx; k++) {
+        int x1 = c->lx_x1[k], x2 = c->lx_x2[k], yy = c->lx_y[k], num = x2 - x1 + 1;
+        for (i = 0; i < num; i++) {
+            int xx = i + x1;
+            sub[i] = -ax[xx * ncy + yy] * del_f;
+            dia[i] = 1 + (ax[(xx + 1) * ncy + yy] + bx[xx * ncy + yy]) * del_f;
+            sup[i] = -bx[(xx + 1) * ncy + yy] * del_f;
+            rhs[i] = n[xx * ncy + yy] + src[xx * ncy + yy] * c->dtc - (fy[xx * ngy + yy + 1] - fy[xx * ngy + yy]) * c->dtc / c->dy;
+        }
+        if (!orc_tridiag(sub, dia, sup, rhs, x, gam, num)) continue;
+        for (i = 0; i < num; i++)
+            n[(i + x1) * ncy + yy] = (c->PlasmaRegion[(i + x1) * ncy + yy] == 1 && x[i] > 1e9) ? x[i] : 0;
+    }
+    free(sub); free(dia); free(sup); free(rhs); free(x); free(gam);
+}
+static void fluid_solve_y(orc_case *c, int f) {
+    int k, j, ncy = c->ncy, ngy = c->ngy;
+    size_t NX = (size_t)c->ngx * ncy, NY = (size_t)c->ncx * ngy, NC = (size_t)c->ncx * ncy;
+    real *n = c->fden + f * NC;
+    const real *ay = c->fay + f * NY, *by = c->fby + f * NY, *fx = c->fflx + f * NX, *src = c->fsrc + f * NC;
+    real del_f = c->dtc / c->dy;
+    int m = c->ngy + 1;
+    real *sub = malloc(m * sizeof(real)), *dia = malloc(m * sizeof(real)), *sup = malloc(m * sizeof(real));
+    real *rhs = malloc(m * sizeof(real)), *x = malloc(m * sizeof(real)), *gam = malloc(m * sizeof(real));
+    for (k = 0; k < c->nly; k++) {
+        int xx = c->ly_x[k], y1 = c->ly_y1[k], y2 = c->ly_y2[k], num = y2 - y1 + 1;
+        for (j = 0; j < num; j++) {
+            int yy = j + y1;
+            sub[j] = -ay[xx * ngy + yy] * del_f;
+            dia[j] = 1 + (ay[xx * ngy + yy + 1] + by[xx * ngy + yy]) * del_f;
+            sup[j] = -by[xx * ngy + yy + 1] * del_f;
+            rhs[j] = n[xx * ncy + yy] + src[xx * ncy + yy] * c->dtc - (fx[(xx + 1) * ncy + yy] - fx[xx * ncy + yy]) * c->dtc / c->dx;
+        }
+        if (!orc_tridiag(sub, dia, sup, rhs, x, gam, num)) continue;
+        for (j = 0; j < num; j++)
+            n[xx * ncy + j + y1] = (c->PlasmaRegion[xx * ncy + j + y1] == 1 && x[j] > 1e9) ? x[j] : 0;
+    }
+    free(sub); free(dia); free(sup); free(rhs); free(x); free(gam);
+}
+
+/* Solve_Continuity_eqn (src/cuda_Fluid.cu:34-42) */
+void orc_fluid_continuity(orc_case *c) {
+    int f;
+    for (f = 0; f < c->nfsp; f++) {
+        fluid_solve_x(c, f);
+        orc_fluid_flux_x(c, f);
+        fluid_solve_y(c, f);
+        orc_fluid_flux_y(c, f);
+    }
+}
+
+/* The fluid part of the diagnostic (argon; src/cuda_Diagnostic.cu:44-110,
+ * 534-560, 634-642): every step adds the electron node density to the
+ * window; the (N_ave+1)-th call averages it and the collision rates since
+ * the last average, computes the Ar* source per node
+ *   S = R2 - R4 - k7 n_e n - 2 R8 - k9 n_Ar n
+ * (R: averaged event rates, k: rate constants, n: the node metastable
+ * density of the last sync), hands it to the cells and switches the
+ * continuity solve on; once on, the cell densities go to the nodes and the
+ * electron null-collision probability P2 follows them. */
+void orc_fluid_diag(orc_case *c) {
+    int g, r, G = c->Gsize, T = c->TnRct, ngy = c->ngy, ncy = c->ncy;
+    int avg = c->nave_count == c->deck.n_ave;
+    for (g = 0; g < G; g++) {
+        c->win_den[g] += c->den[g];
+        if (!avg) continue;
+        real ave_e = c->Denscale[0] / (real)c->deck.n_ave * c->win_den[g];
+        real rate = c->Denscale[0] / (real)c->deck.n_ave / c->dt;
+        real R[9];
+        for (r = 0; r < 9; r++) {
+            R[r] = (r == 7) ? 0 : rate * (real)(c->mccr[(size_t)g * T + r] - c->mccr_mark[(size_t)g * T + r]);
+        }
+        for (r = 0; r < T; r++) c->mccr_mark[(size_t)g * T + r] = c->mccr[(size_t)g * T + r];
+        c->win_den[g] = 0;
+        real Eden = (real)(0.25 * ave_e);
+        real nm = c->FluidN[g];
+        real k7 = c->RR[7] * Eden * nm;
+        real k9 = c->RR[9] * c->BackDen1[g] * nm;
+        real S = R[2] - R[4] - k7 - 2 * R[8] - k9;
+        int x = g / ngy, y = g % ngy;
+        if (x < c->ncx && y < ncy) c->fsrc[x * ncy + y] = S;
+    }
+    if (avg) { c->nave_count = 0; c->conti_flag = 1; }
+    else c->nave_count++;
+    if (!c->conti_flag) return;
+    /* Sync_Fluid_GFCtoGFG_forDen (src/cuda_Fluid.cu:302-320) and the P2 of
+     * the electron null test (src/cuda_mccAr.cu:57-58) */
+    {
+        int i, j;
+        for (i = 0; i < c->ncx; i++)
+            for (j = 0; j < ncy; j++) {
+                real v = c->fden[i * ncy + j];
+                c->FluidN[i * ngy + j] = v;
+                if (i == c->ncx - 1 || j == ncy - 1) c->FluidN[(i + 1) * ngy + j + 1] = v;
+            }
+        double dtm = (double)c->dt_mcc;
+        for (g = 0; g < G; g++)
+            c->P2[g] = c->P1[g] + (real)(1.0 - dm_exp(-dtm * (double)c->sigv[1] * (double)c->FluidN[g]));
+    }
+}
+
+/* one step of the fluid coupling after the deposit: the continuity solve
+ * (once switched on), then its part of the diagnostic */
+void orc_fluid_step(orc_case *c) {
+    if (c->conti_flag) orc_fluid_continuity(c);
+    orc_fluid_diag(c);
 }
 
 /* ================= OpenMP step (CPU baseline only) =========================
@@ -1006,6 +1182,17 @@ static int lookup(orc_case *c, const char *name, orc_arr *out) {
     ENT("CondCharge", c->CondCharge, (long)c->nsp * c->CondNUMR, rs) ENT("Lost", c->Lost, c->nsp, rs) ENT("mccr", c->mccr, G * c->TnRct, 8)
     ENT("sum_den", c->sum_den, (long)c->nsp * G, 8) ENT("sum_phi", c->sum_phi, G, 8)
     ENT("sum_ex", c->sum_ex, G, 8) ENT("sum_ey", c->sum_ey, G, 8) ENT("lu", c->lu, (long)c->A_size * (2 * c->lu_bw + 1), 8)
+    {
+        long NF = c->nfsp, NC = (long)c->ncx * c->ncy, NX = (long)c->ngx * c->ncy, NY = (long)c->ncx * c->ngy;
+        ENT("fl_x1", c->lx_x1, c->nlx, 4) ENT("fl_x2", c->lx_x2, c->nlx, 4) ENT("fl_xy", c->lx_y, c->nlx, 4)
+        ENT("fl_yx", c->ly_x, c->nly, 4) ENT("fl_y1", c->ly_y1, c->nly, 4) ENT("fl_y2", c->ly_y2, c->nly, 4)
+        ENT("fl_xg1", c->lx_f1, NF * c->nlx, rs) ENT("fl_xg2", c->lx_f2, NF * c->nlx, rs)
+        ENT("fl_yg1", c->ly_f1, NF * c->nly, rs) ENT("fl_yg2", c->ly_f2, NF * c->nly, rs)
+        ENT("fl_D", c->fD, NF * NC, rs) ENT("fl_den", c->fden, NF * NC, rs) ENT("fl_src", c->fsrc, NF * NC, rs)
+        ENT("fl_ax", c->fax, NF * NX, rs) ENT("fl_bx", c->fbx, NF * NX, rs) ENT("fl_flux_x", c->fflx, NF * NX, rs)
+        ENT("fl_ay", c->fay, NF * NY, rs) ENT("fl_by", c->fby, NF * NY, rs) ENT("fl_flux_y", c->ffly, NF * NY, rs)
+        ENT("fl_win", c->win_den, G, rs)
+    }
     if (!strncmp(name, "sp", 2) && name[2] >= '0' && name[2] <= '9' && name[3] == '.') {
         int s = name[2] - '0';
         if (s >= c->nsp) return -1;
@@ -1048,6 +1235,12 @@ double orc_scalar(orc_case *c, const char *name) {
     if (!strcmp(name, "nvel")) return c->nvel;
     if (!strcmp(name, "TnRct")) return c->TnRct;
     if (!strcmp(name, "n_accum")) return c->n_accum;
+    if (!strcmp(name, "dtc")) return (double)c->dtc;
+    if (!strcmp(name, "fl_nx")) return c->nlx;
+    if (!strcmp(name, "fl_ny")) return c->nly;
+    if (!strcmp(name, "nfsp")) return c->nfsp;
+    if (!strcmp(name, "conti_flag")) return c->conti_flag;
+    if (!strcmp(name, "nave_count")) return c->nave_count;
     if (!strncmp(name, "np", 2)) return c->sp[name[2] - '0'].n;
     if (!strncmp(name, "cap", 3)) return c->sp[name[3] - '0'].cap;
     if (!strncmp(name, "next_id", 7)) return c->sp[name[7] - '0'].next_id;
@@ -1060,6 +1253,7 @@ double orc_scalar(orc_case *c, const char *name) {
 }
 
 void orc_set_np(orc_case *c, int isp, int n) { c->sp[isp].n = n; }
+void orc_set_fluid(orc_case *c, int on) { c->fluid_on = on; }
 void orc_set_tstep(orc_case *c, long tstep) { c->tstep = tstep; c->t = (double)tstep * (double)c->dt; }
 int orc_sizeof_deck(void) { return (int)sizeof(orc_deck); }
 int orc_sizeof_real(void) { return (int)sizeof(real); }

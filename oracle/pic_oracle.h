@@ -66,6 +66,7 @@ typedef struct {
     double sp_dens[ORC_MAXSP], sp_temp[ORC_MAXSP], sp_np2c[ORC_MAXSP];
     int sp_ratio[ORC_MAXSP], sp_maxnp[ORC_MAXSP];
     int dt_pic;
+    int dt_conti;                 /* continuity time step in PIC steps (TimeStep_Conti) */
     double pcg_tol;
     int n_ave, n_smt, np_limit;
     double coll_flag[ORC_MAXRCT];
@@ -150,6 +151,19 @@ typedef struct {
     int err;
     /* banded LU of the field operator (FP64) */
     double *lu; int lu_bw;
+    /* metastable fluid (src/Fluid.c, src/cuda_Fluid.cu): x lines (fixed y)
+     * and y lines (fixed x) of plasma cells with their end factors per fluid
+     * species; per species the cell diffusion coefficient and density
+     * [ncx][ncy], the x-face coefficients and fluxes [ngx][ncy], the y-face
+     * ones [ncx][ngy], the source [ncx][ncy]; the averaging window */
+    real dtc;
+    int nlx, nly;
+    int *lx_x1, *lx_x2, *lx_y, *ly_x, *ly_y1, *ly_y2;
+    real *lx_f1, *lx_f2, *ly_f1, *ly_f2;            /* nfsp x lines */
+    real *fD, *fden, *fsrc, *fax, *fbx, *fflx, *fay, *fby, *ffly;
+    int fluid_on, nave_count, conti_flag;
+    real *win_den;                                  /* Gsize: electron density summed over the window */
+    int64_t *mccr_mark;                             /* Gsize x TnRct: counts at the last average */
 } orc_case;
 
 /* Collision step of the oxygen chemistries, standalone: O2 (gas type 1,

@@ -38,7 +38,7 @@ def test_python_mirror_binds():
               "step", "run_staged", "set_step", "info", "particles", "set_particles", "array", "set_array", "sync",
               "close", "eedf_sample", "stage_timing", "stage_times", "last_field_solve", "field_solver_kind",
               "field_solver_shape", "diag_sample", "diag_history",
-              "dump_save", "dump_load"):
+              "dump_save", "dump_load", "fluid"):
         assert callable(getattr(picmcc.Case, m, None)), f"Case.{m} is missing"
     for m in ("set_rhs", "solve", "sync", "result", "close"):
         assert callable(getattr(picmcc.PoissonBatch, m, None)), f"PoissonBatch.{m} is missing"

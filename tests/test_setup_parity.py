@@ -10,7 +10,9 @@ from picmcc import FP32_REFINED, FP64_PLAIN, HostSetup, PicError, decks
 
 SETUP_ARRAYS = ["Boundary", "CondID", "Face", "DensRegion", "Area", "Temp", "PlasmaRegion", "eps_r", "eps",
                 "StructIdx", "A_idx", "Ai", "Aj", "Aval", "Atemp", "Mdiag", "cond_b", "temp_b", "lap_phi",
-                "BackDen1", "BackVel1", "FluidN", "xe", "xee", "cx", "sigv", "P1", "P2", "Pion", "vsave"]
+                "BackDen1", "BackVel1", "FluidN", "xe", "xee", "cx", "sigv", "P1", "P2", "Pion", "vsave",
+                "fl_x1", "fl_x2", "fl_xy", "fl_yx", "fl_y1", "fl_y2", "fl_xg1", "fl_xg2", "fl_yg1", "fl_yg2",
+                "fl_D", "fl_den", "fl_ax", "fl_bx", "fl_ay", "fl_by", "fl_flux_x", "fl_flux_y"]
 
 
 def deck_variants():
@@ -75,7 +77,7 @@ def test_setup_matches_oracle(name, prec, pn):
             x = h.array(f"sp{s}.{k}")
             y = o.arr(f"sp{s}.{k}")[:len(x)]
             assert np.array_equal(x, y), f"{name}/{pn}: sp{s}.{k}"
-    for sc in ("dt", "dx", "dy", "MCCn", "dt_mcc", "nvel", "A_size", "CondNUMR"):
+    for sc in ("dt", "dx", "dy", "MCCn", "dt_mcc", "nvel", "A_size", "CondNUMR", "dtc", "fl_nx", "fl_ny", "nfsp"):
         assert h.scalar(sc) == o.scalar(sc), sc
 
 
