@@ -865,18 +865,33 @@ struct Case : CaseBase {
             w.i32((int)(recs[s].size() / 6));
             w.put(recs[s].data(), recs[s].size());
         }
-        // fluid species: the metastable densities are the deck's (no fluid
-        // equations run here), their sources and fluxes zero
-        w.i32(0);
-        std::vector<float> row((size_t)std::max(ngy, ngx));
+        // fluid species: with the fluid on, its live density, source, face
+        // fluxes and continuity flag; otherwise the set-up's densities and
+        // fluxes, sources zero
+        std::vector<R> fden = S.fl_den, fsrc(S.fl_den.size(), (R)0), ffx = S.fl_flux_x, ffy = S.fl_flux_y;
+        FluidState fst{0, 0};
+        if (fl.state) {
+            if (int e = fetch(fden, fl.den, S.fl_den.size())) return e;
+            if (int e = fetch(fsrc, (const R *)fl.src, S.fl_den.size())) return e;
+            if (int e = fetch(ffx, (const R *)fl.flux_x, S.fl_flux_x.size())) return e;
+            if (int e = fetch(ffy, (const R *)fl.flux_y, S.fl_flux_y.size())) return e;
+            HIPCHK(hipMemcpy(&fst, fl.state, sizeof fst, hipMemcpyDeviceToHost));
+        }
+        w.i32(fst.conti);
+        std::vector<float> row((size_t)std::max(ngy, ngx) * std::max(ngx, ngy));
         for (int fs = 0; fs < S.nfsp; ++fs) {
+            const size_t o = (size_t)fs * ncx * ncy;
             for (int i = 0; i < ncx; ++i) {
-                for (int j = 0; j < ncy; ++j) row[j] = (float)S.FluidN[(size_t)fs * G + i * ngy + j];
+                for (int j = 0; j < ncy; ++j) row[j] = (float)fden[o + (size_t)i * ncy + j];
                 w.put(row.data(), ncy);
-                w.zeros(ncy);
+                for (int j = 0; j < ncy; ++j) row[j] = (float)fsrc[o + (size_t)i * ncy + j];
+                w.put(row.data(), ncy);
             }
-            w.zeros((size_t)ngx * ncy);
-            w.zeros((size_t)ncx * ngy);
+            const size_t ox = (size_t)fs * ngx * ncy, oy = (size_t)fs * ncx * ngy;
+            for (size_t k = 0; k < (size_t)ngx * ncy; ++k) row[k] = (float)ffx[ox + k];
+            w.put(row.data(), (size_t)ngx * ncy);
+            for (size_t k = 0; k < (size_t)ncx * ngy; ++k) row[k] = (float)ffy[oy + k];
+            w.put(row.data(), (size_t)ncx * ngy);
         }
         // history: the retained diagnostics records (pic_diag_sample), oldest first
         const int o_np = 2, o_q = 2 + 2 * nsp, o_ab = o_q + nc, o_v = o_ab + nsp * nc;
@@ -1016,8 +1031,18 @@ struct Case : CaseBase {
             r.get(recs[s].data(), recs[s].size());
             if (!r.ok) return early("particles");
         }
-        (void)r.i32();   // fluid continuity flag
-        r.skip((size_t)S.nfsp * ((size_t)2 * ncx * ncy + (size_t)ngx * ncy + (size_t)ncx * ngy));
+        // fluid: restored when the case runs the fluid, else read past
+        const int conti = r.i32();
+        std::vector<float> fden((size_t)S.nfsp * ncx * ncy), fsrc(fden.size());
+        std::vector<float> ffx((size_t)S.nfsp * ngx * ncy), ffy((size_t)S.nfsp * ncx * ngy);
+        for (int fs = 0; fs < S.nfsp; ++fs) {
+            for (int i = 0; i < ncx; ++i) {
+                r.get(fden.data() + ((size_t)fs * ncx + i) * ncy, ncy);
+                r.get(fsrc.data() + ((size_t)fs * ncx + i) * ncy, ncy);
+            }
+            r.get(ffx.data() + (size_t)fs * ngx * ncy, (size_t)ngx * ncy);
+            r.get(ffy.data() + (size_t)fs * ncx * ngy, (size_t)ncx * ngy);
+        }
         const int hc = r.i32();
         (void)r.i32();
         if (!r.ok) return early("fluid");
@@ -1095,8 +1120,22 @@ struct Case : CaseBase {
         HIPCHK(hipMemcpy(Ey, hy.data(), G * sizeof(R), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(tot, ht.data(), G * sizeof(R), hipMemcpyHostToDevice));
         if (S.A_size) HIPCHK(hipMemcpy(x64, hx64.data(), hx64.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (fl.state) {
+            auto up = [&](R *dst, const std::vector<float> &v) {
+                std::vector<R> t(v.begin(), v.end());
+                return hipMemcpy(dst, t.data(), t.size() * sizeof(R), hipMemcpyHostToDevice);
+            };
+            HIPCHK(up(fl.den, fden));
+            HIPCHK(up(fl.src, fsrc));
+            HIPCHK(up(fl.flux_x, ffx));
+            HIPCHK(up(fl.flux_y, ffy));
+            const FluidState fst{0, conti ? 1 : 0};
+            HIPCHK(hipMemcpy(fl.state, &fst, sizeof fst, hipMemcpyHostToDevice));
+            HIPCHK(hipMemset(fl.win, 0, (size_t)G * sizeof(R)));
+        }
         // counters start again at the dump's step
         HIPCHK(hipMemsetAsync(mccr, 0, (size_t)G * S.TnRct * sizeof(unsigned long long), st));
+        if (fl.state) HIPCHK(hipMemsetAsync(fl.mark, 0, (size_t)G * S.TnRct * sizeof(unsigned long long), st));
         HIPCHK(hipMemsetAsync(condcharge, 0, (size_t)nsp * nc * sizeof(int), st));
         HIPCHK(hipMemsetAsync(lost, 0, (size_t)nsp * sizeof(int), st));
         HIPCHK(hipMemsetAsync(sigma_hits, 0, (size_t)nsp * G * sizeof(unsigned long long), st));

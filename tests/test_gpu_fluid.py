@@ -82,3 +82,33 @@ def test_oxygen_cases_are_refused():
     with pytest.raises(PicError, match="built for argon"):
         c.fluid(True)
     c.close()
+
+
+def test_dump_carries_the_fluid(tmp_path):
+    import dump_v000 as V
+    d = deck()
+    a = Case(d, FP32_REFINED)
+    a.fluid(True)
+    a.step(9)
+    a.sync()
+    path = str(tmp_path / "f.dmp")
+    a.dump_save(path)
+    inf = a.info()
+    D = V.Dims(inf.nsp, 1, inf.ngx - 1, inf.ngy - 1, inf.CondNUMR, inf.TnRct)
+    f = V.read(path, D)
+    assert f["conti_flag"] == 1 and f["trailing_bytes"] == 0
+    ncx, ncy = inf.ngx - 1, inf.ngy - 1
+    assert np.array_equal(f["fluid"][0]["den"], a.array("fl_den").reshape(ncx, ncy))
+    assert np.array_equal(f["fluid"][0]["source"], a.array("fl_src").reshape(ncx, ncy))
+    assert np.array_equal(f["fluid"][0]["flux_x"], a.array("fl_flux_x").reshape(ncx + 1, ncy))
+    assert np.array_equal(f["fluid"][0]["flux_y"], a.array("fl_flux_y").reshape(ncx, ncy + 1))
+    b = Case(d, FP32_REFINED)
+    b.fluid(True)
+    b.dump_load(path)
+    for k in ("fl_den", "fl_src", "fl_flux_x", "fl_flux_y"):
+        assert np.array_equal(b.array(k), a.array(k)), k
+    assert int(b.array("fl_state")[1]) == 1
+    b.step(3)
+    b.sync()
+    a.close()
+    b.close()
