@@ -180,11 +180,13 @@ class EnsembleServer:
         self._members = []      # its _Active entries
         self.rebuilds = 0
 
-    def run(self, requests, keep_state=None):
+    def run(self, requests, keep_state=None, on_retire=None):
         """Serve `requests` in real time; returns one record per case.
-        keep_state(rid) -> bool: return that case's final particles and density."""
+        keep_state(rid) -> bool: return that case's final particles and density.
+        on_retire(record): called as each case finishes."""
         if not CONCURRENT:
             raise RuntimeError(DISABLED_REASON)
+        self._on_retire = on_retire
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         active, creating, results = [], [], []
         used = 0
@@ -242,6 +244,8 @@ class EnsembleServer:
         a.case.close()
         active.remove(a)
         results.append(rec)
+        if getattr(self, "_on_retire", None):
+            self._on_retire(rec)
         return a.footprint
 
     def _feed_batch(self, active, results, keep_state, t0):
