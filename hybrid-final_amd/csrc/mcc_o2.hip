@@ -209,34 +209,64 @@ constexpr int O2_QCAP = 128;    // queued colliders per wave (< 64 wait while a 
 // tests 64 particles at a time; the colliders go to its LDS queue, and
 // whenever 64 are queued they collide together, one per lane, so the long
 // collision code runs on full wavefronts.
+// A wave takes O2_CPW consecutive cells of one species: their particles are
+// one contiguous range (cells of a column are consecutive nodes; the node
+// between two columns starts no cell), walked 64 at a time, so species with
+// few particles per cell still fill the wave.  Queue entries carry the
+// cell's index within the wave; event, product and detachment counts are
+// kept per (cell, row) in LDS.
+constexpr int O2_CPW = 4;
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, int isp, uint32_t step,
                                                     unsigned long long *mccr, int *pcount, int *delreq) {
     __shared__ int q_kk[O2_WAVES][O2_QCAP];
     __shared__ unsigned char q_j[O2_WAVES][O2_QCAP];
-    __shared__ int c_r[O2_WAVES][CHEM_MAXROWS + 1];   // events per reaction row; last = detachments
-    __shared__ int c_p[O2_WAVES][CHEM_MAXSP];         // products per species
-    int lane;
-    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
-    if (c < 0) return;
+    __shared__ unsigned char q_c[O2_WAVES][O2_QCAP];
+    __shared__ int c_r[O2_WAVES][O2_CPW][CHEM_MAXROWS + 1];   // events per (cell, row); last = detachments
+    __shared__ int c_p[O2_WAVES][O2_CPW][CHEM_MAXSP];         // products per (cell, species)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (isp < 0) isp = blockIdx.y;
-    const int wv = threadIdx.x >> 6;
+    const int w0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)) * O2_CPW;
+    if (w0 >= k.Csize) return;
     const PView<R> p = set.p[isp];
     uint8_t *fate = set.fate[isp];
     R *stash = set.stash[isp];
-    int k0, k1;
-    cell_range(p, c, k0, k1);
+    int cell[O2_CPW], lo[O2_CPW + 1];
+#pragma unroll
+    for (int q = 0; q < O2_CPW; ++q) {
+        const int wq = w0 + q;
+        cell[q] = wq < k.Csize ? (wq / k.ncy) * k.ngy + (wq % k.ncy) : -1;
+    }
+    {
+        int a0 = 0, a1 = 0;
+        cell_range(p, cell[0], a0, a1);
+        lo[0] = a0;
+        lo[1] = a1;
+#pragma unroll
+        for (int q = 1; q < O2_CPW; ++q) {
+            int b0 = lo[q], b1 = lo[q];
+            if (cell[q] >= 0) cell_range(p, cell[q], b0, b1);
+            lo[q + 1] = max(b1, lo[q]);
+        }
+    }
+    const int hi = lo[O2_CPW];
     if (isp != 0 && !k.ions) {
-        for (int kk = k0 + lane; kk < k1; kk += 64) fate[kk] = 0;
+        for (int kk = lo[0] + lane; kk < hi; kk += 64) fate[kk] = 0;
         return;
     }
-    for (int i = lane; i <= CHEM_MAXROWS; i += 64) c_r[wv][i] = 0;
-    if (lane < CHEM_MAXSP) c_p[wv][lane] = 0;
+    auto cell_of = [&](int q) {
+        int c = cell[0];
+#pragma unroll
+        for (int r = 1; r < O2_CPW; ++r) c = q == r ? cell[r] : c;
+        return c;
+    };
+    for (int i = lane; i < O2_CPW * (CHEM_MAXROWS + 1); i += 64) (&c_r[wv][0][0])[i] = 0;
+    for (int i = lane; i < O2_CPW * CHEM_MAXSP; i += 64) (&c_p[wv][0][0])[i] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     const size_t cap = (size_t)p.cap;
     const int g0 = kGroup0[k.chem][isp], ng = kNGroup[k.chem][isp];
-    const int tg0 = g0;   // target entries and handler groups share their order
-    const R *P = k.prob + (size_t)c * k.ntgt + tg0;
-    const R Tprob = P[ng - 1];
     const int draws = ((k.sub_mask >> isp) & 1) ? k.MCCn : 1;
     const uint32_t tag = mcc_tag(isp);
     const unsigned long long lt = lanes_below(lane);
@@ -247,6 +277,9 @@ __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, 
         if (lane >= n) return;
         const int kk = q_kk[wv][q0 + lane];
         const int jc = q_j[wv][q0 + lane];
+        const int kq = q_c[wv][q0 + lane];
+        const int c = cell_of(kq);
+        const R *P = k.prob + (size_t)c * k.ntgt + g0;   // target entries and handler groups share their order
         Stream st;
         st.init(k.seed, p.id[kk], step, tag);
         st.seek(jc);
@@ -259,7 +292,7 @@ __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, 
         const int r = o2_handle<R>(k, isp, g0 + flag, c, st, vx, vy, vz, dead, ps, pv, np, detach);
         uint8_t f = 0;
         if (r >= 0) {
-            atomicAdd(&c_r[wv][r], 1);
+            atomicAdd(&c_r[wv][kq][r], 1);
             if (dead) f |= O2F_DEAD;
             else { p.vx[kk] = vx; p.vy[kk] = vy; p.vz[kk] = vz; }
             for (int q = 0; q < np; ++q) {
@@ -267,17 +300,21 @@ __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, 
                 stash[(3 * q) * cap + kk] = pv[q][0];
                 stash[(3 * q + 1) * cap + kk] = pv[q][1];
                 stash[(3 * q + 2) * cap + kk] = pv[q][2];
-                atomicAdd(&c_p[wv][ps[q]], 1);
+                atomicAdd(&c_p[wv][kq][ps[q]], 1);
             }
-            if (detach) atomicAdd(&c_r[wv][CHEM_MAXROWS], 1);
+            if (detach) atomicAdd(&c_r[wv][kq][CHEM_MAXROWS], 1);
         }
         fate[kk] = f;
     };
 
-    for (int b = k0; b < k1; b += 64) {
+    for (int b = lo[0]; b < hi; b += 64) {
         const int kk = b + lane;
+        int kq = 0;
+#pragma unroll
+        for (int q = 1; q < O2_CPW; ++q) kq += kk >= lo[q] ? 1 : 0;
         int jc = -1;
-        if (kk < k1) {
+        if (kk < hi) {
+            const R Tprob = k.prob[(size_t)cell_of(kq) * k.ntgt + g0 + ng - 1];
             Stream st;
             st.init(k.seed, p.id[kk], step, tag);
             for (int j = 0; j < draws; ++j)
@@ -289,6 +326,7 @@ __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, 
             const int pos = qn + __popcll(m & lt);
             q_kk[wv][pos] = kk;
             q_j[wv][pos] = (unsigned char)jc;
+            q_c[wv][pos] = (unsigned char)kq;
         }
         qn += __popcll(m);
         __builtin_amdgcn_wave_barrier();
@@ -301,15 +339,20 @@ __global__ __launch_bounds__(256) void k_o2_collide(O2Const<R> k, O2Set<R> set, 
     if (qn > 0) collide(0, qn);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (int i = lane; i < CHEM_MAXROWS; i += 64) {
-        const int n = c_r[wv][i];
-        if (n) atomicAdd(&mccr[(size_t)c * k.TnRct + i], (unsigned long long)n);
+    for (int t = lane; t < O2_CPW * CHEM_MAXROWS; t += 64) {
+        const int q = t / CHEM_MAXROWS, i = t % CHEM_MAXROWS;
+        const int n = c_r[wv][q][i];
+        if (n) atomicAdd(&mccr[(size_t)cell_of(q) * k.TnRct + i], (unsigned long long)n);
     }
-    if (lane < k.nsp) {
-        const int n = c_p[wv][lane];
-        if (n) atomicAdd(&pcount[(size_t)lane * k.G + c], n);
+    for (int t = lane; t < O2_CPW * k.nsp; t += 64) {
+        const int q = t / k.nsp, sp = t % k.nsp;
+        const int n = c_p[wv][q][sp];
+        if (n) atomicAdd(&pcount[(size_t)sp * k.G + cell_of(q)], n);
     }
-    if (lane == 0 && c_r[wv][CHEM_MAXROWS]) atomicAdd(&delreq[c], c_r[wv][CHEM_MAXROWS]);
+    if (lane < O2_CPW) {
+        const int n = c_r[wv][lane][CHEM_MAXROWS];
+        if (n) atomicAdd(&delreq[cell_of(lane)], n);
+    }
 }
 
 // The reference removes a random O- of the cell for each detachment
@@ -526,8 +569,9 @@ void launch_o2_probs(const O2Const<R> &k, hipStream_t s) {
 template <typename R>
 void launch_o2_collide(const O2Const<R> &k, const O2Set<R> &set, int isp, uint32_t step, unsigned long long *mccr,
                        int *pcount, int *delreq, hipStream_t s) {
-    hipLaunchKernelGGL(k_o2_collide<R>, dim3(wave_blocks(k.Csize), isp < 0 ? k.nsp : 1), dim3(64 * O2_WAVES), 0, s,
-                       k, set, isp, step, mccr, pcount, delreq);
+    const int nb = (k.Csize + O2_CPW * O2_WAVES - 1) / (O2_CPW * O2_WAVES);
+    hipLaunchKernelGGL(k_o2_collide<R>, dim3(nb, isp < 0 ? k.nsp : 1), dim3(64 * O2_WAVES), 0, s, k, set, isp, step,
+                       mccr, pcount, delreq);
 }
 template <typename R>
 void launch_o2_victims(const O2Const<R> &k, const O2Set<R> &set, uint32_t step, const int *delreq,
