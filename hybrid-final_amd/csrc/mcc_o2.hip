@@ -1,12 +1,14 @@
 // Collision step of the oxygen chemistries (O2, gas type 1; Ar/O2, gas
 // type 2) for gfx950.
 //
-// Five launches per step, all keyed on one 64-lane wave per cell:
+// Five launches per step:
 //   probs    per node, the cumulative target probabilities of every species
 //            (20 or 25 values; particle targets use this step's den)
-//   collide  every species in one launch (one grid row each): null-collision
-//            test per particle; the colliders of a wave are queued in LDS
-//            and collided 64 at a time on full wavefronts (target, reaction,
+//   collide  every species in one launch (one grid row each), one 64-lane
+//            wave per four consecutive cells walked as one particle range:
+//            null-collision test per particle; the colliders of a wave are
+//            queued in LDS and collided 64 at a time on full wavefronts
+//            (target, reaction,
 //            outcome), written in place (survivors) or as a fate byte and up
 //            to two products (velocities in a stash indexed like the parent);
 //            event, product and detachment counts gather in LDS and reach
@@ -14,9 +16,10 @@
 //   victims  per cell with electron detachments: removes that many O- of
 //            the cell, smallest Philox keys first
 //   totals   per node: survivors plus products of every species (then scans)
-//   scatter  every species in one launch: survivors in old order, then
-//            products by (source species, parent, slot); new ids follow the
-//            global (cell, source species, parent, slot) order
+//   scatter  every species in one launch (one grid row per target species,
+//            one wave per four cells): survivors in old order, then products
+//            by (source species, parent, slot); new ids follow the global
+//            (cell, source species, parent, slot) order
 // Every count is an integer atomic or a ballot, so the result is the same
 // for any schedule.  Reference: src/cuda_mcc.cu:21-96 (MCC_ArO2_Basic,
 // MCC_O2_Basic), src/cuda_mccO2.cu:2-977, src/cuda_mccArO2.cu:51-1265
@@ -465,64 +468,93 @@ __global__ __launch_bounds__(256) void k_o2_totals(O2Const<R> k, O2Set<R> set, c
 // walk over the parents of every source species, whose products are placed
 // into all target species with one wave scan each: new bucket = survivors in
 // old order, then products by (source species, parent, slot).
+// One wave per (O2_CPW consecutive cells, target species t): the survivors
+// of t in those cells, walked as one range, then the products that become
+// t, by source species, parent and slot, for each of the cells that has any
+// (the product scan tells which).  Survivors keep their order; products
+// follow them in (source species, parent, slot) order, with ids from the
+// global product scan.
 template <typename R>
 __global__ __launch_bounds__(256) void k_o2_scatter_all(O2Const<R> k, O2Set<R> set, O2Set<R> dst, const int *pstart,
                                                         const uint32_t *next_id, int *err) {
-    int lane;
-    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
-    if (c < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.y;
+    const int w0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)) * O2_CPW;
+    if (w0 >= k.Csize || t >= k.nsp) return;
     const unsigned long long lt = lanes_below(lane);
     const int nsp = k.nsp;
-    int base[CHEM_MAXSP], id0[CHEM_MAXSP], prun[CHEM_MAXSP];
+    int cell[O2_CPW], lo[O2_CPW + 1], run[O2_CPW], ns[O2_CPW];
 #pragma unroll
-    for (int t = 0; t < CHEM_MAXSP; ++t) {
-        base[t] = 0; id0[t] = 0; prun[t] = 0;
-        if (t >= nsp) continue;
-        const PView<R> p = set.p[t];
-        const PView<R> d = dst.p[t];
-        const uint8_t *ft = set.fate[t];
-        int k0, k1;
-        cell_range(p, c, k0, k1);
-        const int ns = d.cell_start[c];
-        int run = 0;
-        for (int b = k0; b < k1; b += 64) {
-            const int kk = b + lane;
-            const bool keep = kk < k1 && !(ft[kk] & (O2F_DEAD | O2F_VICTIM));
-            const unsigned long long m = __ballot(keep);
-            if (keep) {
-                const int pos = ns + run + __popcll(m & lt);
-                if (pos >= d.cap) atomicOr(err, 1);
-                else {
-                    d.x[pos] = p.x[kk]; d.y[pos] = p.y[kk];
-                    d.vx[pos] = p.vx[kk]; d.vy[pos] = p.vy[kk]; d.vz[pos] = p.vz[kk]; d.id[pos] = p.id[kk];
-                }
-            }
-            run += __popcll(m);
-        }
-        base[t] = ns + run;
-        id0[t] = (int)(next_id[t] + (uint32_t)pstart[(size_t)t * (k.G + 1) + c]);
+    for (int q = 0; q < O2_CPW; ++q) {
+        const int wq = w0 + q;
+        cell[q] = wq < k.Csize ? (wq / k.ncy) * k.ngy + (wq % k.ncy) : -1;
+        run[q] = 0;
     }
-    for (int s2 = 0; s2 < nsp; ++s2) {
-        const PView<R> q = set.p[s2];
-        const uint8_t *fq = set.fate[s2];
-        const R *sq = set.stash[s2];
-        const size_t qcap = (size_t)q.cap;
-        int q0, q1;
-        cell_range(q, c, q0, q1);
-        for (int b = q0; b < q1; b += 64) {
-            const int kk = b + lane;
-            int pr0 = -1, pr1 = -1;
-            if (kk < q1) {
-                const uint8_t f = fq[kk];
-                pr0 = o2_fate_prod(f, 0);
-                pr1 = o2_fate_prod(f, 1);
-            }
-            // no products in this chunk: nothing to place
-            if (!__ballot(pr0 >= 0)) continue;
+    const PView<R> p = set.p[t];
+    const PView<R> d = dst.p[t];
+    const uint8_t *ft = set.fate[t];
+    {
+        int a0 = 0, a1 = 0;
+        cell_range(p, cell[0], a0, a1);
+        lo[0] = a0;
+        lo[1] = a1;
 #pragma unroll
-            for (int t = 0; t < CHEM_MAXSP; ++t) {
-                if (t >= nsp) continue;
-                const int n0 = pr0 == t, n1 = pr1 == t;
+        for (int q = 1; q < O2_CPW; ++q) {
+            int b0 = lo[q], b1 = lo[q];
+            if (cell[q] >= 0) cell_range(p, cell[q], b0, b1);
+            lo[q + 1] = max(b1, lo[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < O2_CPW; ++q) ns[q] = cell[q] >= 0 ? d.cell_start[cell[q]] : 0;
+    const int hi = lo[O2_CPW];
+    // survivors of t; the lanes of one cell are a contiguous run
+    for (int b = lo[0]; b < hi; b += 64) {
+        const int kk = b + lane;
+        int kq = 0;
+#pragma unroll
+        for (int q = 1; q < O2_CPW; ++q) kq += kk >= lo[q] ? 1 : 0;
+        const bool keep = kk < hi && !(ft[kk] & (O2F_DEAD | O2F_VICTIM));
+        int pos = 0;
+#pragma unroll
+        for (int q = 0; q < O2_CPW; ++q) {
+            const unsigned long long m = __ballot(keep && kq == q);
+            if (keep && kq == q) pos = ns[q] + run[q] + __popcll(m & lt);
+            run[q] += __popcll(m);
+        }
+        if (keep) {
+            if (pos >= d.cap) atomicOr(err, 1);
+            else {
+                d.x[pos] = p.x[kk]; d.y[pos] = p.y[kk];
+                d.vx[pos] = p.vx[kk]; d.vy[pos] = p.vy[kk]; d.vz[pos] = p.vz[kk]; d.id[pos] = p.id[kk];
+            }
+        }
+    }
+    // products that become t
+    const int *ps = pstart + (size_t)t * (k.G + 1);
+    for (int q = 0; q < O2_CPW; ++q) {
+        const int c = cell[q];
+        if (c < 0) continue;
+        const int p0 = ps[c];
+        if (ps[c + 1] == p0) continue;   // none in this cell
+        const int base = ns[q] + run[q];
+        const uint32_t id0 = next_id[t] + (uint32_t)p0;
+        int prun = 0;
+        for (int s2 = 0; s2 < nsp; ++s2) {
+            const PView<R> src = set.p[s2];
+            const uint8_t *fq = set.fate[s2];
+            const R *sq = set.stash[s2];
+            const size_t qcap = (size_t)src.cap;
+            int q0, q1;
+            cell_range(src, c, q0, q1);
+            for (int b = q0; b < q1; b += 64) {
+                const int kk = b + lane;
+                int n0 = 0, n1 = 0;
+                if (kk < q1) {
+                    const uint8_t f = fq[kk];
+                    n0 = o2_fate_prod(f, 0) == t;
+                    n1 = o2_fate_prod(f, 1) == t;
+                }
                 const int cnt = n0 + n1;
                 if (!__ballot(cnt > 0)) continue;
                 int incl = cnt;
@@ -532,22 +564,21 @@ __global__ __launch_bounds__(256) void k_o2_scatter_all(O2Const<R> k, O2Set<R> s
                     if (lane >= off) incl += u;
                 }
                 const int tot = __shfl(incl, 63, 64);
-                int r = prun[t] + incl - cnt;
-                const PView<R> d = dst.p[t];
+                int r = prun + incl - cnt;
                 for (int slot = 0; slot < 2; ++slot) {
                     if (!(slot ? n1 : n0)) continue;
-                    const int pos = base[t] + r;
+                    const int pos = base + r;
                     if (pos >= d.cap) atomicOr(err, 1);
                     else {
-                        d.x[pos] = q.x[kk]; d.y[pos] = q.y[kk];
+                        d.x[pos] = src.x[kk]; d.y[pos] = src.y[kk];
                         d.vx[pos] = sq[(3 * slot) * qcap + kk];
                         d.vy[pos] = sq[(3 * slot + 1) * qcap + kk];
                         d.vz[pos] = sq[(3 * slot + 2) * qcap + kk];
-                        d.id[pos] = (uint32_t)id0[t] + (uint32_t)r;
+                        d.id[pos] = id0 + (uint32_t)r;
                     }
                     ++r;
                 }
-                prun[t] += tot;
+                prun += tot;
             }
         }
     }
@@ -586,8 +617,9 @@ void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, const int *pcoun
 template <typename R>
 void launch_o2_scatter_all(const O2Const<R> &k, const O2Set<R> &set, const O2Set<R> &dst, const int *pstart,
                            const uint32_t *next_id, int *err, hipStream_t s) {
-    hipLaunchKernelGGL(k_o2_scatter_all<R>, dim3(wave_blocks(k.Csize)), dim3(64 * O2_WAVES), 0, s, k, set, dst,
-                       pstart, next_id, err);
+    const int nb = (k.Csize + O2_CPW * O2_WAVES - 1) / (O2_CPW * O2_WAVES);
+    hipLaunchKernelGGL(k_o2_scatter_all<R>, dim3(nb, k.nsp), dim3(64 * O2_WAVES), 0, s, k, set, dst, pstart, next_id,
+                       err);
 }
 void launch_o2_bump_ids(uint32_t *next_id, const int *pstart, int G, int nsp, hipStream_t s) {
     hipLaunchKernelGGL(k_o2_bump, dim3(1), dim3(64), 0, s, next_id, pstart, G, nsp);
