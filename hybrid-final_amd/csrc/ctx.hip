@@ -1172,8 +1172,9 @@ struct Case : CaseBase {
         HIPCHK(hipStreamSynchronize(st));
         FluidDev<R> d{};
         d.ncx = S.ncx; d.ncy = S.ncy; d.ngx = S.ngx; d.ngy = S.ngy; d.G = G; d.nf = S.nfsp;
-        d.nx = S.fx_n; d.ny = S.fy_n; d.n_ave = S.deck.n_ave; d.TnRct = S.TnRct;
+        d.nx = S.fx_n; d.ny = S.fy_n; d.n_ave = S.deck.n_ave; d.TnRct = S.TnRct; d.gas = gas;
         d.dtc = S.dtc; d.dx = S.dx; d.dy = S.dy; d.dt = S.dt; d.denscale_e = S.Denscale[0];
+        for (int k = 0; k < nsp && k < 8; ++k) d.dsc[k] = S.Denscale[k];
         d.RR7 = S.RR[7]; d.RR9 = S.RR[9];
         d.dtm = (double)S.dt_mcc; d.sigv1 = (double)S.sigv[1];
         int *ip = nullptr;
@@ -1194,6 +1195,8 @@ struct Case : CaseBase {
         if (int e = fl_upload(rp, S.fl_ay)) return e; d.ay = rp;
         if (int e = fl_upload(rp, S.fl_by)) return e; d.by = rp;
         if (int e = fl_upload(rp, S.BackDen1)) return e; d.BackDen1 = rp;
+        if (int e = fl_upload(rp, S.BackDen2.empty() ? S.BackDen1 : S.BackDen2)) return e; d.BackDen2 = rp;
+        if (int e = fl_upload(rp, std::vector<R>(S.RR, S.RR + 100))) return e; d.RR = rp;
         if (int e = fl_upload(d.den, S.fl_den)) return e;
         if (int e = fl_upload(d.src, std::vector<R>(), S.fl_den.size())) return e;
         if (int e = fl_upload(d.flux_x, S.fl_flux_x)) return e;
@@ -1202,7 +1205,13 @@ struct Case : CaseBase {
         if (int e = fl_upload(d.xs, std::vector<R>(), scratch)) return e;
         if (int e = fl_upload(d.gs, std::vector<R>(), scratch)) return e;
         if (int e = fl_upload(d.win, std::vector<R>(), (size_t)G)) return e;
-        if (int e = fl_upload(d.nodeN, std::vector<R>(S.FluidN.begin(), S.FluidN.begin() + G))) return e;
+        // the node densities: argon's own array (its MCC reads P2); the
+        // oxygen collision steps read theirs from the array the fluid updates
+        if (gas == 0) {
+            if (int e = fl_upload(d.nodeN, std::vector<R>(S.FluidN.begin(), S.FluidN.begin() + G))) return e;
+        } else {
+            d.nodeN = o2fluid;
+        }
         if (int e = fl_upload(d.mark, std::vector<unsigned long long>(), (size_t)G * S.TnRct)) return e;
         // the event counts so far are not part of the first window
         HIPCHK(hipMemcpy(d.mark, mccr, (size_t)G * S.TnRct * sizeof(unsigned long long), hipMemcpyDeviceToDevice));
@@ -1211,8 +1220,6 @@ struct Case : CaseBase {
         return 0;
     }
     int fluid_enable(int on) override {
-        if (on && gas != 0)
-            return fail(-75, "the metastable fluid coupling is built for argon cases (gas type 0)");
         if (on && S.deck.n_ave <= 0) return fail(-75, "the metastable fluid needs AverageIter > 0");
         if (on)
             if (int e = fluid_alloc()) return e;
@@ -1496,7 +1503,7 @@ struct Case : CaseBase {
             if (n == "fl_flux_x") return D(fl.flux_x, (size_t)S.nfsp * S.ngx * S.ncy, rs, 1);
             if (n == "fl_flux_y") return D(fl.flux_y, (size_t)S.nfsp * S.ncx * S.ngy, rs, 1);
             if (n == "fl_win") return D(fl.win, G, rs, 1);
-            if (n == "FluidN") return D(fl.nodeN, G, rs, 1, false);
+            if (n == "FluidN") return D(fl.nodeN, (size_t)(gas == 0 ? 1 : S.nfsp) * G, rs, 1, false);
             if (n == "P2") return D(P2, G, rs, 1, false);
             if (n == "fl_state") return D(fl.state, 2, 4, 0, false);
         }

@@ -190,11 +190,12 @@ struct FluidState {
 };
 template <typename R>
 struct FluidDev {
-    int ncx, ncy, ngx, ngy, G, nf, nx, ny, n_ave, TnRct;
+    int ncx, ncy, ngx, ngy, G, nf, nx, ny, n_ave, TnRct, gas;
     R dtc, dx, dy, dt, denscale_e, RR7, RR9;
+    R dsc[8];   // Denscale of every particle species
     double dtm, sigv1;
     const int *x_x1, *x_x2, *x_y, *y_x, *y_y1, *y_y2, *plasma;
-    const R *x_g1, *x_g2, *y_g1, *y_g2, *ax, *bx, *ay, *by, *BackDen1;
+    const R *x_g1, *x_g2, *y_g1, *y_g2, *ax, *bx, *ay, *by, *BackDen1, *BackDen2, *RR;
     R *den, *src, *flux_x, *flux_y, *xs, *gs, *win, *nodeN;
     unsigned long long *mark;
     FluidState *state;

@@ -898,12 +898,13 @@ void fluid_setup(Setup<R> &s) {
     s.fx_n = (int)s.fx_x2.size();
     s.fy_n = (int)s.fy_y2.size();
     // wall flux factors (src/Fluid.c:113-186): 0 at a Neumann node on the
-    // domain edge, the species' quenching probability everywhere else
+    // domain edge, the species' quenching probability everywhere else (a
+    // float in the reference, FG.Gamma1, in FP64 cases too)
     s.fx_g1.assign((size_t)nf * s.fx_n, 0); s.fx_g2.assign((size_t)nf * s.fx_n, 0);
     s.fy_g1.assign((size_t)nf * s.fy_n, 0); s.fy_g2.assign((size_t)nf * s.fy_n, 0);
     auto neu = [&](int g) { return s.Boundary[g] == NEUMANN; };
     for (int f = 0; f < nf; ++f) {
-        const R gam = (R)s.deck.fl_quench[f];
+        const R gam = (R)(float)s.deck.fl_quench[f];
         for (int k = 0; k < s.fx_n; ++k) {
             const int x1 = s.fx_x1[k], x2 = s.fx_x2[k], yy = s.fx_y[k];
             const bool n1 = neu(x1 * ngy + yy) || neu(x1 * ngy + yy + 1);

@@ -272,7 +272,8 @@ class Case:
         return dict(zip(self.STAGES, list(ms))), n.value
 
     def fluid(self, on=True):
-        """Switches the metastable fluid coupling on or off (argon cases)."""
+        """Switches the neutral fluid coupling on or off (Ar*; O2A, O2B, OP,
+        OD for O2; Ar* and those four for Ar/O2)."""
         _check(lib().pic_fluid_enable(self.h, int(bool(on))))
 
     def dump_save(self, path):

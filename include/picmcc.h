@@ -96,17 +96,20 @@ int pic_diag_sample(pic_ctx *ctx);
 /* copies up to max_records of the history, oldest first, into out
  * (max_records x *rec_len doubles; out may be NULL to query the sizes) */
 int pic_diag_history(pic_ctx *ctx, double *out, int max_records, int *nrec, int *rec_len);
-/* Metastable fluid (the reference's continuity solver and source update of
+/* Neutral fluid (the reference's continuity solver and source update of
  * its warm-up loop, src/cuda_main.cu:84, src/cuda_Fluid.cu:34-178,
- * src/cuda_Diagnostic.cu:37-110, 634-642), argon cases: with on != 0 every
- * step, after the deposit, adds the electron density to the averaging
- * window; every AverageIter+1-th step averages it and the collision rates,
- * forms the Ar* source and switches the continuity solve on; from then on
- * each step solves the Ar* continuity equation (ADI, Scharfetter-Gummel,
- * time step dt x TimeStep_Conti), hands the density to the nodes and
- * updates the electron null-collision probability.  -75 for O2 and Ar/O2
- * cases.  Arrays: fl_den, fl_src, fl_flux_x, fl_flux_y, fl_win, FluidN, P2,
- * fl_state (window count, solve on). */
+ * src/cuda_Diagnostic.cu:37-110, 548-744): Ar* for argon; O2A, O2B, OP, OD
+ * for O2; Ar*, O2A, O2B, OP, OD for Ar/O2.  With on != 0 every step, after
+ * the deposit, adds the electron density to the averaging window; every
+ * AverageIter+1-th step averages it and the collision rates, forms every
+ * fluid species' source and switches the continuity solve on; from then on
+ * each step solves the species' continuity equations (ADI,
+ * Scharfetter-Gummel, time step dt x TimeStep_Conti) and hands the
+ * densities to the nodes, where argon's electron null-collision
+ * probability and the oxygen collision steps' target densities follow
+ * them.  -75 when AverageIter is 0.  Arrays: fl_den, fl_src, fl_flux_x,
+ * fl_flux_y (species-major), fl_win, FluidN, P2 (argon), fl_state (window
+ * count, solve on). */
 int pic_fluid_enable(pic_ctx *ctx, int on);
 /* Checkpoint in the reference's dump format, version 0.0.0 (V000_DUMP,
  * src/cuda_SaveDump.cu:5-106, with the version header of SaveDumpFile,

@@ -162,6 +162,11 @@ def deck_to_struct(deck):
         ac = _obj(cc, "ArgonCase")
         for r, k in enumerate(AR_KEYS):
             d.coll_flag[r] = _num(ac, k)
+    else:
+        # "N.reaction" keys (src/start.c:1036-1230): the number is the index
+        ac = _obj(cc, {1: "OxygenCase", 2: "Argon/OxygenCase"}[d.gas])
+        for k, v in ac.items():
+            d.coll_flag[int(k.split(".")[0])] = float(v)
     return d
 
 

@@ -103,12 +103,45 @@ static int derive_constants(orc_case *c) {
         c->bg_pres[0] = (real)d->total_pres;
         c->bg_dens[0] = (real)(NPERTORR * (double)c->bg_pres[0] / ((double)(real)d->bg_temp[0] + DBLMIN));
         c->fl_mass[0] = (real)(39.950 * AMU);
-        c->fl_dens[0] = (real)d->fl_dens[0];
         c->mass[0] = (real)(5.485799486e-4 * AMU); c->q[0] = (real)(-1.0 * CQ);
         c->mass[1] = (real)(39.9500 * AMU);        c->q[1] = (real)(1.0 * CQ);
+    } else if (d->gas == 1) {
+        /* O2 (start.c:420-550): O2 background; fluid O2A, O2B, OP, OD;
+         * particles e, O2+, O+, O- */
+        c->nsp = 4; c->nfsp = 4;
+        c->bg_mass[0] = (real)(32.0 * AMU);
+        c->bg_pres[0] = (real)d->total_pres;
+        c->bg_dens[0] = (real)(NPERTORR * (double)c->bg_pres[0] / ((double)(real)d->bg_temp[1] + DBLMIN));
+        c->fl_mass[0] = c->fl_mass[1] = (real)(32.0 * AMU);
+        c->fl_mass[2] = c->fl_mass[3] = (real)(16.0 * AMU);
+        c->mass[0] = (real)(5.485799486e-4 * AMU); c->q[0] = (real)(-1.0 * CQ);
+        c->mass[1] = (real)(32.0 * AMU);           c->q[1] = (real)(1.0 * CQ);
+        c->mass[2] = (real)(16.0 * AMU);           c->q[2] = (real)(1.0 * CQ);
+        c->mass[3] = (real)(16.0 * AMU);           c->q[3] = (real)(-1.0 * CQ);
     } else {
-        return -19;   /* O2 and Ar/O2 chemistry: not restated yet */
+        /* Ar/O2 (start.c:551-725): the O2 share of the pressure is
+         * r_O2 / (r_Ar + r_O2) (float), Ar takes 1 - that; fluid Ar*, O2A,
+         * O2B, OP, OD; particles e, Ar+, O2+, O+, O- */
+        real r1 = (real)d->bg_ratio[0], r2 = (real)d->bg_ratio[1];
+        real f2 = r2 / (r1 + r2);
+        real f1 = (real)(1.0 - (double)f2);
+        c->nsp = 5; c->nfsp = 5;
+        c->bg_mass[0] = (real)(39.950 * AMU);
+        c->bg_pres[0] = (real)d->total_pres * f1;
+        c->bg_dens[0] = (real)(NPERTORR * (double)c->bg_pres[0] / ((double)(real)d->bg_temp[0] + DBLMIN));
+        c->bg_mass[1] = (real)(32.0 * AMU);
+        c->bg_pres[1] = (real)d->total_pres * f2;
+        c->bg_dens[1] = (real)(NPERTORR * (double)c->bg_pres[1] / ((double)(real)d->bg_temp[1] + DBLMIN));
+        c->fl_mass[0] = (real)(39.950 * AMU);
+        c->fl_mass[1] = c->fl_mass[2] = (real)(32.0 * AMU);
+        c->fl_mass[3] = c->fl_mass[4] = (real)(16.0 * AMU);
+        c->mass[0] = (real)(5.485799486e-4 * AMU); c->q[0] = (real)(-1.0 * CQ);
+        c->mass[1] = (real)(39.950 * AMU);         c->q[1] = (real)(1.0 * CQ);
+        c->mass[2] = (real)(32.0 * AMU);           c->q[2] = (real)(1.0 * CQ);
+        c->mass[3] = (real)(16.0 * AMU);           c->q[3] = (real)(1.0 * CQ);
+        c->mass[4] = (real)(16.0 * AMU);           c->q[4] = (real)(-1.0 * CQ);
     }
+    for (i = 0; i < c->nfsp; ++i) c->fl_dens[i] = (real)d->fl_dens[i];
     for (i = 0; i < c->nsp; ++i) {
         c->np2c[i] = (real)d->sp_np2c[i];
         real temp = (real)d->sp_temp[i];
@@ -457,6 +490,10 @@ static int laplace_presolves(orc_case *c) {
         double T = (double)c->Temp[g];
         c->BackDen1[g] = (real)(NPERTORR * (double)c->bg_pres[0] / (T * 8.6142e-5 + DBLMIN));
         c->BackVel1[g] = (real)sqrt(T * 1.38e-23 / (double)c->bg_mass[0]);
+        if (c->deck.gas == 2) {   /* Calculate_2GasPara, src/cuda_Field.cu:669-678 */
+            c->BackDen2[g] = (real)(NPERTORR * (double)c->bg_pres[1] / (T * 8.6142e-5 + DBLMIN));
+            c->BackVel2[g] = (real)sqrt(T * 1.38e-23 / (double)c->bg_mass[1]);
+        }
     }
     free(b); free(x);
     return 0;
@@ -539,6 +576,24 @@ static void argon_tables(orc_case *c) {
     c->mofM[4] = me / mAs; c->mofM[5] = mAr / mi; c->mofM[6] = mAr / mi; c->mofM[7] = me / mAs;
     c->mofM[8] = mAs / mAr; c->mofM[9] = mAr / mAs;
     c->RR[7] = c->Flag[7] * (real)2.0e-13; c->RR[8] = c->Flag[8] * (real)6.2e-16; c->RR[9] = c->Flag[9] * (real)3.0e-21;
+}
+
+/* ---------- O2 and Ar/O2 rate constants (src/PhysicsData.c:2406-2414,
+ *            3256-3275): the reactions after the tabulated ones (58 of O2,
+ *            68 of Ar/O2) have a rate constant, RR = Flag x k.  The oracle
+ *            keeps only these for the two gases: it restates their fluid
+ *            (orc_fluid_diag); their collision step is orc_mcc_o2. ---------- */
+static void oxygen_rates(orc_case *c) {
+    static const double k_o2[9] = {6.90e-16, 8.00e-18, 2.56e-17, 1.00e-18, 4.20e-17, 1.30e-22, 2.20e-24,
+                                   5.50e-29, 2.20e-24};
+    static const double k_aro2[20] = {2.00e-13, 2.70e-13, 6.90e-16, 8.00e-18, 2.56e-17, 1.00e-18, 4.20e-17,
+                                      1.30e-22, 2.20e-24, 5.50e-29, 2.20e-24, 6.40e-18, 5.10e-17, 6.20e-16,
+                                      3.00e-21, 8.10e-18, 4.10e-17, 1.01e-16, 1.14e-16, 1.12e-15};
+    int o2 = c->deck.gas == 1, r;
+    int ntab = o2 ? 58 : 68;
+    c->TnRct = o2 ? 67 : 88;
+    for (r = 0; r < c->TnRct; ++r) { c->Flag[r] = (real)c->deck.coll_flag[r]; if (c->Flag[r] < 0) c->Flag[r] = 0; }
+    for (r = ntab; r < c->TnRct; ++r) c->RR[r] = (real)((double)c->Flag[r] * (o2 ? k_o2 : k_aro2)[r - ntab]);
 }
 
 /* ---------- null-collision constants: src/cuda_mcc.cu:202-291 ---------- */
@@ -864,6 +919,12 @@ int orc_setup(orc_case *c) {
     e = laplace_presolves(c);
     if (e) return e;
     fluid_densities(c);
+    if (c->deck.gas != 0) {
+        /* oxygen gases: backgrounds, fluid and rate constants (no particles) */
+        oxygen_rates(c);
+        fluid_setup(c);
+        return 0;
+    }
     argon_tables(c);
     null_collision(c);
     maxwell_table(c);

@@ -809,6 +809,7 @@ void orc_fluid_step(orc_case *c);
 
 /* ================= one timestep: src/cuda_main.cu:112-235 ================= */
 int orc_step(orc_case *c) {
+    if (c->deck.gas != 0) return -19;   /* oxygen gases: fluid only (their collision step is orc_mcc_o2) */
     c->tstep++;
     c->t = (double)c->tstep * (double)c->dt;
     orc_field(c);
@@ -943,53 +944,172 @@ void orc_fluid_continuity(orc_case *c) {
     }
 }
 
-/* The fluid part of the diagnostic (argon; src/cuda_Diagnostic.cu:44-110,
- * 534-560, 634-642): every step adds the electron node density to the
- * window; the (N_ave+1)-th call averages it and the collision rates since
- * the last average, computes the Ar* source per node
- *   S = R2 - R4 - k7 n_e n - 2 R8 - k9 n_Ar n
- * (R: averaged event rates, k: rate constants, n: the node metastable
- * density of the last sync), hands it to the cells and switches the
- * continuity solve on; once on, the cell densities go to the nodes and the
- * electron null-collision probability P2 follows them. */
+/* Averaged event rates (src/cuda_Diagnostic.cu:548-633): reaction r's count
+ * over the window, times Denscale of the species whose collisions count it
+ * over N_ave dt.  rate_range covers reactions r0..r1-1 of one species
+ * ((Denscale / N_ave) / dt, then times the count); rate_late is the form the
+ * reference uses for its last few entries (Denscale times (1 / N_ave / dt),
+ * then times the count). */
+static void rate_range(orc_case *c, real *M, int g, int r0, int r1, int sp) {
+    int r, T = c->TnRct;
+    real t = c->Denscale[sp] / (real)c->deck.n_ave / c->dt;
+    for (r = r0; r < r1; r++) M[r] = t * (real)(c->mccr[(size_t)g * T + r] - c->mccr_mark[(size_t)g * T + r]);
+}
+static void rate_late(orc_case *c, real *M, int g, int r, int sp) {
+    int T = c->TnRct;
+    real t = (real)1 / (real)c->deck.n_ave / c->dt;
+    M[r] = c->Denscale[sp] * t * (real)(c->mccr[(size_t)g * T + r] - c->mccr_mark[(size_t)g * T + r]);
+}
+
+/* The fluid sources of one node, per fluid species (src/cuda_Diagnostic.cu:
+ * 634-744).  M holds the averaged rates; the rate-constant reactions are
+ * set here from the node's fluid (last sync) and background densities, then
+ * each species' source is its production minus its loss as the reference
+ * sums them.  Eden = 0.25 x the averaged electron density. */
+static void fluid_sources(orc_case *c, int g, real *M, real Eden, real *S) {
+    int G = c->Gsize;
+    const real *F = c->FluidN;
+    real B1 = c->BackDen1[g];
+    if (c->deck.gas == 0) {
+        /* Ar*: S = R2 - R4 - k7 n_e n - 2 R8 - k9 n_Ar n */
+        real nm = F[g];
+        real k7 = c->RR[7] * Eden * nm;
+        real k9 = c->RR[9] * B1 * nm;
+        S[0] = M[2] - M[4] - k7 - 2 * M[8] - k9;
+    } else if (c->deck.gas == 1) {
+        /* O2A, O2B, OP, OD (src/cuda_Diagnostic.cu:643-683) */
+        real sum, diff;
+        M[59] = c->RR[59] * F[2 * G + g] * F[3 * G + g];
+        M[60] = c->RR[60] * F[3 * G + g] * B1;
+        M[61] = c->RR[61] * F[3 * G + g] * B1;
+        M[62] = c->RR[62] * F[3 * G + g] * B1;
+        M[63] = c->RR[63] * F[g] * F[2 * G + g];
+        M[64] = c->RR[64] * F[g] * B1;
+        M[65] = c->RR[65] * F[g] * F[g];
+        M[66] = c->RR[66] * F[G + g] * B1;
+        sum = M[3] + M[61];
+        diff = M[14] + M[15] + M[16] + M[17] + M[18] + M[19] + M[20] + M[21] + M[46] + M[51] + M[56] + M[63] + M[64]
+             + 2 * M[65];
+        S[0] = sum - diff;
+        sum = M[4] + M[62];
+        diff = M[22] + M[23] + M[24] + M[25] + M[26] + M[27] + M[28] + M[29] + M[52] + M[57] + M[58] + M[66];
+        S[1] = sum - diff;
+        sum = M[6] + 2 * M[7] + M[8] + M[11] + M[13] + M[15] + 2 * M[18] + M[19] + M[21] + M[23] + 2 * M[26] + M[27]
+            + M[29] + M[30] + M[31] + M[40] + M[42] + M[44] + 2 * M[45] + M[46] + M[50] + M[53] + M[56] + M[57]
+            + M[58] + M[59] + M[60] + M[61] + M[62];
+        diff = M[33] + M[34] + M[35] + M[36] + M[37] + M[38] + M[43] + M[47];
+        S[2] = sum - diff;
+        sum = M[8] + 2 * M[9] + M[19] + 2 * M[20] + M[27] + 2 * M[28] + M[31] + M[33];
+        diff = M[39] + M[40] + M[59] + M[60] + M[61] + M[62];
+        S[3] = sum - diff;
+    } else {
+        /* Ar*, O2A, O2B, OP, OD (src/cuda_Diagnostic.cu:684-744) */
+        real sum, diff, B2 = c->BackDen2[g];
+        real N1 = F[g], N2 = F[G + g], N3 = F[2 * G + g], N4 = F[3 * G + g], N5 = F[4 * G + g];
+        M[68] = c->RR[68] * Eden * N1;
+        M[71] = c->RR[71] * N4 * N5;
+        M[72] = c->RR[72] * N5 * B2;
+        M[73] = c->RR[73] * N5 * B2;
+        M[74] = c->RR[74] * N5 * B2;
+        M[75] = c->RR[75] * N2 * N4;
+        M[76] = c->RR[76] * N2 * B2;
+        M[77] = c->RR[77] * N2 * N2;
+        M[78] = c->RR[78] * N3 * B2;
+        M[82] = c->RR[82] * N1 * B1;
+        M[83] = c->RR[83] * N1 * N4;
+        M[84] = c->RR[84] * N1 * N4;
+        M[85] = c->RR[85] * N1 * B2;
+        M[86] = c->RR[86] * N1 * B2;
+        M[87] = c->RR[87] * N1 * B2;
+        sum = M[2];
+        diff = M[4] + M[68] + 2 * M[81] + M[82] + M[83] + M[84] + M[85] + M[86] + M[87];
+        S[0] = sum - diff;
+        sum = M[8] + M[73];
+        diff = M[19] + M[20] + M[21] + M[22] + M[23] + M[24] + M[25] + M[26] + M[51] + M[56] + M[30] + M[75] + M[76]
+             + 2 * M[77];
+        S[1] = sum - diff;
+        sum = M[9] + M[74];
+        diff = M[27] + M[28] + M[29] + M[30] + M[31] + M[32] + M[33] + M[34] + M[57] + M[64] + M[70] + M[78];
+        S[2] = sum - diff;
+        sum = M[11] + 2 * M[12] + M[13] + M[16] + M[18] + M[20] + 2 * M[23] + M[24] + M[26] + M[28] + 2 * M[31]
+            + M[32] + M[34] + M[35] + M[36] + M[45] + M[47] + M[49] + 2 * M[50] + M[51] + M[55] + M[60] + M[63]
+            + M[64] + M[70] + M[71] + M[72] + M[73] + M[74] + M[69] + 2 * M[85] + M[86];
+        diff = M[38] + M[39] + M[40] + M[41] + M[42] + M[43] + M[48] + M[52] + M[79] + M[83];
+        S[3] = sum - diff;
+        sum = M[13] + 2 * M[14] + M[24] + 2 * M[25] + M[32] + 2 * M[33] + M[36] + M[38] + M[83] + M[86];
+        diff = M[44] + M[45] + M[71] + M[72] + M[73] + M[74];
+        S[4] = sum - diff;
+    }
+}
+
+/* The fluid part of the diagnostic (src/cuda_Diagnostic.cu:44-110,
+ * 534-744): every step adds the electron node density to the window; the
+ * (N_ave+1)-th call averages it and the collision rates since the last
+ * average (per gas: argon 0-6 and 8 by the electron Denscale; O2 0-40 e,
+ * 41-46 O-, 47-52 O2+, 53-57 O+, 58 O-; Ar/O2 0-45 e, 46-51 O-, 52-59 O2+,
+ * 60-64 O+, 65-67 Ar+, 69-70 O+, 79-80 Ar+, 81 e), computes every fluid
+ * species' source per node (fluid_sources), hands it to the cells and
+ * switches the continuity solve on; once on, the cell densities of every
+ * fluid species go to the nodes, and for argon the electron null-collision
+ * probability P2 follows the Ar* density (the oxygen collision steps read
+ * the node densities themselves). */
 void orc_fluid_diag(orc_case *c) {
-    int g, r, G = c->Gsize, T = c->TnRct, ngy = c->ngy, ncy = c->ncy;
+    int g, r, f, G = c->Gsize, T = c->TnRct, ngy = c->ngy, ncy = c->ncy, nf = c->nfsp, gas = c->deck.gas;
     int avg = c->nave_count == c->deck.n_ave;
+    size_t NC = (size_t)c->ncx * ncy;
     for (g = 0; g < G; g++) {
         c->win_den[g] += c->den[g];
         if (!avg) continue;
         real ave_e = c->Denscale[0] / (real)c->deck.n_ave * c->win_den[g];
-        real rate = c->Denscale[0] / (real)c->deck.n_ave / c->dt;
-        real R[9];
-        for (r = 0; r < 9; r++) {
-            R[r] = (r == 7) ? 0 : rate * (real)(c->mccr[(size_t)g * T + r] - c->mccr_mark[(size_t)g * T + r]);
+        real M[ORC_MAXRCT], S[ORC_MAXFSP];
+        for (r = 0; r < T; r++) M[r] = 0;
+        if (gas == 0) {
+            rate_range(c, M, g, 0, 7, 0);
+            rate_range(c, M, g, 8, 9, 0);
+        } else if (gas == 1) {
+            rate_range(c, M, g, 0, 41, 0);
+            rate_range(c, M, g, 41, 47, 3);
+            rate_range(c, M, g, 47, 53, 1);
+            rate_range(c, M, g, 53, 58, 2);
+            rate_late(c, M, g, 58, 3);
+        } else {
+            rate_range(c, M, g, 0, 46, 0);
+            rate_range(c, M, g, 46, 52, 4);
+            rate_range(c, M, g, 52, 60, 2);
+            rate_range(c, M, g, 60, 65, 3);
+            rate_range(c, M, g, 65, 68, 1);
+            rate_late(c, M, g, 69, 3);
+            rate_late(c, M, g, 70, 3);
+            rate_late(c, M, g, 79, 1);
+            rate_late(c, M, g, 80, 1);
+            rate_late(c, M, g, 81, 0);
         }
         for (r = 0; r < T; r++) c->mccr_mark[(size_t)g * T + r] = c->mccr[(size_t)g * T + r];
         c->win_den[g] = 0;
-        real Eden = (real)(0.25 * ave_e);
-        real nm = c->FluidN[g];
-        real k7 = c->RR[7] * Eden * nm;
-        real k9 = c->RR[9] * c->BackDen1[g] * nm;
-        real S = R[2] - R[4] - k7 - 2 * R[8] - k9;
+        fluid_sources(c, g, M, (real)(0.25 * ave_e), S);
         int x = g / ngy, y = g % ngy;
-        if (x < c->ncx && y < ncy) c->fsrc[x * ncy + y] = S;
+        if (x < c->ncx && y < ncy)
+            for (f = 0; f < nf; f++) c->fsrc[f * NC + x * ncy + y] = S[f];
     }
     if (avg) { c->nave_count = 0; c->conti_flag = 1; }
     else c->nave_count++;
     if (!c->conti_flag) return;
     /* Sync_Fluid_GFCtoGFG_forDen (src/cuda_Fluid.cu:302-320) and the P2 of
-     * the electron null test (src/cuda_mccAr.cu:57-58) */
+     * the argon electron null test (src/cuda_mccAr.cu:57-58) */
     {
         int i, j;
-        for (i = 0; i < c->ncx; i++)
-            for (j = 0; j < ncy; j++) {
-                real v = c->fden[i * ncy + j];
-                c->FluidN[i * ngy + j] = v;
-                if (i == c->ncx - 1 || j == ncy - 1) c->FluidN[(i + 1) * ngy + j + 1] = v;
-            }
-        double dtm = (double)c->dt_mcc;
-        for (g = 0; g < G; g++)
-            c->P2[g] = c->P1[g] + (real)(1.0 - dm_exp(-dtm * (double)c->sigv[1] * (double)c->FluidN[g]));
+        for (f = 0; f < nf; f++)
+            for (i = 0; i < c->ncx; i++)
+                for (j = 0; j < ncy; j++) {
+                    real v = c->fden[f * NC + i * ncy + j];
+                    c->FluidN[(size_t)f * G + i * ngy + j] = v;
+                    if (i == c->ncx - 1 || j == ncy - 1) c->FluidN[(size_t)f * G + (i + 1) * ngy + j + 1] = v;
+                }
+        if (gas == 0) {
+            double dtm = (double)c->dt_mcc;
+            for (g = 0; g < G; g++)
+                c->P2[g] = c->P1[g] + (real)(1.0 - dm_exp(-dtm * (double)c->sigv[1] * (double)c->FluidN[g]));
+        }
     }
 }
 
@@ -1172,7 +1292,10 @@ static int lookup(orc_case *c, const char *name, orc_arr *out) {
     ENT("Atemp", c->Atemp_val, c->A_size > 0 ? c->Ai[c->A_size] : 0, rs)
     ENT("Mdiag", c->Mdiag, c->A_size, rs) ENT("cond_b", c->cond_b, (long)c->CondNUMR * c->A_size, rs)
     ENT("temp_b", c->temp_b, c->A_size, rs) ENT("lap_phi", c->lap_phi, (long)c->CondNUMR * G, rs)
-    ENT("BackDen1", c->BackDen1, G, rs) ENT("BackVel1", c->BackVel1, G, rs) ENT("FluidN", c->FluidN, (long)c->nfsp * G, rs)
+    ENT("BackDen1", c->BackDen1, G, rs) ENT("BackVel1", c->BackVel1, G, rs)
+    ENT("BackDen2", c->BackDen2, c->BackDen2 ? G : 0, rs) ENT("BackVel2", c->BackVel2, c->BackVel2 ? G : 0, rs)
+    ENT("RR", c->RR, c->TnRct, rs)
+    ENT("Denscale", c->Denscale, c->nsp, rs) ENT("FluidN", c->FluidN, (long)c->nfsp * G, rs)
     ENT("xe", c->xe, c->N_LOGX, rs) ENT("xee", c->xee, c->N_LOGX, rs) ENT("cx", c->cx, (long)c->N_LOGX * c->TnRct, rs)
     ENT("sigv", c->sigv, 3, rs) ENT("P1", c->P1, G, rs) ENT("P2", c->P2, G, rs) ENT("Pion", c->Pion, G, rs)
     ENT("vsave", c->vsave, c->nvel, rs) ENT("phi", c->phi, G, rs) ENT("TotPot", c->TotPot, G, rs)

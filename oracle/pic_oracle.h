@@ -21,9 +21,15 @@
  * into it, a mirrored offset of 1 is kept below 1, and particles deleted at
  * walls are counted (Lost).  The others are preserved.
  *
+ * O2 and Ar/O2 decks: orc_create sets up the grid, the field operator, the
+ * backgrounds, the fluid and the rate constants only (no particles;
+ * orc_step returns -19); the oxygen collision step is orc_mcc_o2 on
+ * host-set-up inputs, the oxygen fluid orc_fluid_step.
+ *
  * How each part is pinned (reference-built fixtures, known answers, closed
- * forms, statistics of the collision channels) is listed with the tests in
- * tests/test_oracle_pins.py and tests/test_setup_parity.py.
+ * forms, statistics of the collision channels, the reaction equations) is
+ * listed with the tests in tests/test_oracle_pins.py,
+ * tests/test_setup_parity.py and tests/test_fluid_pins.py.
  */
 #ifndef PIC_ORACLE_H
 #define PIC_ORACLE_H
@@ -37,6 +43,7 @@ typedef REAL real;
 #define ORC_MAXL 32      /* max boundary / conductor / source / dielectric lines */
 #define ORC_MAXSP 5
 #define ORC_MAXRCT 100
+#define ORC_MAXFSP 5   /* fluid species: Ar* | O2A O2B OP OD | Ar* O2A O2B OP OD */
 
 /* Flat copy of the JSON case deck (src/start.c:19-1431).  Filled from the
  * deck by oracle/oracle.py with parson's "missing key reads as 0" rule. */

@@ -176,3 +176,144 @@ def test_source_window_and_rates():
     assert np.all(o.arr("fl_win") == 0)
     # the node density did not move yet (no continuity solve has run): P2 stays the set-up's
     assert np.array_equal(o.arr("P2"), p2_setup)
+
+
+# ------------------------------------------------------------ oxygen gases
+# The reaction equations are the deck's Collision_On_Off keys ("N.lhs>rhs",
+# hybrid-final_amd/csrc/coll_keys.inc; the reference reads the same keys,
+# src/start.c:1036-1230).  Fluid species: O2 gas O2A O2B OP OD; Ar/O2 Ar*
+# (the metastable) O2A O2B OP OD.  Naming in the keys: "O" in
+# "e+O2B>e+2O" is ground-state O(3P); "Ar*m" is the metastable among the
+# tabulated reactions and "Ar*" among the rate-constant ones (68 on).
+FLUID = {1: ["O2A", "O2B", "OP", "OD"], 2: ["ARM", "O2A", "O2B", "OP", "OD"]}
+PROJ = {1: {"e": 0, "O2^": 1, "O^": 2, "O-": 3}, 2: {"e": 0, "Ar^": 1, "O2^": 2, "O^": 3, "O-": 4}}
+NTAB = {1: 58, 2: 68}
+# reactions the reference averages without a collision counting them
+# (src/cuda_Diagnostic.cu:586-588, 621-632), with the species whose Denscale
+# it uses: 69 and 70 (O- projectiles) take O+'s, 81 (Ar* + Ar*) the electron's
+LATE = {1: {58: 3}, 2: {69: 3, 70: 3, 79: 1, 80: 1, 81: 0}}
+# where the reference's sums differ from the equations: its Ar/O2 O2A loss
+# lists reaction 30 (e+O2B>e+O2) where reaction 63 (O^+O2A>O2^+OP) belongs
+# (src/cuda_Diagnostic.cu:722-724)
+QUIRK = {1: {}, 2: {("O2A", 30): -1, ("O2A", 63): 0}}
+
+
+def reactions(gas):
+    import os
+    import re
+    path = os.path.join(os.path.dirname(__file__), "..", "hybrid-final_amd", "csrc", "coll_keys.inc")
+    text = open(path).read()
+    body = re.search(r"pic_coll_keys%d\[\d+\] = \{(.*?)\};" % gas, text).group(1)
+    keys = re.findall(r'"([^"]*)"', body)
+    out = []
+    for r, k in enumerate(keys):
+        num, eq = k.split(".", 1)
+        assert int(num) == r
+        lhs, rhs = eq.strip().split(">")
+
+        def side(t):
+            cnt = {}
+            for tok in t.split("+"):
+                m = re.match(r"(\d*)(.*)$", tok)
+                n, name = int(m.group(1) or 1), m.group(2)
+                if name == "O":
+                    name = "OP"
+                if name == "Ar*m" or (name == "Ar*" and r >= NTAB[gas]):
+                    name = "ARM"
+                if name:
+                    cnt[name] = cnt.get(name, 0) + n
+            return cnt
+        # the projectile: the charged species on the left ("Ar+Ar^>..." too)
+        proj = next((t for t in lhs.split("+") if t in PROJ[gas]), None)
+        out.append((proj, side(lhs), side(rhs)))
+    return out
+
+
+def net(gas, r, f, eqs):
+    if (f, r) in QUIRK[gas]:
+        return QUIRK[gas][(f, r)]
+    _, l, rt = eqs[r]
+    return rt.get(f, 0) - l.get(f, 0)
+
+
+def oxy_oracle(gas):
+    d = decks.oxygen_deck(gas=gas, ncx=6, ncy=4, ppc=2)
+    d["SimulationMethod"]["AverageIter"] = 1
+    return Oracle(d, "f64")
+
+
+@pytest.mark.parametrize("gas", [1, 2], ids=["O2", "ArO2"])
+def test_oxygen_sources_follow_the_reaction_equations(gas):
+    """One reaction's events at a time, fluid densities zero: every fluid
+    species' source is (its net change in the equation) x (events) x the
+    Denscale of the projectile species / (N_ave dt); reactions with a rate
+    constant ignore counted events."""
+    o = oxy_oracle(gas)
+    eqs = reactions(gas)
+    G, T = int(o.scalar("ngx") * o.scalar("ngy")), int(o.scalar("TnRct"))
+    assert len(eqs) == T
+    ncx, ncy, ngy = int(o.scalar("ngx")) - 1, int(o.scalar("ngy")) - 1, int(o.scalar("ngy"))
+    nf = len(FLUID[gas])
+    ds, dt = o.arr("Denscale"), o.scalar("dt")
+    assert len(set(ds.tolist())) > 2, "the species' Denscale should differ"
+    mccr = o.arr("mccr").reshape(G, T)
+    src = o.arr("fl_src").reshape(nf, ncx, ncy)
+    o.arr("fl_den")[:] = 0
+    o.arr("FluidN")[:] = 0
+    o.orc_fluid_diag()                    # the window's accumulation step
+    counted = set(range(NTAB[gas])) | set(LATE[gas])
+    for r in range(T):
+        mccr[:, r] += 3
+        o.orc_fluid_diag()                # the averaging step
+        proj = eqs[r][0]
+        sp = LATE[gas].get(r, PROJ[gas].get(proj))
+        for fi, f in enumerate(FLUID[gas]):
+            want = net(gas, r, f, eqs) * 3 * ds[sp] / 1 / dt if r in counted else 0.0
+            got = src[fi]
+            assert np.allclose(got, want, rtol=1e-12, atol=0), (r, proj, f, got.ravel()[0], want)
+        o.orc_fluid_diag()                # next window's accumulation
+
+
+@pytest.mark.parametrize("gas", [1, 2], ids=["O2", "ArO2"])
+def test_oxygen_rate_constant_sources(gas):
+    """No counted events; random node fluid densities: each species' source
+    is the sum over the rate-constant reactions of (net change) x RR x the
+    densities of the equation's two reactants (fluid species, the O2 or Ar
+    background, or 0.25 x the averaged electron density)."""
+    o = oxy_oracle(gas)
+    eqs = reactions(gas)
+    G, T = int(o.scalar("ngx") * o.scalar("ngy")), int(o.scalar("TnRct"))
+    ncx, ncy, ngy = int(o.scalar("ngx")) - 1, int(o.scalar("ngy")) - 1, int(o.scalar("ngy"))
+    nf = len(FLUID[gas])
+    RR, ds = o.arr("RR"), o.arr("Denscale")
+    assert np.all(RR[NTAB[gas]:] > 0)
+    rng = np.random.default_rng(9)
+    den = o.arr("den")
+    den[:G] = rng.uniform(0, 40, G)
+    o.arr("fl_den")[:] = 0
+    FN = o.arr("FluidN").reshape(nf, G)
+    o.orc_fluid_diag()
+    Fv = rng.uniform(1e14, 1e16, (nf, G))
+    FN[:] = Fv
+    o.orc_fluid_diag()
+    eden = 0.25 * (ds[0] / 1 * (2 * den[:G]))
+    bg = {"O2": o.arr("BackDen1") if gas == 1 else o.arr("BackDen2")}
+    if gas == 2:
+        bg["Ar"] = o.arr("BackDen1")
+    src = o.arr("fl_src").reshape(nf, ncx, ncy)
+    for fi, f in enumerate(FLUID[gas]):
+        want = np.zeros(G)
+        for r in range(NTAB[gas], T):
+            if r in LATE[gas]:
+                continue
+            k = net(gas, r, f, eqs)
+            if k == 0:
+                continue
+            fac = RR[r] * np.ones(G)
+            for name, n in eqs[r][1].items():
+                for _ in range(n):
+                    fac = fac * (Fv[FLUID[gas].index(name)] if name in FLUID[gas] else eden if name == "e" else bg[name])
+            want += k * fac
+        nodes = [i * ngy + j for i in range(ncx) for j in range(ncy)]
+        scale = np.max(np.abs(want)) + 1e-300
+        assert np.all(np.abs(src[fi].ravel() - want[nodes]) <= 1e-12 * scale), f

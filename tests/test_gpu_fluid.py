@@ -1,37 +1,49 @@
 """The metastable fluid on the GPU (fluid.hip) against the oracle's fluid
-step, bit for bit in FP32 and FP64: from the same electron densities and
+step, bit for bit in FP32 and FP64, for argon (Ar*), O2 (O2A, O2B, OP, OD)
+and Ar/O2 (Ar*, O2A, O2B, OP, OD): from the same electron densities and
 collision counts, step after step through the averaging window, the switch
-to the continuity solve, the sweeps, the face fluxes, the node sync and the
-updated null-collision probability.  Then whole steps with the fluid on:
-graph replays equal kernel-by-kernel steps, the fluid moves, and O2 cases
-are refused."""
+to the continuity solve, the sweeps, the face fluxes, every species' source
+and node sync, and for argon the updated null-collision probability.  Then
+whole steps with the fluid on: graph replays equal kernel-by-kernel steps
+(argon), the fluid moves and feeds the oxygen collision steps' node
+densities, and the dump carries every species."""
 import numpy as np
 import pytest
 
 from oracle import Oracle
-from picmcc import Case, FP32_REFINED, FP64_PLAIN, PicError, decks
+from picmcc import Case, FP32_REFINED, FP64_PLAIN, decks
 
 pytestmark = pytest.mark.gpu
 
 
-def deck():
-    d = decks.small(ncx=20, ncy=14, ppc=20)
+def deck(gas=0):
+    if gas == 0:
+        d = decks.small(ncx=20, ncy=14, ppc=20)
+    else:
+        d = decks.oxygen_deck(gas=gas, ncx=20, ncy=14, ppc=10)
     d["SimulationMethod"]["AverageIter"] = 3
     return d
 
 
+GASES = [0, 1, 2]
+GAS_IDS = ["Ar", "O2", "ArO2"]
+
+
+@pytest.mark.parametrize("gas", GASES, ids=GAS_IDS)
 @pytest.mark.parametrize("prec,pn", [(FP32_REFINED, "f32"), (FP64_PLAIN, "f64")], ids=["fp32", "fp64"])
-def test_fluid_step_equals_the_oracle(prec, pn):
-    d = deck()
+def test_fluid_step_equals_the_oracle(prec, pn, gas):
+    d = deck(gas)
     c = Case(d, prec)
     o = Oracle(d, pn)
     c.fluid(True)
     inf = c.info()
     G, T, nsp = inf.ngx * inf.ngy, inf.TnRct, inf.nsp
+    nf = int(o.scalar("nfsp"))
     rng = np.random.default_rng(11)
     den = np.zeros(nsp * G)
     mccr = np.zeros((G, T), dtype=np.int64)
     real = np.float32 if pn == "f32" else np.float64
+    names = ("fl_den", "fl_src", "fl_flux_x", "fl_flux_y", "fl_win") + (("P2",) if gas == 0 else ())
     for step in range(12):
         den[:G] = rng.uniform(0, 80, G)
         mccr += rng.integers(0, 5, size=mccr.shape)
@@ -43,11 +55,17 @@ def test_fluid_step_equals_the_oracle(prec, pn):
         o.orc_fluid_step()
         st = c.array("fl_state")
         assert (int(st[0]), int(st[1])) == (int(o.scalar("nave_count")), int(o.scalar("conti_flag"))), step
-        for name in ("fl_den", "fl_src", "fl_flux_x", "fl_flux_y", "fl_win", "P2"):
+        for name in names:
             a, b = c.array(name), o.arr(name)
+            assert a.shape == b.shape, name
             assert np.array_equal(a, b), f"step {step}: {name} differs at {np.nonzero(a != b)[0][:5]}"
-        assert np.array_equal(c.array("FluidN"), o.arr("FluidN")[:G]), f"step {step}: node density"
+        assert np.array_equal(c.array("FluidN"), o.arr("FluidN")[:(1 if gas == 0 else nf) * G]), \
+            f"step {step}: node density"
     assert int(o.scalar("conti_flag")) == 1
+    if gas:
+        # every species' source and density moved
+        src = c.array("fl_src").reshape(nf, -1)
+        assert all(np.any(src[f] != 0) for f in range(nf))
     c.close()
 
 
@@ -77,16 +95,47 @@ def test_steps_with_the_fluid():
             assert np.array_equal(a["parts"][s][k], b["parts"][s][k]), f"species {s} {k}"
 
 
-def test_oxygen_cases_are_refused():
-    c = Case(decks.oxygen_deck(gas=1, ncx=8, ncy=6, ppc=10), FP32_REFINED)
-    with pytest.raises(PicError, match="built for argon"):
+@pytest.mark.parametrize("gas", [1, 2], ids=["O2", "ArO2"])
+def test_oxygen_steps_with_the_fluid(gas):
+    """Whole staged steps of an oxygen case with the fluid on: the
+    continuity solve runs, the node densities the collision step reads
+    follow the cells, and two runs of the same deck agree bit for bit."""
+    d = deck(gas)
+    runs = []
+    for _ in range(2):
+        c = Case(d, FP32_REFINED)
         c.fluid(True)
-    c.close()
+        den0 = c.array("fl_den").copy()
+        node0 = c.array("FluidN").copy()
+        c.step(10)
+        c.sync()
+        st = c.array("fl_state")
+        assert int(st[1]) == 1
+        r = {k: c.array(k).copy() for k in ("fl_den", "fl_src", "FluidN")}
+        assert not np.array_equal(r["fl_den"], den0), "the continuity solve did not run"
+        assert not np.array_equal(r["FluidN"], node0), "the node densities did not follow the fluid"
+        inf = c.info()
+        G = inf.ngx * inf.ngy
+        ncx, ncy = inf.ngx - 1, inf.ngy - 1
+        nf = r["FluidN"].size // G
+        # the node densities are the cells' (node (i, j) takes cell (i, j))
+        FN = r["FluidN"].reshape(nf, inf.ngx, inf.ngy)
+        assert np.array_equal(FN[:, :ncx, :ncy], r["fl_den"].reshape(nf, ncx, ncy))
+        r["parts"] = [c.particles(s) for s in range(inf.nsp)]
+        runs.append(r)
+        c.close()
+    a, b = runs
+    for k in ("fl_den", "fl_src", "FluidN"):
+        assert np.array_equal(a[k], b[k]), k
+    for s in range(len(a["parts"])):
+        for k in ("x", "y", "vx", "vy", "vz", "id", "cell_start"):
+            assert np.array_equal(a["parts"][s][k], b["parts"][s][k]), f"species {s} {k}"
 
 
-def test_dump_carries_the_fluid(tmp_path):
+@pytest.mark.parametrize("gas", GASES, ids=GAS_IDS)
+def test_dump_carries_the_fluid(tmp_path, gas):
     import dump_v000 as V
-    d = deck()
+    d = deck(gas)
     a = Case(d, FP32_REFINED)
     a.fluid(True)
     a.step(9)
@@ -94,14 +143,16 @@ def test_dump_carries_the_fluid(tmp_path):
     path = str(tmp_path / "f.dmp")
     a.dump_save(path)
     inf = a.info()
-    D = V.Dims(inf.nsp, 1, inf.ngx - 1, inf.ngy - 1, inf.CondNUMR, inf.TnRct)
+    nf = V.NFSP[gas]
+    D = V.Dims(inf.nsp, nf, inf.ngx - 1, inf.ngy - 1, inf.CondNUMR, inf.TnRct)
     f = V.read(path, D)
-    assert f["conti_flag"] == 1 and f["trailing_bytes"] == 0
+    assert f["conti_flag"] == 1 and f["trailing_bytes"] == 0 and f["gas"] == gas
     ncx, ncy = inf.ngx - 1, inf.ngy - 1
-    assert np.array_equal(f["fluid"][0]["den"], a.array("fl_den").reshape(ncx, ncy))
-    assert np.array_equal(f["fluid"][0]["source"], a.array("fl_src").reshape(ncx, ncy))
-    assert np.array_equal(f["fluid"][0]["flux_x"], a.array("fl_flux_x").reshape(ncx + 1, ncy))
-    assert np.array_equal(f["fluid"][0]["flux_y"], a.array("fl_flux_y").reshape(ncx, ncy + 1))
+    for k in range(nf):
+        assert np.array_equal(f["fluid"][k]["den"], a.array("fl_den").reshape(nf, ncx, ncy)[k]), k
+        assert np.array_equal(f["fluid"][k]["source"], a.array("fl_src").reshape(nf, ncx, ncy)[k]), k
+        assert np.array_equal(f["fluid"][k]["flux_x"], a.array("fl_flux_x").reshape(nf, ncx + 1, ncy)[k]), k
+        assert np.array_equal(f["fluid"][k]["flux_y"], a.array("fl_flux_y").reshape(nf, ncx, ncy + 1)[k]), k
     b = Case(d, FP32_REFINED)
     b.fluid(True)
     b.dump_load(path)

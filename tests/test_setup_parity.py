@@ -108,3 +108,34 @@ def test_deck_validation_errors(mutate, word):
 def test_malformed_json_is_an_error():
     with pytest.raises(PicError):
         HostSetup('{"Geometry": {')
+
+
+OXY_FLUID_ARRAYS = ["Temp", "BackDen1", "BackVel1", "FluidN", "fl_x1", "fl_x2", "fl_xy", "fl_yx", "fl_y1", "fl_y2",
+                    "fl_xg1", "fl_xg2", "fl_yg1", "fl_yg2", "fl_D", "fl_den", "fl_ax", "fl_bx", "fl_ay", "fl_by",
+                    "fl_flux_x", "fl_flux_y"]
+
+
+@pytest.mark.parametrize("gas", [1, 2], ids=["O2", "ArO2"])
+@pytest.mark.parametrize("prec,pn", [(FP32_REFINED, "f32"), (FP64_PLAIN, "f64")])
+def test_oxygen_fluid_setup_matches_oracle(gas, prec, pn):
+    """O2 and Ar/O2 cases: backgrounds (Ar/O2 with its two-gas split), the
+    four or five fluid species' lines, wall factors, diffusion and
+    Scharfetter-Gummel coefficients, initial densities and fluxes, and the
+    rate constants of the reactions after the tabulated ones."""
+    d = decks.oxygen_deck(gas=gas, ncx=10, ncy=7, ppc=2)
+    ns = d["GasSpecies"]["NeutralSpecies"]
+    for k, name in enumerate([n for n in ("Ar*", "O2A", "O2B", "OP", "OD") if n in ns]):
+        ns[name]["Quenching"] = 0.1 * (k + 1)      # a distinct wall factor per species
+    h = HostSetup(d, prec, load=False)
+    o = Oracle(d, pn)
+    names = OXY_FLUID_ARRAYS + (["BackDen2", "BackVel2"] if gas == 2 else [])
+    for a in names:
+        x, y = h.array(a), o.arr(a)
+        assert x.shape == y.shape, a
+        assert np.array_equal(x, y.astype(x.dtype)), f"gas {gas}/{pn}: {a}"
+    T = int(o.scalar("TnRct"))
+    assert h.scalar("TnRct") == T
+    ntab = 58 if gas == 1 else 68
+    assert np.array_equal(h.array("RR")[ntab:T], o.arr("RR")[ntab:T])
+    for sc in ("dtc", "fl_nx", "fl_ny", "nfsp"):
+        assert h.scalar(sc) == o.scalar(sc), sc
