@@ -668,6 +668,24 @@ struct Case : CaseBase {
         HIPCHK(hipGetLastError());
         return 0;
     }
+    // Move and SortBounndary of a whole step: per species the push and the
+    // destination count in one pass, then the rest of the sort.  The same
+    // result as move() then sort_boundary() (no species' sort depends on
+    // another's push); the stage entry points keep the two passes apart.
+    int move_sort() {
+        for (int s = 0; s < nsp; ++s) {
+            launch_push_count<R>(kc, cur[s], s, tag[s], key[s], cnt[s], condcharge, st);
+            launch_totals(G, inv_start, inv_ent, cnt[s], nullptr, total, alive, nullptr, st);
+            launch_scan(total, nxt[s].cell_start, G, scan_tmp, st);
+            launch_assign_bases(G, inv_start, inv_ent, cnt[s], nullptr, nxt[s].cell_start, alive, nullptr, base[s],
+                                nullptr, nullptr, st);
+            launch_sort_scatter<R>(kc, cur[s], nxt[s], tag[s], key[s], base[s], nxt[s].cap, d_err, st);
+            swap_species(s);
+        }
+        tag_valid = 0;
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     // oxygen chemistries: collide every species, remove the O- taken by
     // electron detachment, then rebuild every bucket with the products
     int mcc_o2() {
@@ -1317,8 +1335,8 @@ struct Case : CaseBase {
             // and Ar/O2 cases run the stages one by one
             for (int i = 0; i < n; ++i) {
                 int e;
-                if ((e = advance_time()) || (e = field_solve()) || (e = efield()) || (e = move()) ||
-                    (e = sort_boundary()) || (e = mcc()) || (e = deposit()) || (e = diagnostic()))
+                if ((e = advance_time()) || (e = field_solve()) || (e = efield()) || (e = move_sort()) ||
+                    (e = mcc()) || (e = deposit()) || (e = diagnostic()))
                     return e;
             }
             return 0;

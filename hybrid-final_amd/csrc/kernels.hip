@@ -135,6 +135,55 @@ __global__ __launch_bounds__(256) void k_sort_count(DevConst<R> k, PView<R> p, i
     }
 }
 
+// ============================================================ staged: Move + Sort count
+// k_push and k_sort_count in one pass while the particle is in registers
+// (whole staged steps): the same positions, tags, keys, counts and wall
+// side effects, one read of the particle instead of two.
+template <typename R>
+__global__ __launch_bounds__(256) void k_push_count(DevConst<R> k, PView<R> p, int isp, uint8_t *tag, int *key,
+                                                    int *cnt, int *condcharge) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    int k0, k1;
+    cell_range(p, c, k0, k1);
+    const unsigned long long lt = lanes_below(lane);
+    int run[ROUTE_SLOTS];
+#pragma unroll
+    for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] = 0;
+    if (k0 < k1) {
+        CellE<R> e;
+        e.load(k.Ex, k.Ey, c, k.ngy);
+        const R As = k.Ascale[isp];
+        for (int base = k0; base < k1; base += 64) {
+            const int kk = base + lane;
+            int slot = -1;
+            if (kk < k1) {
+                R x = p.x[kk], y = p.y[kk], vx = p.vx[kk], vy = p.vy[kk];
+                const int code = push_particle<R>(e.ews, e.ewn, e.ees, e.een, e.yws, e.ywn, e.yes, e.yen, As, k.dt_dx,
+                                                  k.dt_dy, x, y, vx, vy);
+                p.x[kk] = x; p.y[kk] = y; p.vx[kk] = vx; p.vy[kk] = vy;
+                tag[kk] = (uint8_t)code;
+                const Route rt = k.route[c * ROUTE_SLOTS + code];
+                if (rt.dest >= 0) slot = rt.info & 15;
+                else wall_effects<R>(k, rt, isp, c, code, x, y, vx, vy, condcharge);
+            }
+            int rank = -1;
+#pragma unroll
+            for (int s = 0; s < ROUTE_SLOTS; ++s) {
+                const unsigned long long m = __ballot(slot == s);
+                if (slot == s) rank = run[s] + __popcll(m & lt);
+                run[s] += __popcll(m);
+            }
+            if (kk < k1) key[kk] = slot < 0 ? -1 : ((slot << 24) | rank);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < ROUTE_SLOTS; ++s) cnt[c * ROUTE_SLOTS + s] = run[s];
+    }
+}
+
 // ============================================================ bases
 // Per destination cell d: new total, then the base offset of every
 // (source cell, slot) entry feeding d in canonical order.  prod_cnt may be
@@ -1456,6 +1505,11 @@ void launch_sort_count(const DevConst<R> &k, const PView<R> &p, int isp, const u
     hipLaunchKernelGGL(k_sort_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag, key, cnt, cc);
 }
 template <typename R>
+void launch_push_count(const DevConst<R> &k, const PView<R> &p, int isp, uint8_t *tag, int *key, int *cnt, int *cc,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_push_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag, key, cnt, cc);
+}
+template <typename R>
 void launch_sort_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const uint8_t *tag,
                          const int *key, const int *base, int cap, int *err, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_scatter<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, q, tag, key, base, cap, err);
@@ -1633,6 +1687,8 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
 
 #define INST(R)                                                                                                         \
     template void launch_push<R>(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);                 \
+    template void launch_push_count<R>(const DevConst<R> &, const PView<R> &, int, uint8_t *, int *, int *, int *,     \
+                                       hipStream_t);                                                                  \
     template void launch_sort_count<R>(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *,      \
                                        int *, hipStream_t);                                                            \
     template void launch_sort_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const uint8_t *,     \

@@ -91,6 +91,8 @@ struct VoltConst {
 
 template <typename R> void launch_push(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);
 template <typename R>
+void launch_push_count(const DevConst<R> &, const PView<R> &, int, uint8_t *, int *, int *, int *, hipStream_t);
+template <typename R>
 void launch_sort_count(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *, int *, hipStream_t);
 template <typename R>
 void launch_sort_scatter(const DevConst<R> &, const PView<R> &, const PView<R> &, const uint8_t *, const int *,
