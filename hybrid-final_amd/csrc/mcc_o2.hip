@@ -15,7 +15,8 @@
 //            global memory once per wave
 //   victims  per cell with electron detachments: removes that many O- of
 //            the cell, smallest Philox keys first
-//   totals   per node: survivors plus products of every species (then scans)
+//   totals   per node: survivors plus products of every species (one grid
+//            row per species, one wave per four cells; then scans)
 //   scatter  every species in one launch (one grid row per target species,
 //            one wave per four cells): survivors in old order, then products
 //            by (source species, parent, slot); new ids follow the global
@@ -445,22 +446,50 @@ __global__ __launch_bounds__(256) void k_o2_victims(O2Const<R> k, O2Set<R> set, 
 
 // per node and species: survivors plus products (total is nsp x G; nodes
 // that start no cell keep the value already there)
+// One wave per (O2_CPW consecutive cells, species t): the survivors of t in
+// those cells, walked as one range and counted per cell, plus the cell's
+// products that become t.
 template <typename R>
 __global__ __launch_bounds__(256) void k_o2_totals(O2Const<R> k, O2Set<R> set, const int *pcount, int *total) {
-    int lane;
-    const int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
-    if (c < 0) return;
-    for (int t = 0; t < k.nsp; ++t) {
-        const PView<R> p = set.p[t];
-        const uint8_t *fate = set.fate[t];
-        int k0, k1;
-        cell_range(p, c, k0, k1);
-        int n = 0;
-        for (int b = k0; b < k1; b += 64) {
-            const int kk = b + lane;
-            n += __popcll(__ballot(kk < k1 && !(fate[kk] & (O2F_DEAD | O2F_VICTIM))));
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.y;
+    const int w0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)) * O2_CPW;
+    if (w0 >= k.Csize || t >= k.nsp) return;
+    const PView<R> p = set.p[t];
+    const uint8_t *fate = set.fate[t];
+    int cell[O2_CPW], lo[O2_CPW + 1], n[O2_CPW];
+#pragma unroll
+    for (int q = 0; q < O2_CPW; ++q) {
+        const int wq = w0 + q;
+        cell[q] = wq < k.Csize ? (wq / k.ncy) * k.ngy + (wq % k.ncy) : -1;
+        n[q] = 0;
+    }
+    {
+        int a0 = 0, a1 = 0;
+        cell_range(p, cell[0], a0, a1);
+        lo[0] = a0;
+        lo[1] = a1;
+#pragma unroll
+        for (int q = 1; q < O2_CPW; ++q) {
+            int b0 = lo[q], b1 = lo[q];
+            if (cell[q] >= 0) cell_range(p, cell[q], b0, b1);
+            lo[q + 1] = max(b1, lo[q]);
         }
-        if (lane == 0) total[(size_t)t * k.G + c] = n + pcount[(size_t)t * k.G + c];
+    }
+    const int hi = lo[O2_CPW];
+    for (int b = lo[0]; b < hi; b += 64) {
+        const int kk = b + lane;
+        int kq = 0;
+#pragma unroll
+        for (int q = 1; q < O2_CPW; ++q) kq += kk >= lo[q] ? 1 : 0;
+        const bool alive = kk < hi && !(fate[kk] & (O2F_DEAD | O2F_VICTIM));
+#pragma unroll
+        for (int q = 0; q < O2_CPW; ++q) n[q] += __popcll(__ballot(alive && kq == q));
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < O2_CPW; ++q)
+            if (cell[q] >= 0) total[(size_t)t * k.G + cell[q]] = n[q] + pcount[(size_t)t * k.G + cell[q]];
     }
 }
 
@@ -612,7 +641,8 @@ void launch_o2_victims(const O2Const<R> &k, const O2Set<R> &set, uint32_t step, 
 }
 template <typename R>
 void launch_o2_totals(const O2Const<R> &k, const O2Set<R> &set, const int *pcount, int *total, hipStream_t s) {
-    hipLaunchKernelGGL(k_o2_totals<R>, dim3(wave_blocks(k.Csize)), dim3(64 * O2_WAVES), 0, s, k, set, pcount, total);
+    const int nb = (k.Csize + O2_CPW * O2_WAVES - 1) / (O2_CPW * O2_WAVES);
+    hipLaunchKernelGGL(k_o2_totals<R>, dim3(nb, k.nsp), dim3(64 * O2_WAVES), 0, s, k, set, pcount, total);
 }
 template <typename R>
 void launch_o2_scatter_all(const O2Const<R> &k, const O2Set<R> &set, const O2Set<R> &dst, const int *pstart,
