@@ -672,13 +672,27 @@ struct Case : CaseBase {
     // destination count in one pass, then the rest of the sort.  The same
     // result as move() then sort_boundary() (no species' sort depends on
     // another's push); the stage entry points keep the two passes apart.
+    // The bookkeeping between the two passes runs for all species at once:
+    // one totals launch, the scans three arrays per pass, one bases launch
+    // (the per-species totals live in o2total, which the collision step
+    // clears before its own use).
     int move_sort() {
+        SortSpecies j{};
+        const int *in[CHEM_MAXSP];
+        int *out[CHEM_MAXSP];
         for (int s = 0; s < nsp; ++s) {
             launch_push_count<R>(kc, cur[s], s, tag[s], key[s], cnt[s], condcharge, st);
-            launch_totals(G, inv_start, inv_ent, cnt[s], nullptr, total, alive, nullptr, st);
-            launch_scan(total, nxt[s].cell_start, G, scan_tmp, st);
-            launch_assign_bases(G, inv_start, inv_ent, cnt[s], nullptr, nxt[s].cell_start, alive, nullptr, base[s],
-                                nullptr, nullptr, st);
+            j.cnt[s] = cnt[s];
+            j.total[s] = o2total + (size_t)s * G;
+            j.start[s] = nxt[s].cell_start;
+            j.base[s] = base[s];
+            in[s] = j.total[s];
+            out[s] = nxt[s].cell_start;
+        }
+        launch_totals_sp(G, nsp, inv_start, inv_ent, j, st);
+        for (int a = 0; a < nsp; a += 3) launch_scan_multi(std::min(3, nsp - a), in + a, out + a, G, scan_tmp, st);
+        launch_assign_bases_sp(G, nsp, inv_start, inv_ent, j, st);
+        for (int s = 0; s < nsp; ++s) {
             launch_sort_scatter<R>(kc, cur[s], nxt[s], tag[s], key[s], base[s], nxt[s].cap, d_err, st);
             swap_species(s);
         }

@@ -480,6 +480,33 @@ __global__ __launch_bounds__(BASES_TILE) void k_pair_scan_bases_m(const PairBase
 }
 
 
+// Totals and bases of the staged sort for every species in one launch each
+// (no products; grid row = species): the same per-destination sums and
+// entry bases as k_totals and k_assign_bases.
+__global__ void k_totals_sp(int Gsize, const int *inv_start, const int *inv_ent, SortSpecies j) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (d >= Gsize) return;
+    const int *cnt = j.cnt[s];
+    int a = 0;
+    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) a += cnt[inv_ent[e]];
+    j.total[s][d] = a;
+}
+
+__global__ void k_assign_bases_sp(int Gsize, const int *inv_start, const int *inv_ent, SortSpecies j) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (d >= Gsize) return;
+    const int *cnt = j.cnt[s];
+    int *base = j.base[s];
+    int run = j.start[s][d];
+    for (int e = inv_start[d]; e < inv_start[d + 1]; ++e) {
+        const int ent = inv_ent[e];
+        base[ent] = run;
+        run += cnt[ent];
+    }
+}
+
 // ============================================================ staged: Sort scatter
 template <typename R>
 __global__ __launch_bounds__(256) void k_sort_scatter(DevConst<R> k, PView<R> p, PView<R> q, const uint8_t *tag,
@@ -1674,6 +1701,15 @@ void launch_bases_pair(int G, const int *inv_start, const int *inv_ent, const in
 int scan_tmp_ints(int n) { return 3 * ((n + BASES_TILE - 1) / BASES_TILE) + 3; }
 // exclusive scans of k <= 3 arrays of n ints: out[a][0..n], out[a][n] = total;
 // tmp holds scan_tmp_ints(n) ints
+void launch_totals_sp(int Gsize, int nsp, const int *inv_start, const int *inv_ent, const SortSpecies &j,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_totals_sp, dim3(node_blocks(Gsize), nsp), dim3(256), 0, s, Gsize, inv_start, inv_ent, j);
+}
+void launch_assign_bases_sp(int Gsize, int nsp, const int *inv_start, const int *inv_ent, const SortSpecies &j,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_assign_bases_sp, dim3(node_blocks(Gsize), nsp), dim3(256), 0, s, Gsize, inv_start, inv_ent,
+                       j);
+}
 void launch_scan_multi(int k, const int *const *in, int *const *out, int n, int *tmp, hipStream_t s) {
     const int nt = std::max(1, (n + SCAN_TILE - 1) / SCAN_TILE);
     ScanJobs J{};

@@ -140,6 +140,18 @@ void launch_assign_bases(int, const int *, const int *, const int *, const int *
 void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s);
 int scan_tmp_ints(int n);
 void launch_scan_multi(int k, const int *const *in, int *const *out, int n, int *tmp, hipStream_t s);
+// the staged sort's per-species arrays, for its all-species launches
+constexpr int SORT_MAXSP = 8;
+struct SortSpecies {
+    const int *cnt[SORT_MAXSP];
+    int *total[SORT_MAXSP];
+    const int *start[SORT_MAXSP];
+    int *base[SORT_MAXSP];
+};
+void launch_totals_sp(int Gsize, int nsp, const int *inv_start, const int *inv_ent, const SortSpecies &j,
+                      hipStream_t s);
+void launch_assign_bases_sp(int Gsize, int nsp, const int *inv_start, const int *inv_ent, const SortSpecies &j,
+                            hipStream_t s);
 // fused step, between the passes: per-destination totals of both species,
 // their scans (cell_start of both species, product order) and the base of
 // every (source cell, slot) entry; tmp holds scan_tmp_ints(G) ints
