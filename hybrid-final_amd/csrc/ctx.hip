@@ -677,11 +677,14 @@ struct Case : CaseBase {
     // (the per-species totals live in o2total, which the collision step
     // clears before its own use).
     int move_sort() {
+        if (nsp > 5) return fail(-19, "staged whole steps take at most 5 species");
         SortSpecies j{};
+        StagedSet<R> set{};
         const int *in[CHEM_MAXSP];
         int *out[CHEM_MAXSP];
         for (int s = 0; s < nsp; ++s) {
-            launch_push_count<R>(kc, cur[s], s, tag[s], key[s], cnt[s], condcharge, st);
+            set.p[s] = cur[s]; set.q[s] = nxt[s]; set.tag[s] = tag[s]; set.key[s] = key[s]; set.cnt[s] = cnt[s];
+            set.base[s] = base[s];
             j.cnt[s] = cnt[s];
             j.total[s] = o2total + (size_t)s * G;
             j.start[s] = nxt[s].cell_start;
@@ -689,13 +692,12 @@ struct Case : CaseBase {
             in[s] = j.total[s];
             out[s] = nxt[s].cell_start;
         }
+        launch_push_count<R>(kc, set, nsp, condcharge, st);
         launch_totals_sp(G, nsp, inv_start, inv_ent, j, st);
         for (int a = 0; a < nsp; a += 3) launch_scan_multi(std::min(3, nsp - a), in + a, out + a, G, scan_tmp, st);
         launch_assign_bases_sp(G, nsp, inv_start, inv_ent, j, st);
-        for (int s = 0; s < nsp; ++s) {
-            launch_sort_scatter<R>(kc, cur[s], nxt[s], tag[s], key[s], base[s], nxt[s].cap, d_err, st);
-            swap_species(s);
-        }
+        launch_sort_scatter_all<R>(kc, set, nsp, d_err, st);
+        for (int s = 0; s < nsp; ++s) swap_species(s);
         tag_valid = 0;
         HIPCHK(hipGetLastError());
         return 0;

@@ -137,14 +137,18 @@ __global__ __launch_bounds__(256) void k_sort_count(DevConst<R> k, PView<R> p, i
 
 // ============================================================ staged: Move + Sort count
 // k_push and k_sort_count in one pass while the particle is in registers
-// (whole staged steps): the same positions, tags, keys, counts and wall
-// side effects, one read of the particle instead of two.
+// (whole staged steps), every species in one launch (grid row = species):
+// the same positions, tags, keys, counts and wall side effects, one read of
+// the particle instead of two.
 template <typename R>
-__global__ __launch_bounds__(256) void k_push_count(DevConst<R> k, PView<R> p, int isp, uint8_t *tag, int *key,
-                                                    int *cnt, int *condcharge) {
+__global__ __launch_bounds__(256) void k_push_count(DevConst<R> k, StagedSet<R> set, int *condcharge) {
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
+    const int isp = blockIdx.y;
+    const PView<R> p = set.p[isp];
+    uint8_t *tag = set.tag[isp];
+    int *key = set.key[isp], *cnt = set.cnt[isp];
     int k0, k1;
     cell_range(p, c, k0, k1);
     const unsigned long long lt = lanes_below(lane);
@@ -514,6 +518,33 @@ __global__ __launch_bounds__(256) void k_sort_scatter(DevConst<R> k, PView<R> p,
     int lane;
     int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
     if (c < 0) return;
+    int k0, k1;
+    cell_range(p, c, k0, k1);
+    for (int kk = k0 + lane; kk < k1; kk += 64) {
+        int ky = key[kk];
+        if (ky < 0) continue;
+        int slot = ky >> 24, rank = ky & 0xffffff;
+        int code = tag[kk];
+        Route rt = k.route[c * ROUTE_SLOTS + code];
+        R x = p.x[kk], y = p.y[kk], vx = p.vx[kk], vy = p.vy[kk];
+        apply_reflect<R>(rt.info, x, y, vx, vy);
+        int pos = base[c * ROUTE_SLOTS + slot] + rank;
+        if (pos >= cap) { atomicOr(err, 1); continue; }
+        q.x[pos] = x; q.y[pos] = y; q.vx[pos] = vx; q.vy[pos] = vy; q.vz[pos] = p.vz[kk]; q.id[pos] = p.id[kk];
+    }
+}
+
+// k_sort_scatter for every species in one launch (grid row = species)
+template <typename R>
+__global__ __launch_bounds__(256) void k_sort_scatter_all(DevConst<R> k, StagedSet<R> set, int *err) {
+    int lane;
+    int c = wave_cell(k.ncy, k.ngy, k.Csize, lane);
+    if (c < 0) return;
+    const int isp = blockIdx.y;
+    const PView<R> p = set.p[isp], q = set.q[isp];
+    const uint8_t *tag = set.tag[isp];
+    const int *key = set.key[isp], *base = set.base[isp];
+    const int cap = q.cap;
     int k0, k1;
     cell_range(p, c, k0, k1);
     for (int kk = k0 + lane; kk < k1; kk += 64) {
@@ -1532,9 +1563,12 @@ void launch_sort_count(const DevConst<R> &k, const PView<R> &p, int isp, const u
     hipLaunchKernelGGL(k_sort_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag, key, cnt, cc);
 }
 template <typename R>
-void launch_push_count(const DevConst<R> &k, const PView<R> &p, int isp, uint8_t *tag, int *key, int *cnt, int *cc,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_push_count<R>, dim3(cell_blocks(k.Csize)), dim3(256), 0, s, k, p, isp, tag, key, cnt, cc);
+void launch_push_count(const DevConst<R> &k, const StagedSet<R> &set, int nsp, int *cc, hipStream_t s) {
+    hipLaunchKernelGGL(k_push_count<R>, dim3(cell_blocks(k.Csize), nsp), dim3(256), 0, s, k, set, cc);
+}
+template <typename R>
+void launch_sort_scatter_all(const DevConst<R> &k, const StagedSet<R> &set, int nsp, int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_scatter_all<R>, dim3(cell_blocks(k.Csize), nsp), dim3(256), 0, s, k, set, err);
 }
 template <typename R>
 void launch_sort_scatter(const DevConst<R> &k, const PView<R> &p, const PView<R> &q, const uint8_t *tag,
@@ -1723,8 +1757,8 @@ void launch_scan(const int *in, int *out, int n, int *tmp, hipStream_t s) {
 
 #define INST(R)                                                                                                         \
     template void launch_push<R>(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);                 \
-    template void launch_push_count<R>(const DevConst<R> &, const PView<R> &, int, uint8_t *, int *, int *, int *,     \
-                                       hipStream_t);                                                                  \
+    template void launch_push_count<R>(const DevConst<R> &, const StagedSet<R> &, int, int *, hipStream_t);          \
+    template void launch_sort_scatter_all<R>(const DevConst<R> &, const StagedSet<R> &, int, int *, hipStream_t);    \
     template void launch_sort_count<R>(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *,      \
                                        int *, hipStream_t);                                                            \
     template void launch_sort_scatter<R>(const DevConst<R> &, const PView<R> &, const PView<R> &, const uint8_t *,     \

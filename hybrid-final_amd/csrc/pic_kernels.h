@@ -90,8 +90,19 @@ struct VoltConst {
 };
 
 template <typename R> void launch_push(const DevConst<R> &, const PView<R> &, int, uint8_t *, hipStream_t);
+// every species of a staged step, for the all-species launches of a whole
+// step (push + count, sort scatter)
 template <typename R>
-void launch_push_count(const DevConst<R> &, const PView<R> &, int, uint8_t *, int *, int *, int *, hipStream_t);
+struct StagedSet {
+    PView<R> p[5], q[5];
+    uint8_t *tag[5];
+    int *key[5], *cnt[5];
+    const int *base[5];
+};
+template <typename R>
+void launch_push_count(const DevConst<R> &, const StagedSet<R> &, int nsp, int *condcharge, hipStream_t);
+template <typename R>
+void launch_sort_scatter_all(const DevConst<R> &, const StagedSet<R> &, int nsp, int *err, hipStream_t);
 template <typename R>
 void launch_sort_count(const DevConst<R> &, const PView<R> &, int, const uint8_t *, int *, int *, int *, hipStream_t);
 template <typename R>
