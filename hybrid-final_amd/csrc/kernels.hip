@@ -152,9 +152,7 @@ __global__ __launch_bounds__(256) void k_push_count(DevConst<R> k, StagedSet<R> 
     int k0, k1;
     cell_range(p, c, k0, k1);
     const unsigned long long lt = lanes_below(lane);
-    int run[ROUTE_SLOTS];
-#pragma unroll
-    for (int s = 0; s < ROUTE_SLOTS; ++s) run[s] = 0;
+    int run = 0;   // lane s < ROUTE_SLOTS: survivors of slot s so far
     if (k0 < k1) {
         CellE<R> e;
         e.load(k.Ex, k.Ey, c, k.ngy);
@@ -172,20 +170,23 @@ __global__ __launch_bounds__(256) void k_push_count(DevConst<R> k, StagedSet<R> 
                 if (rt.dest >= 0) slot = rt.info & 15;
                 else wall_effects<R>(k, rt, isp, c, code, x, y, vx, vy, condcharge);
             }
+            // ranks over the slots present in these 64 (usually 2-4 of
+            // the 9): lane s holds slot s's running count
             int rank = -1;
-#pragma unroll
-            for (int s = 0; s < ROUTE_SLOTS; ++s) {
-                const unsigned long long m = __ballot(slot == s);
-                if (slot == s) rank = run[s] + __popcll(m & lt);
-                run[s] += __popcll(m);
+            unsigned long long pend = __ballot(slot >= 0);
+            while (pend) {
+                const int first = __builtin_ctzll(pend);
+                const int sl = __builtin_amdgcn_readfirstlane(__shfl(slot, first, 64));
+                const unsigned long long m = __ballot(slot == sl);
+                const int before = __builtin_amdgcn_readlane(run, sl);
+                if (slot == sl) rank = before + __popcll(m & lt);
+                if (lane == sl) run += __popcll(m);
+                pend &= ~m;
             }
             if (kk < k1) key[kk] = slot < 0 ? -1 : ((slot << 24) | rank);
         }
     }
-    if (lane == 0) {
-#pragma unroll
-        for (int s = 0; s < ROUTE_SLOTS; ++s) cnt[c * ROUTE_SLOTS + s] = run[s];
-    }
+    if (lane < ROUTE_SLOTS) cnt[c * ROUTE_SLOTS + lane] = run;
 }
 
 // ============================================================ bases
